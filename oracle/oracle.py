@@ -1,0 +1,218 @@
+"""Oracle front end: exact lowering (lowering.py) + C numerics (pi_oracle.c).
+
+TEST INFRASTRUCTURE ONLY -- used by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker, never by the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from . import lowering as low
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libpi_oracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+        _LIB.oracle_steady_state.restype = C.c_int
+        _LIB.oracle_expv.restype = C.c_int
+        _LIB.oracle_solve_dynamics.restype = C.c_int
+        _LIB.oracle_operator_scale.restype = C.c_double
+    return _LIB
+
+
+class _CSCSet(C.Structure):
+    _fields_ = [("cp_base", C.c_void_p), ("colptr", C.c_void_p), ("rows", C.c_void_p), ("vals", C.c_void_p)]
+
+
+class _Plan(C.Structure):
+    _fields_ = [
+        ("ns", C.c_int32), ("dims", C.c_void_p), ("offs", C.c_void_p), ("n", C.c_int64),
+        ("has_h", C.c_int32), ("has_q", C.c_int32), ("H", _CSCSet), ("Q", _CSCSet),
+        ("ncg", C.c_int32), ("cg_scale", C.c_void_p), ("K", _CSCSet),
+        ("nob", C.c_int32), ("ob_scale", C.c_void_p), ("ob_br_base", C.c_void_p),
+        ("br_out", C.c_void_p), ("br_in", C.c_void_p), ("br_r", C.c_void_p),
+        ("br_use_support", C.c_void_p), ("br_sup_base", C.c_void_p),
+        ("sup_rows", C.c_void_p), ("sup_cols", C.c_void_p), ("sup_vals", C.c_void_p),
+        ("br_dense_base", C.c_void_p), ("dense", C.c_void_p), ("max_dim", C.c_int64),
+    ]
+
+
+class _SteadyInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("restarts", C.c_int64), ("applications", C.c_int64),
+                ("converged", C.c_int32), ("residual", C.c_double), ("normalized_residual", C.c_double),
+                ("trace_error", C.c_double), ("operator_scale", C.c_double), ("linear_residual", C.c_double)]
+
+
+class _ExpvInfo(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("accepted", C.c_int64), ("rejected", C.c_int64),
+                ("applications", C.c_int64), ("factorizations", C.c_int64), ("attempts", C.c_int64),
+                ("reached", C.c_double), ("estimated_error", C.c_double)]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _cscset(blocks):
+    cp_base, colptr, rows, vals = [], [], [], []
+    for B in blocks:
+        cp_base.append(len(colptr))
+        base = len(rows)
+        colptr.extend(base + p for p in B.colptr)
+        rows.extend(B.rows)
+        vals.extend(B.vals)
+    arrs = (np.array(cp_base, dtype=np.int64), np.array(colptr, dtype=np.int64),
+            np.array(rows, dtype=np.int32), np.array(vals, dtype=np.complex128))
+    return arrs
+
+
+class OraclePlan:
+    """Flattened FusedStaticPIKernel handed to the C oracle."""
+
+    def __init__(self, plan: low.FusedPlan):
+        b = plan.basis
+        self.basis = b
+        self.plan = plan
+        keep = []
+        ns = len(b.sectors)
+        dims = np.array(b.dims, dtype=np.int64)
+        offs = np.array(b.offsets[:-1], dtype=np.int64)
+        keep += [dims, offs]
+        P = _Plan()
+        P.ns = ns
+        P.dims = _ptr(dims)
+        P.offs = _ptr(offs)
+        P.n = b.n
+        P.max_dim = int(max(b.dims))
+
+        def setcsc(field, blocks):
+            arrs = _cscset(blocks)
+            keep.extend(arrs)
+            setattr(P, field, _CSCSet(_ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]), _ptr(arrs[3])))
+
+        P.has_h = plan.H is not None
+        P.has_q = plan.Q is not None
+        empty = [low.CSC(m, m, [0] * (m + 1), [], []) for m in b.dims]
+        setcsc("H", plan.H if plan.H is not None else empty)
+        setcsc("Q", plan.Q if plan.Q is not None else empty)
+        P.ncg = len(plan.collective_gains)
+        cg_scale = np.array([g[0] for g in plan.collective_gains] or [0], dtype=np.complex128)
+        keep.append(cg_scale)
+        P.cg_scale = _ptr(cg_scale)
+        setcsc("K", [blk for g in plan.collective_gains for blk in g[1]] or empty)
+        P.nob = len(plan.onebody_gains)
+        ob_scale = np.array([g[0] for g in plan.onebody_gains] or [0], dtype=np.complex128)
+        base = [0]
+        br_out, br_in, br_r, use, sup_base, sup_rows, sup_cols, sup_vals, dbase, dense = (
+            [], [], [], [], [0], [], [], [], [], [])
+        dense_len = 0
+        for _, branches in plan.onebody_gains:
+            for (li, ni, r, Cm, u, sup) in branches:
+                br_out.append(li)
+                br_in.append(ni)
+                br_r.append(r)
+                use.append(1 if u else 0)
+                for (row, col, v) in (sup if u else ()):
+                    sup_rows.append(row)
+                    sup_cols.append(col)
+                    sup_vals.append(v)
+                sup_base.append(len(sup_rows))
+                dbase.append(dense_len)
+                if not u:
+                    dense.append(np.asarray(Cm, dtype=np.complex128).reshape(-1, order="F"))
+                    dense_len += Cm.size
+            base.append(len(br_out))
+        arrs = dict(
+            ob_scale=ob_scale, ob_br_base=np.array(base, dtype=np.int64),
+            br_out=np.array(br_out or [0], dtype=np.int32), br_in=np.array(br_in or [0], dtype=np.int32),
+            br_r=np.array(br_r or [0], dtype=np.float64), br_use_support=np.array(use or [0], dtype=np.int32),
+            br_sup_base=np.array(sup_base, dtype=np.int64), sup_rows=np.array(sup_rows or [0], dtype=np.int32),
+            sup_cols=np.array(sup_cols or [0], dtype=np.int32),
+            sup_vals=np.array(sup_vals or [0], dtype=np.complex128),
+            br_dense_base=np.array(dbase or [0], dtype=np.int64),
+            dense=(np.concatenate(dense) if dense else np.zeros(1, dtype=np.complex128)))
+        for k, v in arrs.items():
+            keep.append(v)
+            setattr(P, k, _ptr(v))
+        self._keep = keep
+        self._P = P
+        t_idx, t_val = low.trace_functional(b)
+        self.t_idx = t_idx
+        self.t_val = t_val
+
+    @property
+    def n(self):
+        return self.basis.n
+
+    def apply(self, x, nthreads=1):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.empty_like(x)
+        if nthreads > 1:
+            lib().oracle_apply_threaded(C.byref(self._P), _ptr(x), _ptr(y), C.c_int(nthreads))
+        else:
+            lib().oracle_apply(C.byref(self._P), _ptr(x), _ptr(y))
+        return y
+
+    def operator_scale(self, nthreads=1):
+        return lib().oracle_operator_scale(C.byref(self._P), C.c_int(nthreads))
+
+    def steady_state(self, krylovdim=30, maxiter=500, atol=0.0, rtol=1e-10, operator_scale=0.0, nthreads=1):
+        x = np.zeros(self.n, dtype=np.complex128)
+        info = _SteadyInfo()
+        lib().oracle_steady_state(C.byref(self._P), C.c_int(nthreads), _ptr(self.t_idx), _ptr(self.t_val),
+                                  C.c_int64(len(self.t_idx)), C.c_int(krylovdim), C.c_int64(maxiter),
+                                  C.c_double(atol), C.c_double(rtol), C.c_double(operator_scale), _ptr(x),
+                                  C.byref(info))
+        return x, {k: getattr(info, k) for k, _ in _SteadyInfo._fields_}
+
+    def expv(self, b, t, krylovdim=30, atol=1e-10, rtol=1e-8, safety=0.9, max_steps=10000, nthreads=1):
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        y = np.empty_like(b)
+        info = _ExpvInfo()
+        lib().oracle_expv(C.byref(self._P), C.c_int(nthreads), _ptr(y), _ptr(b), C.c_double(t),
+                          C.c_int(krylovdim), C.c_double(atol), C.c_double(rtol), C.c_double(safety),
+                          C.c_int64(max_steps), C.byref(info))
+        return y, {k: getattr(info, k) for k, _ in _ExpvInfo._fields_}
+
+    def solve_dynamics(self, rho0, ts, observables, krylovdim=30, atol=1e-10, rtol=1e-8, safety=0.9,
+                       max_steps=10000, nthreads=1):
+        rho = np.array(rho0, dtype=np.complex128, copy=True)
+        ts = np.ascontiguousarray(ts, dtype=np.float64)
+        obs = np.ascontiguousarray(np.array(observables, dtype=np.complex128).reshape(len(observables), -1))
+        out = np.zeros((len(ts), len(observables)), dtype=np.complex128)
+        info = _ExpvInfo()
+        lib().oracle_solve_dynamics(C.byref(self._P), C.c_int(nthreads), _ptr(rho), _ptr(ts),
+                                    C.c_int64(len(ts)), _ptr(obs), C.c_int32(len(observables)), _ptr(out),
+                                    C.c_int(krylovdim), C.c_double(atol), C.c_double(rtol),
+                                    C.c_double(safety), C.c_int64(max_steps), C.byref(info))
+        return out, rho, {k: getattr(info, k) for k, _ in _ExpvInfo._fields_}
+
+
+def expm(A):
+    A = np.asfortranarray(A, dtype=np.complex128)
+    q = A.shape[0]
+    E = np.zeros((q, q), dtype=np.complex128, order="F")
+    lib().oracle_expm(C.c_int(q), _ptr(A), _ptr(E))
+    return E
+
+
+def compile_plan(N, d, terms):
+    basis = low.Basis(N, d)
+    geom = low.OneBodyGeometry(basis)
+    plan = low.compile_model(basis, terms, geom)
+    return OraclePlan(plan), geom
