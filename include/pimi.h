@@ -1,0 +1,225 @@
+/*
+ * pimi.h -- C ABI of the MI355X backend for the permutation-invariant (PI)
+ * Liouvillian hot path: Schur-block Liouvillian apply inside GMRES
+ * stationary-state solves and adaptive Krylov exponential dynamics.
+ *
+ * Plain pointers and sizes only; complex scalars are interleaved
+ * (re, im) pairs of double or float.  Every function returns a pimi_status;
+ * pimi_last_error() gives a thread-local message for the last failure.
+ *
+ * Reference interfaces each entry point replaces (PermutationalInvariantDynamics.jl):
+ *   pimi_compile              compile(model; backend=...)        src/liouvillian.jl:4040-4098
+ *                              with LiouvillianPlan lowering      src/liouvillian.jl:1290-1371,1700-1851
+ *   pimi_plan_create_fused    upload of an already lowered FusedStaticPIKernel
+ *                              (accelerate hook)                  src/accelerators.jl:348-386
+ *   pimi_apply                apply!(y, compiled, x, t, p, work) src/liouvillian.jl:3215-3251
+ *                              threaded_apply!                    src/threaded_apply.jl:564-623
+ *   pimi_steady_gmres         krylov_steady_state / _gmres!      src/krylov.jl:732-891
+ *   pimi_expv_series          solve_dynamics(...; ExpvAlgorithm) src/highlevel.jl:559-733
+ *                              with krylov_expv!                  src/krylov_extensions.jl:1452-1568
+ *   pimi_trace_functional     _trace_functional                  src/source_protocol.jl:15-34
+ *   pimi_collective_operator  collective_operator                src/geometry.jl:1282-1308
+ */
+#ifndef PIMI_H
+#define PIMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t pimi_status;
+enum {
+    PIMI_OK = 0,
+    PIMI_EINVAL = 1,         /* ArgumentError */
+    PIMI_EDIM = 2,           /* DimensionMismatch */
+    PIMI_ENOMEM = 3,         /* memory budget or allocation failure (checked before allocating) */
+    PIMI_EDEVICE = 4,        /* HIP runtime error */
+    PIMI_EBUSY = 5,          /* workspace already in use by another caller */
+    PIMI_ENOTCONVERGED = 6,  /* solver did not converge; info is filled */
+    PIMI_ERANGE = 7          /* exact coefficient outside the supported exact range */
+};
+
+typedef enum { PIMI_C64 = 0, PIMI_C32 = 1 } pimi_precision;
+
+typedef struct pimi_context pimi_context;
+typedef struct pimi_plan pimi_plan;
+typedef struct pimi_workspace pimi_workspace;
+
+/* Model terms (src/terms.jl:118-160).  op/op2 are d*d complex matrices,
+ * column-major, interleaved doubles. */
+typedef enum {
+    PIMI_TERM_LOCAL_H = 0,          /* LocalHamiltonian(op; rate, hbar) */
+    PIMI_TERM_COLLECTIVE_H = 1,     /* CollectiveHamiltonian(op; rate, hbar) */
+    PIMI_TERM_LOCAL_JUMP = 2,       /* LocalJump(op; rate) */
+    PIMI_TERM_COLLECTIVE_JUMP = 3,  /* CollectiveJump(op; rate) */
+    PIMI_TERM_DIRECT_PRODUCT_H = 4  /* DirectPIHamiltonian(collective(op)*collective(op2); rate, hbar) */
+} pimi_term_kind;
+
+typedef struct {
+    int32_t kind;
+    int32_t reserved;
+    const double *op;
+    const double *op2;
+    double rate;
+    double hbar;
+} pimi_term;
+
+typedef struct {
+    int32_t N;
+    int32_t d;
+    int32_t nterms;
+    int32_t reserved;
+    const pimi_term *terms;
+} pimi_model_desc;
+
+/* A lowered FusedStaticPIKernel (src/liouvillian.jl:539-545) as flat arrays.
+ * CSC block k occupies colptr[cp_base[k] .. cp_base[k]+m], whose entries index
+ * rows/vals directly.  Collective-gain blocks are gain-major (g*ns + s).
+ * One-body branch b of gain g (ob_br_base[g] <= b < ob_br_base[g+1]) maps
+ * input sector br_in[b] into output sector br_out[b] with scale
+ * ob_scale[g]*br_r[b]; its contraction C (m_out x m_in) is given either as an
+ * exact support list (br_use_support[b] = 1, entries sup_base[b]..sup_base[b+1])
+ * or densely, column-major, at dense[br_dense_base[b]]. */
+typedef struct {
+    const int64_t *cp_base;
+    const int64_t *colptr;
+    const int32_t *rows;
+    const double *vals;   /* complex */
+} pimi_csc_set;
+
+typedef struct {
+    int32_t ns;
+    int32_t has_h;
+    int32_t has_q;
+    int32_t ncg;
+    int32_t nob;
+    int32_t reserved;
+    const int64_t *dims;
+    pimi_csc_set H, Q, K;
+    const double *cg_scale;       /* complex, ncg */
+    const double *ob_scale;       /* complex, nob */
+    const int64_t *ob_br_base;    /* nob+1 */
+    const int32_t *br_out, *br_in;
+    const double *br_r;
+    const int32_t *br_use_support;
+    const int64_t *br_sup_base;   /* nbr+1 */
+    const int32_t *sup_rows, *sup_cols;
+    const double *sup_vals;       /* complex */
+    const int64_t *br_dense_base; /* nbr */
+    const double *dense;          /* complex */
+} pimi_fused_desc;
+
+typedef struct {
+    int64_t n;              /* PI dimension */
+    int32_t nsectors;
+    int32_t max_block;
+    int64_t nterms;         /* stencil terms over all sectors */
+    int64_t stencil_entries;/* sum over outputs of A-row x B-row widths */
+    int64_t table_bytes_c64;/* device stencil table bytes (complex FP64 tables) */
+    int64_t table_bytes_c32;
+} pimi_plan_info;
+
+typedef enum {
+    PIMI_PRECOND_NONE = 0,   /* plain restarted GMRES, as the reference's default */
+    PIMI_PRECOND_SECTOR = 1  /* right preconditioning by the per-sector inverse of X -> G X + X G^H,
+                                G = -iH - Q/2 with Q replaced by its diagonal in the H eigenbasis */
+} pimi_preconditioner;
+
+typedef struct {
+    int32_t krylovdim;      /* GMRESAlgorithm(krylovdim=30, ...) */
+    int32_t preconditioner; /* pimi_preconditioner */
+    int64_t maxiter;        /* GMRESAlgorithm(maxiter=500) */
+    double atol, rtol;      /* convergence: |r| <= atol + rtol |b| */
+    double operator_scale;  /* <= 0: deterministic 3-probe estimate */
+    /* mixed precision (pimi_precision of the workspace = PIMI_C32):
+       inner tolerance and maximum outer refinement sweeps */
+    double inner_rtol;
+    int64_t max_refinements;
+} pimi_gmres_opts;
+
+typedef struct {
+    int32_t krylovdim;      /* ExpvAlgorithm(krylovdim=30, ...) */
+    int32_t reserved;
+    double atol, rtol, safety;
+    int64_t max_steps;
+} pimi_expv_opts;
+
+typedef struct {
+    int32_t converged;
+    int32_t reserved;
+    int64_t iterations, restarts, applications;
+    int64_t accepted, rejected, factorizations, refinements;
+    double residual;            /* |L x|_2 */
+    double normalized_residual; /* |L x|_2 / operator_scale */
+    double trace_error;         /* |t.x - 1| */
+    double operator_scale;
+    double linear_residual;
+    double estimated_error;
+    double seconds;
+} pimi_solve_info;
+
+const char *pimi_last_error(void);
+const char *pimi_version(void);
+
+pimi_status pimi_context_create(int32_t device_ordinal, pimi_context **out);
+void pimi_context_destroy(pimi_context *ctx);
+
+/* compile(model; backend=:mi355x): lower the model on the host (exact
+ * rationals) and upload the stencil tables once.  ctx == NULL gives a
+ * host-only plan (lowering, export, trace functional, collective operators;
+ * no workspaces). */
+pimi_status pimi_compile(pimi_context *ctx, const pimi_model_desc *model, pimi_plan **out);
+/* Upload an already lowered fused plan (e.g. the reference's own LiouvillianPlan). */
+pimi_status pimi_plan_create_fused(pimi_context *ctx, const pimi_fused_desc *desc, pimi_plan **out);
+pimi_status pimi_plan_info_get(const pimi_plan *plan, pimi_plan_info *info);
+/* dims[ns] per-sector block dimensions */
+pimi_status pimi_plan_dims(const pimi_plan *plan, int64_t *dims);
+/* trace functional: nnz = sum of dims; idx/vals host buffers of that length */
+pimi_status pimi_trace_functional(const pimi_plan *plan, int64_t *idx, double *vals);
+/* PI coefficient vector of sum_i X_i for a d x d complex X (host in, host out) */
+pimi_status pimi_collective_operator(const pimi_plan *plan, const double *X, double *out);
+/* fused plan export of a compiled model (host arrays); lengths via pimi_plan_export_sizes */
+pimi_status pimi_plan_export_sizes(const pimi_plan *plan, int64_t *sizes /* 12 entries */);
+pimi_status pimi_plan_export(const pimi_plan *plan, int64_t *cp_base, int64_t *colptr, int32_t *rows,
+                             double *vals, double *cg_scale, double *ob_scale, int64_t *ob_br_base,
+                             int32_t *br_out, int32_t *br_in, double *br_r, int32_t *br_use_support,
+                             int64_t *br_sup_base, int32_t *sup_rows, int32_t *sup_cols, double *sup_vals,
+                             int64_t *br_dense_base, double *dense);
+void pimi_plan_destroy(pimi_plan *plan);
+
+/* One workspace per concurrent caller: owns a HIP stream and the Krylov basis. */
+pimi_status pimi_workspace_create(pimi_plan *plan, int32_t precision, int32_t krylovdim, pimi_workspace **out);
+void pimi_workspace_destroy(pimi_workspace *ws);
+
+/* y = L x for ncols column vectors of length n (column-major).
+ * on_device = 1: x, y are device pointers (complex of the workspace
+ * precision); 0: host pointers (complex double). */
+pimi_status pimi_apply(pimi_workspace *ws, const void *x, void *y, int64_t ncols, int32_t on_device);
+
+/* Stationary state: solve (L/s + v t^H) rho = v from rho0 = v.  rho_out is a
+ * host buffer of n complex doubles. */
+pimi_status pimi_steady_gmres(pimi_workspace *ws, const pimi_gmres_opts *opts, double *rho_out,
+                              pimi_solve_info *info);
+
+/* Dynamics: rho(t_k) by expv over each interval, recording
+ * obs_out[k*nobs + o] = <obs_o, rho(t_k)>.  rho0/obs/rho_final host buffers. */
+pimi_status pimi_expv_series(pimi_workspace *ws, const double *rho0, const double *ts, int64_t nts,
+                             const pimi_expv_opts *opts, int32_t nobs, const double *obs, double *obs_out,
+                             double *rho_final, pimi_solve_info *info);
+
+/* Throughput probe: `reps` back-to-back applies to ncols synthetic columns
+ * resident on the device, timed with HIP events on the workspace stream;
+ * returns the average milliseconds per apply launch.  No counterpart in the
+ * reference (measurement only). */
+pimi_status pimi_apply_bench(pimi_workspace *ws, int64_t ncols, int32_t reps, double *ms_avg);
+
+/* Time of the last apply kernel launch sequence, in milliseconds (HIP events
+ * recorded on the workspace stream around pimi_apply). */
+pimi_status pimi_workspace_last_apply_ms(const pimi_workspace *ws, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

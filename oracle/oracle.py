@@ -171,13 +171,42 @@ class OraclePlan:
     def operator_scale(self, nthreads=1):
         return lib().oracle_operator_scale(C.byref(self._P), C.c_int(nthreads))
 
-    def steady_state(self, krylovdim=30, maxiter=500, atol=0.0, rtol=1e-10, operator_scale=0.0, nthreads=1):
+    def sector_preconditioner(self):
+        """(U, inv_den) in PI layout from LAPACK eigh of each fused
+        Hamiltonian block (independent of the product's eigensolver)."""
+        if getattr(self, "_pc", None) is None:
+            b = self.basis
+            U = np.zeros(b.n, dtype=np.complex128)
+            inv = np.zeros(b.n, dtype=np.complex128)
+            for s, m in enumerate(b.dims):
+                Hs = self.plan.H[s].dense() if self.plan.H is not None else np.zeros((m, m), complex)
+                Qs = self.plan.Q[s].dense() if self.plan.Q is not None else np.zeros((m, m), complex)
+                Hs = 0.5 * (Hs + Hs.conj().T)
+                d, Us = np.linalg.eigh(Hs)
+                q = np.real(np.einsum("ka,kc,ca->a", Us.conj(), Qs, Us))
+                den = -1j * (d[:, None] - d[None, :]) - 0.5 * (q[:, None] + q[None, :])
+                scale = max(np.max(np.abs(d) + np.abs(q)), 1e-300)
+                iv = np.where(np.abs(den) > 1e-13 * scale, 1.0 / np.where(den == 0, 1, den), 0)
+                o0, o1 = b.offsets[s], b.offsets[s + 1]
+                U[o0:o1] = Us.reshape(-1, order="F")
+                inv[o0:o1] = iv.reshape(-1, order="F")
+            self._pc = (U, inv)
+        return self._pc
+
+    def steady_state(self, krylovdim=30, maxiter=500, atol=0.0, rtol=1e-10, operator_scale=0.0, nthreads=1,
+                     preconditioner=None):
         x = np.zeros(self.n, dtype=np.complex128)
         info = _SteadyInfo()
+        pu = pi = None
+        if preconditioner == "sector":
+            U, inv = self.sector_preconditioner()
+            pu, pi = _ptr(U), _ptr(inv)
+        elif preconditioner is not None:
+            raise ValueError("unknown preconditioner")
         lib().oracle_steady_state(C.byref(self._P), C.c_int(nthreads), _ptr(self.t_idx), _ptr(self.t_val),
                                   C.c_int64(len(self.t_idx)), C.c_int(krylovdim), C.c_int64(maxiter),
                                   C.c_double(atol), C.c_double(rtol), C.c_double(operator_scale), _ptr(x),
-                                  C.byref(info))
+                                  C.byref(info), pu, pi)
         return x, {k: getattr(info, k) for k, _ in _SteadyInfo._fields_}
 
     def expv(self, b, t, krylovdim=30, atol=1e-10, rtol=1e-8, safety=0.9, max_steps=10000, nthreads=1):

@@ -314,6 +314,53 @@ double oracle_operator_scale(const oplan *P, int nthreads) {
     return scale;
 }
 
+/* ---------------- sector preconditioner ----------------
+ * Right preconditioner of the stationary solve (an addition of this build;
+ * the reference's GMRES accepts a preconditioner but has no such one):
+ * per sector, R -> scale * U ((U^H R U) .* inv_den) U^H with U the
+ * eigenvectors of the fused Hamiltonian block and inv_den[a,b] =
+ * 1/(-i(d_a-d_b) - (q_a+q_b)/2), q = diag(U^H Q U).  U and inv_den come in
+ * the PI block layout. */
+typedef struct { const cplx *U, *inv_den; double scale; } sector_pc;
+
+static void pc_apply(const oplan *P, const sector_pc *pc, const cplx *r, cplx *out, int nthreads) {
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int s = 0; s < P->ns; ++s) {
+        int64_t m = P->dims[s], o = P->offs[s];
+        const cplx *U = pc->U + o, *R = r + o, *Dn = pc->inv_den + o;
+        cplx *T1 = malloc(sizeof(cplx) * m * m), *T2 = malloc(sizeof(cplx) * m * m);
+        /* T1 = U^H R */
+        for (int64_t j = 0; j < m; ++j)
+            for (int64_t i = 0; i < m; ++i) {
+                cplx acc = 0;
+                for (int64_t k = 0; k < m; ++k) acc += conj(U[k + i * m]) * R[k + j * m];
+                T1[i + j * m] = acc;
+            }
+        /* T2 = (T1 U) .* inv_den * scale */
+        for (int64_t j = 0; j < m; ++j)
+            for (int64_t i = 0; i < m; ++i) {
+                cplx acc = 0;
+                for (int64_t k = 0; k < m; ++k) acc += T1[i + k * m] * U[k + j * m];
+                T2[i + j * m] = acc * Dn[i + j * m] * pc->scale;
+            }
+        /* T1 = U T2 */
+        for (int64_t j = 0; j < m; ++j)
+            for (int64_t i = 0; i < m; ++i) {
+                cplx acc = 0;
+                for (int64_t k = 0; k < m; ++k) acc += U[i + k * m] * T2[k + j * m];
+                T1[i + j * m] = acc;
+            }
+        /* out = T1 U^H */
+        for (int64_t j = 0; j < m; ++j)
+            for (int64_t i = 0; i < m; ++i) {
+                cplx acc = 0;
+                for (int64_t k = 0; k < m; ++k) acc += T1[i + k * m] * conj(U[j + k * m]);
+                out[o + i + j * m] = acc;
+            }
+        free(T1); free(T2);
+    }
+}
+
 /* ---------------- GMRES(m) with MGS2 (src/krylov.jl:732-786) ---------------- */
 
 typedef struct {
@@ -356,7 +403,7 @@ static void complex_givens(cplx a, cplx b, double *c, cplx *s) {
 
 /* Solve (apply) x = b by restarted GMRES; returns converged flag. */
 static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, double atol, double rtol,
-                 int64_t *iters_out, int64_t *restarts_out, double *res_out) {
+                 int64_t *iters_out, int64_t *restarts_out, double *res_out, const sector_pc *pc) {
     int64_t n = op->P->n;
     cplx *V = malloc(sizeof(cplx) * n * (m + 1));
     cplx *H = malloc(sizeof(cplx) * (m + 1) * m);
@@ -364,6 +411,7 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
     cplx *sn = malloc(sizeof(cplx) * m), *yv = malloc(sizeof(cplx) * m);
     double *cs = malloc(sizeof(double) * m);
     cplx *w = malloc(sizeof(cplx) * n), *r = malloc(sizeof(cplx) * n);
+    cplx *z = pc ? malloc(sizeof(cplx) * n) : NULL;
 #define Hij(i, j) H[(i) + (j) * (m + 1)]
     double bnorm = dznrm2(b, n);
     double tol = atol + rtol * bnorm;
@@ -381,7 +429,11 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
         int kdone = 0;
         int jmax = (int)((maxiter - iterations) < m ? (maxiter - iterations) : m);
         for (int j = 0; j < jmax; ++j) {
-            steady_apply(op, V + (int64_t)j * n, w);
+            if (pc) {
+                pc_apply(op->P, pc, V + (int64_t)j * n, z, op->nthreads);
+                steady_apply(op, z, w);
+            } else
+                steady_apply(op, V + (int64_t)j * n, w);
             for (int pass = 0; pass < 2; ++pass)
                 for (int i = 0; i <= j; ++i) {
                     cplx hij = zdot(V + (int64_t)i * n, w, n);
@@ -417,10 +469,15 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
             yv[i] = sacc / Hij(i, i);
         }
         for (int64_t k = 0; k < n; ++k) {
-            cplx z = 0;
-            for (int jj = 0; jj < kdone; ++jj) z += V[(int64_t)jj * n + k] * yv[jj];
-            x[k] += z;
+            cplx zz = 0;
+            for (int jj = 0; jj < kdone; ++jj) zz += V[(int64_t)jj * n + k] * yv[jj];
+            w[k] = zz;
         }
+        if (pc) {
+            pc_apply(op->P, pc, w, z, op->nthreads);
+            for (int64_t k = 0; k < n; ++k) x[k] += z[k];
+        } else
+            for (int64_t k = 0; k < n; ++k) x[k] += w[k];
         restarts++;
         steady_apply(op, x, w);
         for (int64_t i = 0; i < n; ++i) r[i] = b[i] - w[i];
@@ -430,7 +487,7 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
 done:
 #undef Hij
     *iters_out = iterations; *restarts_out = restarts; *res_out = beta;
-    free(V); free(H); free(g); free(sn); free(yv); free(cs); free(w); free(r);
+    free(V); free(H); free(g); free(sn); free(yv); free(cs); free(w); free(r); free(z);
     return conv;
 }
 
@@ -438,7 +495,7 @@ done:
  * operator_scale <= 0 selects the deterministic probe estimate. */
 int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const double *tvals, int64_t nt,
                         int krylovdim, int64_t maxiter, double atol, double rtol, double operator_scale,
-                        cplx *x_out, steady_info *info) {
+                        cplx *x_out, steady_info *info, const cplx *pc_U, const cplx *pc_inv) {
     int64_t n = P->n;
     cplx *tv = malloc(sizeof(cplx) * nt), *vv = malloc(sizeof(cplx) * nt);
     cplx tt = 0;
@@ -450,7 +507,8 @@ int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const
     for (int64_t k = 0; k < nt; ++k) b[tidx[k]] += vv[k];
     memcpy(x_out, b, sizeof(cplx) * n);
     int64_t it, rs; double res;
-    int conv = gmres(&op, x_out, b, krylovdim, maxiter, atol, rtol, &it, &rs, &res);
+    sector_pc pc = {pc_U, pc_inv, scale};
+    int conv = gmres(&op, x_out, b, krylovdim, maxiter, atol, rtol, &it, &rs, &res, pc_U ? &pc : NULL);
     cplx *w = malloc(sizeof(cplx) * n);
     do_apply(P, x_out, w, nthreads);
     double resid = dznrm2(w, n);

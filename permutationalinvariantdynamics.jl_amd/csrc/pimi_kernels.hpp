@@ -1,0 +1,196 @@
+// Device kernels for the PI Liouvillian apply and the Krylov BLAS-1/2 steps.
+//
+// Complex vectors are arrays of cx<R> (R = double or float), 8/16-byte pairs.
+// All reductions are fixed-shape trees (wave shuffles + LDS), so results are
+// deterministic from run to run.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pimi {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cx {
+    R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R>
+__device__ __forceinline__ cx<R> csub(cx<R> a, cx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R>
+__device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+// conj(a) * b
+template <typename R>
+__device__ __forceinline__ cx<R> cjmul(cx<R> a, cx<R> b) { return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }
+
+struct StencilTermDev {
+    int32_t in_sector;
+    int32_t flags;  // bit0: A identity, bit1: B identity
+    int32_t wa, wb;
+    int64_t a_base, b_base;
+    double s_re, s_im;
+};
+
+constexpr int kApplyBlock = 256;
+constexpr int kRedBlock = 256;
+constexpr int kNumXcd = 8;
+
+// y[i] = out_scale * (L x)[i] for one PI vector; one output coordinate per lane.
+// Tiles of kApplyBlock consecutive coordinates are spread so that XCD k
+// (workgroup id mod 8) handles one contiguous slab of tiles.
+template <typename R>
+__global__ __launch_bounds__(kApplyBlock) void k_apply(
+    const cx<R> *__restrict__ x, cx<R> *__restrict__ y, int64_t n, const int64_t *__restrict__ offs,
+    const int32_t *__restrict__ dims, const int32_t *__restrict__ tile_sector, int64_t ntiles,
+    int64_t tiles_per_xcd, const int64_t *__restrict__ term_base, const StencilTermDev *__restrict__ terms,
+    const int32_t *__restrict__ col, const cx<R> *__restrict__ val, R out_scale, int64_t ncols) {
+    const int64_t bid = blockIdx.x;
+    const int64_t tile = (bid % kNumXcd) * tiles_per_xcd + bid / kNumXcd;
+    if (tile >= ntiles) return;
+    const int64_t i = tile * kApplyBlock + threadIdx.x;
+    if (i >= n) return;
+    int s = tile_sector[tile];
+    while (i >= offs[s + 1]) ++s;
+    const int64_t m = dims[s];
+    const int64_t loc = i - offs[s];
+    const int64_t a = loc % m, b = loc / m;
+    const int64_t t0 = term_base[s], t1 = term_base[s + 1];
+    for (int64_t cidx = blockIdx.y; cidx < ncols; cidx += gridDim.y) {
+        const cx<R> *xc = x + cidx * n;
+        cx<R> acc = {0, 0};
+        for (int64_t t = t0; t < t1; ++t) {
+            const StencilTermDev T = terms[t];
+            const cx<R> *Xn = xc + offs[T.in_sector];
+            const int64_t mn = dims[T.in_sector];
+            cx<R> tacc = {0, 0};
+            if (T.flags & 1) {
+                for (int kb = 0; kb < T.wb; ++kb) {
+                    const int64_t q = T.b_base + kb * m + b;
+                    tacc = cadd(tacc, cjmul(val[q], Xn[a + (int64_t)col[q] * mn]));
+                }
+            } else if (T.flags & 2) {
+                for (int ka = 0; ka < T.wa; ++ka) {
+                    const int64_t q = T.a_base + ka * m + a;
+                    tacc = cadd(tacc, cmul(val[q], Xn[col[q] + b * mn]));
+                }
+            } else {
+                for (int kb = 0; kb < T.wb; ++kb) {
+                    const int64_t qb = T.b_base + kb * m + b;
+                    const int64_t dcol = (int64_t)col[qb] * mn;
+                    cx<R> inner = {0, 0};
+                    for (int ka = 0; ka < T.wa; ++ka) {
+                        const int64_t qa = T.a_base + ka * m + a;
+                        inner = cadd(inner, cmul(val[qa], Xn[col[qa] + dcol]));
+                    }
+                    tacc = cadd(tacc, cjmul(val[qb], inner));
+                }
+            }
+            acc = cadd(acc, cmul(cx<R>{(R)T.s_re, (R)T.s_im}, tacc));
+        }
+        y[cidx * n + i] = cx<R>{out_scale * acc.x, out_scale * acc.y};
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over the block; the result is valid in every thread.  `sh` must hold
+// blockDim.x/64 entries; callers separate consecutive uses with __syncthreads.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    T r = 0;
+    for (int k = 0; k < nw; ++k) r += sh[k];
+    return r;
+}
+
+// Grouped per-sector complex GEMM over PI-layout blocks:
+//   C_s = op(A_s) op(B_s) [.* D_s * dscale]   for every tile of every sector,
+// op = identity (0) or conjugate transpose (1).  A tile is a TS x TS patch of
+// one sector's m x m output block; tiles[k] = (sector, tile row, tile col).
+constexpr int kGemmTile = 32;
+
+struct GemmTile {
+    int32_t sector, ti, tj, pad;
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_sector_gemm(const cx<R> *__restrict__ A, const cx<R> *__restrict__ B,
+                                                     cx<R> *__restrict__ Cm, const int64_t *__restrict__ offs,
+                                                     const int32_t *__restrict__ dims,
+                                                     const GemmTile *__restrict__ tiles, int transA, int transB,
+                                                     const cx<R> *__restrict__ D, R dscale) {
+    constexpr int TS = kGemmTile;
+    __shared__ cx<R> As[TS][TS + 1];
+    __shared__ cx<R> Bs[TS][TS + 1];
+    const GemmTile t = tiles[blockIdx.x];
+    const int64_t m = dims[t.sector];
+    const int64_t base = offs[t.sector];
+    const int64_t row0 = (int64_t)t.ti * TS, col0 = (int64_t)t.tj * TS;
+    const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
+    cx<R> acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = cx<R>{0, 0};
+    for (int64_t k0 = 0; k0 < m; k0 += TS) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            int r, k;
+            cx<R> v = {0, 0};
+            if (transA == 0) {
+                r = e & 31; k = e >> 5;
+                const int64_t gr = row0 + r, gk = k0 + k;
+                if (gr < m && gk < m) v = A[base + gr + gk * m];
+            } else {
+                k = e & 31; r = e >> 5;
+                const int64_t gr = row0 + r, gk = k0 + k;
+                if (gr < m && gk < m) { v = A[base + gk + gr * m]; v.y = -v.y; }
+            }
+            As[r][k] = v;
+            int c;
+            cx<R> w = {0, 0};
+            if (transB == 0) {
+                k = e & 31; c = e >> 5;
+                const int64_t gk = k0 + k, gc = col0 + c;
+                if (gk < m && gc < m) w = B[base + gk + gc * m];
+            } else {
+                c = e & 31; k = e >> 5;
+                const int64_t gk = k0 + k, gc = col0 + c;
+                if (gk < m && gc < m) { w = B[base + gc + gk * m]; w.y = -w.y; }
+            }
+            Bs[k][c] = w;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int k = 0; k < TS; ++k) {
+            const cx<R> a = As[tx][k];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = cadd(acc[q], cmul(a, Bs[k][ty + 8 * q]));
+        }
+        __syncthreads();
+    }
+    const int64_t gr = row0 + tx;
+    if (gr >= m) return;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t gc = col0 + ty + 8 * q;
+        if (gc >= m) continue;
+        const int64_t o = base + gr + gc * m;
+        cx<R> v = acc[q];
+        if (D) {
+            const cx<R> dd = D[o];
+            v = cmul(v, cx<R>{dd.x * dscale, dd.y * dscale});
+        }
+        Cm[o] = v;
+    }
+}
+
+}  // namespace pimi

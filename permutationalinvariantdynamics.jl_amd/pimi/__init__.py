@@ -1,0 +1,13 @@
+"""pimi: MI355X backend for the permutation-invariant Liouvillian hot path.
+
+The numeric work runs in libpimi (HIP kernels for gfx950 behind the C ABI in
+include/pimi.h); this package mirrors the reference package's API around it.
+"""
+from ._native import PIMI_C32, PIMI_C64, NotConverged, PIMIError  # noqa: F401
+from .api import (  # noqa: F401
+    CollectiveHamiltonian, CollectiveJump, CollectiveOperator, CollectiveProduct, CompiledPIModel,
+    DirectPIHamiltonian, DynamicsStreamResult, ExpvAlgorithm, GMRESAlgorithm, LocalHamiltonian, LocalJump, Models,
+    PIBasis, PIModel, PIOperator, PIState, SteadyStateResult, collective_operator, compile,
+    computational_product_state, expect, qubit_ensemble_model, saved_times, solve_dynamics, spin_matrices,
+    stationary_state,
+)

@@ -1,0 +1,189 @@
+"""GPU parity: libpimi on device 0 against the oracle on the same inputs.
+
+Tolerances: complex FP64 apply <= 1e-13 relative (max-norm); FP32 apply
+<= 5e-6 relative; steady states and dynamics <= 1e-9 relative after both
+solvers converge to rtol 1e-12 with atol 0.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pimi
+import planutil as PU
+from oracle import lowering as L
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SP2 = pimi.spin_matrices(2)
+SP3 = pimi.spin_matrices(3)
+
+
+def _c2_model(N):
+    return pimi.qubit_ensemble_model(N, hamiltonian=SP2.jz + SP2.jx, emission=0.1, dephasing=0.2, pumping=0.02,
+                                     collective_emission=0.1)
+
+
+def _c2_terms():
+    sp = L.spin_matrices(2)
+    return L.qubit_ensemble_terms(hamiltonian=sp["jz"] + sp["jx"], emission=0.1, dephasing=0.2, pumping=0.02,
+                                  collective_emission=0.1)
+
+
+def _c1_model():
+    return pimi.Models.driven_qubits(8, drive=1.0, detuning=1.0, emission=0.1, dephasing=0.2, pumping=0.0)
+
+
+def _c1_terms():
+    return L.driven_qubits_terms(drive=1.0, detuning=1.0, emission=0.1, dephasing=0.2, pumping=0.0)
+
+
+def _qutrit(N):
+    B = pimi.PIBasis(N, 3)
+    Jz = pimi.collective_operator(B, SP3.jz)
+    model = pimi.PIModel(B, [pimi.LocalHamiltonian(1.0 * SP3.jz + 0.7 * SP3.jx),
+                             pimi.DirectPIHamiltonian(Jz * Jz, rate=0.3 / N), pimi.LocalJump(SP3.jm, rate=0.1)])
+    b = L.Basis(N, 3)
+    g = L.OneBodyGeometry(b)
+    s3 = L.spin_matrices(3)
+    jz = L.collective_operator(g, s3["jz"])
+    terms = [L.Term("local_h", 1.0 * s3["jz"] + 0.7 * s3["jx"]),
+             L.Term("direct_h", L.physical_blocks(b, L.pi_product(b, jz, jz)), 0.3 / N),
+             L.Term("local_jump", s3["jm"], 0.1)]
+    plan = O.OraclePlan(L.compile_model(b, terms, g))
+    return model, plan
+
+
+def _rand(n, seed, cols=None):
+    rng = np.random.default_rng(seed)
+    shape = (n,) if cols is None else (n, cols)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _relmax(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("N", [1, 2, 8, 20, 61, 120])
+def test_apply_qubits_fp64(N):
+    print(f"apply qubits N={N}", flush=True)
+    comp = pimi.compile(_c2_model(N))
+    ref, _ = O.compile_plan(N, 2, _c2_terms())
+    x = _rand(ref.n, N)
+    assert _relmax(comp.apply(x), ref.apply(x)) <= 1e-13
+
+
+@pytest.mark.parametrize("N", [2, 4, 8])
+def test_apply_qutrits_fp64(N):
+    print(f"apply qutrits N={N}", flush=True)
+    model, ref = _qutrit(N)
+    comp = pimi.compile(model)
+    x = _rand(ref.n, 100 + N)
+    assert _relmax(comp.apply(x), ref.apply(x)) <= 1e-13
+
+
+def test_apply_multicolumn_equals_columns():
+    comp = pimi.compile(_c2_model(20))
+    X = _rand(comp.info["n"], 3, cols=3)
+    Y = comp.apply(X)
+    for k in range(3):
+        np.testing.assert_array_equal(Y[:, k], comp.apply(X[:, k]))
+
+
+def test_apply_fp32():
+    comp = pimi.compile(_c2_model(40))
+    ref, _ = O.compile_plan(40, 2, _c2_terms())
+    x = _rand(ref.n, 5)
+    assert _relmax(comp.apply(x, precision=pimi.PIMI_C32), ref.apply(x)) <= 5e-6
+
+
+def test_fused_upload_path():
+    """The reference-form fused plan (here the oracle's lowering) uploaded
+    through pimi_plan_create_fused gives the same apply."""
+    import pimi._native as N
+    ref, _ = O.compile_plan(20, 2, _c2_terms())
+    P = ref._P
+    desc = N.FusedDesc()
+    desc.ns = P.ns
+    desc.has_h, desc.has_q, desc.ncg, desc.nob = P.has_h, P.has_q, P.ncg, P.nob
+    desc.dims = P.dims
+    for f in ("H", "Q", "K"):
+        s = getattr(P, f)
+        setattr(desc, f, N.CSCSet(s.cp_base, s.colptr, s.rows, s.vals))
+    for f in ("cg_scale", "ob_scale", "ob_br_base", "br_out", "br_in", "br_r", "br_use_support", "br_sup_base",
+              "sup_rows", "sup_cols", "sup_vals", "br_dense_base", "dense"):
+        setattr(desc, f, getattr(P, f))
+    ctx = pimi.api._context(0)
+    plan = C.c_void_p()
+    N.check(N.lib().pimi_plan_create_fused(ctx, C.byref(desc), C.byref(plan)))
+    ws = C.c_void_p()
+    N.check(N.lib().pimi_workspace_create(plan, 0, 4, C.byref(ws)))
+    x = np.ascontiguousarray(_rand(ref.n, 11))
+    y = np.empty_like(x)
+    N.check(N.lib().pimi_apply(ws, N.ptr(x), N.ptr(y), C.c_int64(1), C.c_int32(0)))
+    N.lib().pimi_workspace_destroy(ws)
+    N.lib().pimi_plan_destroy(plan)
+    assert _relmax(y, ref.apply(x)) <= 1e-13
+
+
+def test_apply_rejects_alias():
+    import pimi._native as N
+    comp = pimi.compile(_c2_model(4))
+    ws = comp.workspace()
+    x = np.zeros(comp.info["n"], dtype=complex)
+    with pytest.raises(pimi.PIMIError):
+        N.check(N.lib().pimi_apply(ws, N.ptr(x), N.ptr(x), C.c_int64(1), C.c_int32(0)))
+
+
+@pytest.mark.parametrize("N", [8, 20])
+def test_steady_state_parity(N):
+    print(f"steady N={N}", flush=True)
+    model = _c1_model() if N == 8 else _c2_model(N)
+    terms = _c1_terms() if N == 8 else _c2_terms()
+    comp = pimi.compile(model)
+    ref, _ = O.compile_plan(N, 2, terms)
+    res = pimi.stationary_state(comp, algorithm=pimi.GMRESAlgorithm(30, 5000), atol=0.0, rtol=1e-12,
+                                return_info=True)
+    x_ref, info_ref = ref.steady_state(krylovdim=30, maxiter=5000, atol=0.0, rtol=1e-12)
+    assert res.info["converged"] and info_ref["converged"]
+    assert abs(res.info["operator_scale"] - info_ref["operator_scale"]) <= 1e-12 * info_ref["operator_scale"]
+    err = np.linalg.norm(res.state.data - x_ref) / np.linalg.norm(x_ref)
+    assert err <= 1e-9, err
+    assert res.info["normalized_residual"] <= 1e-12 * max(np.linalg.norm(res.state.data), 1.0)
+
+
+def test_steady_state_not_converged_raises():
+    comp = pimi.compile(_c2_model(10))
+    with pytest.raises(pimi.NotConverged):
+        pimi.stationary_state(comp, algorithm=pimi.GMRESAlgorithm(5, 3), atol=0.0, rtol=1e-14)
+
+
+def test_expv_series_parity_c1():
+    comp = pimi.compile(_c1_model())
+    ref, g = O.compile_plan(8, 2, _c1_terms())
+    b = comp.basis
+    rho0 = pimi.computational_product_state(b, 2)
+    sp = L.spin_matrices(2)
+    obs_local = L.collective_operator(g, sp["jp"] @ sp["jm"])
+    jp = L.collective_operator(g, sp["jp"])
+    jm = L.collective_operator(g, sp["jm"])
+    obs_coll = L.pi_product(ref.basis, jp, jm)
+    alg = pimi.ExpvAlgorithm(krylovdim=30, atol=0.0, rtol=1e-12)
+    res = pimi.solve_dynamics(comp, rho0, (0.0, 4.0), saveat=0.1, algorithm=alg,
+                              observables={"n": SP2.jp @ SP2.jm, "JpJm": pimi.PIOperator(b, obs_coll)})
+    ts = res.t
+    assert len(ts) == 41
+    out, rho_ref, info = ref.solve_dynamics(rho0.data, ts, [obs_local, obs_coll], krylovdim=30, atol=0.0,
+                                            rtol=1e-12)
+    assert _relmax(res.values["n"], out[:, 0]) <= 1e-9
+    assert _relmax(res.values["JpJm"], out[:, 1]) <= 1e-9
+    assert np.linalg.norm(res.final_state.data - rho_ref) <= 1e-9 * np.linalg.norm(rho_ref)
+
+
+def test_collective_operator_matches_oracle():
+    comp = pimi.compile(_c2_model(12))
+    b = L.Basis(12, 2)
+    g = L.OneBodyGeometry(b)
+    for X in (SP2.jx, SP2.jz, SP2.jm):
+        np.testing.assert_array_equal(comp.collective_operator(X).data, L.collective_operator(g, X))
