@@ -137,7 +137,16 @@ typedef struct {
        inner tolerance and maximum outer refinement sweeps */
     double inner_rtol;
     int64_t max_refinements;
+    /* PIMI_CRITERION_REFERENCE: |r| <= atol + rtol |b| and |L x|/s <= atol + rtol max(|x|, 1)
+         (src/krylov.jl:743-746,868-873);
+       PIMI_CRITERION_BACKWARD:  |r| <= atol + rtol (|b| + |x|) and |L x|/(s |x|) <= atol + rtol,
+         the scale-invariant relative residual; at large N |b| = 2^(-N/2) is far below
+         eps |x|, where the reference rule cannot be met in floating point. */
+    int32_t criterion;
+    int32_t reserved2;
 } pimi_gmres_opts;
+
+enum { PIMI_CRITERION_REFERENCE = 0, PIMI_CRITERION_BACKWARD = 1 };
 
 typedef struct {
     int32_t krylovdim;      /* ExpvAlgorithm(krylovdim=30, ...) */
@@ -158,6 +167,7 @@ typedef struct {
     double linear_residual;
     double estimated_error;
     double seconds;
+    double relative_residual;   /* |L x|_2 / (operator_scale |x|_2) */
 } pimi_solve_info;
 
 const char *pimi_last_error(void);

@@ -194,7 +194,7 @@ class OraclePlan:
         return self._pc
 
     def steady_state(self, krylovdim=30, maxiter=500, atol=0.0, rtol=1e-10, operator_scale=0.0, nthreads=1,
-                     preconditioner=None):
+                     preconditioner=None, criterion="reference"):
         x = np.zeros(self.n, dtype=np.complex128)
         info = _SteadyInfo()
         pu = pi = None
@@ -206,7 +206,7 @@ class OraclePlan:
         lib().oracle_steady_state(C.byref(self._P), C.c_int(nthreads), _ptr(self.t_idx), _ptr(self.t_val),
                                   C.c_int64(len(self.t_idx)), C.c_int(krylovdim), C.c_int64(maxiter),
                                   C.c_double(atol), C.c_double(rtol), C.c_double(operator_scale), _ptr(x),
-                                  C.byref(info), pu, pi)
+                                  C.byref(info), pu, pi, C.c_int(1 if criterion == "backward" else 0))
         return x, {k: getattr(info, k) for k, _ in _SteadyInfo._fields_}
 
     def expv(self, b, t, krylovdim=30, atol=1e-10, rtol=1e-8, safety=0.9, max_steps=10000, nthreads=1):

@@ -402,8 +402,32 @@ static void complex_givens(cplx a, cplx b, double *c, cplx *s) {
 }
 
 /* Solve (apply) x = b by restarted GMRES; returns converged flag. */
+/* norm of the current iterate x + M^{-1} V y (or x + V y), y from the rotated
+ * least-squares problem of the first k Arnoldi steps */
+static double iterate_norm(steady_op *op, const cplx *x, const cplx *V, const cplx *H, const cplx *g, int m, int k,
+                           const sector_pc *pc, cplx *tmp, cplx *z) {
+    int64_t n = op->P->n;
+    cplx *yv = malloc(sizeof(cplx) * k);
+    for (int i = k - 1; i >= 0; --i) {
+        cplx acc = g[i];
+        for (int jj = i + 1; jj < k; ++jj) acc -= H[i + jj * (m + 1)] * yv[jj];
+        yv[i] = acc / H[i + i * (m + 1)];
+    }
+    for (int64_t t = 0; t < n; ++t) {
+        cplx zz = 0;
+        for (int jj = 0; jj < k; ++jj) zz += V[(int64_t)jj * n + t] * yv[jj];
+        tmp[t] = zz;
+    }
+    if (pc) pc_apply(op->P, pc, tmp, z, op->nthreads);
+    else memcpy(z, tmp, sizeof(cplx) * n);
+    for (int64_t t = 0; t < n; ++t) z[t] += x[t];
+    double r = dznrm2(z, n);
+    free(yv);
+    return r;
+}
+
 static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, double atol, double rtol,
-                 int64_t *iters_out, int64_t *restarts_out, double *res_out, const sector_pc *pc) {
+                 int64_t *iters_out, int64_t *restarts_out, double *res_out, const sector_pc *pc, int backward) {
     int64_t n = op->P->n;
     cplx *V = malloc(sizeof(cplx) * n * (m + 1));
     cplx *H = malloc(sizeof(cplx) * (m + 1) * m);
@@ -411,10 +435,11 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
     cplx *sn = malloc(sizeof(cplx) * m), *yv = malloc(sizeof(cplx) * m);
     double *cs = malloc(sizeof(double) * m);
     cplx *w = malloc(sizeof(cplx) * n), *r = malloc(sizeof(cplx) * n);
-    cplx *z = pc ? malloc(sizeof(cplx) * n) : NULL;
+    cplx *z = malloc(sizeof(cplx) * n), *tmp = malloc(sizeof(cplx) * n);
 #define Hij(i, j) H[(i) + (j) * (m + 1)]
     double bnorm = dznrm2(b, n);
-    double tol = atol + rtol * bnorm;
+    double xnorm = dznrm2(x, n);
+    double tol = atol + rtol * (backward ? bnorm + xnorm : bnorm);
     int64_t iterations = 0, restarts = 0;
     steady_apply(op, x, w);
     for (int64_t i = 0; i < n; ++i) r[i] = b[i] - w[i];
@@ -461,6 +486,10 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
             g[j + 1] = -conj(s) * g1 + c * g2;
             iterations++;
             kdone = j + 1;
+            if (backward && (kdone % 25 == 0 || cabs(g[j + 1]) <= tol)) {
+                xnorm = iterate_norm(op, x, V, H, g, m, kdone, pc, tmp, z);
+                tol = atol + rtol * (bnorm + xnorm);
+            }
             if (cabs(g[j + 1]) <= tol) break;
         }
         for (int i = kdone - 1; i >= 0; --i) {
@@ -482,12 +511,13 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
         steady_apply(op, x, w);
         for (int64_t i = 0; i < n; ++i) r[i] = b[i] - w[i];
         beta = dznrm2(r, n);
+        if (backward) { xnorm = dznrm2(x, n); tol = atol + rtol * (bnorm + xnorm); }
         if (beta <= tol) { conv = 1; break; }
     }
 done:
 #undef Hij
     *iters_out = iterations; *restarts_out = restarts; *res_out = beta;
-    free(V); free(H); free(g); free(sn); free(yv); free(cs); free(w); free(r); free(z);
+    free(V); free(H); free(g); free(sn); free(yv); free(cs); free(w); free(r); free(z); free(tmp);
     return conv;
 }
 
@@ -495,7 +525,7 @@ done:
  * operator_scale <= 0 selects the deterministic probe estimate. */
 int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const double *tvals, int64_t nt,
                         int krylovdim, int64_t maxiter, double atol, double rtol, double operator_scale,
-                        cplx *x_out, steady_info *info, const cplx *pc_U, const cplx *pc_inv) {
+                        cplx *x_out, steady_info *info, const cplx *pc_U, const cplx *pc_inv, int backward) {
     int64_t n = P->n;
     cplx *tv = malloc(sizeof(cplx) * nt), *vv = malloc(sizeof(cplx) * nt);
     cplx tt = 0;
@@ -508,7 +538,12 @@ int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const
     memcpy(x_out, b, sizeof(cplx) * n);
     int64_t it, rs; double res;
     sector_pc pc = {pc_U, pc_inv, scale};
-    int conv = gmres(&op, x_out, b, krylovdim, maxiter, atol, rtol, &it, &rs, &res, pc_U ? &pc : NULL);
+    int conv = gmres(&op, x_out, b, krylovdim, maxiter, atol, rtol, &it, &rs, &res, pc_U ? &pc : NULL, backward);
+    if (backward) {  /* restore the trace row, weighted by |v| in the residual */
+        cplx tr0 = 0;
+        for (int64_t k = 0; k < nt; ++k) tr0 += conj(tv[k]) * x_out[tidx[k]];
+        for (int64_t i = 0; i < n; ++i) x_out[i] /= tr0;
+    }
     cplx *w = malloc(sizeof(cplx) * n);
     do_apply(P, x_out, w, nthreads);
     double resid = dznrm2(w, n);
@@ -517,7 +552,10 @@ int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const
     for (int64_t k = 0; k < nt; ++k) tr += conj(tv[k]) * x_out[tidx[k]];
     double terr = cabs(tr - 1.0);
     double xn = dznrm2(x_out, n);
-    double tol = atol + rtol * (xn > 1.0 ? xn : 1.0);
+    double bn = 0;
+    for (int64_t k = 0; k < nt; ++k) bn += creal(conj(vv[k]) * vv[k]);
+    bn = sqrt(bn);
+    double tol = backward ? atol + rtol * (xn + bn) : atol + rtol * (xn > 1.0 ? xn : 1.0);
     info->converged = conv && nres <= tol && terr <= atol + rtol;
     info->iterations = it; info->restarts = rs; info->applications = op.applications;
     info->residual = resid; info->normalized_residual = nres; info->trace_error = terr;

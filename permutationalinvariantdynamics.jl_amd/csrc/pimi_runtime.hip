@@ -682,8 +682,44 @@ struct GmresResult {
 };
 
 // Restarted GMRES(m) on the bordered operator (src/krylov.jl:732-786); x, b on device.
+// Restarted GMRES(m) on the bordered operator (src/krylov.jl:732-786); x, b on
+// device.  With `precond` the sector preconditioner is applied on the right
+// (x = x0 + M^{-1} V y), so the GMRES residual is the true residual.  With
+// `backward` the stopping threshold is atol + rtol (|b| + |x_k|), where |x_k|
+// of the current iterate is formed on the device every kXnormEvery steps.
+constexpr int kXnormEvery = 25;
+
+static double iterate_norm(pimi_workspace *W, const std::vector<cd> &Hm, const std::vector<cd> &g, int m, int k,
+                           bool precond, double invscale) {
+    const int64_t n = W->plan->n;
+    std::vector<cd> yv(k);
+    for (int i = k - 1; i >= 0; --i) {
+        cd acc = g[i];
+        for (int jj = i + 1; jj < k; ++jj) acc -= Hm[(size_t)i + (size_t)jj * (m + 1)] * yv[jj];
+        cd piv = Hm[(size_t)i + (size_t)i * (m + 1)];
+        if (piv == cd(0, 0)) throw Error(PIMI_EINVAL, "singular GMRES Hessenberg");
+        yv[i] = acc / piv;
+    }
+    std::vector<cx<double>> yc(k);
+    for (int i = 0; i < k; ++i) yc[i] = cx<double>{yv[i].real(), yv[i].imag()};
+    h2d(W, W->coef.p, yc.data(), sizeof(cx<double>) * k);
+    HIPCHK(hipMemcpyAsync(W->trial.p, W->x.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
+    if (precond) {
+        hipLaunchKernelGGL(k_gemv<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, W->tmp.p, W->V.p, n, k,
+                           W->coef.p, n, 1.0, 1);
+        apply_precond(W, W->tmp.p, W->zbuf.p, 1.0 / invscale);
+        hipLaunchKernelGGL(k_axpy<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, W->trial.p, W->zbuf.p,
+                           n, 1.0, 0.0);
+    } else {
+        hipLaunchKernelGGL(k_gemv<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, W->trial.p, W->V.p, n,
+                           k, W->coef.p, n, 1.0, 0);
+    }
+    dev_norm<double>(W, W->trial.p, n, W->nscal.p + 5);
+    return read_scalar(W, W->nscal.p + 5);
+}
+
 static GmresResult gmres_device(pimi_workspace *W, double invscale, double atol, double rtol, int64_t maxiter,
-                                int64_t &applications, bool precond) {
+                                int64_t &applications, bool precond, bool backward) {
     const int64_t n = W->plan->n;
     const int m = W->m;
     cx<double> *V = W->V.p;
@@ -691,15 +727,18 @@ static GmresResult gmres_device(pimi_workspace *W, double invscale, double atol,
     std::vector<double> cs(m);
     auto Hij = [&](int i, int j) -> cd & { return H[(size_t)i + (size_t)j * (m + 1)]; };
     dev_norm<double>(W, W->b.p, n, W->nscal.p);
-    double bnorm = read_scalar(W, W->nscal.p);
-    double tol = atol + rtol * bnorm;
+    dev_norm<double>(W, W->x.p, n, W->nscal.p + 1);
+    d2h(W, W->hpin, W->nscal.p, 2 * sizeof(double));
+    const double bnorm = W->hpin[0];
+    double xnorm = W->hpin[1];
+    auto threshold = [&]() { return atol + rtol * (backward ? bnorm + xnorm : bnorm); };
     int64_t iterations = 0, restarts = 0;
     steady_apply(W, W->x.p, W->w.p, invscale);
     applications++;
     hipLaunchKernelGGL(k_sub<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, W->r.p, W->b.p, W->w.p, n);
     dev_norm<double>(W, W->r.p, n, W->nscal.p);
     double beta = read_scalar(W, W->nscal.p);
-    if (beta <= tol) return {true, 0, 0, beta};
+    if (beta <= threshold()) return {true, 0, 0, beta};
     std::vector<cx<double>> hcol((size_t)m + 2);
     while (iterations < maxiter) {
         std::fill(H.begin(), H.end(), cd(0, 0));
@@ -749,10 +788,13 @@ static GmresResult gmres_device(pimi_workspace *W, double invscale, double atol,
             g[j + 1] = -std::conj(s) * g1 + c * g2;
             iterations++;
             kdone = j + 1;
+            const double est = std::abs(g[j + 1]);
+            if (backward && (kdone % kXnormEvery == 0 || est <= threshold()) && Hij(j, j) != cd(0, 0))
+                xnorm = iterate_norm(W, H, g, m, kdone, precond, invscale);
             if (W->trace_progress && (iterations % 50 == 0))
-                std::fprintf(stderr, "[pimi] gmres iteration %lld  residual/|b| %.3e\n", (long long)iterations,
-                             std::abs(g[j + 1]) / std::max(bnorm, 1e-300));
-            if (std::abs(g[j + 1]) <= tol) break;
+                std::fprintf(stderr, "[pimi] gmres iteration %lld  residual %.3e  |b| %.3e  |x| %.3e\n",
+                             (long long)iterations, est, bnorm, xnorm);
+            if (est <= threshold()) break;
         }
         for (int i = kdone - 1; i >= 0; --i) {
             cd acc = g[i];
@@ -780,8 +822,11 @@ static GmresResult gmres_device(pimi_workspace *W, double invscale, double atol,
         hipLaunchKernelGGL(k_sub<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, W->r.p, W->b.p, W->w.p,
                            n);
         dev_norm<double>(W, W->r.p, n, W->nscal.p);
-        beta = read_scalar(W, W->nscal.p);
-        if (beta <= tol) return {true, iterations, restarts, beta};
+        dev_norm<double>(W, W->x.p, n, W->nscal.p + 1);
+        d2h(W, W->hpin, W->nscal.p, 2 * sizeof(double));
+        beta = W->hpin[0];
+        xnorm = W->hpin[1];
+        if (beta <= threshold()) return {true, iterations, restarts, beta};
     }
     return {false, iterations, restarts, beta};
 }
@@ -827,7 +872,24 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, double *rh
     const bool precond = o->preconditioner == PIMI_PRECOND_SECTOR;
     if (o->preconditioner != PIMI_PRECOND_NONE && !precond) throw Error(PIMI_EINVAL, "unknown preconditioner");
     if (precond) ensure_preconditioner(W);
-    GmresResult gr = gmres_device(W, 1.0 / scale, o->atol, o->rtol, o->maxiter, apps, precond);
+    const bool backward = o->criterion == PIMI_CRITERION_BACKWARD;
+    if (o->criterion != PIMI_CRITERION_REFERENCE && !backward) throw Error(PIMI_EINVAL, "unknown criterion");
+    GmresResult gr = gmres_device(W, 1.0 / scale, o->atol, o->rtol, o->maxiter, apps, precond, backward);
+    if (backward) {
+        // The trace row of the bordered system carries weight |v| = |t|^{-1}; under the
+        // scale-invariant rule it is restored exactly by x <- x / (t . x), which leaves
+        // |L x| / |x| unchanged.
+        std::vector<cx<double>> xs((size_t)n);
+        d2h(W, xs.data(), W->x.p, sizeof(cx<double>) * n);
+        cd tr0(0, 0);
+        for (size_t k = 0; k < P->tidx.size(); ++k) tr0 += P->tval[k] * cd(xs[P->tidx[k]].x, xs[P->tidx[k]].y);
+        if (tr0 == cd(0, 0)) throw Error(PIMI_ENOTCONVERGED, "stationary iterate has zero trace");
+        for (auto &z : xs) {
+            cd v = cd(z.x, z.y) / tr0;
+            z = cx<double>{v.real(), v.imag()};
+        }
+        h2d(W, W->x.p, xs.data(), sizeof(cx<double>) * n);
+    }
     // acceptance (src/krylov.jl:868-873)
     launch_apply<double>(W, W->x.p, W->w.p, 1.0);
     apps++;
@@ -842,7 +904,10 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, double *rh
     for (size_t k = 0; k < P->tidx.size(); ++k) tr += P->tval[k] * cd(xh[P->tidx[k]].x, xh[P->tidx[k]].y);
     double terr = std::abs(tr - 1.0);
     double nres = resid / scale;
-    double tol = o->atol + o->rtol * std::max(xn, 1.0);
+    double bn = 0;
+    for (double v : vv) bn += v * v;
+    bn = std::sqrt(bn);
+    double tol = backward ? o->atol + o->rtol * (xn + bn) : o->atol + o->rtol * std::max(xn, 1.0);
     std::memcpy(rho_out, xh.data(), sizeof(cx<double>) * n);
     std::memset(info, 0, sizeof(*info));
     info->converged = gr.converged && nres <= tol && terr <= o->atol + o->rtol;
@@ -854,6 +919,7 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, double *rh
     info->trace_error = terr;
     info->operator_scale = scale;
     info->linear_residual = gr.residual;
+    info->relative_residual = xn > 0 ? nres / xn : INFINITY;
     info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!info->converged) {
         char buf[256];

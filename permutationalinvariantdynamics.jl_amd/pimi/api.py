@@ -443,7 +443,8 @@ class SteadyStateResult:
 
 
 def stationary_state(compiled: CompiledPIModel, algorithm=None, atol=1e-10, rtol=1e-8, operator_scale=None,
-                     return_info=False, precision=N.PIMI_C64, inner_rtol=1e-5, max_refinements=50):
+                     return_info=False, precision=N.PIMI_C64, inner_rtol=1e-5, max_refinements=50,
+                     criterion="reference"):
     """Trace-fixed stationary state by restarted GMRES on the device
     (src/krylov.jl:797-891).  Raises NotConverged like the reference."""
     algorithm = algorithm or GMRESAlgorithm()
@@ -451,8 +452,11 @@ def stationary_state(compiled: CompiledPIModel, algorithm=None, atol=1e-10, rtol
         raise ValueError("the mi355x backend solves stationary states with GMRESAlgorithm")
     ws = compiled.workspace(precision, algorithm.krylovdim)
     pc = N.PRECOND_SECTOR if algorithm.preconditioner == "sector" else N.PRECOND_NONE
+    if criterion not in ("reference", "backward"):
+        raise ValueError("criterion must be \"reference\" or \"backward\"")
+    crit = N.CRITERION_BACKWARD if criterion == "backward" else N.CRITERION_REFERENCE
     opts = N.GmresOpts(algorithm.krylovdim, pc, algorithm.maxiter, float(atol), float(rtol),
-                       float(operator_scale or 0.0), float(inner_rtol), int(max_refinements))
+                       float(operator_scale or 0.0), float(inner_rtol), int(max_refinements), crit, 0)
     rho = np.zeros(len(compiled.basis), dtype=np.complex128)
     info = N.SolveInfo()
     N.check(N.lib().pimi_steady_gmres(ws, C.byref(opts), N.ptr(rho), C.byref(info)))
