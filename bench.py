@@ -41,6 +41,18 @@ def build_model(pimi):
                                      pumping=0.02, collective_emission=0.1)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs (calibrated by a device copy), or None."""
+    path = os.path.join(ROOT, "profiles", "k_apply_traffic_n120_fp64.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+    except OSError:
+        return None
+    return d["traffic_bytes"] if d.get("kernel") == kernel else None
+
+
 def apply_roofline(pimi, comp, reps=200):
     N = pimi._native
     ws = comp.workspace(pimi.PIMI_C64, KRYLOV)
@@ -93,12 +105,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--profile", action="store_true", help="apply + copy calibration only (counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
     import pimi
     comp = pimi.compile(build_model(pimi), backend="mi355x", device=0)
+    if args.profile:
+        # counter runs: 20 event-timed applies, then 20 device copies of 64 MiB
+        # (the calibration kernel for the byte counters)
+        N = pimi._native
+        ms_apply, bytes_apply, achieved = apply_roofline(pimi, comp, reps=20)
+        ms = C.c_double()
+        N.check(N.lib().pimi_copy_bench(comp.workspace(pimi.PIMI_C64, KRYLOV), C.c_int64(64 << 20),
+                                        C.c_int32(20), C.byref(ms)))
+        print(json.dumps({"apply_ms": ms_apply, "apply_algorithmic_bytes": bytes_apply, "apply_GBps": achieved,
+                          "copy_bytes": 64 << 20, "copy_ms": ms.value,
+                          "copy_GBps": 2 * (64 << 20) / (ms.value * 1e-3) / 1e9}))
+        return
     alg = pimi.GMRESAlgorithm(krylovdim=KRYLOV, maxiter=KRYLOV, preconditioner="sector")
 
     def solve():
@@ -147,7 +172,7 @@ def main():
         "relative_residual": max(i["relative_residual"] for i in infos),
         "trace_error": max(i["trace_error"] for i in infos),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("pimi::k_apply<double>"),
                      "kernel": "k_apply<double>", "avg_ms": ms_apply, "algorithmic_bytes": bytes_apply},
         "cpu_baseline": cpu,
     }

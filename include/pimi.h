@@ -197,6 +197,10 @@ pimi_status pimi_plan_export(const pimi_plan *plan, int64_t *cp_base, int64_t *c
                              int32_t *br_out, int32_t *br_in, double *br_r, int32_t *br_use_support,
                              int64_t *br_sup_base, int32_t *sup_rows, int32_t *sup_cols, double *sup_vals,
                              int64_t *br_dense_base, double *dense);
+/* Host copies (n complex each, PI block layout) of the stationary-solve
+ * preconditioner factors: U (eigenvectors of each fused Hamiltonian block)
+ * and inv_den (see pimi_preconditioner).  No counterpart in the reference. */
+pimi_status pimi_sector_preconditioner_factors(const pimi_plan *plan, double *U, double *inv_den);
 void pimi_plan_destroy(pimi_plan *plan);
 
 /* One workspace per concurrent caller: owns a HIP stream and the Krylov basis. */
@@ -224,6 +228,10 @@ pimi_status pimi_expv_series(pimi_workspace *ws, const double *rho0, const doubl
  * returns the average milliseconds per apply launch.  No counterpart in the
  * reference (measurement only). */
 pimi_status pimi_apply_bench(pimi_workspace *ws, int64_t ncols, int32_t reps, double *ms_avg);
+
+/* Device-to-device copy of `bytes` bytes, `reps` times, event-timed: the
+ * calibration kernel for HBM byte counters.  Measurement only. */
+pimi_status pimi_copy_bench(pimi_workspace *ws, int64_t bytes, int32_t reps, double *ms_avg);
 
 /* Time of the last apply kernel launch sequence, in milliseconds (HIP events
  * recorded on the workspace stream around pimi_apply). */

@@ -1395,6 +1395,16 @@ pimi_status pimi_plan_export(const pimi_plan *P, int64_t *cp_base, int64_t *colp
     });
 }
 
+pimi_status pimi_sector_preconditioner_factors(const pimi_plan *P, double *U, double *inv_den) {
+    return guarded([&] {
+        if (!P || !U || !inv_den) throw Error(PIMI_EINVAL, "null argument");
+        std::vector<cd> u, inv;
+        build_sector_preconditioner(P->host, u, inv);
+        std::memcpy(U, u.data(), sizeof(cd) * u.size());
+        std::memcpy(inv_den, inv.data(), sizeof(cd) * inv.size());
+    });
+}
+
 void pimi_plan_destroy(pimi_plan *P) { delete P; }
 
 pimi_status pimi_workspace_create(pimi_plan *P, int32_t precision, int32_t krylovdim, pimi_workspace **out) {
@@ -1513,6 +1523,28 @@ pimi_status pimi_apply_bench(pimi_workspace *W, int64_t ncols, int32_t reps, dou
         sync(W);
         HIPCHK(hipEventRecord(W->ev0, W->stream));
         for (int r = 0; r < reps; ++r) run();
+        HIPCHK(hipEventRecord(W->ev1, W->stream));
+        sync(W);
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, W->ev0, W->ev1));
+        *ms_avg = (double)ms / reps;
+    });
+}
+
+pimi_status pimi_copy_bench(pimi_workspace *W, int64_t bytes, int32_t reps, double *ms_avg) {
+    return guarded([&] {
+        if (!W || !ms_avg || bytes < 1 || reps < 1) throw Error(PIMI_EINVAL, "bad argument");
+        BusyGuard guard(W->busy);
+        HIPCHK(hipSetDevice(W->plan->ctx->device));
+        DevBuf<char> a, b;
+        a.alloc((size_t)bytes);
+        b.alloc((size_t)bytes);
+        HIPCHK(hipMemsetAsync(a.p, 1, (size_t)bytes, W->stream));
+        HIPCHK(hipMemcpyAsync(b.p, a.p, (size_t)bytes, hipMemcpyDeviceToDevice, W->stream));
+        sync(W);
+        HIPCHK(hipEventRecord(W->ev0, W->stream));
+        for (int r = 0; r < reps; ++r)
+            HIPCHK(hipMemcpyAsync(b.p, a.p, (size_t)bytes, hipMemcpyDeviceToDevice, W->stream));
         HIPCHK(hipEventRecord(W->ev1, W->stream));
         sync(W);
         float ms = 0;

@@ -220,3 +220,44 @@ def test_preconditioned_n120_backward():
     x = res.state.data
     r = np.linalg.norm(ref.apply(x, nthreads=8)) / (res.info["operator_scale"] * np.linalg.norm(x))
     assert r <= 2e-8
+
+
+def _gold():
+    import json
+    import os
+    return json.load(open(os.path.join(os.path.dirname(__file__), "golden", "closed_form.json")))
+
+
+def test_c1_dynamics_against_closed_form():
+    gold = _gold()["c1"]
+    comp = pimi.compile(_c1_model())
+    b = comp.basis
+    jp = comp.collective_operator(SP2.jp).data
+    jm = comp.collective_operator(SP2.jm).data
+    idx, tv = comp.trace_functional()
+    # (A*B) coefficient blocks = C_A C_B / sqrt(f); sqrt(f) is the trace weight of each sector
+    jj = np.zeros_like(jp)
+    for s, m in enumerate(b.dims):
+        o0, o1 = b.offsets[s], b.offsets[s + 1]
+        sf = tv[sum(b.dims[:s])]
+        jj[o0:o1] = ((jp[o0:o1].reshape(m, m, order="F") @ jm[o0:o1].reshape(m, m, order="F")) / sf).reshape(
+            -1, order="F")
+    res = pimi.solve_dynamics(comp, pimi.computational_product_state(b, 2), (0.0, 4.0), saveat=0.1,
+                              algorithm=pimi.ExpvAlgorithm(30, 0.0, 1e-12),
+                              observables={"n": SP2.jp @ SP2.jm, "JpJm": pimi.PIOperator(b, jj)})
+    np.testing.assert_allclose(res.t, gold["t"], rtol=0, atol=0)
+    np.testing.assert_allclose(res.values["n"].real, gold["n"], rtol=1e-10)
+    np.testing.assert_allclose(res.values["JpJm"].real, gold["JpJm"], rtol=1e-10)
+
+
+def test_local_steady_n120_against_closed_form():
+    """N=120, local terms only: the steady state is a product state, so the
+    device solve is pinned by the 4x4 single-qubit Lindbladian."""
+    gold = _gold()["local_steady_120"]
+    comp = pimi.compile(pimi.qubit_ensemble_model(120, hamiltonian=SP2.jz + SP2.jx, emission=0.1, dephasing=0.2,
+                                                  pumping=0.02))
+    res = pimi.stationary_state(comp, algorithm=pimi.GMRESAlgorithm(1000, 1000, "sector"), atol=0.0, rtol=1e-10,
+                                return_info=True, criterion="backward")
+    n_op = comp.collective_operator(SP2.jp @ SP2.jm)
+    val = pimi.expect(n_op, res.state).real
+    assert abs(val - gold["steady_n"]) <= 1e-8 * gold["steady_n"], (val, gold["steady_n"])

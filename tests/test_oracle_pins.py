@@ -1,0 +1,244 @@
+"""Pins of the oracle against mathematics that does not depend on it.
+
+* combinatorial identities and CG isometries (reference: test/test_combinatorics.jl,
+  test/test_cgc.jl);
+* brute-force d^N Hilbert-space Lindbladians, linked to PI coordinates only by
+  physical pairings with products of collective operators (a spanning set by
+  Schur--Weyl duality);
+* invariants: trace and Hermiticity preservation (test/test_liouvillian_properties.jl);
+* closed-form product-state dynamics and steady states of local-only models
+  (tests/golden/closed_form.json, written by tests/golden/make_closed_form.py);
+* dense-matrix routes for the Krylov solvers (test/test_krylov.jl,
+  test/test_krylov_extensions.jl).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import bruteforce as bf
+from oracle import lowering as L
+from oracle import oracle as O
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "closed_form.json")))
+
+
+@pytest.mark.parametrize("N,d", [(1, 2), (5, 2), (12, 2), (3, 3), (6, 3), (4, 4), (3, 5)])
+def test_basis_dimension_identities(N, d):
+    b = L.Basis(N, d)
+    assert b.n == math.comb(N + d * d - 1, N)
+    for p, pats in zip(b.sectors, b.patterns):
+        assert len(pats) == L.unitary_group_dimension(p)
+        assert sorted(pats) == list(pats)
+    # Schur-Weyl: sum_lambda f^lambda dim(U_lambda) = d^N
+    assert sum(L.symmetric_group_dimension(p) * m for p, m in zip(b.sectors, b.dims)) == d ** N
+    # sectors in descending lexicographic order
+    assert b.sectors == sorted(b.sectors, reverse=True)
+
+
+@pytest.mark.parametrize("N,d", [(6, 2), (4, 3), (3, 4)])
+def test_branch_weight_is_multiplicity_ratio(N, d):
+    from fractions import Fraction
+    for lam in L.partitions(N, d):
+        for r in L.removable_corners(lam):
+            mu = L.remove_corner(lam, r)
+            w = L.one_box_branch_weight(lam, mu)
+            assert w == Fraction(N * L.symmetric_group_dimension(mu), L.symmetric_group_dimension(lam))
+
+
+@pytest.mark.parametrize("N,d", [(3, 2), (4, 2), (3, 3), (4, 3), (2, 4), (3, 4)])
+def test_cg_isometry(N, d):
+    """For each parent shape and child: sum over (child pattern, label) of
+    CG products is the identity on the parent patterns."""
+    b = L.Basis(N, d)
+    for s, lam in enumerate(b.sectors):
+        par = b.patterns[s]
+        for mu in b.removed[s]:
+            ch = L.gt_patterns(mu)
+            M = np.array([[L.cgc(c, j, p, d) for c in ch for j in range(d)] for p in par])
+            np.testing.assert_allclose(M @ M.T, np.eye(len(par)), atol=1e-14)
+
+
+def test_qubit_collective_blocks_are_spin_matrices():
+    """For qubits each sector is a spin-j irrep: collective jz/jx blocks have
+    the spectra of spin-j matrices and satisfy [Jx, Jy] = i Jz."""
+    b = L.Basis(9, 2)
+    g = L.OneBodyGeometry(b)
+    sp = L.spin_matrices(2)
+    for s, p in enumerate(b.sectors):
+        j = (p[0] - p[1]) / 2
+        Jz = L.collective_block_dense(g, sp["jz"], s)
+        Jx = L.collective_block_dense(g, sp["jx"], s)
+        Jy = L.collective_block_dense(g, sp["jy"], s)
+        np.testing.assert_allclose(np.sort(np.linalg.eigvalsh(Jz)), np.arange(-j, j + 1), atol=1e-13)
+        np.testing.assert_allclose(np.sort(np.linalg.eigvalsh(Jx)), np.arange(-j, j + 1), atol=1e-12)
+        np.testing.assert_allclose(Jx @ Jy - Jy @ Jx, 1j * Jz, atol=1e-12)
+
+
+def _bruteforce_check(N, d, terms_pi, terms_full, degree, steps=4):
+    b = L.Basis(N, d)
+    g = L.OneBodyGeometry(b)
+    P = O.OraclePlan(L.compile_model(b, terms_pi, g))
+    Lf = bf.lindbladian(N, d, terms_full)
+    coll_pi = {(i, j): L.collective_operator(g, bf.unit(d, i, j)) for i in range(d) for j in range(d)}
+    coll_f = {(i, j): bf.collective(bf.unit(d, i, j), N, d) for i in range(d) for j in range(d)}
+    Dh = d ** N
+    ident = L.collective_operator(g, np.eye(d) / N)
+    Opi, Of = [], []
+    for w in bf.collective_monomials(d, degree):
+        a, A = ident.copy(), np.eye(Dh, dtype=complex)
+        for ij in w:
+            a = L.pi_product(b, a, coll_pi[ij])
+            A = A @ coll_f[ij]
+        Opi.append(a)
+        Of.append(A)
+    G = np.array(Opi).conj()
+    assert np.linalg.matrix_rank(G, 1e-9) == b.n  # the pairings determine PI vectors
+
+    def to_pi(R):
+        return np.linalg.lstsq(G, np.array([np.trace(O_.conj().T @ R) for O_ in Of]), rcond=None)[0]
+
+    rng = np.random.default_rng(N * 10 + d)
+    psi = rng.standard_normal(d) + 1j * rng.standard_normal(d)
+    phi = rng.standard_normal(d) + 1j * rng.standard_normal(d)
+    fp, ff = psi, phi
+    for _ in range(N - 1):
+        fp, ff = np.kron(fp, psi), np.kron(ff, phi)
+    rho = np.outer(fp, ff.conj())
+    x, v = to_pi(rho), bf.vec(rho)
+    for _ in range(steps):
+        x, v = P.apply(x), Lf @ v
+        ref = to_pi(bf.unvec(v, Dh))
+        assert np.abs(x - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_bruteforce_qubits_all_term_kinds():
+    sp = L.spin_matrices(2)
+    H = 1.0 * sp["jz"] + 0.7 * sp["jx"]
+    tp = [L.Term("collective_h", H), L.Term("local_jump", sp["jm"], 0.1), L.Term("local_jump", sp["jz"], 0.2),
+          L.Term("local_jump", sp["jp"], 0.02), L.Term("collective_jump", sp["jm"], 0.1),
+          L.Term("collective_jump", sp["jz"], 0.03)]
+    tf = [("collective_h", H, 1.0), ("local_jump", sp["jm"], 0.1), ("local_jump", sp["jz"], 0.2),
+          ("local_jump", sp["jp"], 0.02), ("collective_jump", sp["jm"], 0.1), ("collective_jump", sp["jz"], 0.03)]
+    _bruteforce_check(4, 2, tp, tf, 4)
+
+
+def test_bruteforce_qutrits_with_direct_term():
+    s3 = L.spin_matrices(3)
+    N = 3
+    b = L.Basis(N, 3)
+    g = L.OneBodyGeometry(b)
+    jz = L.collective_operator(g, s3["jz"])
+    tp = [L.Term("local_h", 1.0 * s3["jz"] + 0.7 * s3["jx"]),
+          L.Term("direct_h", L.physical_blocks(b, L.pi_product(b, jz, jz)), 0.3 / N),
+          L.Term("local_jump", s3["jm"], 0.1)]
+    Jzf = bf.collective(s3["jz"], N, 3)
+    tf = [("local_h", 1.0 * s3["jz"] + 0.7 * s3["jx"], 1.0), ("full_h", Jzf @ Jzf, 0.3 / N),
+          ("local_jump", s3["jm"], 0.1)]
+    _bruteforce_check(N, 3, tp, tf, 3)
+
+
+def _c2(N):
+    sp = L.spin_matrices(2)
+    return O.compile_plan(N, 2, L.qubit_ensemble_terms(hamiltonian=sp["jz"] + sp["jx"], emission=0.1,
+                                                       dephasing=0.2, pumping=0.02, collective_emission=0.1))
+
+
+def _hermitian_adjoint(b, x):
+    out = np.empty_like(x)
+    for s, m in enumerate(b.dims):
+        o0, o1 = b.offsets[s], b.offsets[s + 1]
+        out[o0:o1] = x[o0:o1].reshape(m, m, order="F").conj().T.reshape(-1, order="F")
+    return out
+
+
+@pytest.mark.parametrize("N", [6, 17])
+def test_invariants_trace_and_hermiticity(N):
+    P, _ = _c2(N)
+    rng = np.random.default_rng(N)
+    x = rng.standard_normal(P.n) + 1j * rng.standard_normal(P.n)
+    y = P.apply(x)
+    # trace preservation: t . (L x) = 0
+    assert abs(np.dot(P.t_val, y[P.t_idx])) <= 1e-13 * np.linalg.norm(y) * np.linalg.norm(P.t_val)
+    # Hermiticity preservation: L(x^dagger) = (L x)^dagger
+    np.testing.assert_allclose(P.apply(_hermitian_adjoint(P.basis, x)), _hermitian_adjoint(P.basis, y),
+                               rtol=0, atol=1e-12 * np.abs(y).max())
+
+
+def test_threaded_apply_is_bitwise_serial():
+    P, _ = _c2(30)
+    x = np.random.default_rng(3).standard_normal(P.n) + 0j
+    assert np.array_equal(P.apply(x), P.apply(x, nthreads=5))
+
+
+def test_c1_dynamics_closed_form():
+    gold = GOLD["c1"]
+    P, g = O.compile_plan(8, 2, L.driven_qubits_terms(drive=1.0, detuning=1.0, emission=0.1, dephasing=0.2,
+                                                      pumping=0.0))
+    sp = L.spin_matrices(2)
+    obs = [L.collective_operator(g, sp["jp"] @ sp["jm"]),
+           L.pi_product(P.basis, L.collective_operator(g, sp["jp"]), L.collective_operator(g, sp["jm"]))]
+    rho0 = L.computational_product_state(P.basis, 2)
+    out, _, info = P.solve_dynamics(rho0, np.array(gold["t"]), obs, krylovdim=30, atol=0.0, rtol=1e-12)
+    assert info["converged"]
+    np.testing.assert_allclose(out[:, 0].real, gold["n"], rtol=1e-10, atol=0)
+    np.testing.assert_allclose(out[:, 1].real, gold["JpJm"], rtol=1e-10, atol=0)
+    x, sinfo = P.steady_state(krylovdim=30, maxiter=5000, atol=0.0, rtol=1e-12)
+    assert sinfo["converged"]
+    assert abs(np.vdot(obs[0], x).real - gold["steady_n"]) <= 1e-9 * gold["steady_n"]
+    assert abs(np.vdot(obs[1], x).real - gold["steady_JpJm"]) <= 1e-9 * gold["steady_JpJm"]
+
+
+def test_local_steady_closed_form_preconditioned():
+    gold = GOLD["local_steady_20"]
+    sp = L.spin_matrices(2)
+    P, g = O.compile_plan(20, 2, L.qubit_ensemble_terms(hamiltonian=sp["jz"] + sp["jx"], emission=0.1,
+                                                        dephasing=0.2, pumping=0.02))
+    x, info = P.steady_state(krylovdim=300, maxiter=300, atol=0.0, rtol=1e-11, preconditioner="sector",
+                             criterion="backward")
+    assert info["converged"]
+    n_op = L.collective_operator(g, sp["jp"] @ sp["jm"])
+    jj = L.pi_product(P.basis, L.collective_operator(g, sp["jp"]), L.collective_operator(g, sp["jm"]))
+    assert abs(np.vdot(n_op, x).real - gold["steady_n"]) <= 1e-8 * gold["steady_n"]
+    assert abs(np.vdot(jj, x).real - gold["steady_JpJm"]) <= 1e-8 * gold["steady_JpJm"]
+
+
+def test_steady_state_matches_dense_null_space():
+    P, _ = _c2(6)
+    n = P.n
+    M = np.array([P.apply(np.eye(n, dtype=complex)[:, k]) for k in range(n)]).T
+    w, V = np.linalg.eig(M)
+    v = V[:, np.argmin(np.abs(w))]
+    v = v / np.dot(P.t_val, v[P.t_idx])
+    x, info = P.steady_state(krylovdim=30, maxiter=3000, atol=0.0, rtol=1e-12)
+    assert info["converged"]
+    assert np.linalg.norm(x - v) <= 1e-9 * np.linalg.norm(v)
+
+
+def test_expv_matches_dense_exponential():
+    P, _ = _c2(5)
+    n = P.n
+    M = np.array([P.apply(np.eye(n, dtype=complex)[:, k]) for k in range(n)]).T
+    b = np.random.default_rng(2).standard_normal(n) + 0j
+    y, info = P.expv(b, 2.5, krylovdim=20, atol=0.0, rtol=1e-12)
+    assert info["converged"]
+    ref = sla.expm(2.5 * M) @ b
+    assert np.linalg.norm(y - ref) <= 1e-10 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("scale", [1e-3, 0.3, 3.0, 40.0])
+def test_small_expm_against_scipy(scale):
+    A = scale * (np.random.default_rng(5).standard_normal((9, 9)) + 1j * np.random.default_rng(6).standard_normal((9, 9)))
+    np.testing.assert_allclose(O.expm(A), sla.expm(A), rtol=0, atol=1e-12 * np.abs(sla.expm(A)).max())
+
+
+def test_reference_gmres30_stalls_at_n20():
+    """The reference's default restarted GMRES(30) does not reach 1e-8 on the
+    N=20 member of the benchmark model family (it plateaus near 4e-5)."""
+    P, _ = _c2(20)
+    x, info = P.steady_state(krylovdim=30, maxiter=3000, atol=0.0, rtol=1e-8)
+    assert not info["converged"]
+    assert info["normalized_residual"] > 1e-6
