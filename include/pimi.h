@@ -123,8 +123,12 @@ typedef struct {
 
 typedef enum {
     PIMI_PRECOND_NONE = 0,   /* plain restarted GMRES, as the reference's default */
-    PIMI_PRECOND_SECTOR = 1  /* right preconditioning by the per-sector inverse of X -> G X + X G^H,
+    PIMI_PRECOND_SECTOR = 1, /* right preconditioning by the per-sector inverse of X -> G X + X G^H,
                                 G = -iH - Q/2 with Q replaced by its diagonal in the H eigenbasis */
+    PIMI_PRECOND_SECTOR_INNER = 2 /* flexible GMRES whose directions come from `inner_steps` steps of
+                                     PIMI_PRECOND_SECTOR-preconditioned GMRES on the sector-block-diagonal
+                                     part of L, in the workspace precision (complex FP32 workspaces: FP32
+                                     inner solves, FP64 outer iteration and residuals) */
 } pimi_preconditioner;
 
 typedef struct {
@@ -143,7 +147,7 @@ typedef struct {
          the scale-invariant relative residual; at large N |b| = 2^(-N/2) is far below
          eps |x|, where the reference rule cannot be met in floating point. */
     int32_t criterion;
-    int32_t reserved2;
+    int32_t inner_steps;    /* PIMI_PRECOND_SECTOR_INNER: inner GMRES steps per outer step */
 } pimi_gmres_opts;
 
 enum { PIMI_CRITERION_REFERENCE = 0, PIMI_CRITERION_BACKWARD = 1 };

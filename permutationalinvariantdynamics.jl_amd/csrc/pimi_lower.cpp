@@ -576,7 +576,7 @@ RowLists rows_of_dense(const std::vector<cd> &A, int64_t m, int64_t n) {
 }
 }  // namespace
 
-void build_stencil(const FusedPlan &P, Stencil &st) {
+void build_stencil(const FusedPlan &P, Stencil &st, bool sector_diagonal_only) {
     st = Stencil();
     st.term_base.push_back(0);
     for (int l = 0; l < P.ns; ++l) {
@@ -613,6 +613,7 @@ void build_stencil(const FusedPlan &P, Stencil &st) {
         for (size_t g = 0; g < P.ob.size(); ++g)
             for (const Branch &b : P.ob[g]) {
                 if (b.out != l) continue;
+                if (sector_diagonal_only && b.in != l) continue;
                 int64_t mn = P.dims[b.in];
                 RowLists R = rows_of_dense(b.dense, m, mn);
                 cd s = P.ob_scale[g] * b.r;

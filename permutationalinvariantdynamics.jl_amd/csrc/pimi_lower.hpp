@@ -111,7 +111,9 @@ struct Stencil {
     int64_t entries_per_apply = 0;   // sum over outputs of wa*wb
 };
 
-void build_stencil(const FusedPlan &plan, Stencil &st);
+// sector_diagonal_only: keep only terms whose input sector is the output
+// sector (the sector-block-diagonal part of L, used by the inner solver).
+void build_stencil(const FusedPlan &plan, Stencil &st, bool sector_diagonal_only = false);
 
 // sqrt(RN(f)) for the hook-length dimension of a partition (exact big integer).
 double sqrt_symmetric_group_dimension(const Partition &p);

@@ -287,6 +287,13 @@ struct pimi_plan {
     pimi::DevBuf<pimi::cx<float>> pU32, pInv32;
     pimi::DevBuf<pimi::GemmTile> gtiles;
     int64_t ngtiles = 0;
+    // sector-block-diagonal part of L (inner solver), built on first use
+    bool diag_ready = false;
+    pimi::DevBuf<int64_t> dg_term_base;
+    pimi::DevBuf<pimi::StencilTermDev> dg_terms;
+    pimi::DevBuf<int32_t> dg_col;
+    pimi::DevBuf<pimi::cx<double>> dg_val64;
+    pimi::DevBuf<pimi::cx<float>> dg_val32;
 };
 
 struct pimi_workspace {
@@ -303,6 +310,11 @@ struct pimi_workspace {
     pimi::DevBuf<pimi::cx<double>> partial, hvec, hacc, coef;
     pimi::DevBuf<double> npartial, nscal, d_vval;
     pimi::DevBuf<pimi::cx<double>> zbuf, pt1, pt2;  // preconditioner work vectors
+    // flexible GMRES: outer search directions Z and the inner solver's vectors
+    pimi::DevBuf<pimi::cx<double>> Zo, Hin;
+    pimi::DevBuf<char> inner_mem;  // (kin+1) basis + 4 work vectors in the inner precision
+    pimi::DevBuf<double> inner_norms;
+    int inner_k = 0;
     double *hpin = nullptr;  // pinned host staging (small results)
     bool trace_progress = false;  // PIMI_TRACE=1: solver progress lines on stderr
 };
@@ -400,16 +412,41 @@ static void finalize_plan(pimi_plan *P) {
 // ---------------------------------------------------------------------------
 // device operations on a workspace
 // ---------------------------------------------------------------------------
+static void ensure_diag_stencil(pimi_plan *P) {
+    if (P->diag_ready) return;
+    Stencil st;
+    build_stencil(P->host, st, true);
+    std::vector<StencilTermDev> terms(st.terms.size());
+    for (size_t k = 0; k < terms.size(); ++k) {
+        const StencilTerm &a = st.terms[k];
+        terms[k] = StencilTermDev{a.in_sector, a.flags, a.wa, a.wb, a.a_base, a.b_base, a.s_re, a.s_im};
+    }
+    std::vector<cx<double>> v64(st.val.size());
+    std::vector<cx<float>> v32(st.val.size());
+    for (size_t k = 0; k < v64.size(); ++k) {
+        v64[k] = cx<double>{st.val[k].real(), st.val[k].imag()};
+        v32[k] = cx<float>{(float)st.val[k].real(), (float)st.val[k].imag()};
+    }
+    P->dg_term_base.upload(st.term_base.data(), st.term_base.size());
+    P->dg_terms.upload(terms.data(), terms.size());
+    P->dg_col.upload(st.col.data(), st.col.size());
+    P->dg_val64.upload(v64.data(), v64.size());
+    P->dg_val32.upload(v32.data(), v32.size());
+    P->diag_ready = true;
+}
+
+// y = out_scale * L x (diag = false) or out_scale * D x, D the sector-block-diagonal part of L
 template <typename R>
-static void launch_apply(pimi_workspace *W, const cx<R> *x, cx<R> *y, double out_scale, int64_t ncols = 1) {
+static void launch_apply(pimi_workspace *W, const cx<R> *x, cx<R> *y, double out_scale, int64_t ncols = 1,
+                         bool diag = false) {
     pimi_plan *P = W->plan;
     const cx<R> *val;
-    if constexpr (sizeof(R) == 8) val = P->val64.p;
-    else val = P->val32.p;
+    if constexpr (sizeof(R) == 8) val = diag ? P->dg_val64.p : P->val64.p;
+    else val = diag ? P->dg_val32.p : P->val32.p;
     dim3 grid((unsigned)(P->tiles_per_xcd * kNumXcd), (unsigned)std::min<int64_t>(ncols, 65535));
     hipLaunchKernelGGL(k_apply<R>, grid, dim3(kApplyBlock), 0, W->stream, x, y, P->n, P->offs.p, P->dims.p,
-                       P->tile_sector.p, P->ntiles, P->tiles_per_xcd, P->term_base.p, P->terms.p, P->col.p, val,
-                       (R)out_scale, ncols);
+                       P->tile_sector.p, P->ntiles, P->tiles_per_xcd, diag ? P->dg_term_base.p : P->term_base.p,
+                       diag ? P->dg_terms.p : P->terms.p, diag ? P->dg_col.p : P->col.p, val, (R)out_scale, ncols);
     HIPCHK(hipGetLastError());
 }
 
@@ -660,19 +697,27 @@ static void ensure_preconditioner(pimi_workspace *W) {
 }
 
 // out = scale * U ((U^H r U) .* inv_den) U^H, sector by sector
-static void apply_precond(pimi_workspace *W, const cx<double> *r, cx<double> *out, double scale) {
+template <typename R>
+static void apply_precond_t(pimi_workspace *W, const cx<R> *r, cx<R> *out, double scale, cx<R> *t1, cx<R> *t2) {
     pimi_plan *P = W->plan;
     dim3 g((unsigned)P->ngtiles), b(256);
-    const cx<double> *nul = nullptr;
-    hipLaunchKernelGGL(k_sector_gemm<double>, g, b, 0, W->stream, P->pU64.p, r, W->pt1.p, P->offs.p, P->dims.p,
-                       P->gtiles.p, 1, 0, nul, 1.0);
-    hipLaunchKernelGGL(k_sector_gemm<double>, g, b, 0, W->stream, W->pt1.p, P->pU64.p, W->pt2.p, P->offs.p,
-                       P->dims.p, P->gtiles.p, 0, 0, P->pInv64.p, scale);
-    hipLaunchKernelGGL(k_sector_gemm<double>, g, b, 0, W->stream, P->pU64.p, W->pt2.p, W->pt1.p, P->offs.p,
-                       P->dims.p, P->gtiles.p, 0, 0, nul, 1.0);
-    hipLaunchKernelGGL(k_sector_gemm<double>, g, b, 0, W->stream, W->pt1.p, P->pU64.p, out, P->offs.p, P->dims.p,
-                       P->gtiles.p, 0, 1, nul, 1.0);
+    const cx<R> *nul = nullptr;
+    const cx<R> *U, *Inv;
+    if constexpr (sizeof(R) == 8) { U = P->pU64.p; Inv = P->pInv64.p; }
+    else { U = P->pU32.p; Inv = P->pInv32.p; }
+    hipLaunchKernelGGL(k_sector_gemm<R>, g, b, 0, W->stream, U, r, t1, P->offs.p, P->dims.p, P->gtiles.p, 1, 0, nul,
+                       (R)1);
+    hipLaunchKernelGGL(k_sector_gemm<R>, g, b, 0, W->stream, t1, U, t2, P->offs.p, P->dims.p, P->gtiles.p, 0, 0, Inv,
+                       (R)scale);
+    hipLaunchKernelGGL(k_sector_gemm<R>, g, b, 0, W->stream, U, t2, t1, P->offs.p, P->dims.p, P->gtiles.p, 0, 0, nul,
+                       (R)1);
+    hipLaunchKernelGGL(k_sector_gemm<R>, g, b, 0, W->stream, t1, U, out, P->offs.p, P->dims.p, P->gtiles.p, 0, 1, nul,
+                       (R)1);
     HIPCHK(hipGetLastError());
+}
+
+static void apply_precond(pimi_workspace *W, const cx<double> *r, cx<double> *out, double scale) {
+    apply_precond_t<double>(W, r, out, scale, W->pt1.p, W->pt2.p);
 }
 
 struct GmresResult {
@@ -831,6 +876,220 @@ static GmresResult gmres_device(pimi_workspace *W, double invscale, double atol,
     return {false, iterations, restarts, beta};
 }
 
+// ---------------------------------------------------------------------------
+// Inner-outer stationary solve.  The outer iteration is flexible GMRES on the
+// bordered operator; each outer direction z_j approximately solves D z = v_j
+// with `kin` steps of right-preconditioned GMRES (sector preconditioner) on
+// D/s, D the sector-block-diagonal part of L.  The inner solve runs in the
+// workspace precision (complex FP32 for PIMI_C32: the mixed-precision form),
+// the outer iteration, its basis and all residuals in FP64.
+// ---------------------------------------------------------------------------
+static void ensure_inner(pimi_workspace *W, int kin) {
+    const int64_t n = W->plan->n;
+    const size_t es = W->prec == PIMI_C32 ? sizeof(cx<float>) : sizeof(cx<double>);
+    if (W->inner_k < kin) {
+        W->inner_mem.alloc(es * (size_t)n * (size_t)(kin + 1 + 4));
+        W->Hin.alloc((size_t)(kin + 1) * kin);
+        W->inner_norms.alloc((size_t)kin + 2);
+        W->inner_k = kin;
+    }
+    if (!W->Zo.p) W->Zo.alloc((size_t)n * W->m);
+}
+
+static void small_lsq(int k, int ldh, const std::vector<cd> &Hm, double beta, std::vector<cd> &y) {
+    // min |beta e1 - H y| for an upper Hessenberg (k+1) x k H by Givens rotations
+    std::vector<cd> R(Hm), g(k + 1, cd(0, 0));
+    g[0] = beta;
+    auto at = [&](int i, int j) -> cd & { return R[(size_t)i + (size_t)j * ldh]; };
+    for (int j = 0; j < k; ++j) {
+        double c;
+        cd sn;
+        complex_givens(at(j, j), at(j + 1, j), c, sn);
+        for (int jj = j; jj < k; ++jj) {
+            cd h1 = at(j, jj), h2 = at(j + 1, jj);
+            at(j, jj) = c * h1 + sn * h2;
+            at(j + 1, jj) = -std::conj(sn) * h1 + c * h2;
+        }
+        cd g1 = g[j], g2 = g[j + 1];
+        g[j] = c * g1 + sn * g2;
+        g[j + 1] = -std::conj(sn) * g1 + c * g2;
+    }
+    y.assign(k, cd(0, 0));
+    for (int i = k - 1; i >= 0; --i) {
+        cd acc = g[i];
+        for (int jj = i + 1; jj < k; ++jj) acc -= at(i, jj) * y[jj];
+        y[i] = at(i, i) == cd(0, 0) ? cd(0, 0) : acc / at(i, i);
+    }
+}
+
+template <typename R>
+static void inner_solve(pimi_workspace *W, const cx<double> *r64, cx<double> *z64, int kin, double scale) {
+    const int64_t n = W->plan->n;
+    cx<R> *base = reinterpret_cast<cx<R> *>(W->inner_mem.p);
+    cx<R> *Vi = base, *zt = base + (size_t)n * (kin + 1), *wt = zt + n, *t1 = wt + n, *t2 = t1 + n;
+    double *nrm = W->inner_norms.p;  // nrm[0..kin-1]: subdiagonal, nrm[kin]: beta
+    if constexpr (sizeof(R) == 8) {
+        dev_norm<double>(W, r64, n, nrm + kin);
+        hipLaunchKernelGGL(k_scale_div<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, Vi, r64, n,
+                           nrm + kin);
+    } else {
+        hipLaunchKernelGGL((k_convert<R, double>), dim3(1024), dim3(256), 0, W->stream, Vi, r64, n);
+        dev_norm<R>(W, Vi, n, nrm + kin);
+        hipLaunchKernelGGL(k_scale_div<R>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, Vi, Vi, n, nrm + kin);
+    }
+    HIPCHK(hipMemsetAsync(W->Hin.p, 0, sizeof(cx<double>) * (size_t)(kin + 1) * kin, W->stream));
+    for (int i = 0; i < kin; ++i) {
+        apply_precond_t<R>(W, Vi + (size_t)i * n, zt, scale, t1, t2);
+        launch_apply<R>(W, zt, wt, 1.0 / scale, 1, true);
+        dev_cgs2<R>(W, Vi, n, i + 1, wt, n, W->Hin.p + (size_t)i * (kin + 1));
+        dev_norm<R>(W, wt, n, nrm + i);
+        hipLaunchKernelGGL(k_scale_div<R>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, Vi + (size_t)(i + 1) * n,
+                           wt, n, nrm + i);
+    }
+    std::vector<cx<double>> hh((size_t)(kin + 1) * kin);
+    std::vector<double> nn((size_t)kin + 1);
+    HIPCHK(hipMemcpyAsync(hh.data(), W->Hin.p, sizeof(cx<double>) * hh.size(), hipMemcpyDeviceToHost, W->stream));
+    HIPCHK(hipMemcpyAsync(nn.data(), nrm, sizeof(double) * nn.size(), hipMemcpyDeviceToHost, W->stream));
+    sync(W);
+    const double beta = nn[kin];
+    int k = kin;
+    std::vector<cd> Hm((size_t)(kin + 1) * kin, cd(0, 0));
+    for (int j = 0; j < kin; ++j) {
+        for (int i = 0; i <= j; ++i) Hm[(size_t)i + (size_t)j * (kin + 1)] = cd(hh[(size_t)i + (size_t)j * (kin + 1)].x,
+                                                                             hh[(size_t)i + (size_t)j * (kin + 1)].y);
+        Hm[(size_t)(j + 1) + (size_t)j * (kin + 1)] = nn[j];
+        if (nn[j] == 0) { k = j + 1; break; }  // invariant subspace: later columns are unused
+    }
+    std::vector<cd> y;
+    if (beta == 0) { HIPCHK(hipMemsetAsync(z64, 0, sizeof(cx<double>) * n, W->stream)); return; }
+    small_lsq(k, kin + 1, Hm, beta, y);
+    std::vector<cx<double>> yc(k);
+    for (int i = 0; i < k; ++i) yc[i] = cx<double>{y[i].real(), y[i].imag()};
+    h2d(W, W->coef.p, yc.data(), sizeof(cx<double>) * k);
+    hipLaunchKernelGGL(k_gemv<R>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, wt, Vi, n, k, W->coef.p, n, 1.0,
+                       1);
+    if constexpr (sizeof(R) == 8) {
+        apply_precond_t<R>(W, wt, z64, scale, t1, t2);
+    } else {
+        apply_precond_t<R>(W, wt, zt, scale, t1, t2);
+        hipLaunchKernelGGL((k_convert<double, R>), dim3(1024), dim3(256), 0, W->stream, z64, zt, n);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol, double rtol, int64_t maxiter,
+                                 int64_t &applications, bool backward, int kin) {
+    const int64_t n = W->plan->n;
+    const int m = W->m;
+    const double scale = 1.0 / invscale;
+    ensure_inner(W, kin);
+    cx<double> *V = W->V.p, *Z = W->Zo.p;
+    auto inner = [&](const cx<double> *r, cx<double> *z) {
+        if (W->prec == PIMI_C32) inner_solve<float>(W, r, z, kin, scale);
+        else inner_solve<double>(W, r, z, kin, scale);
+        applications += kin;
+    };
+    std::vector<cd> H((size_t)(m + 1) * m), g(m + 1), sn(m), yv(m);
+    std::vector<double> cs(m);
+    auto Hij = [&](int i, int j) -> cd & { return H[(size_t)i + (size_t)j * (m + 1)]; };
+    dev_norm<double>(W, W->b.p, n, W->nscal.p);
+    dev_norm<double>(W, W->x.p, n, W->nscal.p + 1);
+    d2h(W, W->hpin, W->nscal.p, 2 * sizeof(double));
+    const double bnorm = W->hpin[0];
+    double xnorm = W->hpin[1];
+    auto threshold = [&]() { return atol + rtol * (backward ? bnorm + xnorm : bnorm); };
+    auto form_x = [&](int k, const std::vector<cd> &Hr, const std::vector<cd> &gr, cx<double> *dst, bool add_to_x) {
+        std::vector<cd> yy(k);
+        for (int i = k - 1; i >= 0; --i) {
+            cd acc = gr[i];
+            for (int jj = i + 1; jj < k; ++jj) acc -= Hr[(size_t)i + (size_t)jj * (m + 1)] * yy[jj];
+            yy[i] = acc / Hr[(size_t)i + (size_t)i * (m + 1)];
+        }
+        std::vector<cx<double>> yc(k);
+        for (int i = 0; i < k; ++i) yc[i] = cx<double>{yy[i].real(), yy[i].imag()};
+        h2d(W, W->coef.p, yc.data(), sizeof(cx<double>) * k);
+        if (!add_to_x) HIPCHK(hipMemcpyAsync(dst, W->x.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
+        hipLaunchKernelGGL(k_gemv<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, dst, Z, n, k, W->coef.p,
+                           n, 1.0, 0);
+    };
+    int64_t iterations = 0, restarts = 0;
+    steady_apply(W, W->x.p, W->w.p, invscale);
+    applications++;
+    hipLaunchKernelGGL(k_sub<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, W->r.p, W->b.p, W->w.p, n);
+    dev_norm<double>(W, W->r.p, n, W->nscal.p);
+    double beta = read_scalar(W, W->nscal.p);
+    if (beta <= threshold()) return {true, 0, 0, beta};
+    std::vector<cx<double>> hcol((size_t)m + 2);
+    while (iterations < maxiter) {
+        std::fill(H.begin(), H.end(), cd(0, 0));
+        std::fill(g.begin(), g.end(), cd(0, 0));
+        W->hpin[0] = beta;
+        h2d(W, W->nscal.p + 1, W->hpin, sizeof(double));
+        hipLaunchKernelGGL(k_scale_div<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, V, W->r.p, n,
+                           W->nscal.p + 1);
+        g[0] = beta;
+        int kdone = 0;
+        int jmax = (int)std::min<int64_t>(m, maxiter - iterations);
+        for (int j = 0; j < jmax; ++j) {
+            inner(V + (int64_t)j * n, Z + (int64_t)j * n);
+            steady_apply(W, Z + (int64_t)j * n, W->w.p, invscale);
+            applications++;
+            HIPCHK(hipMemsetAsync(W->hacc.p, 0, sizeof(cx<double>) * (j + 1), W->stream));
+            dev_cgs2<double>(W, V, n, j + 1, W->w.p, n, W->hacc.p);
+            dev_norm<double>(W, W->w.p, n, W->nscal.p);
+            hipLaunchKernelGGL(k_scale_div<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream,
+                               V + (int64_t)(j + 1) * n, W->w.p, n, W->nscal.p);
+            HIPCHK(hipMemcpyAsync(hcol.data(), W->hacc.p, sizeof(cx<double>) * (j + 1), hipMemcpyDeviceToHost,
+                                  W->stream));
+            HIPCHK(hipMemcpyAsync(W->hpin, W->nscal.p, sizeof(double), hipMemcpyDeviceToHost, W->stream));
+            sync(W);
+            for (int i = 0; i <= j; ++i) Hij(i, j) = cd(hcol[i].x, hcol[i].y);
+            Hij(j + 1, j) = W->hpin[0];
+            for (int i = 0; i < j; ++i) {
+                cd h1 = Hij(i, j), h2 = Hij(i + 1, j);
+                Hij(i, j) = cs[i] * h1 + sn[i] * h2;
+                Hij(i + 1, j) = -std::conj(sn[i]) * h1 + cs[i] * h2;
+            }
+            double c;
+            cd s;
+            complex_givens(Hij(j, j), Hij(j + 1, j), c, s);
+            cs[j] = c;
+            sn[j] = s;
+            cd h1 = Hij(j, j), h2 = Hij(j + 1, j);
+            Hij(j, j) = c * h1 + s * h2;
+            Hij(j + 1, j) = 0;
+            cd g1 = g[j], g2 = g[j + 1];
+            g[j] = c * g1 + s * g2;
+            g[j + 1] = -std::conj(s) * g1 + c * g2;
+            iterations++;
+            kdone = j + 1;
+            const double est = std::abs(g[j + 1]);
+            if (backward && (kdone % 5 == 0 || est <= threshold()) && Hij(j, j) != cd(0, 0)) {
+                form_x(kdone, H, g, W->trial.p, false);
+                dev_norm<double>(W, W->trial.p, n, W->nscal.p + 5);
+                xnorm = read_scalar(W, W->nscal.p + 5);
+            }
+            if (W->trace_progress && (iterations % 10 == 0))
+                std::fprintf(stderr, "[pimi] fgmres iteration %lld  residual %.3e  |b| %.3e  |x| %.3e\n",
+                             (long long)iterations, est, bnorm, xnorm);
+            if (est <= threshold()) break;
+        }
+        form_x(kdone, H, g, W->x.p, true);
+        restarts++;
+        steady_apply(W, W->x.p, W->w.p, invscale);
+        applications++;
+        hipLaunchKernelGGL(k_sub<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, W->r.p, W->b.p, W->w.p,
+                           n);
+        dev_norm<double>(W, W->r.p, n, W->nscal.p);
+        dev_norm<double>(W, W->x.p, n, W->nscal.p + 1);
+        d2h(W, W->hpin, W->nscal.p, 2 * sizeof(double));
+        beta = W->hpin[0];
+        xnorm = W->hpin[1];
+        if (beta <= threshold()) return {true, iterations, restarts, beta};
+    }
+    return {false, iterations, restarts, beta};
+}
+
 static double operator_scale(pimi_workspace *W) {
     const int64_t n = W->plan->n;
     uint64_t st = 0x51ca1eULL;
@@ -870,11 +1129,20 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, double *rh
     HIPCHK(hipMemcpyAsync(W->x.p, W->b.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
     int64_t apps = 3;
     const bool precond = o->preconditioner == PIMI_PRECOND_SECTOR;
-    if (o->preconditioner != PIMI_PRECOND_NONE && !precond) throw Error(PIMI_EINVAL, "unknown preconditioner");
-    if (precond) ensure_preconditioner(W);
+    const bool inner_outer = o->preconditioner == PIMI_PRECOND_SECTOR_INNER;
+    if (o->preconditioner != PIMI_PRECOND_NONE && !precond && !inner_outer)
+        throw Error(PIMI_EINVAL, "unknown preconditioner");
+    if (inner_outer && (o->inner_steps < 1 || o->inner_steps > 64))
+        throw Error(PIMI_EINVAL, "inner_steps must lie in 1..64");
+    if (W->prec == PIMI_C32 && !inner_outer)
+        throw Error(PIMI_EINVAL, "complex FP32 workspaces solve with the inner-outer preconditioner");
+    if (precond || inner_outer) ensure_preconditioner(W);
+    if (inner_outer) ensure_diag_stencil(P);
     const bool backward = o->criterion == PIMI_CRITERION_BACKWARD;
     if (o->criterion != PIMI_CRITERION_REFERENCE && !backward) throw Error(PIMI_EINVAL, "unknown criterion");
-    GmresResult gr = gmres_device(W, 1.0 / scale, o->atol, o->rtol, o->maxiter, apps, precond, backward);
+    GmresResult gr = inner_outer
+                         ? fgmres_device(W, 1.0 / scale, o->atol, o->rtol, o->maxiter, apps, backward, o->inner_steps)
+                         : gmres_device(W, 1.0 / scale, o->atol, o->rtol, o->maxiter, apps, precond, backward);
     if (backward) {
         // The trace row of the bordered system carries weight |v| = |t|^{-1}; under the
         // scale-invariant rule it is restored exactly by x <- x / (t . x), which leaves

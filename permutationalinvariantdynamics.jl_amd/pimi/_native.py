@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libpimi.so")
 
 PIMI_OK, PIMI_EINVAL, PIMI_EDIM, PIMI_ENOMEM, PIMI_EDEVICE, PIMI_EBUSY, PIMI_ENOTCONVERGED, PIMI_ERANGE = range(8)
 PIMI_C64, PIMI_C32 = 0, 1
-PRECOND_NONE, PRECOND_SECTOR = 0, 1
+PRECOND_NONE, PRECOND_SECTOR, PRECOND_SECTOR_INNER = 0, 1, 2
 CRITERION_REFERENCE, CRITERION_BACKWARD = 0, 1
 TERM_LOCAL_H, TERM_COLLECTIVE_H, TERM_LOCAL_JUMP, TERM_COLLECTIVE_JUMP, TERM_DIRECT_PRODUCT_H = range(5)
 
@@ -70,7 +70,7 @@ class PlanInfo(C.Structure):
 class GmresOpts(C.Structure):
     _fields_ = [("krylovdim", C.c_int32), ("preconditioner", C.c_int32), ("maxiter", C.c_int64), ("atol", C.c_double),
                 ("rtol", C.c_double), ("operator_scale", C.c_double), ("inner_rtol", C.c_double),
-                ("max_refinements", C.c_int64), ("criterion", C.c_int32), ("reserved2", C.c_int32)]
+                ("max_refinements", C.c_int64), ("criterion", C.c_int32), ("inner_steps", C.c_int32)]
 
 
 class ExpvOpts(C.Structure):

@@ -266,12 +266,13 @@ class GMRESAlgorithm:
     krylovdim: int = 30
     maxiter: int = 500
     preconditioner: object = None
+    inner_steps: int = 8
 
     def __post_init__(self):
         if self.krylovdim < 1 or self.maxiter < 1:
             raise ValueError("krylovdim and maxiter must be positive")
-        if self.preconditioner not in (None, "sector"):
-            raise ValueError("preconditioner must be nothing or \"sector\" on the mi355x backend")
+        if self.preconditioner not in (None, "sector", "sector_inner"):
+            raise ValueError("preconditioner must be nothing, \"sector\" or \"sector_inner\" on the mi355x backend")
 
 
 @dataclass
@@ -451,12 +452,14 @@ def stationary_state(compiled: CompiledPIModel, algorithm=None, atol=1e-10, rtol
     if not isinstance(algorithm, GMRESAlgorithm):
         raise ValueError("the mi355x backend solves stationary states with GMRESAlgorithm")
     ws = compiled.workspace(precision, algorithm.krylovdim)
-    pc = N.PRECOND_SECTOR if algorithm.preconditioner == "sector" else N.PRECOND_NONE
+    pc = {None: N.PRECOND_NONE, "sector": N.PRECOND_SECTOR,
+          "sector_inner": N.PRECOND_SECTOR_INNER}[algorithm.preconditioner]
     if criterion not in ("reference", "backward"):
         raise ValueError("criterion must be \"reference\" or \"backward\"")
     crit = N.CRITERION_BACKWARD if criterion == "backward" else N.CRITERION_REFERENCE
     opts = N.GmresOpts(algorithm.krylovdim, pc, algorithm.maxiter, float(atol), float(rtol),
-                       float(operator_scale or 0.0), float(inner_rtol), int(max_refinements), crit, 0)
+                       float(operator_scale or 0.0), float(inner_rtol), int(max_refinements), crit,
+                       int(algorithm.inner_steps))
     rho = np.zeros(len(compiled.basis), dtype=np.complex128)
     info = N.SolveInfo()
     N.check(N.lib().pimi_steady_gmres(ws, C.byref(opts), N.ptr(rho), C.byref(info)))
