@@ -237,6 +237,11 @@ pimi_status pimi_apply_bench(pimi_workspace *ws, int64_t ncols, int32_t reps, do
  * calibration kernel for HBM byte counters.  Measurement only. */
 pimi_status pimi_copy_bench(pimi_workspace *ws, int64_t bytes, int32_t reps, double *ms_avg);
 
+/* y = scale * M^{-1} x with the sector preconditioner on the device (host
+ * buffers, complex double; computed in the workspace precision).  For tests;
+ * no counterpart in the reference. */
+pimi_status pimi_precond_apply(pimi_workspace *ws, const double *x, double *y, double scale);
+
 /* Time of the last apply kernel launch sequence, in milliseconds (HIP events
  * recorded on the workspace stream around pimi_apply). */
 pimi_status pimi_workspace_last_apply_ms(const pimi_workspace *ws, double *ms);

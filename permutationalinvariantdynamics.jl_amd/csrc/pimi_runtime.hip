@@ -169,14 +169,13 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_norm(const double *__restr
     if (threadIdx.x == 0) *out = sqrt(s);
 }
 
-// dst = src / *denom  (skipped when *denom == 0)
+// dst = src / *denom; a zero denominator (exact breakdown) gives dst = 0
 template <typename R>
-__global__ __launch_bounds__(kRedBlock) void k_scale_div(cx<R> *__restrict__ dst, const cx<R> *__restrict__ src,
-                                                         int64_t n, const double *__restrict__ denom) {
+__global__ __launch_bounds__(kRedBlock) void k_scale_div(cx<R> *dst, const cx<R> *src, int64_t n,
+                                                         const double *__restrict__ denom) {
     const double d = *denom;
-    if (d == 0.0) return;
     for (int64_t k = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kRedBlock)
-        dst[k] = cx<R>{(R)(src[k].x / d), (R)(src[k].y / d)};
+        dst[k] = d == 0.0 ? cx<R>{0, 0} : cx<R>{(R)(src[k].x / d), (R)(src[k].y / d)};
 }
 
 // r = b - w
@@ -217,6 +216,83 @@ __global__ __launch_bounds__(1024) void k_trace_axpy(cx<R> *__restrict__ y, cons
         cx<R> yy = y[idx[k]];
         y[idx[k]] = cx<R>{(R)(yy.x + re * v[k]), (R)(yy.y + im * v[k])};
     }
+}
+
+// zero a buffer of complex doubles (a kernel, so that it can live inside a
+// captured graph, where memset nodes are avoided)
+__global__ void k_zero_c64(cx<double> *__restrict__ p, int64_t count) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (int64_t)gridDim.x * blockDim.x)
+        p[k] = cx<double>{0, 0};
+}
+
+// min |beta e1 - H y| for the inner solver's (k+1) x k upper Hessenberg H,
+// H(i, j) at Hin[i + j (k+1)] for i <= j, subdiagonal in nrm[0..k-1], beta in
+// nrm[k].  One lane; k <= 32.  A zero subdiagonal ends the space early.
+__global__ void k_inner_lsq(const cx<double> *__restrict__ Hin, const double *__restrict__ nrm, int k,
+                            cx<double> *__restrict__ y) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    constexpr int KMAX = 32;
+    double Rr[(KMAX + 1) * KMAX / 2 + KMAX], Ri[(KMAX + 1) * KMAX / 2 + KMAX];  // packed upper triangle
+    double gr[KMAX + 1], gi[KMAX + 1];
+    auto idx = [](int i, int j) { return j * (j + 1) / 2 + i; };  // i <= j
+    int kk = k;
+    for (int j = 0; j < k; ++j)
+        if (nrm[j] == 0.0) { kk = j + 1; break; }
+    for (int i = 0; i <= kk; ++i) { gr[i] = 0; gi[i] = 0; }
+    gr[0] = nrm[k];
+    // rotate column by column
+    double csv[KMAX], snr[KMAX], sni[KMAX];
+    for (int j = 0; j < kk; ++j) {
+        // load column j (rows 0..j) and subdiagonal
+        for (int i = 0; i <= j; ++i) { Rr[idx(i, j)] = Hin[i + (int64_t)j * (k + 1)].x; Ri[idx(i, j)] = Hin[i + (int64_t)j * (k + 1)].y; }
+        double sub_r = nrm[j], sub_i = 0.0;
+        for (int i = 0; i < j; ++i) {  // previous rotations
+            double h1r = Rr[idx(i, j)], h1i = Ri[idx(i, j)];
+            double h2r = Rr[idx(i + 1, j)], h2i = Ri[idx(i + 1, j)];
+            double c = csv[i], sr = snr[i], si = sni[i];
+            // h1' = c h1 + s h2 ; h2' = -conj(s) h1 + c h2
+            Rr[idx(i, j)] = c * h1r + (sr * h2r - si * h2i);
+            Ri[idx(i, j)] = c * h1i + (sr * h2i + si * h2r);
+            Rr[idx(i + 1, j)] = -(sr * h1r + si * h1i) + c * h2r;
+            Ri[idx(i + 1, j)] = -(sr * h1i - si * h1r) + c * h2i;
+        }
+        // new rotation zeroing the subdiagonal
+        double ar = Rr[idx(j, j)], ai = Ri[idx(j, j)];
+        double absa = sqrt(ar * ar + ai * ai), absb = sqrt(sub_r * sub_r + sub_i * sub_i);
+        double c, sr, si;
+        if (absb == 0) { c = 1; sr = 0; si = 0; }
+        else if (absa == 0) { c = 0; sr = 1; si = 0; }
+        else {
+            double scale = absa + absb;
+            double nrmab = scale * sqrt((absa / scale) * (absa / scale) + (absb / scale) * (absb / scale));
+            double alr = ar / absa, ali = ai / absa;
+            c = absa / nrmab;
+            // s = alpha * conj(b) / nrmab
+            sr = (alr * sub_r + ali * sub_i) / nrmab;
+            si = (ali * sub_r - alr * sub_i) / nrmab;
+        }
+        csv[j] = c; snr[j] = sr; sni[j] = si;
+        Rr[idx(j, j)] = c * ar + (sr * sub_r - si * sub_i);
+        Ri[idx(j, j)] = c * ai + (sr * sub_i + si * sub_r);
+        double g1r = gr[j], g1i = gi[j];
+        gr[j] = c * g1r;
+        gi[j] = c * g1i;
+        gr[j + 1] = -(sr * g1r + si * g1i);
+        gi[j + 1] = -(sr * g1i - si * g1r);
+    }
+    for (int i = kk - 1; i >= 0; --i) {
+        double accr = gr[i], acci = gi[i];
+        for (int jj = i + 1; jj < kk; ++jj) {
+            double hr = Rr[idx(i, jj)], hi = Ri[idx(i, jj)];
+            accr -= hr * y[jj].x - hi * y[jj].y;
+            acci -= hr * y[jj].y + hi * y[jj].x;
+        }
+        double dr = Rr[idx(i, i)], di = Ri[idx(i, i)];
+        double den = dr * dr + di * di;
+        y[i] = den == 0 ? cx<double>{0, 0}
+                        : cx<double>{(accr * dr + acci * di) / den, (acci * dr - accr * di) / den};
+    }
+    for (int i = kk; i < k; ++i) y[i] = cx<double>{0, 0};
 }
 
 // scatter a sparse vector into a zeroed dense one
@@ -287,6 +363,8 @@ struct pimi_plan {
     pimi::DevBuf<pimi::cx<float>> pU32, pInv32;
     pimi::DevBuf<pimi::GemmTile> gtiles;
     int64_t ngtiles = 0;
+    pimi::DevBuf<pimi::GemmTile> mtiles;  // 16x16 tiles for the matrix-core GEMM
+    int64_t nmtiles = 0;
     // sector-block-diagonal part of L (inner solver), built on first use
     bool diag_ready = false;
     pimi::DevBuf<int64_t> dg_term_base;
@@ -314,7 +392,12 @@ struct pimi_workspace {
     pimi::DevBuf<pimi::cx<double>> Zo, Hin;
     pimi::DevBuf<char> inner_mem;  // (kin+1) basis + 4 work vectors in the inner precision
     pimi::DevBuf<double> inner_norms;
+    pimi::DevBuf<pimi::cx<double>> inner_io, inner_y;  // fixed input/output vectors and LSQ solution
     int inner_k = 0;
+    hipGraphExec_t inner_graph = nullptr;  // captured inner solve (one per (kin, precision))
+    int inner_graph_k = 0, inner_graph_prec = -1;
+    double inner_graph_scale = 0.0;
+    bool use_graphs = true;  // PIMI_NO_GRAPH=1 runs the same kernels eagerly
     double *hpin = nullptr;  // pinned host staging (small results)
     bool trace_progress = false;  // PIMI_TRACE=1: solver progress lines on stderr
 };
@@ -687,6 +770,14 @@ static void ensure_preconditioner(pimi_workspace *W) {
         P->pInv32.upload(i32.data(), i32.size());
         P->gtiles.upload(tiles.data(), tiles.size());
         P->ngtiles = (int64_t)tiles.size();
+        std::vector<GemmTile> mt;
+        for (int s = 0; s < P->ns; ++s) {
+            int nt = (int)((P->host.dims[s] + kMfmaTile - 1) / kMfmaTile);
+            for (int tj = 0; tj < nt; ++tj)
+                for (int ti = 0; ti < nt; ++ti) mt.push_back(GemmTile{s, ti, tj, 0});
+        }
+        P->mtiles.upload(mt.data(), mt.size());
+        P->nmtiles = (int64_t)mt.size();
         P->prec_ready = true;
     }
     if (!W->zbuf.p) {
@@ -700,19 +791,20 @@ static void ensure_preconditioner(pimi_workspace *W) {
 template <typename R>
 static void apply_precond_t(pimi_workspace *W, const cx<R> *r, cx<R> *out, double scale, cx<R> *t1, cx<R> *t2) {
     pimi_plan *P = W->plan;
-    dim3 g((unsigned)P->ngtiles), b(256);
+    dim3 g((unsigned)((P->nmtiles + 3) / 4)), b(256);
     const cx<R> *nul = nullptr;
     const cx<R> *U, *Inv;
     if constexpr (sizeof(R) == 8) { U = P->pU64.p; Inv = P->pInv64.p; }
     else { U = P->pU32.p; Inv = P->pInv32.p; }
-    hipLaunchKernelGGL(k_sector_gemm<R>, g, b, 0, W->stream, U, r, t1, P->offs.p, P->dims.p, P->gtiles.p, 1, 0, nul,
-                       (R)1);
-    hipLaunchKernelGGL(k_sector_gemm<R>, g, b, 0, W->stream, t1, U, t2, P->offs.p, P->dims.p, P->gtiles.p, 0, 0, Inv,
-                       (R)scale);
-    hipLaunchKernelGGL(k_sector_gemm<R>, g, b, 0, W->stream, U, t2, t1, P->offs.p, P->dims.p, P->gtiles.p, 0, 0, nul,
-                       (R)1);
-    hipLaunchKernelGGL(k_sector_gemm<R>, g, b, 0, W->stream, t1, U, out, P->offs.p, P->dims.p, P->gtiles.p, 0, 1, nul,
-                       (R)1);
+    const int64_t nt = P->nmtiles;
+    hipLaunchKernelGGL(k_sector_gemm_mfma<R>, g, b, 0, W->stream, U, r, t1, P->offs.p, P->dims.p, P->mtiles.p, nt, 1,
+                       0, nul, (R)1);
+    hipLaunchKernelGGL(k_sector_gemm_mfma<R>, g, b, 0, W->stream, t1, U, t2, P->offs.p, P->dims.p, P->mtiles.p, nt, 0,
+                       0, Inv, (R)scale);
+    hipLaunchKernelGGL(k_sector_gemm_mfma<R>, g, b, 0, W->stream, U, t2, t1, P->offs.p, P->dims.p, P->mtiles.p, nt, 0,
+                       0, nul, (R)1);
+    hipLaunchKernelGGL(k_sector_gemm_mfma<R>, g, b, 0, W->stream, t1, U, out, P->offs.p, P->dims.p, P->mtiles.p, nt,
+                       0, 1, nul, (R)1);
     HIPCHK(hipGetLastError());
 }
 
@@ -888,93 +980,85 @@ static void ensure_inner(pimi_workspace *W, int kin) {
     const int64_t n = W->plan->n;
     const size_t es = W->prec == PIMI_C32 ? sizeof(cx<float>) : sizeof(cx<double>);
     if (W->inner_k < kin) {
+        if (W->inner_graph) { HIPCHK(hipGraphExecDestroy(W->inner_graph)); W->inner_graph = nullptr; }
         W->inner_mem.alloc(es * (size_t)n * (size_t)(kin + 1 + 4));
+        HIPCHK(hipMemsetAsync(W->inner_mem.p, 0, es * (size_t)n * (size_t)(kin + 1 + 4), W->stream));
         W->Hin.alloc((size_t)(kin + 1) * kin);
         W->inner_norms.alloc((size_t)kin + 2);
+        W->inner_io.alloc((size_t)2 * n);
+        W->inner_y.alloc((size_t)kin);
         W->inner_k = kin;
     }
     if (!W->Zo.p) W->Zo.alloc((size_t)n * W->m);
 }
 
-static void small_lsq(int k, int ldh, const std::vector<cd> &Hm, double beta, std::vector<cd> &y) {
-    // min |beta e1 - H y| for an upper Hessenberg (k+1) x k H by Givens rotations
-    std::vector<cd> R(Hm), g(k + 1, cd(0, 0));
-    g[0] = beta;
-    auto at = [&](int i, int j) -> cd & { return R[(size_t)i + (size_t)j * ldh]; };
-    for (int j = 0; j < k; ++j) {
-        double c;
-        cd sn;
-        complex_givens(at(j, j), at(j + 1, j), c, sn);
-        for (int jj = j; jj < k; ++jj) {
-            cd h1 = at(j, jj), h2 = at(j + 1, jj);
-            at(j, jj) = c * h1 + sn * h2;
-            at(j + 1, jj) = -std::conj(sn) * h1 + c * h2;
-        }
-        cd g1 = g[j], g2 = g[j + 1];
-        g[j] = c * g1 + sn * g2;
-        g[j + 1] = -std::conj(sn) * g1 + c * g2;
-    }
-    y.assign(k, cd(0, 0));
-    for (int i = k - 1; i >= 0; --i) {
-        cd acc = g[i];
-        for (int jj = i + 1; jj < k; ++jj) acc -= at(i, jj) * y[jj];
-        y[i] = at(i, i) == cd(0, 0) ? cd(0, 0) : acc / at(i, i);
-    }
-}
-
+// The inner solve with no host interaction: norms, Hessenberg, least squares
+// and the combination all stay on the device, so the sequence can be
+// captured once as a HIP graph and replayed for every outer step.
+// Reads W->inner_io[0:n], writes W->inner_io[n:2n].
 template <typename R>
-static void inner_solve(pimi_workspace *W, const cx<double> *r64, cx<double> *z64, int kin, double scale) {
+static void inner_solve_dev(pimi_workspace *W, int kin, double scale) {
     const int64_t n = W->plan->n;
+    const cx<double> *rin = W->inner_io.p;
+    cx<double> *zout = W->inner_io.p + n;
     cx<R> *base = reinterpret_cast<cx<R> *>(W->inner_mem.p);
     cx<R> *Vi = base, *zt = base + (size_t)n * (kin + 1), *wt = zt + n, *t1 = wt + n, *t2 = t1 + n;
-    double *nrm = W->inner_norms.p;  // nrm[0..kin-1]: subdiagonal, nrm[kin]: beta
+    double *nrm = W->inner_norms.p;
+    const dim3 gv(grid_for(n)), bv(kRedBlock);
     if constexpr (sizeof(R) == 8) {
-        dev_norm<double>(W, r64, n, nrm + kin);
-        hipLaunchKernelGGL(k_scale_div<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, Vi, r64, n,
-                           nrm + kin);
+        dev_norm<double>(W, rin, n, nrm + kin);
+        hipLaunchKernelGGL(k_scale_div<double>, gv, bv, 0, W->stream, Vi, rin, n, nrm + kin);
     } else {
-        hipLaunchKernelGGL((k_convert<R, double>), dim3(1024), dim3(256), 0, W->stream, Vi, r64, n);
+        hipLaunchKernelGGL((k_convert<R, double>), dim3(1024), dim3(256), 0, W->stream, Vi, rin, n);
         dev_norm<R>(W, Vi, n, nrm + kin);
-        hipLaunchKernelGGL(k_scale_div<R>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, Vi, Vi, n, nrm + kin);
+        hipLaunchKernelGGL(k_scale_div<R>, gv, bv, 0, W->stream, Vi, Vi, n, nrm + kin);
     }
-    HIPCHK(hipMemsetAsync(W->Hin.p, 0, sizeof(cx<double>) * (size_t)(kin + 1) * kin, W->stream));
+    hipLaunchKernelGGL(k_zero_c64, dim3(1), dim3(256), 0, W->stream, W->Hin.p, (int64_t)(kin + 1) * kin);
     for (int i = 0; i < kin; ++i) {
         apply_precond_t<R>(W, Vi + (size_t)i * n, zt, scale, t1, t2);
         launch_apply<R>(W, zt, wt, 1.0 / scale, 1, true);
         dev_cgs2<R>(W, Vi, n, i + 1, wt, n, W->Hin.p + (size_t)i * (kin + 1));
         dev_norm<R>(W, wt, n, nrm + i);
-        hipLaunchKernelGGL(k_scale_div<R>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, Vi + (size_t)(i + 1) * n,
-                           wt, n, nrm + i);
+        hipLaunchKernelGGL(k_scale_div<R>, gv, bv, 0, W->stream, Vi + (size_t)(i + 1) * n, wt, n, nrm + i);
     }
-    std::vector<cx<double>> hh((size_t)(kin + 1) * kin);
-    std::vector<double> nn((size_t)kin + 1);
-    HIPCHK(hipMemcpyAsync(hh.data(), W->Hin.p, sizeof(cx<double>) * hh.size(), hipMemcpyDeviceToHost, W->stream));
-    HIPCHK(hipMemcpyAsync(nn.data(), nrm, sizeof(double) * nn.size(), hipMemcpyDeviceToHost, W->stream));
-    sync(W);
-    const double beta = nn[kin];
-    int k = kin;
-    std::vector<cd> Hm((size_t)(kin + 1) * kin, cd(0, 0));
-    for (int j = 0; j < kin; ++j) {
-        for (int i = 0; i <= j; ++i) Hm[(size_t)i + (size_t)j * (kin + 1)] = cd(hh[(size_t)i + (size_t)j * (kin + 1)].x,
-                                                                             hh[(size_t)i + (size_t)j * (kin + 1)].y);
-        Hm[(size_t)(j + 1) + (size_t)j * (kin + 1)] = nn[j];
-        if (nn[j] == 0) { k = j + 1; break; }  // invariant subspace: later columns are unused
-    }
-    std::vector<cd> y;
-    if (beta == 0) { HIPCHK(hipMemsetAsync(z64, 0, sizeof(cx<double>) * n, W->stream)); return; }
-    small_lsq(k, kin + 1, Hm, beta, y);
-    std::vector<cx<double>> yc(k);
-    for (int i = 0; i < k; ++i) yc[i] = cx<double>{y[i].real(), y[i].imag()};
-    h2d(W, W->coef.p, yc.data(), sizeof(cx<double>) * k);
-    hipLaunchKernelGGL(k_gemv<R>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, wt, Vi, n, k, W->coef.p, n, 1.0,
-                       1);
+    hipLaunchKernelGGL(k_inner_lsq, dim3(1), dim3(64), 0, W->stream, W->Hin.p, nrm, kin, W->inner_y.p);
+    hipLaunchKernelGGL(k_gemv<R>, gv, bv, 0, W->stream, wt, Vi, n, kin, W->inner_y.p, n, 1.0, 1);
     if constexpr (sizeof(R) == 8) {
-        apply_precond_t<R>(W, wt, z64, scale, t1, t2);
+        apply_precond_t<R>(W, wt, zout, scale, t1, t2);
     } else {
         apply_precond_t<R>(W, wt, zt, scale, t1, t2);
-        hipLaunchKernelGGL((k_convert<double, R>), dim3(1024), dim3(256), 0, W->stream, z64, zt, n);
+        hipLaunchKernelGGL((k_convert<double, R>), dim3(1024), dim3(256), 0, W->stream, zout, zt, n);
     }
     HIPCHK(hipGetLastError());
+}
+
+static void run_inner(pimi_workspace *W, int kin, double scale) {
+    auto body = [&]() {
+        if (W->prec == PIMI_C32) inner_solve_dev<float>(W, kin, scale);
+        else inner_solve_dev<double>(W, kin, scale);
+    };
+    if (!W->use_graphs) { body(); return; }
+    if (!W->inner_graph || W->inner_graph_k != kin || W->inner_graph_prec != W->prec ||
+        W->inner_graph_scale != scale) {
+        if (W->inner_graph) { HIPCHK(hipGraphExecDestroy(W->inner_graph)); W->inner_graph = nullptr; }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(W->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(W->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(W->stream, &g));
+        hipError_t e = hipGraphInstantiate(&W->inner_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(e);
+        W->inner_graph_k = kin;
+        W->inner_graph_prec = W->prec;
+        W->inner_graph_scale = scale;
+    }
+    HIPCHK(hipGraphLaunch(W->inner_graph, W->stream));
 }
 
 static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol, double rtol, int64_t maxiter,
@@ -985,8 +1069,9 @@ static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol
     ensure_inner(W, kin);
     cx<double> *V = W->V.p, *Z = W->Zo.p;
     auto inner = [&](const cx<double> *r, cx<double> *z) {
-        if (W->prec == PIMI_C32) inner_solve<float>(W, r, z, kin, scale);
-        else inner_solve<double>(W, r, z, kin, scale);
+        HIPCHK(hipMemcpyAsync(W->inner_io.p, r, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
+        run_inner(W, kin, scale);
+        HIPCHK(hipMemcpyAsync(z, W->inner_io.p + n, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
         applications += kin;
     };
     std::vector<cd> H((size_t)(m + 1) * m), g(m + 1), sn(m), yv(m);
@@ -1132,8 +1217,8 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, double *rh
     const bool inner_outer = o->preconditioner == PIMI_PRECOND_SECTOR_INNER;
     if (o->preconditioner != PIMI_PRECOND_NONE && !precond && !inner_outer)
         throw Error(PIMI_EINVAL, "unknown preconditioner");
-    if (inner_outer && (o->inner_steps < 1 || o->inner_steps > 64))
-        throw Error(PIMI_EINVAL, "inner_steps must lie in 1..64");
+    if (inner_outer && (o->inner_steps < 1 || o->inner_steps > 32))
+        throw Error(PIMI_EINVAL, "inner_steps must lie in 1..32");
     if (W->prec == PIMI_C32 && !inner_outer)
         throw Error(PIMI_EINVAL, "complex FP32 workspaces solve with the inner-outer preconditioner");
     if (precond || inner_outer) ensure_preconditioner(W);
@@ -1695,6 +1780,8 @@ pimi_status pimi_workspace_create(pimi_plan *P, int32_t precision, int32_t krylo
         W->m = (int)std::min<int64_t>(krylovdim, P->n);
         const char *tr = std::getenv("PIMI_TRACE");
         W->trace_progress = tr && tr[0] == '1';
+        const char *ng = std::getenv("PIMI_NO_GRAPH");
+        W->use_graphs = !(ng && ng[0] == '1');
         HIPCHK(hipStreamCreateWithFlags(&W->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&W->ev0));
         HIPCHK(hipEventCreate(&W->ev1));
@@ -1708,6 +1795,7 @@ void pimi_workspace_destroy(pimi_workspace *W) {
     if (!W) return;
     (void)hipSetDevice(W->plan->ctx->device);
     if (W->stream) (void)hipStreamSynchronize(W->stream);
+    if (W->inner_graph) (void)hipGraphExecDestroy(W->inner_graph);
     if (W->hpin) (void)hipHostFree(W->hpin);
     if (W->ev0) (void)hipEventDestroy(W->ev0);
     if (W->ev1) (void)hipEventDestroy(W->ev1);
@@ -1818,6 +1906,27 @@ pimi_status pimi_copy_bench(pimi_workspace *W, int64_t bytes, int32_t reps, doub
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, W->ev0, W->ev1));
         *ms_avg = (double)ms / reps;
+    });
+}
+
+pimi_status pimi_precond_apply(pimi_workspace *W, const double *x, double *y, double scale) {
+    return guarded([&] {
+        if (!W || !x || !y) throw Error(PIMI_EINVAL, "null argument");
+        BusyGuard guard(W->busy);
+        HIPCHK(hipSetDevice(W->plan->ctx->device));
+        ensure_preconditioner(W);
+        const int64_t n = W->plan->n;
+        h2d(W, W->tmp.p, x, sizeof(cx<double>) * n);
+        if (W->prec == PIMI_C32) {
+            DevBuf<cx<float>> a, b, t1, t2;
+            a.alloc(n); b.alloc(n); t1.alloc(n); t2.alloc(n);
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(1024), dim3(256), 0, W->stream, a.p, W->tmp.p, n);
+            apply_precond_t<float>(W, a.p, b.p, scale, t1.p, t2.p);
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, W->stream, W->w.p, b.p, n);
+        } else {
+            apply_precond(W, W->tmp.p, W->w.p, scale);
+        }
+        d2h(W, y, W->w.p, sizeof(cx<double>) * n);
     });
 }
 

@@ -24,7 +24,7 @@ EXPORTS = [
     "pimi_collective_operator", "pimi_plan_export_sizes", "pimi_plan_export", "pimi_plan_destroy",
     "pimi_workspace_create", "pimi_workspace_destroy", "pimi_apply", "pimi_steady_gmres", "pimi_expv_series",
     "pimi_workspace_last_apply_ms", "pimi_apply_bench", "pimi_copy_bench",
-    "pimi_sector_preconditioner_factors",
+    "pimi_sector_preconditioner_factors", "pimi_precond_apply",
 ]
 
 
