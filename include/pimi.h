@@ -143,9 +143,10 @@ typedef struct {
     int64_t max_refinements;
     /* PIMI_CRITERION_REFERENCE: |r| <= atol + rtol |b| and |L x|/s <= atol + rtol max(|x|, 1)
          (src/krylov.jl:743-746,868-873);
-       PIMI_CRITERION_BACKWARD:  |r| <= atol + rtol (|b| + |x|) and |L x|/(s |x|) <= atol + rtol,
-         the scale-invariant relative residual; at large N |b| = 2^(-N/2) is far below
-         eps |x|, where the reference rule cannot be met in floating point. */
+       PIMI_CRITERION_BACKWARD:  |r| <= atol + rtol |x| / 2 and |L x|/s <= atol + rtol |x|, i.e. for
+         atol = 0 the scale-invariant relative residual |L x|/(s |x|) <= rtol (t^H L = 0 bounds
+         |L x|/s by 2 |r|); at large N |b| = 2^(-N/2) is far below eps |x|, where the
+         reference rule cannot be met in floating point. */
     int32_t criterion;
     int32_t inner_steps;    /* PIMI_PRECOND_SECTOR_INNER: inner GMRES steps per outer step */
 } pimi_gmres_opts;

@@ -439,7 +439,9 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
 #define Hij(i, j) H[(i) + (j) * (m + 1)]
     double bnorm = dznrm2(b, n);
     double xnorm = dznrm2(x, n);
-    double tol = atol + rtol * (backward ? bnorm + xnorm : bnorm);
+    /* backward: t^H L = 0 and t.v = 1 give |L x|/s <= 2 |r|, so |r| <= rtol |x| / 2
+       guarantees the relative residual |L x| / (s |x|) <= rtol */
+    double tol = backward ? atol + 0.5 * rtol * xnorm : atol + rtol * bnorm;
     int64_t iterations = 0, restarts = 0;
     steady_apply(op, x, w);
     for (int64_t i = 0; i < n; ++i) r[i] = b[i] - w[i];
@@ -488,7 +490,7 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
             kdone = j + 1;
             if (backward && (kdone % 25 == 0 || cabs(g[j + 1]) <= tol)) {
                 xnorm = iterate_norm(op, x, V, H, g, m, kdone, pc, tmp, z);
-                tol = atol + rtol * (bnorm + xnorm);
+                tol = atol + 0.5 * rtol * xnorm;
             }
             if (cabs(g[j + 1]) <= tol) break;
         }
@@ -511,7 +513,7 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
         steady_apply(op, x, w);
         for (int64_t i = 0; i < n; ++i) r[i] = b[i] - w[i];
         beta = dznrm2(r, n);
-        if (backward) { xnorm = dznrm2(x, n); tol = atol + rtol * (bnorm + xnorm); }
+        if (backward) { xnorm = dznrm2(x, n); tol = atol + 0.5 * rtol * xnorm; }
         if (beta <= tol) { conv = 1; break; }
     }
 done:
@@ -552,10 +554,7 @@ int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const
     for (int64_t k = 0; k < nt; ++k) tr += conj(tv[k]) * x_out[tidx[k]];
     double terr = cabs(tr - 1.0);
     double xn = dznrm2(x_out, n);
-    double bn = 0;
-    for (int64_t k = 0; k < nt; ++k) bn += creal(conj(vv[k]) * vv[k]);
-    bn = sqrt(bn);
-    double tol = backward ? atol + rtol * (xn + bn) : atol + rtol * (xn > 1.0 ? xn : 1.0);
+    double tol = backward ? atol + rtol * xn : atol + rtol * (xn > 1.0 ? xn : 1.0);
     info->converged = conv && nres <= tol && terr <= atol + rtol;
     info->iterations = it; info->restarts = rs; info->applications = op.applications;
     info->residual = resid; info->normalized_residual = nres; info->trace_error = terr;

@@ -19,9 +19,13 @@
 #include "pimi_lower.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <limits>
+#include <string>
+#include <thread>
 #include <tuple>
 
 #include "../../include/pimi.h"
@@ -646,58 +650,136 @@ void build_stencil(const FusedPlan &P, Stencil &st, bool sector_diagonal_only) {
 // stationary-solve preconditioner factors
 // ---------------------------------------------------------------------------
 void hermitian_eigh(int64_t m, std::vector<cd> A, std::vector<double> &d, std::vector<cd> &U) {
-    auto at = [&](std::vector<cd> &M, int64_t i, int64_t j) -> cd & { return M[(size_t)(i + j * m)]; };
-    U.assign((size_t)(m * m), cd(0, 0));
-    for (int64_t i = 0; i < m; ++i) at(U, i, i) = 1.0;
-    double scale = 0;
-    for (auto &z : A) scale += std::norm(z);
-    for (int sweep = 0; sweep < 60; ++sweep) {
-        double off = 0;
-        for (int64_t q = 0; q < m; ++q)
-            for (int64_t p = 0; p < q; ++p) off += std::norm(at(A, p, q));
-        if (off <= 1e-32 * scale || off == 0) break;
-        for (int64_t p = 0; p < m; ++p)
-            for (int64_t q = p + 1; q < m; ++q) {
-                cd apq = at(A, p, q);
-                double b = std::abs(apq);
-                if (b == 0 || b * b <= 1e-36 * scale) continue;
-                cd ph = apq / b;                      // e^{i phi}
-                double a = at(A, p, p).real(), dd = at(A, q, q).real();
-                double zeta = (dd - a) / (2 * b);
-                double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));
-                double c = 1 / std::sqrt(1 + t * t), sn = t * c;
-                // J = [[c, sn], [-sn e^{-i phi}, c e^{-i phi}]] on (p, q)
-                cd j00 = c, j01 = sn, j10 = -sn * std::conj(ph), j11 = c * std::conj(ph);
-                for (int64_t i = 0; i < m; ++i) {  // columns: A <- A J
-                    cd x = at(A, i, p), y = at(A, i, q);
-                    at(A, i, p) = x * j00 + y * j10;
-                    at(A, i, q) = x * j01 + y * j11;
+    // 1. Householder reduction A = Q T Q^H, T Hermitian tridiagonal.  With
+    //    v the unit reflector vector, p = A v and beta = v^H A v (real),
+    //    H A H = A - v w^H - w v^H for w = 2 p - 2 beta v.
+    auto at = [m](std::vector<cd> &M, int64_t i, int64_t j) -> cd & { return M[(size_t)(i + j * m)]; };
+    std::vector<cd> Q((size_t)(m * m), cd(0, 0));
+    for (int64_t i = 0; i < m; ++i) at(Q, i, i) = 1.0;
+    std::vector<cd> v((size_t)m), w((size_t)m);
+    for (int64_t k = 0; k + 2 < m; ++k) {
+        double xn = 0;
+        for (int64_t i = k + 1; i < m; ++i) xn += std::norm(at(A, i, k));
+        xn = std::sqrt(xn);
+        if (xn == 0) continue;
+        const cd x0 = at(A, k + 1, k);
+        const cd ph = std::abs(x0) > 0 ? x0 / std::abs(x0) : cd(1, 0);
+        const cd alpha = -ph * xn;
+        for (int64_t i = k + 1; i < m; ++i) v[(size_t)i] = at(A, i, k);
+        v[(size_t)(k + 1)] -= alpha;
+        double vn = 0;
+        for (int64_t i = k + 1; i < m; ++i) vn += std::norm(v[(size_t)i]);
+        vn = std::sqrt(vn);
+        if (vn == 0) continue;
+        for (int64_t i = k + 1; i < m; ++i) v[(size_t)i] /= vn;
+        // p = A v over the trailing block (rows k..m-1; column k handled exactly below)
+        for (int64_t i = k + 1; i < m; ++i) w[(size_t)i] = 0;
+        for (int64_t j = k + 1; j < m; ++j) {
+            const cd vj = v[(size_t)j];
+            if (vj == cd(0, 0)) continue;
+            for (int64_t i = k + 1; i < m; ++i) w[(size_t)i] += at(A, i, j) * vj;
+        }
+        double beta = 0;
+        for (int64_t i = k + 1; i < m; ++i) beta += (std::conj(v[(size_t)i]) * w[(size_t)i]).real();
+        for (int64_t i = k + 1; i < m; ++i) w[(size_t)i] = 2.0 * w[(size_t)i] - 2.0 * beta * v[(size_t)i];
+        for (int64_t j = k + 1; j < m; ++j) {
+            const cd cvj = std::conj(v[(size_t)j]), cwj = std::conj(w[(size_t)j]);
+            for (int64_t i = k + 1; i < m; ++i) at(A, i, j) -= v[(size_t)i] * cwj + w[(size_t)i] * cvj;
+        }
+        // column / row k: H x = alpha e1
+        at(A, k + 1, k) = alpha;
+        at(A, k, k + 1) = std::conj(alpha);
+        for (int64_t i = k + 2; i < m; ++i) { at(A, i, k) = 0; at(A, k, i) = 0; }
+        // Q <- Q H on columns k+1..m-1
+        for (int64_t i = 0; i < m; ++i) {
+            cd qv(0, 0);
+            for (int64_t j = k + 1; j < m; ++j) qv += at(Q, i, j) * v[(size_t)j];
+            qv *= 2.0;
+            for (int64_t j = k + 1; j < m; ++j) at(Q, i, j) -= qv * std::conj(v[(size_t)j]);
+        }
+    }
+    // 2. T = D T_r D^H with T_r real (subdiagonal |e_k|), D = diag of phases.
+    std::vector<double> diag((size_t)m), sub((size_t)m, 0.0);
+    std::vector<cd> ph((size_t)m, cd(1, 0));
+    for (int64_t k = 0; k < m; ++k) diag[(size_t)k] = at(A, k, k).real();
+    for (int64_t k = 0; k + 1 < m; ++k) {
+        const cd e = at(A, k + 1, k);
+        const double ae = std::abs(e);
+        sub[(size_t)k] = ae;
+        ph[(size_t)(k + 1)] = ae > 0 ? ph[(size_t)k] * (e / ae) : ph[(size_t)k];
+    }
+    // 3. T_r = Z diag(d) Z^T by the implicit QL iteration with Wilkinson shifts
+    //    (Bowdler, Martin, Reinsch and Wilkinson 1968).
+    std::vector<double> Z((size_t)(m * m), 0.0);
+    for (int64_t i = 0; i < m; ++i) Z[(size_t)(i + i * m)] = 1.0;
+    std::vector<double> &dd = diag, &e = sub;
+    for (int64_t l = 0; l < m; ++l) {
+        int iter = 0;
+        while (true) {
+            int64_t mm = l;
+            for (; mm + 1 < m; ++mm) {
+                const double t = std::fabs(dd[(size_t)mm]) + std::fabs(dd[(size_t)(mm + 1)]);
+                if (std::fabs(e[(size_t)mm]) <= std::numeric_limits<double>::epsilon() * t) break;
+            }
+            if (mm == l) break;
+            if (++iter > 60) throw Error(PIMI_EINVAL, "tridiagonal QL iteration did not converge");
+            double g = (dd[(size_t)(l + 1)] - dd[(size_t)l]) / (2.0 * e[(size_t)l]);
+            double r = std::hypot(g, 1.0);
+            g = dd[(size_t)mm] - dd[(size_t)l] + e[(size_t)l] / (g + (g >= 0 ? r : -r));
+            double sn = 1.0, c = 1.0, p = 0.0;
+            int64_t i = mm - 1;
+            bool early = false;
+            for (; i >= l; --i) {
+                double f = sn * e[(size_t)i], b = c * e[(size_t)i];
+                r = std::hypot(f, g);
+                e[(size_t)(i + 1)] = r;
+                if (r == 0.0) {
+                    dd[(size_t)(i + 1)] -= p;
+                    e[(size_t)mm] = 0.0;
+                    early = true;
+                    break;
                 }
-                for (int64_t j = 0; j < m; ++j) {  // rows: A <- J^H A
-                    cd x = at(A, p, j), y = at(A, q, j);
-                    at(A, p, j) = std::conj(j00) * x + std::conj(j10) * y;
-                    at(A, q, j) = std::conj(j01) * x + std::conj(j11) * y;
-                }
-                at(A, p, q) = 0;
-                at(A, q, p) = 0;
-                for (int64_t i = 0; i < m; ++i) {
-                    cd x = at(U, i, p), y = at(U, i, q);
-                    at(U, i, p) = x * j00 + y * j10;
-                    at(U, i, q) = x * j01 + y * j11;
+                sn = f / r;
+                c = g / r;
+                g = dd[(size_t)(i + 1)] - p;
+                r = (dd[(size_t)i] - g) * sn + 2.0 * c * b;
+                p = sn * r;
+                dd[(size_t)(i + 1)] = g + p;
+                g = c * r - b;
+                double *zi = &Z[(size_t)(i * m)], *zi1 = &Z[(size_t)((i + 1) * m)];
+                for (int64_t k = 0; k < m; ++k) {
+                    const double t = zi1[k];
+                    zi1[k] = sn * zi[k] + c * t;
+                    zi[k] = c * zi[k] - sn * t;
                 }
             }
+            if (early) continue;
+            dd[(size_t)l] -= p;
+            e[(size_t)l] = g;
+            e[(size_t)mm] = 0.0;
+        }
     }
-    d.resize((size_t)m);
-    for (int64_t i = 0; i < m; ++i) d[(size_t)i] = at(A, i, i).real();
+    // 4. U = Q D Z
+    U.assign((size_t)(m * m), cd(0, 0));
+    for (int64_t k = 0; k < m; ++k) {
+        const cd pk = ph[(size_t)k];
+        for (int64_t i = 0; i < m; ++i) at(Q, i, k) *= pk;
+    }
+    for (int64_t j = 0; j < m; ++j)
+        for (int64_t k = 0; k < m; ++k) {
+            const double z = Z[(size_t)(k + j * m)];
+            if (z == 0.0) continue;
+            for (int64_t i = 0; i < m; ++i) at(U, i, j) += at(Q, i, k) * z;
+        }
+    d = dd;
 }
 
 void build_sector_preconditioner(const FusedPlan &P, std::vector<cd> &U, std::vector<cd> &inv_den) {
     U.assign((size_t)P.n, cd(0, 0));
     inv_den.assign((size_t)P.n, cd(0, 0));
-    for (int s = 0; s < P.ns; ++s) {
+    auto one_sector = [&](int s) {
         int64_t m = P.dims[s];
         std::vector<cd> Hd = P.has_h ? csc_to_dense(P.H[s]) : std::vector<cd>((size_t)(m * m), cd(0, 0));
-        std::vector<cd> Qd = P.has_q ? csc_to_dense(P.Q[s]) : std::vector<cd>((size_t)(m * m), cd(0, 0));
         // Hermitian part of the fused block (it is Hermitian up to rounding)
         for (int64_t j = 0; j < m; ++j)
             for (int64_t i = 0; i <= j; ++i) {
@@ -708,15 +790,19 @@ void build_sector_preconditioner(const FusedPlan &P, std::vector<cd> &U, std::ve
         std::vector<double> d;
         std::vector<cd> Us;
         hermitian_eigh(m, Hd, d, Us);
+        // q_a = (U^H Q U)_{aa}, Q in its sparse column form
         std::vector<double> q((size_t)m, 0.0);
-        for (int64_t a = 0; a < m; ++a) {
-            cd acc(0, 0);
-            for (int64_t c = 0; c < m; ++c) {
-                cd qu(0, 0);
-                for (int64_t b = 0; b < m; ++b) qu += Qd[(size_t)(c + b * m)] * Us[(size_t)(b + a * m)];
-                acc += std::conj(Us[(size_t)(c + a * m)]) * qu;
+        if (P.has_q) {
+            const CSC &Qs = P.Q[s];
+            for (int64_t a = 0; a < m; ++a) {
+                cd acc(0, 0);
+                for (int64_t b = 0; b < m; ++b) {
+                    const cd uba = Us[(size_t)(b + a * m)];
+                    for (int64_t k = Qs.colptr[(size_t)b]; k < Qs.colptr[(size_t)(b + 1)]; ++k)
+                        acc += std::conj(Us[(size_t)(Qs.rows[(size_t)k] + a * m)]) * Qs.vals[(size_t)k] * uba;
+                }
+                q[(size_t)a] = acc.real();
             }
-            q[(size_t)a] = acc.real();
         }
         double dscale = 0;
         for (int64_t a = 0; a < m; ++a) dscale = std::max(dscale, std::fabs(d[(size_t)a]) + std::fabs(q[(size_t)a]));
@@ -728,7 +814,29 @@ void build_sector_preconditioner(const FusedPlan &P, std::vector<cd> &U, std::ve
                 inv_den[(size_t)(P.offs[s] + a + b * m)] = inv;
                 U[(size_t)(P.offs[s] + a + b * m)] = Us[(size_t)(a + b * m)];
             }
-    }
+    };
+    // sectors are independent: largest first over a small thread pool
+    std::vector<int> order((size_t)P.ns);
+    for (int s = 0; s < P.ns; ++s) order[(size_t)s] = s;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return P.dims[a] > P.dims[b]; });
+    unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    int nthreads = (int)std::min<unsigned>(std::min<unsigned>(hw, 16u), (unsigned)std::max(P.ns, 1));
+    std::atomic<int> next{0};
+    std::vector<std::string> errors((size_t)nthreads);
+    auto worker = [&](int tid) {
+        try {
+            for (int k; (k = next.fetch_add(1)) < P.ns;) one_sector(order[(size_t)k]);
+        } catch (const std::exception &ex) {
+            errors[(size_t)tid] = ex.what();
+            next.store(P.ns);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto &th : pool) th.join();
+    for (auto &msg : errors)
+        if (!msg.empty()) throw Error(PIMI_EINVAL, msg);
 }
 
 }  // namespace pimi

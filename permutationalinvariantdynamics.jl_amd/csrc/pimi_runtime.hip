@@ -822,8 +822,10 @@ struct GmresResult {
 // Restarted GMRES(m) on the bordered operator (src/krylov.jl:732-786); x, b on
 // device.  With `precond` the sector preconditioner is applied on the right
 // (x = x0 + M^{-1} V y), so the GMRES residual is the true residual.  With
-// `backward` the stopping threshold is atol + rtol (|b| + |x_k|), where |x_k|
-// of the current iterate is formed on the device every kXnormEvery steps.
+// `backward` the stopping threshold is atol + rtol |x_k| / 2, where |x_k| of
+// the current iterate is formed on the device every kXnormEvery steps: since
+// t^H L = 0 and t.v = 1, the bordered residual r bounds |L x| / s <= 2 |r|,
+// so the threshold guarantees the relative residual |L x| / (s |x|) <= rtol.
 constexpr int kXnormEvery = 25;
 
 static double iterate_norm(pimi_workspace *W, const std::vector<cd> &Hm, const std::vector<cd> &g, int m, int k,
@@ -868,7 +870,7 @@ static GmresResult gmres_device(pimi_workspace *W, double invscale, double atol,
     d2h(W, W->hpin, W->nscal.p, 2 * sizeof(double));
     const double bnorm = W->hpin[0];
     double xnorm = W->hpin[1];
-    auto threshold = [&]() { return atol + rtol * (backward ? bnorm + xnorm : bnorm); };
+    auto threshold = [&]() { return backward ? atol + 0.5 * rtol * xnorm : atol + rtol * bnorm; };
     int64_t iterations = 0, restarts = 0;
     steady_apply(W, W->x.p, W->w.p, invscale);
     applications++;
@@ -1082,7 +1084,7 @@ static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol
     d2h(W, W->hpin, W->nscal.p, 2 * sizeof(double));
     const double bnorm = W->hpin[0];
     double xnorm = W->hpin[1];
-    auto threshold = [&]() { return atol + rtol * (backward ? bnorm + xnorm : bnorm); };
+    auto threshold = [&]() { return backward ? atol + 0.5 * rtol * xnorm : atol + rtol * bnorm; };
     auto form_x = [&](int k, const std::vector<cd> &Hr, const std::vector<cd> &gr, cx<double> *dst, bool add_to_x) {
         std::vector<cd> yy(k);
         for (int i = k - 1; i >= 0; --i) {
@@ -1257,10 +1259,7 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, double *rh
     for (size_t k = 0; k < P->tidx.size(); ++k) tr += P->tval[k] * cd(xh[P->tidx[k]].x, xh[P->tidx[k]].y);
     double terr = std::abs(tr - 1.0);
     double nres = resid / scale;
-    double bn = 0;
-    for (double v : vv) bn += v * v;
-    bn = std::sqrt(bn);
-    double tol = backward ? o->atol + o->rtol * (xn + bn) : o->atol + o->rtol * std::max(xn, 1.0);
+    double tol = backward ? o->atol + o->rtol * xn : o->atol + o->rtol * std::max(xn, 1.0);
     std::memcpy(rho_out, xh.data(), sizeof(cx<double>) * n);
     std::memset(info, 0, sizeof(*info));
     info->converged = gr.converged && nres <= tol && terr <= o->atol + o->rtol;
