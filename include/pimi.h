@@ -143,15 +143,26 @@ typedef struct {
     int64_t max_refinements;
     /* PIMI_CRITERION_REFERENCE: |r| <= atol + rtol |b| and |L x|/s <= atol + rtol max(|x|, 1)
          (src/krylov.jl:743-746,868-873);
-       PIMI_CRITERION_BACKWARD:  |r| <= atol + rtol |x| / 2 and |L x|/s <= atol + rtol |x|, i.e. for
+       PIMI_CRITERION_BACKWARD:  |r| <= (atol + rtol |x|) / 2 and |L x|/s <= atol + rtol |x|, i.e. for
          atol = 0 the scale-invariant relative residual |L x|/(s |x|) <= rtol (t^H L = 0 bounds
          |L x|/s by 2 |r|); at large N |b| = 2^(-N/2) is far below eps |x|, where the
          reference rule cannot be met in floating point. */
     int32_t criterion;
     int32_t inner_steps;    /* PIMI_PRECOND_SECTOR_INNER: inner GMRES steps per outer step */
+    /* PIMI_WEIGHT_NONE: iterate on the PI coefficients x, as the reference does.
+       PIMI_WEIGHT_SECTOR: iterate on u = S x, S = sqrt(f^nu) on sector nu (u_nu = f^nu times
+         the physical block), i.e. on S (L/s + v t^H) S^{-1}; the trace functional becomes 1 on
+         the block diagonals.  Residual norms then weigh each sector by its multiplicity, as
+         traces and observables do; with PIMI_WEIGHT_NONE an error that is small against |x|
+         but lies in a sector of large f^nu (up to 1e17 at N = 120) moves observables by up to
+         sqrt(f^nu) times as much.  No counterpart in the reference.  Needs the backward
+         criterion; the reported relative residual is still |L x| / (s |x|) on x. */
+    int32_t weighting;
+    int32_t reserved;
 } pimi_gmres_opts;
 
 enum { PIMI_CRITERION_REFERENCE = 0, PIMI_CRITERION_BACKWARD = 1 };
+enum { PIMI_WEIGHT_NONE = 0, PIMI_WEIGHT_SECTOR = 1 };
 
 typedef struct {
     int32_t krylovdim;      /* ExpvAlgorithm(krylovdim=30, ...) */
@@ -246,6 +257,54 @@ pimi_status pimi_precond_apply(pimi_workspace *ws, const double *x, double *y, d
 /* Time of the last apply kernel launch sequence, in milliseconds (HIP events
  * recorded on the workspace stream around pimi_apply). */
 pimi_status pimi_workspace_last_apply_ms(const pimi_workspace *ws, double *ms);
+
+/* ---------------------------------------------------------------------------
+ * Batch engine: many independent stationary solves advanced together on one
+ * device, one kernel launch per operation for all of them.  No counterpart in
+ * the reference, which solves one model at a time; each job runs the
+ * stationary solve of src/krylov.jl:797-891 (bordered system, right sector
+ * preconditioner, scale-invariant criterion) as GMRES cycles inside FP64
+ * iterative refinement.  Jobs wait in a FIFO queue and are admitted while
+ * the engine's device memory reservation holds their state (see
+ * pimi_batch_job_bytes); a finished job frees its memory at once.  A job's
+ * result does not depend on which other jobs share the device.
+ * ------------------------------------------------------------------------- */
+typedef struct pimi_batch pimi_batch;
+
+typedef struct {
+    int32_t precision;     /* PIMI_C64, or PIMI_C32: FP32 Arnoldi (basis, apply, preconditioner) in FP64 refinement */
+    int32_t max_jobs;      /* jobs resident at once (0: 1024) */
+    int64_t memory_bytes;  /* device memory reserved for job state (0: three quarters of the free memory) */
+} pimi_batch_opts;
+
+pimi_status pimi_batch_create(pimi_context *ctx, const pimi_batch_opts *opts, pimi_batch **out);
+void pimi_batch_destroy(pimi_batch *batch);
+/* device bytes one job of this plan holds while admitted */
+pimi_status pimi_batch_job_bytes(const pimi_plan *plan, int32_t precision, int32_t krylovdim, int64_t *bytes);
+/* Queue a stationary solve of `plan` (which must outlive the job).  opts:
+ * preconditioner PIMI_PRECOND_SECTOR, criterion PIMI_CRITERION_BACKWARD,
+ * krylovdim = Krylov dimension of one GMRES cycle, inner_rtol in (0, 1) =
+ * residual reduction per cycle, maxiter = Arnoldi steps in total,
+ * max_refinements = cycles, operator_scale 0 = estimate it. */
+pimi_status pimi_batch_submit(pimi_batch *batch, pimi_plan *plan, const pimi_gmres_opts *opts, int64_t *job_id);
+/* Advance up to max_ticks ticks (admitting queued jobs as memory frees); with
+ * stop_when_finished, return after the first tick in which a job finished.
+ * *finished = jobs that finished during the call. */
+pimi_status pimi_batch_run(pimi_batch *batch, int32_t max_ticks, int32_t stop_when_finished, int64_t *finished);
+/* counts[6]: queued, active, finished (not yet fetched), ticks so far,
+ * reserved bytes in use, reservation size */
+pimi_status pimi_batch_counts(pimi_batch *batch, int64_t *counts);
+/* next finished job id, or -1 */
+pimi_status pimi_batch_next_finished(pimi_batch *batch, int64_t *job_id);
+/* Copy a finished job's state (trace one, n complex doubles) and info, and
+ * forget the job; PIMI_ENOTCONVERGED (state and info filled) for a job that
+ * did not converge. */
+pimi_status pimi_batch_fetch(pimi_batch *batch, int64_t job_id, double *rho_out, pimi_solve_info *info);
+/* Throughput probe: one launch applying L to a synthetic vector of every plan
+ * (engine precision), `reps` times, event-timed; algorithmic bytes per launch
+ * = sum over plans of 2 n s + stencil tables.  Measurement only. */
+pimi_status pimi_batch_apply_bench(pimi_batch *batch, pimi_plan *const *plans, int32_t nplans, int32_t reps,
+                                   double *ms_avg, int64_t *algorithmic_bytes);
 
 #ifdef __cplusplus
 }

@@ -193,9 +193,27 @@ class OraclePlan:
             self._pc = (U, inv)
         return self._pc
 
+    def sector_sigma(self):
+        """Per-coordinate sqrt(f^nu): the trace weight of the coordinate's sector."""
+        b = self.basis
+        sig = np.empty(self.n)
+        k = 0
+        for s, m in enumerate(b.dims):
+            sig[b.offsets[s]:b.offsets[s + 1]] = self.t_val[k]
+            k += m
+        return sig
+
     def steady_state(self, krylovdim=30, maxiter=500, atol=0.0, rtol=1e-10, operator_scale=0.0, nthreads=1,
-                     preconditioner=None, criterion="reference"):
+                     preconditioner=None, criterion="reference", weighting=None):
+        """weighting="sector": iterate on u = S x with S = sqrt(f^nu) per sector."""
         x = np.zeros(self.n, dtype=np.complex128)
+        sig = None
+        if weighting == "sector":
+            if criterion != "backward":
+                raise ValueError("sector weighting needs the backward criterion")
+            sig = np.ascontiguousarray(self.sector_sigma())
+        elif weighting is not None:
+            raise ValueError("unknown weighting")
         info = _SteadyInfo()
         pu = pi = None
         if preconditioner == "sector":
@@ -206,7 +224,8 @@ class OraclePlan:
         lib().oracle_steady_state(C.byref(self._P), C.c_int(nthreads), _ptr(self.t_idx), _ptr(self.t_val),
                                   C.c_int64(len(self.t_idx)), C.c_int(krylovdim), C.c_int64(maxiter),
                                   C.c_double(atol), C.c_double(rtol), C.c_double(operator_scale), _ptr(x),
-                                  C.byref(info), pu, pi, C.c_int(1 if criterion == "backward" else 0))
+                                  C.byref(info), pu, pi, C.c_int(1 if criterion == "backward" else 0),
+                                  None if sig is None else _ptr(sig))
         return x, {k: getattr(info, k) for k, _ in _SteadyInfo._fields_}
 
     def expv(self, b, t, krylovdim=30, atol=1e-10, rtol=1e-8, safety=0.9, max_steps=10000, nthreads=1):

@@ -378,14 +378,27 @@ typedef struct {
     const cplx *vval;   /* normalized trace functional t/(t.t) */
     int64_t nt;
     int64_t applications;
+    const double *sigma; /* sector weighting: per-coordinate sqrt(f) (NULL: none) */
+    cplx *scratch;
 } steady_op;
 
+/* y = A z, A = L/s + v t^H, or with sector weighting S A S^{-1}: the input is
+ * unscaled, L applied, the output scaled back (t and v are passed already
+ * transformed: t/sigma and sigma v on the trace support). */
 static void steady_apply(steady_op *op, const cplx *z, cplx *y) {
-    do_apply(op->P, z, y, op->nthreads);
+    const cplx *in = z;
+    if (op->sigma) {
+        for (int64_t i = 0; i < op->P->n; ++i) op->scratch[i] = z[i] / op->sigma[i];
+        in = op->scratch;
+    }
+    do_apply(op->P, in, y, op->nthreads);
     op->applications++;
     cplx alpha = 0;
     for (int64_t k = 0; k < op->nt; ++k) alpha += conj(op->tval[k]) * z[op->tidx[k]];
-    for (int64_t i = 0; i < op->P->n; ++i) y[i] = op->invscale * y[i];
+    if (op->sigma)
+        for (int64_t i = 0; i < op->P->n; ++i) y[i] = op->invscale * (op->sigma[i] * y[i]);
+    else
+        for (int64_t i = 0; i < op->P->n; ++i) y[i] = op->invscale * y[i];
     for (int64_t k = 0; k < op->nt; ++k) y[op->tidx[k]] += alpha * op->vval[k];
 }
 
@@ -439,9 +452,9 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
 #define Hij(i, j) H[(i) + (j) * (m + 1)]
     double bnorm = dznrm2(b, n);
     double xnorm = dznrm2(x, n);
-    /* backward: t^H L = 0 and t.v = 1 give |L x|/s <= 2 |r|, so |r| <= rtol |x| / 2
-       guarantees the relative residual |L x| / (s |x|) <= rtol */
-    double tol = backward ? atol + 0.5 * rtol * xnorm : atol + rtol * bnorm;
+    /* backward: t^H L = 0 and t.v = 1 give |L x|/s <= 2 |r|, so |r| <= (atol + rtol |x|) / 2
+       guarantees |L x| / s <= atol + rtol |x| */
+    double tol = backward ? 0.5 * (atol + rtol * xnorm) : atol + rtol * bnorm;
     int64_t iterations = 0, restarts = 0;
     steady_apply(op, x, w);
     for (int64_t i = 0; i < n; ++i) r[i] = b[i] - w[i];
@@ -490,7 +503,7 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
             kdone = j + 1;
             if (backward && (kdone % 25 == 0 || cabs(g[j + 1]) <= tol)) {
                 xnorm = iterate_norm(op, x, V, H, g, m, kdone, pc, tmp, z);
-                tol = atol + 0.5 * rtol * xnorm;
+                tol = 0.5 * (atol + rtol * xnorm);
             }
             if (cabs(g[j + 1]) <= tol) break;
         }
@@ -513,7 +526,7 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
         steady_apply(op, x, w);
         for (int64_t i = 0; i < n; ++i) r[i] = b[i] - w[i];
         beta = dznrm2(r, n);
-        if (backward) { xnorm = dznrm2(x, n); tol = atol + 0.5 * rtol * xnorm; }
+        if (backward) { xnorm = dznrm2(x, n); tol = 0.5 * (atol + rtol * xnorm); }
         if (beta <= tol) { conv = 1; break; }
     }
 done:
@@ -527,20 +540,49 @@ done:
  * operator_scale <= 0 selects the deterministic probe estimate. */
 int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const double *tvals, int64_t nt,
                         int krylovdim, int64_t maxiter, double atol, double rtol, double operator_scale,
-                        cplx *x_out, steady_info *info, const cplx *pc_U, const cplx *pc_inv, int backward) {
+                        cplx *x_out, steady_info *info, const cplx *pc_U, const cplx *pc_inv, int backward,
+                        const double *sigma) {
     int64_t n = P->n;
     cplx *tv = malloc(sizeof(cplx) * nt), *vv = malloc(sizeof(cplx) * nt);
+    cplx *tw = malloc(sizeof(cplx) * nt), *vw = malloc(sizeof(cplx) * nt);
     cplx tt = 0;
     for (int64_t k = 0; k < nt; ++k) { tv[k] = tvals[k]; tt += conj(tv[k]) * tv[k]; }
     for (int64_t k = 0; k < nt; ++k) vv[k] = tv[k] / tt;
+    /* sector weighting: iterate on u = S x; t -> t / sigma, v -> sigma v */
+    for (int64_t k = 0; k < nt; ++k) {
+        tw[k] = sigma ? tv[k] / sigma[tidx[k]] : tv[k];
+        vw[k] = sigma ? vv[k] * sigma[tidx[k]] : vv[k];
+    }
     double scale = operator_scale > 0 ? operator_scale : oracle_operator_scale(P, nthreads);
-    steady_op op = {P, nthreads, 1.0 / scale, tidx, tv, vv, nt, 0};
+    cplx *scr = sigma ? malloc(sizeof(cplx) * n) : NULL;
+    steady_op op = {P, nthreads, 1.0 / scale, tidx, tw, vw, nt, 0, sigma, scr};
     cplx *b = calloc(n, sizeof(cplx));
-    for (int64_t k = 0; k < nt; ++k) b[tidx[k]] += vv[k];
+    for (int64_t k = 0; k < nt; ++k) b[tidx[k]] += vw[k];
     memcpy(x_out, b, sizeof(cplx) * n);
-    int64_t it, rs; double res;
+    int64_t it = 0, rs = 0; double res = 0;
     sector_pc pc = {pc_U, pc_inv, scale};
-    int conv = gmres(&op, x_out, b, krylovdim, maxiter, atol, rtol, &it, &rs, &res, pc_U ? &pc : NULL, backward);
+    /* with weighting the iteration controls the weighted residual; if
+       |L x| / s <= atol + rtol |x| does not follow on x = u / sigma, continue
+       with a tenfold tighter threshold (at most four times) */
+    int conv = 0;
+    double tight = 1.0;
+    cplx *lw = sigma ? malloc(sizeof(cplx) * n) : NULL;
+    for (int pass = 0;; ++pass) {
+        int64_t it1, rs1;
+        conv = gmres(&op, x_out, b, krylovdim, maxiter - it, atol * tight, rtol * tight, &it1, &rs1, &res,
+                     pc_U ? &pc : NULL, backward);
+        it += it1; rs += rs1;
+        if (!sigma || !conv || pass >= 4 || it >= maxiter) break;
+        for (int64_t i = 0; i < n; ++i) scr[i] = x_out[i] / sigma[i];
+        do_apply(P, scr, lw, nthreads);
+        op.applications++;
+        if (dznrm2(lw, n) / scale <= atol + rtol * dznrm2(scr, n)) break;
+        tight *= 0.1;
+    }
+    free(lw);
+    if (sigma)
+        for (int64_t i = 0; i < n; ++i) x_out[i] /= sigma[i];
+    free(scr); free(tw); free(vw);
     if (backward) {  /* restore the trace row, weighted by |v| in the residual */
         cplx tr0 = 0;
         for (int64_t k = 0; k < nt; ++k) tr0 += conj(tv[k]) * x_out[tidx[k]];

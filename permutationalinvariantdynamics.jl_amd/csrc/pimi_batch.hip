@@ -1,0 +1,1355 @@
+// Batched stationary solves: many independent PI models advanced together,
+// one kernel launch per operation for all of them.
+//
+// A batch engine owns one HIP stream and one device arena.  Jobs (a compiled
+// plan plus GMRES options) wait in a FIFO queue and are admitted while the
+// arena has room for their state (the HBM footprint of their Krylov basis and
+// work vectors) and a slot is free.  Admitted jobs advance in ticks: in one
+// tick every job takes one step of its own state machine, and each kind of
+// step is one launch over all jobs taking it (descriptor arrays give every
+// job its own vectors, tables and sizes).  Each job solves the bordered
+// stationary system (L/s + v t^H) x = v (src/krylov.jl:797-891) by
+// right-preconditioned GMRES cycles inside iterative refinement:
+//
+//   r = v - A x (FP64)  ->  GMRES on A d = r in the engine precision, sector
+//   preconditioner on the right, until |r_k| <= max(inner_rtol |r|,
+//   (atol + rtol |x|) / 2) or the Krylov dimension is used up  ->  x += d
+//
+// With PIMI_C32 the Arnoldi basis, apply and preconditioner are complex FP32
+// (dot products accumulate in FP64); residuals, the solution, Hessenberg
+// least squares and the trace normalization stay FP64.  The stopping rule is
+// the scale-invariant one (t^H L = 0 bounds |L x|/s by 2 |r|), so a finished
+// job has |L x| / (s |x|) <= rtol for atol = 0.
+//
+// Every job's arithmetic depends only on its own data: reductions run over a
+// per-job block count fixed by its dimension, so a job's result does not
+// depend on which other jobs share its launches.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstddef>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <map>
+#include <memory>
+
+#include "pimi_internal.hpp"
+
+namespace pimi {
+
+// ---------------------------------------------------------------------------
+// launch descriptors: one entry per job taking part in a launch, `first` =
+// the job's first block (tile, wave) in the launch, ascending
+// ---------------------------------------------------------------------------
+struct BApply {
+    int64_t first;
+    const void *x;
+    void *y;
+    const int64_t *offs;
+    const int32_t *dims;
+    const int32_t *tile_sector;
+    const int64_t *term_base;
+    const StencilTermDev *terms;
+    const int32_t *col;
+    const void *val;
+    int64_t n;
+    double out_scale;
+};
+
+struct BTrace {  // one block per job
+    const void *z;    // input
+    void *y;          // bordered: y += (t . z) v
+    cx<double> *dot;  // optional output: t . z
+    const int64_t *tidx;
+    const double *tval, *vval;
+    int64_t nt;
+    int32_t bordered, pad;
+};
+
+struct BGemm {
+    int64_t first;
+    const void *A, *B;
+    void *C;
+    const void *D;
+    const int64_t *offs;
+    const int32_t *dims;
+    const GemmTile *tiles;
+    double dscale;
+};
+
+struct BVec {
+    int64_t first;
+    const void *a, *b;
+    void *c, *d;
+    int64_t n;
+    int32_t nblk, ncol;
+    double s0;
+    uint64_t u0;
+};
+
+struct BScatter {  // one block per job
+    cx<double> *b, *x;
+    const int64_t *tidx;
+    const double *vval;
+    int64_t nt;
+};
+
+struct BCopy {  // one block per entry
+    const cx<double> *src;
+    cx<double> *dst;
+    int64_t count;
+};
+
+template <typename T>
+__device__ __forceinline__ int seg_of(const T *__restrict__ seg, int nseg, int64_t b) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (seg[mid].first <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// batched kernels
+// ---------------------------------------------------------------------------
+// y = out_scale * L x for every job; tiles of all jobs laid end to end, the
+// global tile space split into one contiguous slab per XCD.
+template <typename R>
+__global__ __launch_bounds__(kApplyBlock) void kb_apply(const BApply *__restrict__ J, int nj, int64_t total,
+                                                        int64_t per_xcd) {
+    const int64_t bid = blockIdx.x;
+    const int64_t g = (bid % kNumXcd) * per_xcd + bid / kNumXcd;
+    if (g >= total) return;
+    const BApply A = J[seg_of(J, nj, g)];
+    const int64_t lt = g - A.first;
+    const int64_t i = lt * kApplyBlock + threadIdx.x;
+    if (i >= A.n) return;
+    int s = A.tile_sector[lt];
+    while (i >= A.offs[s + 1]) ++s;
+    const int64_t m = A.dims[s], loc = i - A.offs[s];
+    const cx<R> acc = apply_coordinate<R>((const cx<R> *)A.x, s, loc % m, loc / m, m, A.offs, A.dims, A.term_base,
+                                          A.terms, A.col, (const cx<R> *)A.val);
+    const R sc = (R)A.out_scale;
+    ((cx<R> *)A.y)[i] = cx<R>{sc * acc.x, sc * acc.y};
+}
+
+// y = x * w[sector] (w = 1/sigma: back from the weighted coordinates), one
+// coordinate per lane over all jobs' tiles (FP64)
+__global__ __launch_bounds__(kApplyBlock) void kb_sector_scale(const BApply *__restrict__ J, int nj, int64_t total) {
+    const int64_t g = blockIdx.x;
+    if (g >= total) return;
+    const BApply A = J[seg_of(J, nj, g)];
+    const int64_t lt = g - A.first;
+    const int64_t i = lt * kApplyBlock + threadIdx.x;
+    if (i >= A.n) return;
+    int s = A.tile_sector[lt];
+    while (i >= A.offs[s + 1]) ++s;
+    const double f = ((const double *)A.val)[s];
+    const cx<double> v = ((const cx<double> *)A.x)[i];
+    ((cx<double> *)A.y)[i] = cx<double>{v.x * f, v.y * f};
+}
+
+// t . z per job (FP64 accumulation); bordered: y += (t . z) v on the trace support
+template <typename R>
+__global__ __launch_bounds__(256) void kb_trace(const BTrace *__restrict__ T) {
+    __shared__ double sh[256 / 64];
+    const BTrace J = T[blockIdx.x];
+    const cx<R> *z = (const cx<R> *)J.z;
+    double re = 0, im = 0;
+    for (int64_t k = threadIdx.x; k < J.nt; k += 256) {
+        const cx<R> zz = z[J.tidx[k]];
+        re += J.tval[k] * (double)zz.x;
+        im += J.tval[k] * (double)zz.y;
+    }
+    re = block_sum(re, sh);
+    __syncthreads();
+    im = block_sum(im, sh);
+    if (J.dot && threadIdx.x == 0) *J.dot = cx<double>{re, im};
+    if (J.bordered) {
+        cx<R> *y = (cx<R> *)J.y;
+        for (int64_t k = threadIdx.x; k < J.nt; k += 256) {
+            const cx<R> yy = y[J.tidx[k]];
+            y[J.tidx[k]] = cx<R>{(R)(yy.x + re * J.vval[k]), (R)(yy.y + im * J.vval[k])};
+        }
+    }
+}
+
+// grouped sector GEMM tiles of all jobs, one wave per 16x16 tile
+template <typename R>
+__global__ __launch_bounds__(256) void kb_gemm(const BGemm *__restrict__ G, int ng, int64_t total, int transA,
+                                               int transB) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= total) return;  // whole wave leaves together; no block barriers below
+    const BGemm J = G[seg_of(G, ng, w)];
+    const GemmTile t = J.tiles[w - J.first];
+    gemm_tile_mfma<R>((const cx<R> *)J.A, (const cx<R> *)J.B, (cx<R> *)J.C, J.dims[t.sector], J.offs[t.sector], t,
+                      transA, transB, (const cx<R> *)J.D, (R)J.dscale, lane);
+}
+
+// The BLAS-1 kernels below run over "virtual blocks": entry e of the
+// descriptor array owns virtual blocks [first, first + its count), and a
+// launch of at most kMaxGrid workgroups loops over all of them, so that the
+// grid stays within the device's work-item limit for any batch.
+constexpr int64_t kMaxGrid = 1 << 20;
+
+// c[lb] = sum over block lb's elements of |a|^2
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void kb_norm_partial(const BVec *__restrict__ S, int ns, int64_t total) {
+    __shared__ double sh[kRedBlock / 64];
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const BVec J = S[seg_of(S, ns, vb)];
+        const int64_t lb = vb - J.first;
+        const cx<R> *a = (const cx<R> *)J.a;
+        double s = 0;
+        for (int64_t k = lb * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock) {
+            const double x = a[k].x, y = a[k].y;
+            s += x * x + y * y;
+        }
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) ((double *)J.c)[lb] = s;
+        __syncthreads();
+    }
+}
+
+// *d = sqrt(sum of the nblk partials in c); one wave per entry
+__global__ __launch_bounds__(64) void kb_reduce_norm(const BVec *__restrict__ S) {
+    const BVec J = S[blockIdx.x];
+    double s = 0;
+    for (int k = threadIdx.x; k < J.nblk; k += 64) s += ((const double *)J.c)[k];
+    s = wave_sum(s);
+    if (threadIdx.x == 0) *(double *)J.d = sqrt(s);
+}
+
+// c[col * nblk + chunk] = chunk's part of conj(V[:, col]) . w, virtual blocks
+// ordered (chunk, col) with col fastest so that neighbouring blocks share w
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void kb_dots_partial(const BVec *__restrict__ S, int ns, int64_t total) {
+    __shared__ double sh[kRedBlock / 64];
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const BVec J = S[seg_of(S, ns, vb)];
+        const int64_t lb = vb - J.first;
+        const int64_t chunk = lb / J.ncol, c = lb % J.ncol;
+        const cx<R> *v = (const cx<R> *)J.a + c * J.n;
+        const cx<R> *w = (const cx<R> *)J.b;
+        double re = 0, im = 0;
+        for (int64_t k = chunk * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock) {
+            const cx<R> a = v[k], b = w[k];
+            re += (double)a.x * (double)b.x + (double)a.y * (double)b.y;
+            im += (double)a.x * (double)b.y - (double)a.y * (double)b.x;
+        }
+        re = block_sum(re, sh);
+        __syncthreads();
+        im = block_sum(im, sh);
+        if (threadIdx.x == 0) ((cx<double> *)J.c)[c * J.nblk + chunk] = cx<double>{re, im};
+        __syncthreads();
+    }
+}
+
+// h[col] = sum of the column's partials (a); hacc (d): s0 == 0 store, else add
+__global__ __launch_bounds__(64) void kb_reduce_cols(const BVec *__restrict__ S, int ns, int64_t total) {
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const BVec J = S[seg_of(S, ns, vb)];
+        const int64_t c = vb - J.first;
+        const cx<double> *p = (const cx<double> *)J.a + c * J.nblk;
+        double re = 0, im = 0;
+        for (int k = threadIdx.x; k < J.nblk; k += 64) {
+            re += p[k].x;
+            im += p[k].y;
+        }
+        re = wave_sum(re);
+        im = wave_sum(im);
+        if (threadIdx.x == 0) {
+            ((cx<double> *)J.c)[c] = cx<double>{re, im};
+            cx<double> *hacc = (cx<double> *)J.d;
+            if (J.s0 == 0.0) hacc[c] = cx<double>{re, im};
+            else hacc[c] = cx<double>{hacc[c].x + re, hacc[c].y + im};
+        }
+    }
+}
+
+// mode 0: c -= V[:, 0:ncol] b ; mode 1: c = V[:, 0:ncol] b   (b: FP64 coefficients)
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void kb_gemv(const BVec *__restrict__ S, int ns, int64_t total, int mode) {
+    __shared__ cx<double> hs[kColChunk];
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const BVec J = S[seg_of(S, ns, vb)];
+        const int64_t lb = vb - J.first;
+        const cx<R> *V = (const cx<R> *)J.a;
+        const cx<double> *h = (const cx<double> *)J.b;
+        cx<R> *w = (cx<R> *)J.c;
+        for (int64_t k0 = lb * kRedBlock; k0 < J.n; k0 += (int64_t)J.nblk * kRedBlock) {
+            const int64_t k = k0 + threadIdx.x;
+            double re = 0, im = 0;
+            for (int c0 = 0; c0 < J.ncol; c0 += kColChunk) {
+                const int cn = min(kColChunk, J.ncol - c0);
+                __syncthreads();
+                for (int c = threadIdx.x; c < cn; c += kRedBlock) hs[c] = h[c0 + c];
+                __syncthreads();
+                if (k < J.n)
+                    for (int c = 0; c < cn; ++c) {
+                        const cx<R> v = V[k + (int64_t)(c0 + c) * J.n];
+                        re += (double)v.x * hs[c].x - (double)v.y * hs[c].y;
+                        im += (double)v.x * hs[c].y + (double)v.y * hs[c].x;
+                    }
+            }
+            if (k < J.n) {
+                if (mode == 0) w[k] = cx<R>{(R)((double)w[k].x - re), (R)((double)w[k].y - im)};
+                else w[k] = cx<R>{(R)re, (R)im};
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// c = a / *d (a zero denominator, an exact breakdown, gives 0), converting precision
+template <typename Rd, typename Rs>
+__global__ __launch_bounds__(kRedBlock) void kb_scale_div(const BVec *__restrict__ S, int ns, int64_t total) {
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const BVec J = S[seg_of(S, ns, vb)];
+        const int64_t lb = vb - J.first;
+        const double den = *(const double *)J.d;
+        const cx<Rs> *a = (const cx<Rs> *)J.a;
+        cx<Rd> *c = (cx<Rd> *)J.c;
+        for (int64_t k = lb * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock)
+            c[k] = den == 0.0 ? cx<Rd>{0, 0} : cx<Rd>{(Rd)((double)a[k].x / den), (Rd)((double)a[k].y / den)};
+    }
+}
+
+// c = a - b (FP64)
+__global__ __launch_bounds__(kRedBlock) void kb_sub(const BVec *__restrict__ S, int ns, int64_t total) {
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const BVec J = S[seg_of(S, ns, vb)];
+        const int64_t lb = vb - J.first;
+        const cx<double> *a = (const cx<double> *)J.a, *b = (const cx<double> *)J.b;
+        cx<double> *c = (cx<double> *)J.c;
+        for (int64_t k = lb * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock)
+            c[k] = cx<double>{a[k].x - b[k].x, a[k].y - b[k].y};
+    }
+}
+
+// c += a, a in the engine precision, c FP64
+template <typename Rs>
+__global__ __launch_bounds__(kRedBlock) void kb_add(const BVec *__restrict__ S, int ns, int64_t total) {
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const BVec J = S[seg_of(S, ns, vb)];
+        const int64_t lb = vb - J.first;
+        const cx<Rs> *a = (const cx<Rs> *)J.a;
+        cx<double> *c = (cx<double> *)J.c;
+        for (int64_t k = lb * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock)
+            c[k] = cx<double>{c[k].x + (double)a[k].x, c[k].y + (double)a[k].y};
+    }
+}
+
+__global__ __launch_bounds__(kRedBlock) void kb_zero(const BVec *__restrict__ S, int ns, int64_t total) {
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const BVec J = S[seg_of(S, ns, vb)];
+        const int64_t lb = vb - J.first;
+        cx<double> *c = (cx<double> *)J.c;
+        for (int64_t k = lb * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock)
+            c[k] = cx<double>{0, 0};
+    }
+}
+
+// Probe vector for the operator-scale estimate: element e of the vector
+// starting at counter u0 takes outputs u0 + 2e + 1 and u0 + 2e + 2 of the
+// splitmix64 stream seeded 0x51ca1e, through Box-Muller; splitmix64's k-th
+// output depends only on k, so every element is formed independently.
+__device__ __forceinline__ double probe_uniform(uint64_t k) {
+    uint64_t z = 0x51ca1eULL + k * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+__global__ __launch_bounds__(kRedBlock) void kb_probe(const BVec *__restrict__ S, int ns, int64_t total) {
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const BVec J = S[seg_of(S, ns, vb)];
+        const int64_t lb = vb - J.first;
+        cx<double> *c = (cx<double> *)J.c;
+        for (int64_t k = lb * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock) {
+            const double u = probe_uniform(J.u0 + 2 * (uint64_t)k + 1),
+                         v = probe_uniform(J.u0 + 2 * (uint64_t)k + 2);
+            const double r = sqrt(-2.0 * log(u)), a = 6.283185307179586 * v;
+            c[k] = cx<double>{r * cos(a) * 0.7071067811865476, r * sin(a) * 0.7071067811865476};
+        }
+    }
+}
+
+// b = x = v on the trace support (b and x zeroed before)
+__global__ __launch_bounds__(256) void kb_scatter(const BScatter *__restrict__ S) {
+    const BScatter J = S[blockIdx.x];
+    for (int64_t k = threadIdx.x; k < J.nt; k += 256) {
+        J.b[J.tidx[k]] = cx<double>{J.vval[k], 0};
+        J.x[J.tidx[k]] = cx<double>{J.vval[k], 0};
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_gather(const BCopy *__restrict__ S) {
+    const BCopy J = S[blockIdx.x];
+    for (int64_t k = threadIdx.x; k < J.count; k += 256) J.dst[k] = J.src[k];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// First-fit sub-allocator over one device reservation (no hipMalloc/hipFree
+// per job: those synchronize the device).
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+    void init(size_t bytes) {
+        HIPCHK(hipMalloc((void **)&base, bytes));
+        size = bytes;
+        free_.clear();
+        free_[0] = bytes;
+    }
+    bool alloc(size_t bytes, size_t &off) {
+        bytes = align_up(bytes, 256);
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= bytes) {
+                off = it->first;
+                const size_t rest = it->second - bytes;
+                free_.erase(it);
+                if (rest) free_[off + bytes] = rest;
+                used += bytes;
+                return true;
+            }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = align_up(bytes, 256);
+        used -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    ~Arena() {
+        if (base) (void)hipFree(base);
+    }
+};
+
+enum Phase { PH_PROBE = 0, PH_INIT, PH_ARNOLDI, PH_END, PH_FINAL };
+enum JobState { JS_QUEUED = 0, JS_ACTIVE, JS_DONE, JS_FAILED };
+
+// mailbox slots (complex doubles) at the head of each job's mail region
+constexpr int kMailScalars = 8;   // 16 doubles: see the S_* indices
+constexpr int kMailTrace = 8;     // t . x
+constexpr int kMailHacc = 9;      // Hessenberg column h[0..j]
+enum {
+    S_WNORM = 0,  // |w| of the Arnoldi step
+    S_RNORM = 1,  // |r| (FP64 residual of the bordered system)
+    S_XNORM = 2,  // |x|
+    S_LXNORM = 3, // |L x|
+    S_PNORM = 4,  // |p_k|, k = 0..2 (probe vectors)
+    S_LPNORM = 7  // |L p_k|
+};
+
+struct BJob {
+    int64_t id = 0;
+    pimi_plan *plan = nullptr;
+    pimi_gmres_opts o{};
+    int state = JS_QUEUED, phase = PH_PROBE;
+    int kdim = 0, nblk = 1;
+    int64_t n = 0, nt = 0;
+    size_t off = 0, bytes = 0;
+    // device state (inside the arena)
+    cx<double> *x = nullptr, *b = nullptr, *r = nullptr, *w64 = nullptr, *xo = nullptr;
+    char *V = nullptr, *w = nullptr, *z = nullptr, *t1 = nullptr, *t2 = nullptr;  // engine precision
+    cx<double> *partial = nullptr, *h = nullptr, *mail = nullptr;
+    double *npart = nullptr, *vval = nullptr;
+    double *scal() const { return (double *)mail; }
+    // host Krylov state
+    std::vector<cd> H, g, sn, y;
+    std::vector<double> cs;
+    std::vector<double> vv;
+    int j = 0, kend = 0;
+    bool begin_cycle = false, candidate = false, weighted = false;
+    double tight = 1.0;  // threshold factor, lowered when the coefficient-space check fails
+    double scale = 0, beta = 0, beta0 = 0, xnorm = 0;
+    int64_t iterations = 0, cycles = 0, applications = 0;
+    std::chrono::steady_clock::time_point t_submit, t_admit;
+    std::vector<cx<double>> result;
+    pimi_solve_info info{};
+    std::string error;
+};
+
+static int blocks_for(int64_t n) {
+    return (int)std::min<int64_t>(std::max<int64_t>((n + 2047) / 2048, 1), 128);
+}
+
+// Device bytes of one job's state: FP64 x, b, r, w64, xo; the Krylov basis and four
+// work vectors in the engine precision; reduction partials, Hessenberg
+// column, mailbox, coefficients and the bordered-system weights.
+static size_t job_bytes(int64_t n, int64_t nt, int kdim, int prec) {
+    const size_t es = prec == PIMI_C32 ? sizeof(cx<float>) : sizeof(cx<double>);
+    const int nblk = blocks_for(n);
+    size_t b = 0;
+    b += 5 * align_up(sizeof(cx<double>) * n, 256);
+    b += align_up(es * n * (size_t)(kdim + 1), 256) + 4 * align_up(es * n, 256);
+    b += align_up(sizeof(cx<double>) * nblk * (size_t)(kdim + 1), 256);
+    b += 2 * align_up(sizeof(cx<double>) * (size_t)(kdim + 2 + kMailHacc), 256);
+    b += align_up(sizeof(double) * 4 * nblk, 256) + align_up(sizeof(double) * nt, 256);
+    return b;
+}
+
+}  // namespace pimi
+
+struct pimi_batch {
+    pimi_context *ctx = nullptr;
+    int prec = PIMI_C64;
+    int max_jobs = 1024;
+    hipStream_t stream = nullptr;
+    pimi::Arena arena;
+    int64_t next_id = 1;
+    std::deque<pimi::BJob *> queue;
+    std::vector<pimi::BJob *> active;
+    std::map<int64_t, std::unique_ptr<pimi::BJob>> jobs;
+    std::deque<int64_t> finished;
+    char *hstage = nullptr;
+    size_t hstage_cap = 0;
+    pimi::DevBuf<char> dstage;
+    pimi::cx<double> *hmail = nullptr;
+    size_t hmail_cap = 0;
+    pimi::DevBuf<pimi::cx<double>> dmail;
+    std::atomic<int> busy{0};
+    int64_t ticks = 0;
+    bool trace = false;
+    ~pimi_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (hstage) (void)hipHostFree(hstage);
+        if (hmail) (void)hipHostFree(hmail);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace pimi {
+
+// ---------------------------------------------------------------------------
+// one tick: descriptors for every launch are staged on the host, uploaded in
+// one copy, then the launches run; one gather + one copy bring back every
+// job's scalars and Hessenberg column; one synchronization per tick.
+// ---------------------------------------------------------------------------
+struct Tick {
+    pimi_batch *E;
+    std::vector<char> buf;
+    std::vector<std::function<void(const char *)>> ops;
+    // pointer fields (offset in buf) that must hold the device address of
+    // stage data (offset in buf): patched once the device stage is placed
+    std::vector<std::pair<size_t, size_t>> relocs;
+
+    template <typename T>
+    size_t put(const std::vector<T> &v) {
+        const size_t off = align_up(buf.size(), 256);
+        buf.resize(off + sizeof(T) * v.size());
+        if (!v.empty()) std::memcpy(buf.data() + off, v.data(), sizeof(T) * v.size());
+        return off;
+    }
+
+    // BLAS-1 launch over per-job entries; blocks(e) = blocks of entry e
+    template <typename K>
+    void vec(std::vector<BVec> v, K kernel, std::function<int64_t(const BVec &)> blocks, unsigned bdim = kRedBlock) {
+        if (v.empty()) return;
+        int64_t tot = 0;
+        for (auto &e : v) {
+            e.first = tot;
+            tot += blocks(e);
+        }
+        const size_t off = put(v);
+        const int ns = (int)v.size();
+        const unsigned grid = (unsigned)std::min<int64_t>(tot, kMaxGrid);
+        hipStream_t st = E->stream;
+        ops.push_back([=](const char *d) {
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(bdim), 0, st, (const BVec *)(d + off), ns, tot);
+        });
+    }
+    static int64_t nblk_of(const BVec &e) { return e.nblk; }
+
+    // unit: y = L x (the coefficient-space operator); otherwise the job's
+    // solve operator y = L x / s, in weighted coordinates for a weighted job
+    template <typename R>
+    void apply(const std::vector<BJob *> &js, std::function<const void *(BJob *)> in, std::function<void *(BJob *)> out,
+               bool unit) {
+        if (js.empty()) return;
+        std::vector<BApply> v;
+        int64_t tot = 0;
+        for (BJob *J : js) {
+            pimi_plan *P = J->plan;
+            const void *val = sizeof(R) == 8 ? (const void *)P->val64.p : (const void *)P->val32.p;
+            const StencilTermDev *terms = (!unit && J->weighted) ? P->terms_w.p : P->terms.p;
+            v.push_back(BApply{tot, in(J), out(J), P->offs.p, P->dims.p, P->tile_sector.p, P->term_base.p, terms,
+                               P->col.p, val, J->n, unit ? 1.0 : 1.0 / J->scale});
+            tot += P->ntiles;
+        }
+        const size_t off = put(v);
+        const int nj = (int)v.size();
+        const int64_t per_xcd = (tot + kNumXcd - 1) / kNumXcd;
+        hipStream_t st = E->stream;
+        ops.push_back([=](const char *d) {
+            hipLaunchKernelGGL(kb_apply<R>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyBlock), 0, st,
+                               (const BApply *)(d + off), nj, tot, per_xcd);
+        });
+    }
+
+    template <typename R>
+    void trace(const std::vector<BJob *> &js, std::function<const void *(BJob *)> z, std::function<void *(BJob *)> y,
+               bool bordered, bool dot) {
+        if (js.empty()) return;
+        std::vector<BTrace> v;
+        for (BJob *J : js)  // the solve operator of a weighted job has t -> ones (v already S v)
+            v.push_back(BTrace{z(J), y ? y(J) : nullptr, dot ? J->mail + kMailTrace : nullptr, J->plan->d_tidx.p,
+                               (bordered && J->weighted) ? J->plan->d_ones.p : J->plan->d_tval.p, J->vval, J->nt,
+                               bordered ? 1 : 0, 0});
+        const size_t off = put(v);
+        const unsigned nj = (unsigned)v.size();
+        hipStream_t st = E->stream;
+        ops.push_back([=](const char *d) {
+            hipLaunchKernelGGL(kb_trace<R>, dim3(nj), dim3(256), 0, st, (const BTrace *)(d + off));
+        });
+    }
+
+    // out = scale * U ((U^H in U) .* inv_den) U^H per sector, t1/t2 as temporaries
+    template <typename R>
+    void precond(const std::vector<BJob *> &js, std::function<const void *(BJob *)> in,
+                 std::function<void *(BJob *)> out) {
+        if (js.empty()) return;
+        struct G {
+            int ta, tb;
+        };
+        const G shapes[4] = {{1, 0}, {0, 0}, {0, 0}, {0, 1}};
+        for (int stage = 0; stage < 4; ++stage) {
+            std::vector<BGemm> v;
+            int64_t tot = 0;
+            for (BJob *J : js) {
+                pimi_plan *P = J->plan;
+                const void *U = sizeof(R) == 8 ? (const void *)P->pU64.p : (const void *)P->pU32.p;
+                const void *Inv = sizeof(R) == 8 ? (const void *)P->pInv64.p : (const void *)P->pInv32.p;
+                BGemm e{tot, nullptr, nullptr, nullptr, nullptr, P->offs.p, P->dims.p, P->mtiles.p, 1.0};
+                switch (stage) {
+                    case 0: e.A = U; e.B = in(J); e.C = J->t1; break;                            // U^H in
+                    case 1: e.A = J->t1; e.B = U; e.C = J->t2; e.D = Inv; e.dscale = J->scale; break;  // (. U) .* inv
+                    case 2: e.A = U; e.B = J->t2; e.C = J->t1; break;                           // U .
+                    default: e.A = J->t1; e.B = U; e.C = out(J); break;                         // . U^H
+                }
+                v.push_back(e);
+                tot += P->nmtiles;
+            }
+            const size_t off = put(v);
+            const int ng = (int)v.size();
+            const int ta = shapes[stage].ta, tb = shapes[stage].tb;
+            hipStream_t st = E->stream;
+            ops.push_back([=](const char *d) {
+                hipLaunchKernelGGL(kb_gemm<R>, dim3((unsigned)((tot + 3) / 4)), dim3(256), 0, st,
+                                   (const BGemm *)(d + off), ng, tot, ta, tb);
+            });
+        }
+    }
+
+    template <typename R>
+    void norm(const std::vector<BJob *> &js, std::function<const void *(BJob *)> a, int slot, int pslot = 0) {
+        if (js.empty()) return;
+        std::vector<BVec> v;
+        for (BJob *J : js)
+            v.push_back(BVec{0, a(J), nullptr, J->npart + (size_t)pslot * J->nblk, J->scal() + slot, J->n, J->nblk, 1,
+                             0.0, 0});
+        vec(v, kb_norm_partial<R>, nblk_of);
+        const size_t off = put(v);
+        const unsigned nj = (unsigned)v.size();
+        hipStream_t st = E->stream;
+        ops.push_back([=](const char *d) {
+            hipLaunchKernelGGL(kb_reduce_norm, dim3(nj), dim3(64), 0, st, (const BVec *)(d + off));
+        });
+    }
+
+    // CGS2 of w against V[:, 0:j+1]; hacc = accumulated coefficients (mailbox)
+    template <typename R>
+    void cgs2(const std::vector<BJob *> &js) {
+        if (js.empty()) return;
+        for (int pass = 0; pass < 2; ++pass) {
+            std::vector<BVec> d, r, u;
+            for (BJob *J : js) {
+                const int nc = J->j + 1;
+                d.push_back(BVec{0, J->V, J->w, J->partial, nullptr, J->n, J->nblk, nc, 0.0, 0});
+                r.push_back(BVec{0, J->partial, nullptr, J->h, J->mail + kMailHacc, J->n, J->nblk, nc,
+                                 pass == 0 ? 0.0 : 1.0, 0});
+                u.push_back(BVec{0, J->V, J->h, J->w, nullptr, J->n, J->nblk, nc, 0.0, 0});
+            }
+            vec(d, kb_dots_partial<R>, [](const BVec &e) { return (int64_t)e.nblk * e.ncol; });
+            vec(r, kb_reduce_cols, [](const BVec &e) { return (int64_t)e.ncol; }, 64);
+            gemv<R>(u, 0);
+        }
+    }
+
+    template <typename R>
+    size_t gemv(std::vector<BVec> u, int mode) {
+        if (u.empty()) return 0;
+        int64_t tot = 0;
+        for (auto &e : u) {
+            e.first = tot;
+            tot += e.nblk;
+        }
+        const size_t off = put(u);
+        const int ns = (int)u.size();
+        const unsigned grid = (unsigned)std::min<int64_t>(tot, kMaxGrid);
+        hipStream_t st = E->stream;
+        ops.push_back([=](const char *d) {
+            hipLaunchKernelGGL(kb_gemv<R>, dim3(grid), dim3(kRedBlock), 0, st, (const BVec *)(d + off), ns, tot,
+                               mode);
+        });
+        return off;
+    }
+};
+
+template <typename R>
+static char *vcol(BJob *J, int c) {
+    return J->V + sizeof(cx<R>) * (size_t)J->n * (size_t)c;
+}
+
+static void carve(pimi_batch *E, BJob *J) {
+    const size_t es = E->prec == PIMI_C32 ? sizeof(cx<float>) : sizeof(cx<double>);
+    char *p = E->arena.base + J->off;
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += align_up(bytes, 256);
+        return q;
+    };
+    const int64_t n = J->n;
+    J->x = (cx<double> *)take(sizeof(cx<double>) * n);
+    J->b = (cx<double> *)take(sizeof(cx<double>) * n);
+    J->r = (cx<double> *)take(sizeof(cx<double>) * n);
+    J->w64 = (cx<double> *)take(sizeof(cx<double>) * n);
+    J->xo = (cx<double> *)take(sizeof(cx<double>) * n);
+    J->V = take(es * n * (size_t)(J->kdim + 1));
+    J->w = take(es * n);
+    J->z = take(es * n);
+    J->t1 = take(es * n);
+    J->t2 = take(es * n);
+    J->partial = (cx<double> *)take(sizeof(cx<double>) * J->nblk * (size_t)(J->kdim + 1));
+    J->h = (cx<double> *)take(sizeof(cx<double>) * (size_t)(J->kdim + 2 + kMailHacc));
+    J->mail = (cx<double> *)take(sizeof(cx<double>) * (size_t)(J->kdim + 2 + kMailHacc));
+    J->npart = (double *)take(sizeof(double) * 4 * J->nblk);
+    J->vval = (double *)take(sizeof(double) * J->nt);
+    if ((size_t)(p - (E->arena.base + J->off)) > J->bytes) throw Error(PIMI_EINVAL, "job layout exceeds its footprint");
+}
+
+static void admit(pimi_batch *E) {
+    while (!E->queue.empty() && (int)E->active.size() < E->max_jobs) {
+        BJob *J = E->queue.front();
+        size_t off;
+        if (!E->arena.alloc(J->bytes, off)) break;  // FIFO: wait for memory to free up
+        E->queue.pop_front();
+        J->off = off;
+        carve(E, J);
+        HIPCHK(hipMemcpyAsync(J->vval, J->vv.data(), sizeof(double) * J->nt, hipMemcpyHostToDevice, E->stream));
+        J->state = JS_ACTIVE;
+        J->phase = J->o.operator_scale > 0 ? PH_INIT : PH_PROBE;
+        J->scale = J->o.operator_scale;
+        J->t_admit = std::chrono::steady_clock::now();
+        E->active.push_back(J);
+    }
+}
+
+static void finish(pimi_batch *E, BJob *J, bool ok) {
+    J->state = ok ? JS_DONE : JS_FAILED;
+    if ((int64_t)J->result.size() != J->n) {  // failed before its final phase
+        J->result.assign((size_t)J->n, cx<double>{0, 0});
+        std::memset(&J->info, 0, sizeof J->info);
+        J->info.iterations = J->iterations;
+        J->info.applications = J->applications;
+        J->info.operator_scale = J->scale;
+    }
+    E->arena.release(J->off, J->bytes);
+    J->x = J->b = J->r = J->w64 = J->xo = nullptr;
+    J->V = J->w = J->z = J->t1 = J->t2 = nullptr;
+    E->finished.push_back(J->id);
+}
+
+template <typename R>
+static void run_tick(pimi_batch *E) {
+    admit(E);
+    if (E->active.empty()) return;
+    std::vector<BJob *> probe, init, arn, begin, end, fin;
+    for (BJob *J : E->active) switch (J->phase) {
+            case PH_PROBE: probe.push_back(J); break;
+            case PH_INIT: init.push_back(J); break;
+            case PH_ARNOLDI:
+                arn.push_back(J);
+                if (J->begin_cycle) begin.push_back(J);
+                break;
+            case PH_END: end.push_back(J); break;
+            default: fin.push_back(J); break;
+        }
+    Tick T{E, {}, {}};
+    auto W64 = [](BJob *J) -> void * { return J->w64; };
+    auto cX = [](BJob *J) -> const void * { return J->x; };
+    auto cW64 = [](BJob *J) -> const void * { return J->w64; };
+    auto cR = [](BJob *J) -> const void * { return J->r; };
+    auto vecs = [](const std::vector<BJob *> &js, std::function<BVec(BJob *)> f) {
+        std::vector<BVec> v;
+        for (BJob *J : js) v.push_back(f(J));
+        return v;
+    };
+    // -- operator scale: max over three probe vectors of |L p| / |p| (src/krylov.jl:156-169)
+    for (int k = 0; k < 3 && !probe.empty(); ++k) {
+        T.vec(vecs(probe, [&](BJob *J) {
+                  return BVec{0, nullptr, nullptr, J->r, nullptr, J->n, J->nblk, 1, 0.0, (uint64_t)(2 * J->n * k)};
+              }),
+              kb_probe, Tick::nblk_of);
+        T.norm<double>(probe, cR, S_PNORM + k);
+        T.apply<double>(probe, cR, W64, true);
+        T.norm<double>(probe, cW64, S_LPNORM + k, 1);
+    }
+    // -- initial residual of x0 = v
+    if (!init.empty()) {
+        std::vector<BVec> z;
+        for (BJob *J : init) {
+            z.push_back(BVec{0, nullptr, nullptr, J->b, nullptr, J->n, J->nblk, 1, 0.0, 0});
+            z.push_back(BVec{0, nullptr, nullptr, J->x, nullptr, J->n, J->nblk, 1, 0.0, 0});
+        }
+        T.vec(z, kb_zero, Tick::nblk_of);
+        std::vector<BScatter> sc;
+        for (BJob *J : init) sc.push_back(BScatter{J->b, J->x, J->plan->d_tidx.p, J->vval, J->nt});
+        const size_t off = T.put(sc);
+        const unsigned nj = (unsigned)sc.size();
+        hipStream_t st = E->stream;
+        T.ops.push_back([=](const char *d) {
+            hipLaunchKernelGGL(kb_scatter, dim3(nj), dim3(256), 0, st, (const BScatter *)(d + off));
+        });
+    }
+    // -- Arnoldi step (first the new cycle's start vector V0 = r / |r|)
+    T.vec(vecs(begin, [](BJob *J) {
+              return BVec{0, J->r, nullptr, J->V, J->scal() + S_RNORM, J->n, J->nblk, 1, 0.0, 0};
+          }),
+          kb_scale_div<R, double>, Tick::nblk_of);
+    if (!arn.empty()) {
+        T.precond<R>(arn, [](BJob *J) -> const void * { return vcol<R>(J, J->j); },
+                     [](BJob *J) -> void * { return J->z; });
+        T.apply<R>(arn, [](BJob *J) -> const void * { return J->z; }, [](BJob *J) -> void * { return J->w; }, false);
+        T.trace<R>(arn, [](BJob *J) -> const void * { return J->z; }, [](BJob *J) -> void * { return J->w; }, true,
+                   false);
+        T.cgs2<R>(arn);
+        T.norm<R>(arn, [](BJob *J) -> const void * { return J->w; }, S_WNORM);
+        T.vec(vecs(arn, [](BJob *J) {
+                  return BVec{0, J->w, nullptr, vcol<R>(J, J->j + 1), J->scal() + S_WNORM, J->n, J->nblk, 1, 0.0, 0};
+              }),
+              kb_scale_div<R, R>, Tick::nblk_of);
+    }
+    // -- end of a cycle: x += M^{-1} V y, then the FP64 residual
+    if (!end.empty()) {
+        std::vector<BVec> u;
+        std::vector<size_t> yoffs;
+        for (BJob *J : end) {
+            std::vector<cx<double>> yc(J->kend);
+            for (int i = 0; i < J->kend; ++i) yc[i] = cx<double>{J->y[i].real(), J->y[i].imag()};
+            yoffs.push_back(T.put(yc));  // coefficients travel with the descriptors
+            u.push_back(BVec{0, J->V, nullptr, J->w, nullptr, J->n, J->nblk, J->kend, 0.0, 0});
+        }
+        const size_t uoff = T.gemv<R>(u, 1);
+        for (size_t k = 0; k < yoffs.size(); ++k)
+            T.relocs.push_back({uoff + k * sizeof(BVec) + offsetof(BVec, b), yoffs[k]});
+        T.precond<R>(end, [](BJob *J) -> const void * { return J->w; }, [](BJob *J) -> void * { return J->z; });
+        T.vec(vecs(end, [](BJob *J) { return BVec{0, J->z, nullptr, J->x, nullptr, J->n, J->nblk, 1, 0.0, 0}; }),
+              kb_add<R>, Tick::nblk_of);
+    }
+    // -- FP64 bordered residual r = b - (L x / s + (t . x) v): new jobs and cycle ends
+    std::vector<BJob *> resid(init);
+    resid.insert(resid.end(), end.begin(), end.end());
+    if (!resid.empty()) {
+        T.apply<double>(resid, cX, W64, false);
+        T.trace<double>(resid, cX, W64, true, false);
+        T.vec(vecs(resid, [](BJob *J) { return BVec{0, J->b, J->w64, J->r, nullptr, J->n, J->nblk, 1, 0.0, 0}; }),
+              kb_sub, Tick::nblk_of);
+        T.norm<double>(resid, cR, S_RNORM);
+        T.norm<double>(resid, cX, S_XNORM, 1);
+    }
+    // -- final: back to coefficients (x = S^{-1} u for weighted jobs), |L x|, |x| and t . x
+    auto cXo = [](BJob *J) -> const void * { return J->weighted ? (const void *)J->xo : (const void *)J->x; };
+    if (!fin.empty()) {
+        std::vector<BApply> v;
+        int64_t tot = 0;
+        for (BJob *J : fin)
+            if (J->weighted) {
+                pimi_plan *P = J->plan;
+                v.push_back(BApply{tot, J->x, J->xo, P->offs.p, P->dims.p, P->tile_sector.p, nullptr, nullptr,
+                                   nullptr, P->d_inv_sigma.p, J->n, 1.0});
+                tot += P->ntiles;
+            }
+        if (!v.empty()) {
+            const size_t off = T.put(v);
+            const int nj = (int)v.size();
+            hipStream_t st = E->stream;
+            T.ops.push_back([=](const char *d) {
+                hipLaunchKernelGGL(kb_sector_scale, dim3((unsigned)tot), dim3(kApplyBlock), 0, st,
+                                   (const BApply *)(d + off), nj, tot);
+            });
+        }
+        T.apply<double>(fin, cXo, W64, true);
+        T.norm<double>(fin, cW64, S_LXNORM);
+        T.norm<double>(fin, cXo, S_XNORM, 1);
+        T.trace<double>(fin, cXo, nullptr, false, true);
+    }
+    // -- mailbox gather
+    std::vector<BCopy> g;
+    size_t mtot = 0;
+    std::vector<size_t> moff(E->active.size());
+    for (size_t k = 0; k < E->active.size(); ++k) {
+        BJob *J = E->active[k];
+        const int64_t cnt = kMailHacc + (J->phase == PH_ARNOLDI ? J->j + 1 : 0);
+        moff[k] = mtot;
+        g.push_back(BCopy{J->mail, nullptr, cnt});
+        mtot += (size_t)cnt;
+    }
+    if (E->dmail.count < mtot) E->dmail.alloc(std::max<size_t>(mtot, 2 * E->dmail.count));
+    for (size_t k = 0; k < g.size(); ++k) g[k].dst = E->dmail.p + moff[k];
+    {
+        const size_t off = T.put(g);
+        const unsigned ng = (unsigned)g.size();
+        hipStream_t st = E->stream;
+        T.ops.push_back([=](const char *d) {
+            hipLaunchKernelGGL(kb_gather, dim3(ng), dim3(256), 0, st, (const BCopy *)(d + off));
+        });
+    }
+    // -- upload, launch, download
+    if (E->hstage_cap < T.buf.size()) {
+        if (E->hstage) HIPCHK(hipHostFree(E->hstage));
+        E->hstage_cap = std::max<size_t>(T.buf.size(), 2 * E->hstage_cap);
+        HIPCHK(hipHostMalloc((void **)&E->hstage, E->hstage_cap, hipHostMallocDefault));
+    }
+    if (E->dstage.count < T.buf.size()) E->dstage.alloc(std::max<size_t>(T.buf.size(), 2 * E->dstage.count));
+    const char *dbase = E->dstage.p;
+    for (auto &rl : T.relocs) {
+        const void *addr = dbase + rl.second;
+        std::memcpy(T.buf.data() + rl.first, &addr, sizeof addr);
+    }
+    std::memcpy(E->hstage, T.buf.data(), T.buf.size());
+    HIPCHK(hipMemcpyAsync(E->dstage.p, E->hstage, T.buf.size(), hipMemcpyHostToDevice, E->stream));
+    for (auto &op : T.ops) op(dbase);
+    HIPCHK(hipGetLastError());
+    if (E->hmail_cap < mtot) {
+        if (E->hmail) HIPCHK(hipHostFree(E->hmail));
+        E->hmail_cap = std::max<size_t>(mtot, 2 * E->hmail_cap);
+        HIPCHK(hipHostMalloc((void **)&E->hmail, sizeof(cx<double>) * E->hmail_cap, hipHostMallocDefault));
+    }
+    HIPCHK(hipMemcpyAsync(E->hmail, E->dmail.p, sizeof(cx<double>) * mtot, hipMemcpyDeviceToHost, E->stream));
+    for (BJob *J : fin) {
+        J->result.resize((size_t)J->n);
+        HIPCHK(hipMemcpyAsync(J->result.data(), J->weighted ? J->xo : J->x, sizeof(cx<double>) * J->n,
+                              hipMemcpyDeviceToHost, E->stream));
+    }
+    HIPCHK(hipStreamSynchronize(E->stream));
+    E->ticks++;
+    // -- host updates
+    std::vector<BJob *> still;
+    for (size_t k = 0; k < E->active.size(); ++k) {
+        BJob *J = E->active[k];
+        const cx<double> *mb = E->hmail + moff[k];
+        const double *sc = (const double *)mb;
+        auto thr = [&]() { return 0.5 * J->tight * (J->o.atol + J->o.rtol * J->xnorm); };
+        bool done = false, ok = false;
+        switch (J->phase) {
+            case PH_PROBE: {
+                double s = 0;
+                for (int q = 0; q < 3; ++q) s = std::max(s, sc[S_LPNORM + q] / sc[S_PNORM + q]);
+                J->applications += 3;
+                if (!(s > 0) || !std::isfinite(s)) {
+                    J->error = "operator scale estimate is not finite and positive";
+                    done = true;
+                    break;
+                }
+                J->scale = s;
+                J->phase = PH_INIT;
+                break;
+            }
+            case PH_INIT:
+            case PH_END: {
+                if (J->phase == PH_END) J->cycles++;
+                J->applications++;
+                J->beta = sc[S_RNORM];
+                J->xnorm = sc[S_XNORM];
+                if (!std::isfinite(J->beta) || !std::isfinite(J->xnorm)) {
+                    J->error = "non-finite residual";
+                    J->phase = PH_FINAL;
+                    J->candidate = false;
+                } else if (J->beta <= thr()) {
+                    J->phase = PH_FINAL;
+                    J->candidate = true;
+                } else if (J->iterations >= J->o.maxiter || J->cycles >= J->o.max_refinements) {
+                    J->phase = PH_FINAL;
+                    J->candidate = false;
+                } else {
+                    J->phase = PH_ARNOLDI;
+                    J->begin_cycle = true;
+                    J->j = 0;
+                    J->beta0 = J->beta;
+                    std::fill(J->H.begin(), J->H.end(), cd(0, 0));
+                    std::fill(J->g.begin(), J->g.end(), cd(0, 0));
+                    J->g[0] = J->beta;
+                }
+                break;
+            }
+            case PH_ARNOLDI: {
+                J->begin_cycle = false;
+                const int m = J->kdim, jj = J->j;
+                auto Hij = [&](int i, int c) -> cd & { return J->H[(size_t)i + (size_t)c * (m + 1)]; };
+                for (int i = 0; i <= jj; ++i) Hij(i, jj) = cd(mb[kMailHacc + i].x, mb[kMailHacc + i].y);
+                const double wn = sc[S_WNORM];
+                Hij(jj + 1, jj) = wn;
+                for (int i = 0; i < jj; ++i) {
+                    const cd h1 = Hij(i, jj), h2 = Hij(i + 1, jj);
+                    Hij(i, jj) = J->cs[i] * h1 + J->sn[i] * h2;
+                    Hij(i + 1, jj) = -std::conj(J->sn[i]) * h1 + J->cs[i] * h2;
+                }
+                double c;
+                cd s;
+                complex_givens(Hij(jj, jj), Hij(jj + 1, jj), c, s);
+                J->cs[jj] = c;
+                J->sn[jj] = s;
+                const cd h1 = Hij(jj, jj), h2 = Hij(jj + 1, jj);
+                Hij(jj, jj) = c * h1 + s * h2;
+                Hij(jj + 1, jj) = 0;
+                const cd g1 = J->g[jj], g2 = J->g[jj + 1];
+                J->g[jj] = c * g1 + s * g2;
+                J->g[jj + 1] = -std::conj(s) * g1 + c * g2;
+                J->iterations++;
+                J->applications++;
+                const double est = std::abs(J->g[jj + 1]);
+                const double cyc = std::max(thr(), J->o.inner_rtol * J->beta0);
+                if (est <= cyc || jj + 1 >= m || J->iterations >= J->o.maxiter || wn == 0.0 || !std::isfinite(est)) {
+                    int kk = jj + 1;
+                    J->y.assign((size_t)kk, cd(0, 0));
+                    bool singular = false;
+                    for (int i = kk - 1; i >= 0; --i) {
+                        cd acc = J->g[i];
+                        for (int q = i + 1; q < kk; ++q) acc -= Hij(i, q) * J->y[q];
+                        if (Hij(i, i) == cd(0, 0)) {
+                            singular = true;
+                            break;
+                        }
+                        J->y[i] = acc / Hij(i, i);
+                    }
+                    if (singular) {
+                        J->error = "singular GMRES Hessenberg";
+                        J->phase = PH_FINAL;
+                        J->candidate = false;
+                    } else {
+                        J->kend = kk;
+                        J->phase = PH_END;
+                    }
+                } else {
+                    J->j = jj + 1;
+                }
+                if (E->trace && J->iterations % 50 == 0)
+                    std::fprintf(stderr, "[pimi batch] job %lld iteration %lld estimate %.3e |x| %.3e\n",
+                                 (long long)J->id, (long long)J->iterations, est, J->xnorm);
+                break;
+            }
+            default: {  // PH_FINAL
+                J->applications++;
+                const double lx = sc[S_LXNORM], xn = sc[S_XNORM];
+                const cd tr(mb[kMailTrace].x, mb[kMailTrace].y);
+                const double nres = lx / J->scale;
+                ok = J->candidate && J->error.empty() && nres <= J->o.atol + J->o.rtol * xn && tr != cd(0, 0);
+                if (!ok && J->candidate && J->error.empty() && J->weighted && J->tight > 1e-4 &&
+                    J->iterations < J->o.maxiter && J->cycles < J->o.max_refinements) {
+                    // the weighted residual met its threshold but |L x| / (s |x|) did not
+                    // follow: continue the refinement with a tighter threshold
+                    J->tight *= 0.1;
+                    J->phase = PH_ARNOLDI;
+                    J->begin_cycle = true;
+                    J->j = 0;
+                    J->beta0 = J->beta;
+                    std::fill(J->H.begin(), J->H.end(), cd(0, 0));
+                    std::fill(J->g.begin(), J->g.end(), cd(0, 0));
+                    J->g[0] = J->beta;
+                    J->result.clear();
+                    break;
+                }
+                if (tr != cd(0, 0))
+                    for (auto &zz : J->result) {
+                        const cd v = cd(zz.x, zz.y) / tr;
+                        zz = cx<double>{v.real(), v.imag()};
+                    }
+                cd tr2(0, 0);
+                const pimi_plan *P = J->plan;
+                for (size_t q = 0; q < P->tidx.size(); ++q)
+                    tr2 += P->tval[q] * cd(J->result[P->tidx[q]].x, J->result[P->tidx[q]].y);
+                pimi_solve_info &I = J->info;
+                std::memset(&I, 0, sizeof I);
+                I.converged = ok ? 1 : 0;
+                I.iterations = J->iterations;
+                I.restarts = J->cycles;
+                I.refinements = J->cycles;
+                I.applications = J->applications;
+                I.residual = std::abs(tr) > 0 ? lx / std::abs(tr) : lx;
+                I.normalized_residual = std::abs(tr) > 0 ? nres / std::abs(tr) : nres;
+                I.trace_error = std::abs(tr2 - 1.0);
+                I.operator_scale = J->scale;
+                I.linear_residual = J->beta;
+                I.relative_residual = xn > 0 ? nres / xn : INFINITY;
+                I.seconds =
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - J->t_admit).count();
+                if (!ok && J->error.empty()) {
+                    char buf[256];
+                    std::snprintf(buf, sizeof buf,
+                                  "batched stationary solve did not converge in %lld iterations (%lld cycles); "
+                                  "relative residual %.3e",
+                                  (long long)J->iterations, (long long)J->cycles, I.relative_residual);
+                    J->error = buf;
+                }
+                done = true;
+                break;
+            }
+        }
+        if (done) finish(E, J, ok);
+        else still.push_back(J);
+    }
+    E->active.swap(still);
+}
+
+}  // namespace pimi
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using namespace pimi;
+
+extern "C" {
+
+pimi_status pimi_batch_create(pimi_context *ctx, const pimi_batch_opts *opts, pimi_batch **out) {
+    return guarded([&] {
+        if (!ctx || !opts || !out) throw Error(PIMI_EINVAL, "null argument");
+        if (opts->precision != PIMI_C64 && opts->precision != PIMI_C32) throw Error(PIMI_EINVAL, "unknown precision");
+        if (opts->max_jobs < 0 || opts->memory_bytes < 0) throw Error(PIMI_EINVAL, "negative engine limits");
+        HIPCHK(hipSetDevice(ctx->device));
+        std::unique_ptr<pimi_batch> E(new pimi_batch());
+        E->ctx = ctx;
+        E->prec = opts->precision;
+        E->max_jobs = opts->max_jobs > 0 ? opts->max_jobs : 1024;
+        size_t bytes = (size_t)opts->memory_bytes;
+        if (bytes == 0) {
+            size_t fr = 0, tot = 0;
+            HIPCHK(hipMemGetInfo(&fr, &tot));
+            bytes = fr / 4 * 3;
+        }
+        const char *tr = std::getenv("PIMI_TRACE");
+        E->trace = tr && tr[0] == '1';
+        HIPCHK(hipStreamCreate(&E->stream));
+        E->arena.init(align_up(bytes, 256));
+        *out = E.release();
+    });
+}
+
+void pimi_batch_destroy(pimi_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    delete b;
+}
+
+pimi_status pimi_batch_job_bytes(const pimi_plan *plan, int32_t precision, int32_t krylovdim, int64_t *bytes) {
+    return guarded([&] {
+        if (!plan || !bytes) throw Error(PIMI_EINVAL, "null argument");
+        if (precision != PIMI_C64 && precision != PIMI_C32) throw Error(PIMI_EINVAL, "unknown precision");
+        if (krylovdim < 1 || krylovdim > kMaxKrylov) throw Error(PIMI_EINVAL, "krylovdim must lie in 1..4000");
+        *bytes = (int64_t)job_bytes(plan->n, (int64_t)plan->tidx.size(), krylovdim, precision);
+    });
+}
+
+pimi_status pimi_batch_submit(pimi_batch *E, pimi_plan *plan, const pimi_gmres_opts *o, int64_t *job_id) {
+    return guarded([&] {
+        if (!E || !plan || !o || !job_id) throw Error(PIMI_EINVAL, "null argument");
+        BusyGuard guard(E->busy);
+        if (!plan->ctx || plan->ctx->device != E->ctx->device)
+            throw Error(PIMI_EINVAL, "the plan must live on the engine's device");
+        if (o->krylovdim < 1 || o->krylovdim > kMaxKrylov) throw Error(PIMI_EINVAL, "krylovdim must lie in 1..4000");
+        if (o->preconditioner != PIMI_PRECOND_SECTOR)
+            throw Error(PIMI_EINVAL, "batched solves use the sector preconditioner");
+        if (o->criterion != PIMI_CRITERION_BACKWARD)
+            throw Error(PIMI_EINVAL, "batched solves use the scale-invariant (backward) criterion");
+        if (!(o->atol >= 0) || !(o->rtol >= 0)) throw Error(PIMI_EINVAL, "atol and rtol must be nonnegative");
+        if (!(o->inner_rtol > 0) || !(o->inner_rtol < 1)) throw Error(PIMI_EINVAL, "inner_rtol must lie in (0, 1)");
+        if (o->maxiter < 1 || o->max_refinements < 1) throw Error(PIMI_EINVAL, "maxiter and max_refinements must be >= 1");
+        if (!(o->operator_scale >= 0) || !std::isfinite(o->operator_scale))
+            throw Error(PIMI_EINVAL, "operator_scale must be finite and nonnegative");
+        if (o->weighting != PIMI_WEIGHT_NONE && o->weighting != PIMI_WEIGHT_SECTOR)
+            throw Error(PIMI_EINVAL, "unknown weighting");
+        HIPCHK(hipSetDevice(E->ctx->device));
+        std::unique_ptr<BJob> J(new BJob());
+        J->plan = plan;
+        J->o = *o;
+        J->n = plan->n;
+        J->nt = (int64_t)plan->tidx.size();
+        J->kdim = o->krylovdim;
+        J->nblk = blocks_for(J->n);
+        J->bytes = job_bytes(J->n, J->nt, J->kdim, E->prec);
+        if (J->bytes > E->arena.size)
+            throw Error(PIMI_ENOMEM, "the job's device state exceeds the engine's memory reservation");
+        ensure_plan_preconditioner(plan);
+        J->weighted = o->weighting == PIMI_WEIGHT_SECTOR;
+        if (J->weighted) ensure_plan_weighting(plan);
+        double tt = 0;
+        for (double t : plan->tval) tt += t * t;
+        J->vv.resize(plan->tval.size());
+        for (size_t k = 0; k < J->vv.size(); ++k) J->vv[k] = plan->tval[k] / tt;
+        if (J->weighted) {  // v -> S v (trace support ordered by sector, m entries each)
+            size_t k = 0;
+            for (int q = 0; q < plan->ns; ++q)
+                for (int64_t a = 0; a < plan->host.dims[(size_t)q]; ++a, ++k) J->vv[k] *= plan->sigma[(size_t)q];
+        }
+        const int m = J->kdim;
+        J->H.assign((size_t)(m + 1) * m, cd(0, 0));
+        J->g.assign((size_t)m + 1, cd(0, 0));
+        J->sn.assign((size_t)m, cd(0, 0));
+        J->cs.assign((size_t)m, 0.0);
+        J->t_submit = std::chrono::steady_clock::now();
+        J->id = E->next_id++;
+        *job_id = J->id;
+        E->queue.push_back(J.get());
+        E->jobs[J->id] = std::move(J);
+    });
+}
+
+pimi_status pimi_batch_run(pimi_batch *E, int32_t max_ticks, int32_t stop_when_finished, int64_t *finished) {
+    return guarded([&] {
+        if (!E) throw Error(PIMI_EINVAL, "null argument");
+        BusyGuard guard(E->busy);
+        HIPCHK(hipSetDevice(E->ctx->device));
+        const size_t before = E->finished.size();
+        for (int32_t t = 0; t < max_ticks; ++t) {
+            if (E->active.empty() && E->queue.empty()) break;
+            const size_t f0 = E->finished.size();
+            if (E->prec == PIMI_C32) run_tick<float>(E);
+            else run_tick<double>(E);
+            if (stop_when_finished && E->finished.size() > f0) break;
+        }
+        if (finished) *finished = (int64_t)(E->finished.size() - before);
+    });
+}
+
+pimi_status pimi_batch_counts(pimi_batch *E, int64_t *counts) {
+    return guarded([&] {
+        if (!E || !counts) throw Error(PIMI_EINVAL, "null argument");
+        counts[0] = (int64_t)E->queue.size();
+        counts[1] = (int64_t)E->active.size();
+        counts[2] = (int64_t)E->finished.size();
+        counts[3] = E->ticks;
+        counts[4] = (int64_t)E->arena.used;
+        counts[5] = (int64_t)E->arena.size;
+    });
+}
+
+pimi_status pimi_batch_next_finished(pimi_batch *E, int64_t *job_id) {
+    return guarded([&] {
+        if (!E || !job_id) throw Error(PIMI_EINVAL, "null argument");
+        BusyGuard guard(E->busy);
+        if (E->finished.empty()) {
+            *job_id = -1;
+            return;
+        }
+        *job_id = E->finished.front();
+        E->finished.pop_front();
+    });
+}
+
+pimi_status pimi_batch_fetch(pimi_batch *E, int64_t job_id, double *rho_out, pimi_solve_info *info) {
+    pimi_status st = PIMI_OK;
+    std::string msg;
+    pimi_status g = guarded([&] {
+        if (!E || !rho_out || !info) throw Error(PIMI_EINVAL, "null argument");
+        BusyGuard guard(E->busy);
+        auto it = E->jobs.find(job_id);
+        if (it == E->jobs.end()) throw Error(PIMI_EINVAL, "unknown batch job");
+        BJob *J = it->second.get();
+        if (J->state != JS_DONE && J->state != JS_FAILED) throw Error(PIMI_EBUSY, "the batch job has not finished");
+        std::memcpy(rho_out, J->result.data(), sizeof(cx<double>) * (size_t)J->n);
+        *info = J->info;
+        if (J->state == JS_FAILED) {
+            st = PIMI_ENOTCONVERGED;
+            msg = J->error.empty() ? "batched stationary solve did not converge" : J->error;
+        }
+        for (auto q = E->finished.begin(); q != E->finished.end(); ++q)
+            if (*q == job_id) {
+                E->finished.erase(q);
+                break;
+            }
+        E->jobs.erase(it);
+    });
+    if (g != PIMI_OK) return g;
+    if (st != PIMI_OK) set_error(msg);
+    return st;
+}
+
+pimi_status pimi_batch_apply_bench(pimi_batch *E, pimi_plan *const *plans, int32_t nplans, int32_t reps,
+                                   double *ms_avg, int64_t *algorithmic_bytes) {
+    return guarded([&] {
+        if (!E || !plans || nplans < 1 || reps < 1 || !ms_avg || !algorithmic_bytes)
+            throw Error(PIMI_EINVAL, "invalid argument");
+        BusyGuard guard(E->busy);
+        HIPCHK(hipSetDevice(E->ctx->device));
+        const size_t es = E->prec == PIMI_C32 ? sizeof(cx<float>) : sizeof(cx<double>);
+        std::vector<DevBuf<char>> xs((size_t)nplans), ys((size_t)nplans);
+        std::vector<BApply> v;
+        int64_t tot = 0, bytes = 0;
+        for (int32_t k = 0; k < nplans; ++k) {
+            pimi_plan *P = plans[k];
+            if (!P || !P->ctx || P->ctx->device != E->ctx->device)
+                throw Error(PIMI_EINVAL, "every plan must live on the engine's device");
+            xs[k].alloc(es * (size_t)P->n);
+            ys[k].alloc(es * (size_t)P->n);
+            HIPCHK(hipMemsetAsync(xs[k].p, 0x3f, es * (size_t)P->n, E->stream));
+            const void *val = E->prec == PIMI_C32 ? (const void *)P->val32.p : (const void *)P->val64.p;
+            v.push_back(BApply{tot, xs[k].p, ys[k].p, P->offs.p, P->dims.p, P->tile_sector.p, P->term_base.p,
+                               P->terms.p, P->col.p, val, P->n, 1.0});
+            tot += P->ntiles;
+            const int64_t base = (int64_t)(P->st.col.size() * 4 + P->st.terms.size() * sizeof(StencilTermDev) +
+                                           P->st.term_base.size() * 8 + (P->host.dims.size() + 1) * 12);
+            bytes += 2 * P->n * (int64_t)es + base + (int64_t)P->st.val.size() * (int64_t)es;
+        }
+        DevBuf<BApply> dv;
+        dv.alloc(v.size());
+        HIPCHK(hipMemcpyAsync(dv.p, v.data(), sizeof(BApply) * v.size(), hipMemcpyHostToDevice, E->stream));
+        const int64_t per_xcd = (tot + kNumXcd - 1) / kNumXcd;
+        auto launch = [&] {
+            if (E->prec == PIMI_C32)
+                hipLaunchKernelGGL(kb_apply<float>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyBlock), 0,
+                                   E->stream, dv.p, nplans, tot, per_xcd);
+            else
+                hipLaunchKernelGGL(kb_apply<double>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyBlock), 0,
+                                   E->stream, dv.p, nplans, tot, per_xcd);
+        };
+        launch();  // warm-up
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, E->stream));
+        for (int32_t r = 0; r < reps; ++r) launch();
+        HIPCHK(hipEventRecord(e1, E->stream));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms_avg = ms / reps;
+        *algorithmic_bytes = bytes;
+    });
+}
+
+}  // extern "C"

@@ -37,6 +37,49 @@ constexpr int kApplyBlock = 256;
 constexpr int kRedBlock = 256;
 constexpr int kNumXcd = 8;
 
+// (L x)[i] for output coordinate i, sector s, position (a, b) in the m x m
+// block: the sum over the sector's stencil terms of s_t * (A X_in B^H)[a, b].
+template <typename R>
+__device__ __forceinline__ cx<R> apply_coordinate(const cx<R> *__restrict__ xc, int s, int64_t a, int64_t b,
+                                                  int64_t m, const int64_t *__restrict__ offs,
+                                                  const int32_t *__restrict__ dims,
+                                                  const int64_t *__restrict__ term_base,
+                                                  const StencilTermDev *__restrict__ terms,
+                                                  const int32_t *__restrict__ col, const cx<R> *__restrict__ val) {
+    const int64_t t0 = term_base[s], t1 = term_base[s + 1];
+    cx<R> acc = {0, 0};
+    for (int64_t t = t0; t < t1; ++t) {
+        const StencilTermDev T = terms[t];
+        const cx<R> *Xn = xc + offs[T.in_sector];
+        const int64_t mn = dims[T.in_sector];
+        cx<R> tacc = {0, 0};
+        if (T.flags & 1) {
+            for (int kb = 0; kb < T.wb; ++kb) {
+                const int64_t q = T.b_base + kb * m + b;
+                tacc = cadd(tacc, cjmul(val[q], Xn[a + (int64_t)col[q] * mn]));
+            }
+        } else if (T.flags & 2) {
+            for (int ka = 0; ka < T.wa; ++ka) {
+                const int64_t q = T.a_base + ka * m + a;
+                tacc = cadd(tacc, cmul(val[q], Xn[col[q] + b * mn]));
+            }
+        } else {
+            for (int kb = 0; kb < T.wb; ++kb) {
+                const int64_t qb = T.b_base + kb * m + b;
+                const int64_t dcol = (int64_t)col[qb] * mn;
+                cx<R> inner = {0, 0};
+                for (int ka = 0; ka < T.wa; ++ka) {
+                    const int64_t qa = T.a_base + ka * m + a;
+                    inner = cadd(inner, cmul(val[qa], Xn[col[qa] + dcol]));
+                }
+                tacc = cadd(tacc, cjmul(val[qb], inner));
+            }
+        }
+        acc = cadd(acc, cmul(cx<R>{(R)T.s_re, (R)T.s_im}, tacc));
+    }
+    return acc;
+}
+
 // y[i] = out_scale * (L x)[i] for one PI vector; one output coordinate per lane.
 // Tiles of kApplyBlock consecutive coordinates are spread so that XCD k
 // (workgroup id mod 8) handles one contiguous slab of tiles.
@@ -56,41 +99,24 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply(
     const int64_t m = dims[s];
     const int64_t loc = i - offs[s];
     const int64_t a = loc % m, b = loc / m;
-    const int64_t t0 = term_base[s], t1 = term_base[s + 1];
     for (int64_t cidx = blockIdx.y; cidx < ncols; cidx += gridDim.y) {
-        const cx<R> *xc = x + cidx * n;
-        cx<R> acc = {0, 0};
-        for (int64_t t = t0; t < t1; ++t) {
-            const StencilTermDev T = terms[t];
-            const cx<R> *Xn = xc + offs[T.in_sector];
-            const int64_t mn = dims[T.in_sector];
-            cx<R> tacc = {0, 0};
-            if (T.flags & 1) {
-                for (int kb = 0; kb < T.wb; ++kb) {
-                    const int64_t q = T.b_base + kb * m + b;
-                    tacc = cadd(tacc, cjmul(val[q], Xn[a + (int64_t)col[q] * mn]));
-                }
-            } else if (T.flags & 2) {
-                for (int ka = 0; ka < T.wa; ++ka) {
-                    const int64_t q = T.a_base + ka * m + a;
-                    tacc = cadd(tacc, cmul(val[q], Xn[col[q] + b * mn]));
-                }
-            } else {
-                for (int kb = 0; kb < T.wb; ++kb) {
-                    const int64_t qb = T.b_base + kb * m + b;
-                    const int64_t dcol = (int64_t)col[qb] * mn;
-                    cx<R> inner = {0, 0};
-                    for (int ka = 0; ka < T.wa; ++ka) {
-                        const int64_t qa = T.a_base + ka * m + a;
-                        inner = cadd(inner, cmul(val[qa], Xn[col[qa] + dcol]));
-                    }
-                    tacc = cadd(tacc, cjmul(val[qb], inner));
-                }
-            }
-            acc = cadd(acc, cmul(cx<R>{(R)T.s_re, (R)T.s_im}, tacc));
-        }
+        const cx<R> acc = apply_coordinate<R>(x + cidx * n, s, a, b, m, offs, dims, term_base, terms, col, val);
         y[cidx * n + i] = cx<R>{out_scale * acc.x, out_scale * acc.y};
     }
+}
+
+// y[i] = x[i] * w[sector of i], one coordinate per lane over the plan's tiles
+template <typename R>
+__global__ __launch_bounds__(kApplyBlock) void k_sector_scale(const cx<R> *__restrict__ x, cx<R> *__restrict__ y,
+                                                              int64_t n, const int64_t *__restrict__ offs,
+                                                              const int32_t *__restrict__ tile_sector,
+                                                              const double *__restrict__ w) {
+    const int64_t i = (int64_t)blockIdx.x * kApplyBlock + threadIdx.x;
+    if (i >= n) return;
+    int s = tile_sector[blockIdx.x];
+    while (i >= offs[s + 1]) ++s;
+    const R f = (R)w[s];
+    y[i] = cx<R>{x[i].x * f, x[i].y * f};
 }
 
 template <typename T>
@@ -223,20 +249,13 @@ struct MfmaOps<double> {
     static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
+// One 16x16 output tile of a sector GEMM, computed by one wave.
 template <typename R>
-__global__ __launch_bounds__(256) void k_sector_gemm_mfma(const cx<R> *__restrict__ A, const cx<R> *__restrict__ B,
-                                                          cx<R> *__restrict__ Cm, const int64_t *__restrict__ offs,
-                                                          const int32_t *__restrict__ dims,
-                                                          const GemmTile *__restrict__ tiles, int64_t ntiles,
-                                                          int transA, int transB, const cx<R> *__restrict__ D,
-                                                          R dscale) {
+__device__ __forceinline__ void gemm_tile_mfma(const cx<R> *__restrict__ A, const cx<R> *__restrict__ B,
+                                               cx<R> *__restrict__ Cm, int64_t m, int64_t base, GemmTile t,
+                                               int transA, int transB, const cx<R> *__restrict__ D, R dscale,
+                                               int lane) {
     typedef MfmaOps<R> Ops;
-    const int lane = threadIdx.x & 63;
-    const int64_t tid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (tid >= ntiles) return;  // whole wave leaves together; no block barriers below
-    const GemmTile t = tiles[tid];
-    const int64_t m = dims[t.sector];
-    const int64_t base = offs[t.sector];
     const int64_t row0 = (int64_t)t.ti * kMfmaTile, col0 = (int64_t)t.tj * kMfmaTile;
     const int li = lane & 15, lk = lane >> 4;
     const int64_t gi = row0 + li, gj = col0 + li;
@@ -269,6 +288,20 @@ __global__ __launch_bounds__(256) void k_sector_gemm_mfma(const cx<R> *__restric
         }
         Cm[o] = v;
     }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_sector_gemm_mfma(const cx<R> *__restrict__ A, const cx<R> *__restrict__ B,
+                                                          cx<R> *__restrict__ Cm, const int64_t *__restrict__ offs,
+                                                          const int32_t *__restrict__ dims,
+                                                          const GemmTile *__restrict__ tiles, int64_t ntiles,
+                                                          int transA, int transB, const cx<R> *__restrict__ D,
+                                                          R dscale) {
+    const int lane = threadIdx.x & 63;
+    const int64_t tid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tid >= ntiles) return;  // whole wave leaves together; no block barriers below
+    const GemmTile t = tiles[tid];
+    gemm_tile_mfma<R>(A, B, Cm, dims[t.sector], offs[t.sector], t, transA, transB, D, dscale, lane);
 }
 
 }  // namespace pimi

@@ -29,24 +29,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/pimi.h"
-#include "pimi_kernels.hpp"
-#include "pimi_lower.hpp"
+#include "pimi_internal.hpp"
 
 namespace pimi {
 
-static thread_local std::string g_last_error;
-
-#define HIPCHK(expr)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            throw Error(PIMI_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));          \
-    } while (0)
-
 constexpr int kDotBlocks = 240;  // partial-sum blocks per column (multiple of 8 XCDs)
 constexpr int kMaxObservables = 100;
-constexpr int kMaxKrylov = 4000;
 
 // ---------------------------------------------------------------------------
 // Krylov kernels
@@ -90,7 +78,6 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_cols(const cx<double> *__r
 }
 
 // w -= V[:, 0:ncol] h
-constexpr int kColChunk = 256;  // basis columns staged through LDS per pass
 
 template <typename R>
 __global__ __launch_bounds__(kRedBlock) void k_gs_update(cx<R> *__restrict__ w, const cx<R> *__restrict__ V,
@@ -312,67 +299,7 @@ __global__ void k_convert(cx<Rd> *__restrict__ dst, const cx<Rs> *__restrict__ s
 // ---------------------------------------------------------------------------
 // host objects
 // ---------------------------------------------------------------------------
-template <typename T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t count = 0;
-    void alloc(size_t n) {
-        release();
-        if (n == 0) n = 1;
-        HIPCHK(hipMalloc((void **)&p, n * sizeof(T)));
-        count = n;
-    }
-    void upload(const T *h, size_t n, hipStream_t s = 0) {
-        alloc(n);
-        HIPCHK(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        count = 0;
-    }
-    ~DevBuf() { release(); }
-};
-
 }  // namespace pimi
-
-struct pimi_context {
-    int device = 0;
-};
-
-struct pimi_plan {
-    pimi_context *ctx = nullptr;
-    pimi::FusedPlan host;
-    pimi::Stencil st;
-    bool compiled = false;  // true: geometry available (collective operators)
-    int64_t n = 0;
-    int ns = 0;
-    int64_t ntiles = 0, tiles_per_xcd = 0;
-    std::vector<int64_t> tidx;
-    std::vector<double> tval;
-    pimi::DevBuf<int64_t> offs, term_base, d_tidx;
-    pimi::DevBuf<int32_t> dims, tile_sector, col;
-    pimi::DevBuf<pimi::StencilTermDev> terms;
-    pimi::DevBuf<pimi::cx<double>> val64;
-    pimi::DevBuf<pimi::cx<float>> val32;
-    pimi::DevBuf<double> d_tval;
-    // stationary-solve preconditioner factors (built on first use)
-    bool prec_ready = false;
-    pimi::DevBuf<pimi::cx<double>> pU64, pInv64;
-    pimi::DevBuf<pimi::cx<float>> pU32, pInv32;
-    pimi::DevBuf<pimi::GemmTile> gtiles;
-    int64_t ngtiles = 0;
-    pimi::DevBuf<pimi::GemmTile> mtiles;  // 16x16 tiles for the matrix-core GEMM
-    int64_t nmtiles = 0;
-    // sector-block-diagonal part of L (inner solver), built on first use
-    bool diag_ready = false;
-    pimi::DevBuf<int64_t> dg_term_base;
-    pimi::DevBuf<pimi::StencilTermDev> dg_terms;
-    pimi::DevBuf<int32_t> dg_col;
-    pimi::DevBuf<pimi::cx<double>> dg_val64;
-    pimi::DevBuf<pimi::cx<float>> dg_val32;
-};
 
 struct pimi_workspace {
     pimi_plan *plan = nullptr;
@@ -400,43 +327,10 @@ struct pimi_workspace {
     bool use_graphs = true;  // PIMI_NO_GRAPH=1 runs the same kernels eagerly
     double *hpin = nullptr;  // pinned host staging (small results)
     bool trace_progress = false;  // PIMI_TRACE=1: solver progress lines on stderr
+    bool weighted = false;        // current stationary solve iterates on u = S x
 };
 
 namespace pimi {
-
-static void set_error(const std::string &m) { g_last_error = m; }
-
-template <typename F>
-static pimi_status guarded(F &&f) {
-    try {
-        f();
-        return PIMI_OK;
-    } catch (const Error &e) {
-        set_error(e.what());
-        return e.status;
-    } catch (const std::bad_alloc &) {
-        set_error("host allocation failed");
-        return PIMI_ENOMEM;
-    } catch (const std::exception &e) {
-        set_error(e.what());
-        return PIMI_EINVAL;
-    }
-}
-
-struct BusyGuard {
-    std::atomic<int> &f;
-    explicit BusyGuard(std::atomic<int> &flag) : f(flag) {
-        int expected = 0;
-        if (!f.compare_exchange_strong(expected, 1))
-            throw Error(PIMI_EBUSY, "a pimi workspace cannot be used concurrently");
-    }
-    ~BusyGuard() { f.store(0); }
-};
-
-static int grid_for(int64_t n) {
-    int64_t g = (n + kRedBlock - 1) / kRedBlock;
-    return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 2048);
-}
 
 static void finalize_plan(pimi_plan *P) {
     const FusedPlan &F = P->host;
@@ -519,17 +413,20 @@ static void ensure_diag_stencil(pimi_plan *P) {
 }
 
 // y = out_scale * L x (diag = false) or out_scale * D x, D the sector-block-diagonal part of L
+// `weighted`: the sector-weighted operator S L S^{-1} (full stencil only; the
+// sector-diagonal part D is unchanged by a sector-constant S)
 template <typename R>
 static void launch_apply(pimi_workspace *W, const cx<R> *x, cx<R> *y, double out_scale, int64_t ncols = 1,
-                         bool diag = false) {
+                         bool diag = false, bool weighted = false) {
     pimi_plan *P = W->plan;
     const cx<R> *val;
     if constexpr (sizeof(R) == 8) val = diag ? P->dg_val64.p : P->val64.p;
     else val = diag ? P->dg_val32.p : P->val32.p;
+    const StencilTermDev *terms = diag ? P->dg_terms.p : (weighted ? P->terms_w.p : P->terms.p);
     dim3 grid((unsigned)(P->tiles_per_xcd * kNumXcd), (unsigned)std::min<int64_t>(ncols, 65535));
     hipLaunchKernelGGL(k_apply<R>, grid, dim3(kApplyBlock), 0, W->stream, x, y, P->n, P->offs.p, P->dims.p,
                        P->tile_sector.p, P->ntiles, P->tiles_per_xcd, diag ? P->dg_term_base.p : P->term_base.p,
-                       diag ? P->dg_terms.p : P->terms.p, diag ? P->dg_col.p : P->col.p, val, (R)out_scale, ncols);
+                       terms, diag ? P->dg_col.p : P->col.p, val, (R)out_scale, ncols);
     HIPCHK(hipGetLastError());
 }
 
@@ -720,12 +617,14 @@ static void ensure_workspace(pimi_workspace *W) {
     }
 }
 
-// y = A z for the bordered stationary operator: L z / s + (t . z) v
+// y = A z for the bordered stationary operator: L z / s + (t . z) v; with
+// sector weighting (W->weighted) S A S^{-1}: t -> ones on the trace support,
+// v -> sigma v (held in d_vval)
 static void steady_apply(pimi_workspace *W, const cx<double> *z, cx<double> *y, double invscale) {
     pimi_plan *P = W->plan;
-    launch_apply<double>(W, z, y, invscale);
-    hipLaunchKernelGGL(k_trace_axpy<double>, dim3(1), dim3(1024), 0, W->stream, y, z, P->d_tidx.p, P->d_tval.p,
-                       W->d_vval.p, (int64_t)P->tidx.size());
+    launch_apply<double>(W, z, y, invscale, 1, false, W->weighted);
+    hipLaunchKernelGGL(k_trace_axpy<double>, dim3(1), dim3(1024), 0, W->stream, y, z, P->d_tidx.p,
+                       W->weighted ? P->d_ones.p : P->d_tval.p, W->d_vval.p, (int64_t)P->tidx.size());
     HIPCHK(hipGetLastError());
 }
 
@@ -734,19 +633,8 @@ static double read_scalar(pimi_workspace *W, const double *dptr) {
     return W->hpin[0];
 }
 
-static void complex_givens(cd a, cd b, double &c, cd &s) {
-    if (b == cd(0, 0)) { c = 1.0; s = 0; return; }
-    if (a == cd(0, 0)) { c = 0.0; s = 1.0; return; }
-    double scale = std::abs(a) + std::abs(b);
-    cd as = a / scale, bs = b / scale;
-    double normab = scale * std::sqrt(std::norm(as) + std::norm(bs));
-    cd alpha = a / std::abs(a);
-    c = std::abs(a) / normab;
-    s = alpha * std::conj(b) / normab;
-}
 
-static void ensure_preconditioner(pimi_workspace *W) {
-    pimi_plan *P = W->plan;
+void ensure_plan_preconditioner(pimi_plan *P) {
     if (!P->prec_ready) {
         std::vector<cd> U, inv;
         build_sector_preconditioner(P->host, U, inv);
@@ -780,6 +668,32 @@ static void ensure_preconditioner(pimi_workspace *W) {
         P->nmtiles = (int64_t)mt.size();
         P->prec_ready = true;
     }
+}
+
+void ensure_plan_weighting(pimi_plan *P) {
+    if (P->weight_ready) return;
+    const FusedPlan &F = P->host;
+    P->sigma.assign((size_t)F.ns, 1.0);
+    if (!F.geom.sqrt_f.empty())
+        for (int q = 0; q < F.ns; ++q) P->sigma[(size_t)q] = F.geom.sqrt_f[(size_t)q];
+    std::vector<StencilTermDev> tw(P->st.terms.size());
+    for (int q = 0; q < F.ns; ++q)
+        for (int64_t k = P->st.term_base[(size_t)q]; k < P->st.term_base[(size_t)q + 1]; ++k) {
+            const StencilTerm &a = P->st.terms[(size_t)k];
+            const double r = P->sigma[(size_t)q] / P->sigma[(size_t)a.in_sector];
+            tw[(size_t)k] = StencilTermDev{a.in_sector, a.flags, a.wa, a.wb, a.a_base, a.b_base, a.s_re * r, a.s_im * r};
+        }
+    std::vector<double> ones(P->tidx.size(), 1.0), inv((size_t)F.ns);
+    for (int q = 0; q < F.ns; ++q) inv[(size_t)q] = 1.0 / P->sigma[(size_t)q];
+    P->terms_w.upload(tw.data(), tw.size());
+    P->d_ones.upload(ones.data(), ones.size());
+    P->d_inv_sigma.upload(inv.data(), inv.size());
+    P->weight_ready = true;
+}
+
+static void ensure_preconditioner(pimi_workspace *W) {
+    pimi_plan *P = W->plan;
+    ensure_plan_preconditioner(P);
     if (!W->zbuf.p) {
         W->zbuf.alloc(P->n);
         W->pt1.alloc(P->n);
@@ -822,10 +736,10 @@ struct GmresResult {
 // Restarted GMRES(m) on the bordered operator (src/krylov.jl:732-786); x, b on
 // device.  With `precond` the sector preconditioner is applied on the right
 // (x = x0 + M^{-1} V y), so the GMRES residual is the true residual.  With
-// `backward` the stopping threshold is atol + rtol |x_k| / 2, where |x_k| of
+// `backward` the stopping threshold is (atol + rtol |x_k|) / 2, where |x_k| of
 // the current iterate is formed on the device every kXnormEvery steps: since
 // t^H L = 0 and t.v = 1, the bordered residual r bounds |L x| / s <= 2 |r|,
-// so the threshold guarantees the relative residual |L x| / (s |x|) <= rtol.
+// so the threshold guarantees |L x| / s <= atol + rtol |x|.
 constexpr int kXnormEvery = 25;
 
 static double iterate_norm(pimi_workspace *W, const std::vector<cd> &Hm, const std::vector<cd> &g, int m, int k,
@@ -870,7 +784,7 @@ static GmresResult gmres_device(pimi_workspace *W, double invscale, double atol,
     d2h(W, W->hpin, W->nscal.p, 2 * sizeof(double));
     const double bnorm = W->hpin[0];
     double xnorm = W->hpin[1];
-    auto threshold = [&]() { return backward ? atol + 0.5 * rtol * xnorm : atol + rtol * bnorm; };
+    auto threshold = [&]() { return backward ? 0.5 * (atol + rtol * xnorm) : atol + rtol * bnorm; };
     int64_t iterations = 0, restarts = 0;
     steady_apply(W, W->x.p, W->w.p, invscale);
     applications++;
@@ -1084,7 +998,7 @@ static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol
     d2h(W, W->hpin, W->nscal.p, 2 * sizeof(double));
     const double bnorm = W->hpin[0];
     double xnorm = W->hpin[1];
-    auto threshold = [&]() { return backward ? atol + 0.5 * rtol * xnorm : atol + rtol * bnorm; };
+    auto threshold = [&]() { return backward ? 0.5 * (atol + rtol * xnorm) : atol + rtol * bnorm; };
     auto form_x = [&](int k, const std::vector<cd> &Hr, const std::vector<cd> &gr, cx<double> *dst, bool add_to_x) {
         std::vector<cd> yy(k);
         for (int i = k - 1; i >= 0; --i) {
@@ -1205,7 +1119,20 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, double *rh
     for (double t : P->tval) tt += t * t;
     std::vector<double> vv(P->tval.size());
     for (size_t k = 0; k < vv.size(); ++k) vv[k] = P->tval[k] / tt;
-    W->d_vval.upload(vv.data(), vv.size(), W->stream);
+    if (o->weighting != PIMI_WEIGHT_NONE && o->weighting != PIMI_WEIGHT_SECTOR)
+        throw Error(PIMI_EINVAL, "unknown weighting");
+    const bool weighted = o->weighting == PIMI_WEIGHT_SECTOR;
+    if (weighted && o->criterion != PIMI_CRITERION_BACKWARD)
+        throw Error(PIMI_EINVAL, "sector weighting needs the backward criterion");
+    W->weighted = false;
+    std::vector<double> vw(vv);
+    if (weighted) {  // v -> S v on the trace support (ordered by sector, m entries each)
+        ensure_plan_weighting(P);
+        size_t k = 0;
+        for (int q = 0; q < P->ns; ++q)
+            for (int64_t a = 0; a < P->host.dims[(size_t)q]; ++a, ++k) vw[k] = vv[k] * P->sigma[(size_t)q];
+    }
+    W->d_vval.upload(vw.data(), vw.size(), W->stream);
     double scale = o->operator_scale > 0 ? o->operator_scale : operator_scale(W);
     if (!(scale > 0) || !std::isfinite(scale))
         throw Error(PIMI_EINVAL, "operator_scale must be finite and positive");
@@ -1227,15 +1154,47 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, double *rh
     if (inner_outer) ensure_diag_stencil(P);
     const bool backward = o->criterion == PIMI_CRITERION_BACKWARD;
     if (o->criterion != PIMI_CRITERION_REFERENCE && !backward) throw Error(PIMI_EINVAL, "unknown criterion");
-    GmresResult gr = inner_outer
-                         ? fgmres_device(W, 1.0 / scale, o->atol, o->rtol, o->maxiter, apps, backward, o->inner_steps)
-                         : gmres_device(W, 1.0 / scale, o->atol, o->rtol, o->maxiter, apps, precond, backward);
+    W->weighted = weighted;
+    // With sector weighting the iteration controls |S A S^{-1} u - S v| against
+    // |u|; the acceptance below measures |L x| / (s |x|) on x = S^{-1} u.  If
+    // that has not followed, the iteration continues with a tighter threshold.
+    GmresResult gr{false, 0, 0, 0.0};
+    double tight = 1.0;
+    for (int pass = 0;; ++pass) {
+        const int64_t left = o->maxiter - gr.iterations;
+        GmresResult g = inner_outer ? fgmres_device(W, 1.0 / scale, o->atol * tight, o->rtol * tight, left, apps,
+                                                    backward, o->inner_steps)
+                                    : gmres_device(W, 1.0 / scale, o->atol * tight, o->rtol * tight, left, apps,
+                                                   precond, backward);
+        gr.converged = g.converged;
+        gr.iterations += g.iterations;
+        gr.restarts += g.restarts;
+        gr.residual = g.residual;
+        if (!weighted || !g.converged || pass >= 4 || gr.iterations >= o->maxiter) break;
+        hipLaunchKernelGGL(k_sector_scale<double>, dim3((unsigned)P->ntiles), dim3(kApplyBlock), 0, W->stream,
+                           W->x.p, W->tmp.p, n, P->offs.p, P->tile_sector.p, P->d_inv_sigma.p);
+        launch_apply<double>(W, W->tmp.p, W->w.p, 1.0);
+        apps++;
+        dev_norm<double>(W, W->w.p, n, W->nscal.p);
+        dev_norm<double>(W, W->tmp.p, n, W->nscal.p + 1);
+        d2h(W, W->hpin, W->nscal.p, 2 * sizeof(double));
+        if (W->hpin[0] / scale <= o->atol + o->rtol * W->hpin[1]) break;
+        tight *= 0.1;
+    }
+    W->weighted = false;
     if (backward) {
         // The trace row of the bordered system carries weight |v| = |t|^{-1}; under the
         // scale-invariant rule it is restored exactly by x <- x / (t . x), which leaves
-        // |L x| / |x| unchanged.
+        // |L x| / |x| unchanged.  Sector weighting is undone first (x = S^{-1} u).
         std::vector<cx<double>> xs((size_t)n);
         d2h(W, xs.data(), W->x.p, sizeof(cx<double>) * n);
+        if (weighted)
+            for (int q = 0; q < P->ns; ++q) {
+                const double inv = 1.0 / P->sigma[(size_t)q];
+                const int64_t m = P->host.dims[(size_t)q];
+                for (int64_t i = P->host.offs[(size_t)q]; i < P->host.offs[(size_t)q] + m * m; ++i)
+                    xs[(size_t)i] = cx<double>{xs[(size_t)i].x * inv, xs[(size_t)i].y * inv};
+            }
         cd tr0(0, 0);
         for (size_t k = 0; k < P->tidx.size(); ++k) tr0 += P->tval[k] * cd(xs[P->tidx[k]].x, xs[P->tidx[k]].y);
         if (tr0 == cd(0, 0)) throw Error(PIMI_ENOTCONVERGED, "stationary iterate has zero trace");

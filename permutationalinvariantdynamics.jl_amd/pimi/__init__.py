@@ -11,3 +11,4 @@ from .api import (  # noqa: F401
     computational_product_state, expect, qubit_ensemble_model, saved_times, solve_dynamics, spin_matrices,
     stationary_state,
 )
+from .batch import BatchAlgorithm, BatchEngine, batched_stationary_states  # noqa: F401

@@ -15,6 +15,7 @@ PIMI_OK, PIMI_EINVAL, PIMI_EDIM, PIMI_ENOMEM, PIMI_EDEVICE, PIMI_EBUSY, PIMI_ENO
 PIMI_C64, PIMI_C32 = 0, 1
 PRECOND_NONE, PRECOND_SECTOR, PRECOND_SECTOR_INNER = 0, 1, 2
 CRITERION_REFERENCE, CRITERION_BACKWARD = 0, 1
+WEIGHT_NONE, WEIGHT_SECTOR = 0, 1
 TERM_LOCAL_H, TERM_COLLECTIVE_H, TERM_LOCAL_JUMP, TERM_COLLECTIVE_JUMP, TERM_DIRECT_PRODUCT_H = range(5)
 
 # Every symbol the header declares.
@@ -25,6 +26,8 @@ EXPORTS = [
     "pimi_workspace_create", "pimi_workspace_destroy", "pimi_apply", "pimi_steady_gmres", "pimi_expv_series",
     "pimi_workspace_last_apply_ms", "pimi_apply_bench", "pimi_copy_bench",
     "pimi_sector_preconditioner_factors", "pimi_precond_apply",
+    "pimi_batch_create", "pimi_batch_destroy", "pimi_batch_job_bytes", "pimi_batch_submit", "pimi_batch_run",
+    "pimi_batch_counts", "pimi_batch_next_finished", "pimi_batch_fetch", "pimi_batch_apply_bench",
 ]
 
 
@@ -70,12 +73,17 @@ class PlanInfo(C.Structure):
 class GmresOpts(C.Structure):
     _fields_ = [("krylovdim", C.c_int32), ("preconditioner", C.c_int32), ("maxiter", C.c_int64), ("atol", C.c_double),
                 ("rtol", C.c_double), ("operator_scale", C.c_double), ("inner_rtol", C.c_double),
-                ("max_refinements", C.c_int64), ("criterion", C.c_int32), ("inner_steps", C.c_int32)]
+                ("max_refinements", C.c_int64), ("criterion", C.c_int32), ("inner_steps", C.c_int32),
+                ("weighting", C.c_int32), ("reserved", C.c_int32)]
 
 
 class ExpvOpts(C.Structure):
     _fields_ = [("krylovdim", C.c_int32), ("reserved", C.c_int32), ("atol", C.c_double), ("rtol", C.c_double),
                 ("safety", C.c_double), ("max_steps", C.c_int64)]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("max_jobs", C.c_int32), ("memory_bytes", C.c_int64)]
 
 
 class SolveInfo(C.Structure):
@@ -105,11 +113,12 @@ def lib():
         for name in EXPORTS:
             f = getattr(L, name)
             if name not in ("pimi_last_error", "pimi_version", "pimi_context_destroy", "pimi_plan_destroy",
-                            "pimi_workspace_destroy"):
+                            "pimi_workspace_destroy", "pimi_batch_destroy"):
                 f.restype = C.c_int32
         L.pimi_context_destroy.restype = None
         L.pimi_plan_destroy.restype = None
         L.pimi_workspace_destroy.restype = None
+        L.pimi_batch_destroy.restype = None
         _lib = L
     return _lib
 

@@ -445,9 +445,14 @@ class SteadyStateResult:
 
 def stationary_state(compiled: CompiledPIModel, algorithm=None, atol=1e-10, rtol=1e-8, operator_scale=None,
                      return_info=False, precision=N.PIMI_C64, inner_rtol=1e-5, max_refinements=50,
-                     criterion="reference"):
+                     criterion="reference", weighting=None):
     """Trace-fixed stationary state by restarted GMRES on the device
-    (src/krylov.jl:797-891).  Raises NotConverged like the reference."""
+    (src/krylov.jl:797-891).  Raises NotConverged like the reference.
+
+    weighting="sector" iterates on u = S x with S = sqrt(f^nu) on sector nu,
+    so that residual norms weigh sectors by multiplicity as observables do
+    (needs criterion="backward"); None iterates on the PI coefficients as the
+    reference does."""
     algorithm = algorithm or GMRESAlgorithm()
     if not isinstance(algorithm, GMRESAlgorithm):
         raise ValueError("the mi355x backend solves stationary states with GMRESAlgorithm")
@@ -457,9 +462,12 @@ def stationary_state(compiled: CompiledPIModel, algorithm=None, atol=1e-10, rtol
     if criterion not in ("reference", "backward"):
         raise ValueError("criterion must be \"reference\" or \"backward\"")
     crit = N.CRITERION_BACKWARD if criterion == "backward" else N.CRITERION_REFERENCE
+    if weighting not in (None, "sector"):
+        raise ValueError("weighting must be None or \"sector\"")
+    wt = N.WEIGHT_SECTOR if weighting == "sector" else N.WEIGHT_NONE
     opts = N.GmresOpts(algorithm.krylovdim, pc, algorithm.maxiter, float(atol), float(rtol),
                        float(operator_scale or 0.0), float(inner_rtol), int(max_refinements), crit,
-                       int(algorithm.inner_steps))
+                       int(algorithm.inner_steps), wt, 0)
     rho = np.zeros(len(compiled.basis), dtype=np.complex128)
     info = N.SolveInfo()
     N.check(N.lib().pimi_steady_gmres(ws, C.byref(opts), N.ptr(rho), C.byref(info)))
