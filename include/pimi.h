@@ -275,7 +275,21 @@ typedef struct {
     int32_t precision;     /* PIMI_C64, or PIMI_C32: FP32 Arnoldi (basis, apply, preconditioner) in FP64 refinement */
     int32_t max_jobs;      /* jobs resident at once (0: 1024) */
     int64_t memory_bytes;  /* device memory reserved for job state (0: three quarters of the free memory) */
+    int32_t flags;         /* PIMI_BATCH_TIMING: time every launch with HIP events (see pimi_batch_stats) */
+    int32_t reserved;
 } pimi_batch_opts;
+
+enum { PIMI_BATCH_TIMING = 1 };
+/* Operation classes of pimi_batch_stats. */
+enum {
+    PIMI_OP_PRECOND_GEMM = 0, /* sector preconditioner (MFMA GEMMs) */
+    PIMI_OP_APPLY = 1,        /* stencil apply of the Arnoldi step */
+    PIMI_OP_DOTS = 2,         /* Gram-Schmidt projections V^H w */
+    PIMI_OP_UPDATE = 3,       /* Gram-Schmidt updates w -= V h */
+    PIMI_OP_CYCLE_END = 4,    /* x += M^{-1} V y */
+    PIMI_OP_FP64 = 5,         /* FP64 residuals, operator-scale probes, final checks */
+    PIMI_OP_OTHER = 6         /* norms, scalings, trace terms, mailbox */
+};
 
 pimi_status pimi_batch_create(pimi_context *ctx, const pimi_batch_opts *opts, pimi_batch **out);
 void pimi_batch_destroy(pimi_batch *batch);
@@ -294,6 +308,11 @@ pimi_status pimi_batch_run(pimi_batch *batch, int32_t max_ticks, int32_t stop_wh
 /* counts[6]: queued, active, finished (not yet fetched), ticks so far,
  * reserved bytes in use, reservation size */
 pimi_status pimi_batch_counts(pimi_batch *batch, int64_t *counts);
+/* Accumulated statistics per operation class c (PIMI_OP_*): out[c] = device
+ * milliseconds (only with PIMI_BATCH_TIMING), out[8+c] = algorithmic bytes
+ * (each array counted once per launch), out[16+c] = flops; 24 entries.
+ * reset != 0 clears them afterwards.  Measurement only. */
+pimi_status pimi_batch_stats(pimi_batch *batch, double *out, int32_t reset);
 /* next finished job id, or -1 */
 pimi_status pimi_batch_next_finished(pimi_batch *batch, int64_t *job_id);
 /* Copy a finished job's state (trace one, n complex doubles) and info, and

@@ -225,53 +225,6 @@ __global__ __launch_bounds__(64) void kb_reduce_norm(const BVec *__restrict__ S)
     if (threadIdx.x == 0) *(double *)J.d = sqrt(s);
 }
 
-// c[col * nblk + chunk] = chunk's part of conj(V[:, col]) . w, virtual blocks
-// ordered (chunk, col) with col fastest so that neighbouring blocks share w
-template <typename R>
-__global__ __launch_bounds__(kRedBlock) void kb_dots_partial(const BVec *__restrict__ S, int ns, int64_t total) {
-    __shared__ double sh[kRedBlock / 64];
-    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
-        const BVec J = S[seg_of(S, ns, vb)];
-        const int64_t lb = vb - J.first;
-        const int64_t chunk = lb / J.ncol, c = lb % J.ncol;
-        const cx<R> *v = (const cx<R> *)J.a + c * J.n;
-        const cx<R> *w = (const cx<R> *)J.b;
-        double re = 0, im = 0;
-        for (int64_t k = chunk * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock) {
-            const cx<R> a = v[k], b = w[k];
-            re += (double)a.x * (double)b.x + (double)a.y * (double)b.y;
-            im += (double)a.x * (double)b.y - (double)a.y * (double)b.x;
-        }
-        re = block_sum(re, sh);
-        __syncthreads();
-        im = block_sum(im, sh);
-        if (threadIdx.x == 0) ((cx<double> *)J.c)[c * J.nblk + chunk] = cx<double>{re, im};
-        __syncthreads();
-    }
-}
-
-// h[col] = sum of the column's partials (a); hacc (d): s0 == 0 store, else add
-__global__ __launch_bounds__(64) void kb_reduce_cols(const BVec *__restrict__ S, int ns, int64_t total) {
-    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
-        const BVec J = S[seg_of(S, ns, vb)];
-        const int64_t c = vb - J.first;
-        const cx<double> *p = (const cx<double> *)J.a + c * J.nblk;
-        double re = 0, im = 0;
-        for (int k = threadIdx.x; k < J.nblk; k += 64) {
-            re += p[k].x;
-            im += p[k].y;
-        }
-        re = wave_sum(re);
-        im = wave_sum(im);
-        if (threadIdx.x == 0) {
-            ((cx<double> *)J.c)[c] = cx<double>{re, im};
-            cx<double> *hacc = (cx<double> *)J.d;
-            if (J.s0 == 0.0) hacc[c] = cx<double>{re, im};
-            else hacc[c] = cx<double>{hacc[c].x + re, hacc[c].y + im};
-        }
-    }
-}
-
 // mode 0: c -= V[:, 0:ncol] b ; mode 1: c = V[:, 0:ncol] b   (b: FP64 coefficients)
 template <typename R>
 __global__ __launch_bounds__(kRedBlock) void kb_gemv(const BVec *__restrict__ S, int ns, int64_t total, int mode) {
@@ -300,6 +253,159 @@ __global__ __launch_bounds__(kRedBlock) void kb_gemv(const BVec *__restrict__ S,
             if (k < J.n) {
                 if (mode == 0) w[k] = cx<R>{(R)((double)w[k].x - re), (R)((double)w[k].y - im)};
                 else w[k] = cx<R>{(R)re, (R)im};
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Delayed reorthogonalization (one reduction and one update pass per Arnoldi
+// step).  With Q = V[:, 0:j] final, p = V[:, j] the once-orthogonalized
+// candidate and w = B p:
+//   s = Q^H p, pi = p^H p, t = Q^H w, tau = p^H w                 (one pass)
+//   rho = sqrt(pi - |s|^2), e = (tau - s^H t) / rho^2
+//   q_j = (p - Q s) / rho,  u_{j+1} = (w - e p - Q (t - e s)) / rho  (one pass)
+// u_{j+1} is B q_j projected once against [Q, q_j]: B Q s lies in span[Q, q_j].
+
+// partial[(col * nblk + chunk) * 2 + {0, 1}] = chunk's part of conj(V_col) . p
+// and conj(V_col) . w for col = 0..ncol-1, p = V[:, ncol-1]; a = V, b = w
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void kb_dots2_partial(const BVec *__restrict__ S, int ns, int64_t total) {
+    __shared__ double sh[kRedBlock / 64];
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const BVec J = S[seg_of(S, ns, vb)];
+        const int64_t lb = vb - J.first;
+        const int64_t chunk = lb / J.ncol, c = lb % J.ncol;
+        const cx<R> *v = (const cx<R> *)J.a + c * J.n;
+        const cx<R> *p = (const cx<R> *)J.a + (int64_t)(J.ncol - 1) * J.n;
+        const cx<R> *w = (const cx<R> *)J.b;
+        double r0 = 0, i0 = 0, r1 = 0, i1 = 0;
+        for (int64_t k = chunk * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock) {
+            const cx<R> a = v[k], b = p[k], d = w[k];
+            r0 += (double)a.x * (double)b.x + (double)a.y * (double)b.y;
+            i0 += (double)a.x * (double)b.y - (double)a.y * (double)b.x;
+            r1 += (double)a.x * (double)d.x + (double)a.y * (double)d.y;
+            i1 += (double)a.x * (double)d.y - (double)a.y * (double)d.x;
+        }
+        r0 = block_sum(r0, sh);
+        __syncthreads();
+        i0 = block_sum(i0, sh);
+        __syncthreads();
+        r1 = block_sum(r1, sh);
+        __syncthreads();
+        i1 = block_sum(i1, sh);
+        if (threadIdx.x == 0) {
+            cx<double> *out = (cx<double> *)J.c + (c * J.nblk + chunk) * 2;
+            out[0] = cx<double>{r0, i0};
+            out[1] = cx<double>{r1, i1};
+        }
+        __syncthreads();
+    }
+}
+
+// c[col] = V_col^H p, c[ncol + col] = V_col^H w from the partials (a)
+__global__ __launch_bounds__(64) void kb_reduce_cols2(const BVec *__restrict__ S, int ns, int64_t total) {
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const BVec J = S[seg_of(S, ns, vb)];
+        const int64_t c = vb - J.first;
+        const cx<double> *pa = (const cx<double> *)J.a + c * J.nblk * 2;
+        double r0 = 0, i0 = 0, r1 = 0, i1 = 0;
+        for (int k = threadIdx.x; k < J.nblk; k += 64) {
+            r0 += pa[2 * k].x;
+            i0 += pa[2 * k].y;
+            r1 += pa[2 * k + 1].x;
+            i1 += pa[2 * k + 1].y;
+        }
+        r0 = wave_sum(r0);
+        i0 = wave_sum(i0);
+        r1 = wave_sum(r1);
+        i1 = wave_sum(i1);
+        if (threadIdx.x == 0) {
+            cx<double> *o = (cx<double> *)J.c;
+            o[c] = cx<double>{r0, i0};
+            o[J.ncol + c] = cx<double>{r1, i1};
+        }
+    }
+}
+
+// Scalar slots (doubles) of the delayed step in a job's mailbox.
+constexpr int kSlotRho = 10, kSlotInvRho = 11, kSlotE = 12;  // e: complex at 12, 13
+
+// rho, 1/rho, e per job (one wave each) and the update coefficients
+// coef[0:j] = s, coef[j:2j] = t - e s; a = reductions (V^H p, V^H w),
+// c = coef, d = scalar slots, ncol = j + 1
+__global__ __launch_bounds__(64) void kb_dcgs_coef(const BVec *__restrict__ S) {
+    const BVec J = S[blockIdx.x];
+    const int j = J.ncol - 1;
+    const cx<double> *sp = (const cx<double> *)J.a, *tw = sp + J.ncol;
+    double ss = 0, str = 0, sti = 0;
+    for (int c = threadIdx.x; c < j; c += 64) {
+        const cx<double> a = sp[c], b = tw[c];
+        ss += a.x * a.x + a.y * a.y;
+        str += a.x * b.x + a.y * b.y;
+        sti += a.x * b.y - a.y * b.x;
+    }
+    ss = wave_sum(ss);
+    str = wave_sum(str);
+    sti = wave_sum(sti);
+    const double r2 = sp[j].x - ss;
+    const double rho = r2 > 0 ? sqrt(r2) : 0.0;
+    const double inv = rho > 0 ? 1.0 / rho : 0.0;
+    const cx<double> e = rho > 0 ? cx<double>{(tw[j].x - str) / r2, (tw[j].y - sti) / r2} : cx<double>{0, 0};
+    cx<double> *coef = (cx<double> *)J.c;
+    for (int c = threadIdx.x; c < j; c += 64) {
+        const cx<double> a = sp[c], b = tw[c];
+        coef[c] = a;
+        coef[j + c] = cx<double>{b.x - (e.x * a.x - e.y * a.y), b.y - (e.x * a.y + e.y * a.x)};
+    }
+    if (threadIdx.x == 0) {
+        double *slot = (double *)J.d;
+        slot[kSlotRho] = rho;
+        slot[kSlotInvRho] = inv;
+        slot[kSlotE] = e.x;
+        slot[kSlotE + 1] = e.y;
+    }
+}
+
+// V[:, j] <- q_j and V[:, j+1] <- u_{j+1} (a = V, b = coef, c = w, d = slots, ncol = j + 1)
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void kb_dcgs_update(const BVec *__restrict__ S, int ns, int64_t total) {
+    __shared__ cx<double> ca[kColChunk], cb[kColChunk];
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const BVec J = S[seg_of(S, ns, vb)];
+        const int64_t lb = vb - J.first;
+        const int j = J.ncol - 1;
+        cx<R> *V = (cx<R> *)J.a;
+        const cx<double> *coef = (const cx<double> *)J.b;
+        const cx<R> *w = (const cx<R> *)J.c;
+        const double *slot = (const double *)J.d;
+        const double inv = slot[kSlotInvRho], er = slot[kSlotE], ei = slot[kSlotE + 1];
+        cx<R> *pcol = V + (int64_t)j * J.n, *ucol = V + (int64_t)(j + 1) * J.n;
+        for (int64_t k0 = lb * kRedBlock; k0 < J.n; k0 += (int64_t)J.nblk * kRedBlock) {
+            const int64_t k = k0 + threadIdx.x;
+            double ar = 0, ai = 0, br = 0, bi = 0;
+            for (int c0 = 0; c0 < j; c0 += kColChunk) {
+                const int cn = min(kColChunk, j - c0);
+                __syncthreads();
+                for (int c = threadIdx.x; c < cn; c += kRedBlock) {
+                    ca[c] = coef[c0 + c];
+                    cb[c] = coef[j + c0 + c];
+                }
+                __syncthreads();
+                if (k < J.n)
+                    for (int c = 0; c < cn; ++c) {
+                        const cx<R> v = V[k + (int64_t)(c0 + c) * J.n];
+                        ar += (double)v.x * ca[c].x - (double)v.y * ca[c].y;
+                        ai += (double)v.x * ca[c].y + (double)v.y * ca[c].x;
+                        br += (double)v.x * cb[c].x - (double)v.y * cb[c].y;
+                        bi += (double)v.x * cb[c].y + (double)v.y * cb[c].x;
+                    }
+            }
+            if (k < J.n) {
+                const double px = pcol[k].x, py = pcol[k].y, wx = w[k].x, wy = w[k].y;
+                pcol[k] = cx<R>{(R)((px - ar) * inv), (R)((py - ai) * inv)};
+                ucol[k] = cx<R>{(R)((wx - (er * px - ei * py) - br) * inv),
+                                (R)((wy - (er * py + ei * px) - bi) * inv)};
             }
         }
         __syncthreads();
@@ -478,11 +584,12 @@ struct BJob {
     double *npart = nullptr, *vval = nullptr;
     double *scal() const { return (double *)mail; }
     // host Krylov state
-    std::vector<cd> H, g, sn, y;
+    std::vector<cd> H, Hraw, cprov, g, sn, y;  // H: rotated (least squares); Hraw: final columns
     std::vector<double> cs;
     std::vector<double> vv;
     int j = 0, kend = 0;
     bool begin_cycle = false, candidate = false, weighted = false;
+    double table_bytes64 = 0, table_bytes32 = 0, gemm_flops = 0;  // per apply / per preconditioner application
     double tight = 1.0;  // threshold factor, lowered when the coefficient-space check fails
     double scale = 0, beta = 0, beta0 = 0, xnorm = 0;
     int64_t iterations = 0, cycles = 0, applications = 0;
@@ -504,9 +611,9 @@ static size_t job_bytes(int64_t n, int64_t nt, int kdim, int prec) {
     const int nblk = blocks_for(n);
     size_t b = 0;
     b += 5 * align_up(sizeof(cx<double>) * n, 256);
-    b += align_up(es * n * (size_t)(kdim + 1), 256) + 4 * align_up(es * n, 256);
-    b += align_up(sizeof(cx<double>) * nblk * (size_t)(kdim + 1), 256);
-    b += 2 * align_up(sizeof(cx<double>) * (size_t)(kdim + 2 + kMailHacc), 256);
+    b += align_up(es * n * (size_t)(kdim + 2), 256) + 4 * align_up(es * n, 256);
+    b += align_up(sizeof(cx<double>) * nblk * 2 * (size_t)(kdim + 1), 256);
+    b += 2 * align_up(sizeof(cx<double>) * (size_t)(2 * (kdim + 2) + kMailHacc), 256);
     b += align_up(sizeof(double) * 4 * nblk, 256) + align_up(sizeof(double) * nt, 256);
     return b;
 }
@@ -533,8 +640,12 @@ struct pimi_batch {
     std::atomic<int> busy{0};
     int64_t ticks = 0;
     bool trace = false;
+    bool timing = false;  // PIMI_BATCH_TIMING: HIP events between launches, per operation class
+    std::vector<hipEvent_t> events;
+    double stat_ms[8] = {}, stat_bytes[8] = {}, stat_flops[8] = {};
     ~pimi_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto ev : events) (void)hipEventDestroy(ev);
         if (hstage) (void)hipHostFree(hstage);
         if (hmail) (void)hipHostFree(hmail);
         if (stream) (void)hipStreamDestroy(stream);
@@ -548,10 +659,21 @@ namespace pimi {
 // one copy, then the launches run; one gather + one copy bring back every
 // job's scalars and Hessenberg column; one synchronization per tick.
 // ---------------------------------------------------------------------------
+// Operation classes of a tick, for the optional per-class timing (HIP events
+// between launches) and the algorithmic bytes / flops they move.
+enum OpClass { OC_GEMM = 0, OC_APPLY, OC_DOTS, OC_UPDATE, OC_CYCLE_END, OC_FP64, OC_OTHER, OC_COUNT };
+
 struct Tick {
     pimi_batch *E;
     std::vector<char> buf;
     std::vector<std::function<void(const char *)>> ops;
+    std::vector<int> op_class;
+    int cls = OC_OTHER;
+    double bytes[OC_COUNT] = {}, flops[OC_COUNT] = {};
+    void push(std::function<void(const char *)> f) {
+        ops.push_back(std::move(f));
+        op_class.push_back(cls);
+    }
     // pointer fields (offset in buf) that must hold the device address of
     // stage data (offset in buf): patched once the device stage is placed
     std::vector<std::pair<size_t, size_t>> relocs;
@@ -577,7 +699,7 @@ struct Tick {
         const int ns = (int)v.size();
         const unsigned grid = (unsigned)std::min<int64_t>(tot, kMaxGrid);
         hipStream_t st = E->stream;
-        ops.push_back([=](const char *d) {
+        push([=](const char *d) {
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(bdim), 0, st, (const BVec *)(d + off), ns, tot);
         });
     }
@@ -598,12 +720,13 @@ struct Tick {
             v.push_back(BApply{tot, in(J), out(J), P->offs.p, P->dims.p, P->tile_sector.p, P->term_base.p, terms,
                                P->col.p, val, J->n, unit ? 1.0 : 1.0 / J->scale});
             tot += P->ntiles;
+            bytes[cls] += 2.0 * J->n * sizeof(cx<R>) + (sizeof(R) == 8 ? J->table_bytes64 : J->table_bytes32);
         }
         const size_t off = put(v);
         const int nj = (int)v.size();
         const int64_t per_xcd = (tot + kNumXcd - 1) / kNumXcd;
         hipStream_t st = E->stream;
-        ops.push_back([=](const char *d) {
+        push([=](const char *d) {
             hipLaunchKernelGGL(kb_apply<R>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyBlock), 0, st,
                                (const BApply *)(d + off), nj, tot, per_xcd);
         });
@@ -621,7 +744,7 @@ struct Tick {
         const size_t off = put(v);
         const unsigned nj = (unsigned)v.size();
         hipStream_t st = E->stream;
-        ops.push_back([=](const char *d) {
+        push([=](const char *d) {
             hipLaunchKernelGGL(kb_trace<R>, dim3(nj), dim3(256), 0, st, (const BTrace *)(d + off));
         });
     }
@@ -635,6 +758,9 @@ struct Tick {
             int ta, tb;
         };
         const G shapes[4] = {{1, 0}, {0, 0}, {0, 0}, {0, 1}};
+        const int saved = cls;
+        cls = OC_GEMM;
+        for (BJob *J : js) flops[OC_GEMM] += J->gemm_flops;
         for (int stage = 0; stage < 4; ++stage) {
             std::vector<BGemm> v;
             int64_t tot = 0;
@@ -656,46 +782,64 @@ struct Tick {
             const int ng = (int)v.size();
             const int ta = shapes[stage].ta, tb = shapes[stage].tb;
             hipStream_t st = E->stream;
-            ops.push_back([=](const char *d) {
+            push([=](const char *d) {
                 hipLaunchKernelGGL(kb_gemm<R>, dim3((unsigned)((tot + 3) / 4)), dim3(256), 0, st,
                                    (const BGemm *)(d + off), ng, tot, ta, tb);
             });
         }
+        cls = saved;
     }
 
     template <typename R>
     void norm(const std::vector<BJob *> &js, std::function<const void *(BJob *)> a, int slot, int pslot = 0) {
         if (js.empty()) return;
         std::vector<BVec> v;
-        for (BJob *J : js)
+        for (BJob *J : js) {
             v.push_back(BVec{0, a(J), nullptr, J->npart + (size_t)pslot * J->nblk, J->scal() + slot, J->n, J->nblk, 1,
                              0.0, 0});
+            bytes[cls] += (double)J->n * sizeof(cx<R>);
+        }
         vec(v, kb_norm_partial<R>, nblk_of);
         const size_t off = put(v);
         const unsigned nj = (unsigned)v.size();
         hipStream_t st = E->stream;
-        ops.push_back([=](const char *d) {
+        push([=](const char *d) {
             hipLaunchKernelGGL(kb_reduce_norm, dim3(nj), dim3(64), 0, st, (const BVec *)(d + off));
         });
     }
 
-    // CGS2 of w against V[:, 0:j+1]; hacc = accumulated coefficients (mailbox)
+    // One delayed-reorthogonalization Arnoldi step (see kb_dcgs_update):
+    // V[:, j] becomes q_j and V[:, j+1] the next candidate; the reductions
+    // V^H p, V^H w land in the mailbox for the host's Hessenberg update.
     template <typename R>
-    void cgs2(const std::vector<BJob *> &js) {
+    void dcgs(const std::vector<BJob *> &js) {
         if (js.empty()) return;
-        for (int pass = 0; pass < 2; ++pass) {
-            std::vector<BVec> d, r, u;
-            for (BJob *J : js) {
-                const int nc = J->j + 1;
-                d.push_back(BVec{0, J->V, J->w, J->partial, nullptr, J->n, J->nblk, nc, 0.0, 0});
-                r.push_back(BVec{0, J->partial, nullptr, J->h, J->mail + kMailHacc, J->n, J->nblk, nc,
-                                 pass == 0 ? 0.0 : 1.0, 0});
-                u.push_back(BVec{0, J->V, J->h, J->w, nullptr, J->n, J->nblk, nc, 0.0, 0});
-            }
-            vec(d, kb_dots_partial<R>, [](const BVec &e) { return (int64_t)e.nblk * e.ncol; });
-            vec(r, kb_reduce_cols, [](const BVec &e) { return (int64_t)e.ncol; }, 64);
-            gemv<R>(u, 0);
+        const int saved = cls;
+        std::vector<BVec> d, r, c, u;
+        for (BJob *J : js) {
+            const int nc = J->j + 1;
+            bytes[OC_DOTS] += (double)(nc + 1) * J->n * sizeof(cx<R>);
+            bytes[OC_UPDATE] += (double)(J->j + 4) * J->n * sizeof(cx<R>);
+            d.push_back(BVec{0, J->V, J->w, J->partial, nullptr, J->n, J->nblk, nc, 0.0, 0});
+            r.push_back(BVec{0, J->partial, nullptr, J->mail + kMailHacc, nullptr, J->n, J->nblk, nc, 0.0, 0});
+            c.push_back(BVec{0, J->mail + kMailHacc, nullptr, J->h, J->scal(), J->n, J->nblk, nc, 0.0, 0});
+            u.push_back(BVec{0, J->V, J->h, J->w, J->scal(), J->n, J->nblk, nc, 0.0, 0});
         }
+        cls = OC_DOTS;
+        vec(d, kb_dots2_partial<R>, [](const BVec &e) { return (int64_t)e.nblk * e.ncol; });
+        cls = OC_OTHER;
+        vec(r, kb_reduce_cols2, [](const BVec &e) { return (int64_t)e.ncol; }, 64);
+        {
+            const size_t off = put(c);
+            const unsigned nj = (unsigned)c.size();
+            hipStream_t st = E->stream;
+            push([=](const char *dd) {
+                hipLaunchKernelGGL(kb_dcgs_coef, dim3(nj), dim3(64), 0, st, (const BVec *)(dd + off));
+            });
+        }
+        cls = OC_UPDATE;
+        vec(u, kb_dcgs_update<R>, nblk_of);
+        cls = saved;
     }
 
     template <typename R>
@@ -710,7 +854,7 @@ struct Tick {
         const int ns = (int)u.size();
         const unsigned grid = (unsigned)std::min<int64_t>(tot, kMaxGrid);
         hipStream_t st = E->stream;
-        ops.push_back([=](const char *d) {
+        push([=](const char *d) {
             hipLaunchKernelGGL(kb_gemv<R>, dim3(grid), dim3(kRedBlock), 0, st, (const BVec *)(d + off), ns, tot,
                                mode);
         });
@@ -737,14 +881,14 @@ static void carve(pimi_batch *E, BJob *J) {
     J->r = (cx<double> *)take(sizeof(cx<double>) * n);
     J->w64 = (cx<double> *)take(sizeof(cx<double>) * n);
     J->xo = (cx<double> *)take(sizeof(cx<double>) * n);
-    J->V = take(es * n * (size_t)(J->kdim + 1));
+    J->V = take(es * n * (size_t)(J->kdim + 2));
     J->w = take(es * n);
     J->z = take(es * n);
     J->t1 = take(es * n);
     J->t2 = take(es * n);
-    J->partial = (cx<double> *)take(sizeof(cx<double>) * J->nblk * (size_t)(J->kdim + 1));
-    J->h = (cx<double> *)take(sizeof(cx<double>) * (size_t)(J->kdim + 2 + kMailHacc));
-    J->mail = (cx<double> *)take(sizeof(cx<double>) * (size_t)(J->kdim + 2 + kMailHacc));
+    J->partial = (cx<double> *)take(sizeof(cx<double>) * J->nblk * 2 * (size_t)(J->kdim + 1));
+    J->h = (cx<double> *)take(sizeof(cx<double>) * (size_t)(2 * (J->kdim + 2) + kMailHacc));
+    J->mail = (cx<double> *)take(sizeof(cx<double>) * (size_t)(2 * (J->kdim + 2) + kMailHacc));
     J->npart = (double *)take(sizeof(double) * 4 * J->nblk);
     J->vval = (double *)take(sizeof(double) * J->nt);
     if ((size_t)(p - (E->arena.base + J->off)) > J->bytes) throw Error(PIMI_EINVAL, "job layout exceeds its footprint");
@@ -808,6 +952,7 @@ static void run_tick(pimi_batch *E) {
         return v;
     };
     // -- operator scale: max over three probe vectors of |L p| / |p| (src/krylov.jl:156-169)
+    T.cls = OC_FP64;
     for (int k = 0; k < 3 && !probe.empty(); ++k) {
         T.vec(vecs(probe, [&](BJob *J) {
                   return BVec{0, nullptr, nullptr, J->r, nullptr, J->n, J->nblk, 1, 0.0, (uint64_t)(2 * J->n * k)};
@@ -830,11 +975,12 @@ static void run_tick(pimi_batch *E) {
         const size_t off = T.put(sc);
         const unsigned nj = (unsigned)sc.size();
         hipStream_t st = E->stream;
-        T.ops.push_back([=](const char *d) {
+        T.push([=](const char *d) {
             hipLaunchKernelGGL(kb_scatter, dim3(nj), dim3(256), 0, st, (const BScatter *)(d + off));
         });
     }
     // -- Arnoldi step (first the new cycle's start vector V0 = r / |r|)
+    T.cls = OC_OTHER;
     T.vec(vecs(begin, [](BJob *J) {
               return BVec{0, J->r, nullptr, J->V, J->scal() + S_RNORM, J->n, J->nblk, 1, 0.0, 0};
           }),
@@ -842,21 +988,20 @@ static void run_tick(pimi_batch *E) {
     if (!arn.empty()) {
         T.precond<R>(arn, [](BJob *J) -> const void * { return vcol<R>(J, J->j); },
                      [](BJob *J) -> void * { return J->z; });
+        T.cls = OC_APPLY;
         T.apply<R>(arn, [](BJob *J) -> const void * { return J->z; }, [](BJob *J) -> void * { return J->w; }, false);
+        T.cls = OC_OTHER;
         T.trace<R>(arn, [](BJob *J) -> const void * { return J->z; }, [](BJob *J) -> void * { return J->w; }, true,
                    false);
-        T.cgs2<R>(arn);
-        T.norm<R>(arn, [](BJob *J) -> const void * { return J->w; }, S_WNORM);
-        T.vec(vecs(arn, [](BJob *J) {
-                  return BVec{0, J->w, nullptr, vcol<R>(J, J->j + 1), J->scal() + S_WNORM, J->n, J->nblk, 1, 0.0, 0};
-              }),
-              kb_scale_div<R, R>, Tick::nblk_of);
+        T.dcgs<R>(arn);
     }
     // -- end of a cycle: x += M^{-1} V y, then the FP64 residual
     if (!end.empty()) {
+        T.cls = OC_CYCLE_END;
         std::vector<BVec> u;
         std::vector<size_t> yoffs;
         for (BJob *J : end) {
+            T.bytes[OC_CYCLE_END] += (double)(J->kend + 1) * J->n * (E->prec == PIMI_C32 ? 8 : 16);
             std::vector<cx<double>> yc(J->kend);
             for (int i = 0; i < J->kend; ++i) yc[i] = cx<double>{J->y[i].real(), J->y[i].imag()};
             yoffs.push_back(T.put(yc));  // coefficients travel with the descriptors
@@ -872,6 +1017,7 @@ static void run_tick(pimi_batch *E) {
     // -- FP64 bordered residual r = b - (L x / s + (t . x) v): new jobs and cycle ends
     std::vector<BJob *> resid(init);
     resid.insert(resid.end(), end.begin(), end.end());
+    T.cls = OC_FP64;
     if (!resid.empty()) {
         T.apply<double>(resid, cX, W64, false);
         T.trace<double>(resid, cX, W64, true, false);
@@ -896,7 +1042,7 @@ static void run_tick(pimi_batch *E) {
             const size_t off = T.put(v);
             const int nj = (int)v.size();
             hipStream_t st = E->stream;
-            T.ops.push_back([=](const char *d) {
+            T.push([=](const char *d) {
                 hipLaunchKernelGGL(kb_sector_scale, dim3((unsigned)tot), dim3(kApplyBlock), 0, st,
                                    (const BApply *)(d + off), nj, tot);
             });
@@ -907,12 +1053,13 @@ static void run_tick(pimi_batch *E) {
         T.trace<double>(fin, cXo, nullptr, false, true);
     }
     // -- mailbox gather
+    T.cls = OC_OTHER;
     std::vector<BCopy> g;
     size_t mtot = 0;
     std::vector<size_t> moff(E->active.size());
     for (size_t k = 0; k < E->active.size(); ++k) {
         BJob *J = E->active[k];
-        const int64_t cnt = kMailHacc + (J->phase == PH_ARNOLDI ? J->j + 1 : 0);
+        const int64_t cnt = kMailHacc + (J->phase == PH_ARNOLDI ? 2 * (J->j + 1) : 0);
         moff[k] = mtot;
         g.push_back(BCopy{J->mail, nullptr, cnt});
         mtot += (size_t)cnt;
@@ -923,7 +1070,7 @@ static void run_tick(pimi_batch *E) {
         const size_t off = T.put(g);
         const unsigned ng = (unsigned)g.size();
         hipStream_t st = E->stream;
-        T.ops.push_back([=](const char *d) {
+        T.push([=](const char *d) {
             hipLaunchKernelGGL(kb_gather, dim3(ng), dim3(256), 0, st, (const BCopy *)(d + off));
         });
     }
@@ -935,13 +1082,24 @@ static void run_tick(pimi_batch *E) {
     }
     if (E->dstage.count < T.buf.size()) E->dstage.alloc(std::max<size_t>(T.buf.size(), 2 * E->dstage.count));
     const char *dbase = E->dstage.p;
+    const bool timing = E->timing;
+    if (timing)
+        while (E->events.size() < T.ops.size() + 1) {
+            hipEvent_t ev;
+            HIPCHK(hipEventCreate(&ev));
+            E->events.push_back(ev);
+        }
     for (auto &rl : T.relocs) {
         const void *addr = dbase + rl.second;
         std::memcpy(T.buf.data() + rl.first, &addr, sizeof addr);
     }
     std::memcpy(E->hstage, T.buf.data(), T.buf.size());
     HIPCHK(hipMemcpyAsync(E->dstage.p, E->hstage, T.buf.size(), hipMemcpyHostToDevice, E->stream));
-    for (auto &op : T.ops) op(dbase);
+    if (timing) HIPCHK(hipEventRecord(E->events[0], E->stream));
+    for (size_t k = 0; k < T.ops.size(); ++k) {
+        T.ops[k](dbase);
+        if (timing) HIPCHK(hipEventRecord(E->events[k + 1], E->stream));
+    }
     HIPCHK(hipGetLastError());
     if (E->hmail_cap < mtot) {
         if (E->hmail) HIPCHK(hipHostFree(E->hmail));
@@ -956,6 +1114,16 @@ static void run_tick(pimi_batch *E) {
     }
     HIPCHK(hipStreamSynchronize(E->stream));
     E->ticks++;
+    for (int c = 0; c < OC_COUNT; ++c) {
+        E->stat_bytes[c] += T.bytes[c];
+        E->stat_flops[c] += T.flops[c];
+    }
+    if (timing)
+        for (size_t k = 0; k < T.ops.size(); ++k) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, E->events[k], E->events[k + 1]));
+            E->stat_ms[T.op_class[k]] += ms;
+        }
     // -- host updates
     std::vector<BJob *> still;
     for (size_t k = 0; k < E->active.size(); ++k) {
@@ -1000,50 +1168,81 @@ static void run_tick(pimi_batch *E) {
                     J->j = 0;
                     J->beta0 = J->beta;
                     std::fill(J->H.begin(), J->H.end(), cd(0, 0));
+                    std::fill(J->Hraw.begin(), J->Hraw.end(), cd(0, 0));
                     std::fill(J->g.begin(), J->g.end(), cd(0, 0));
                     J->g[0] = J->beta;
                 }
                 break;
             }
             case PH_ARNOLDI: {
+                // Tick j finalized Hessenberg column j-1 (first-pass coefficients
+                // c_{j-1} + reorthogonalization s, subdiagonal rho) and produced the
+                // first-pass coefficients of column j:
+                //   c_j = ((t - H[0:j, 0:j] s) / rho ; e - s_{j-1})
                 J->begin_cycle = false;
-                const int m = J->kdim, jj = J->j;
-                auto Hij = [&](int i, int c) -> cd & { return J->H[(size_t)i + (size_t)c * (m + 1)]; };
-                for (int i = 0; i <= jj; ++i) Hij(i, jj) = cd(mb[kMailHacc + i].x, mb[kMailHacc + i].y);
-                const double wn = sc[S_WNORM];
-                Hij(jj + 1, jj) = wn;
-                for (int i = 0; i < jj; ++i) {
-                    const cd h1 = Hij(i, jj), h2 = Hij(i + 1, jj);
-                    Hij(i, jj) = J->cs[i] * h1 + J->sn[i] * h2;
-                    Hij(i + 1, jj) = -std::conj(J->sn[i]) * h1 + J->cs[i] * h2;
-                }
-                double c;
-                cd s;
-                complex_givens(Hij(jj, jj), Hij(jj + 1, jj), c, s);
-                J->cs[jj] = c;
-                J->sn[jj] = s;
-                const cd h1 = Hij(jj, jj), h2 = Hij(jj + 1, jj);
-                Hij(jj, jj) = c * h1 + s * h2;
-                Hij(jj + 1, jj) = 0;
-                const cd g1 = J->g[jj], g2 = J->g[jj + 1];
-                J->g[jj] = c * g1 + s * g2;
-                J->g[jj + 1] = -std::conj(s) * g1 + c * g2;
-                J->iterations++;
                 J->applications++;
-                const double est = std::abs(J->g[jj + 1]);
-                const double cyc = std::max(thr(), J->o.inner_rtol * J->beta0);
-                if (est <= cyc || jj + 1 >= m || J->iterations >= J->o.maxiter || wn == 0.0 || !std::isfinite(est)) {
-                    int kk = jj + 1;
+                const int m = J->kdim, jj = J->j;
+                auto Hr = [&](int i, int c) -> cd & { return J->H[(size_t)i + (size_t)c * (m + 1)]; };
+                auto Hw = [&](int i, int c) -> cd & { return J->Hraw[(size_t)i + (size_t)c * (m + 1)]; };
+                const cx<double> *sv = mb + kMailHacc, *tv = sv + (jj + 1);
+                auto S_ = [&](int i) { return cd(sv[i].x, sv[i].y); };
+                auto T_ = [&](int i) { return cd(tv[i].x, tv[i].y); };
+                const double rho = sc[kSlotRho];
+                const cd e(sc[kSlotE], sc[kSlotE + 1]);
+                bool end_cycle = false;
+                double est = INFINITY;
+                if (jj >= 1) {
+                    for (int i = 0; i < jj; ++i) Hw(i, jj - 1) = J->cprov[(size_t)i] + S_(i);
+                    Hw(jj, jj - 1) = rho;
+                    for (int i = 0; i <= jj; ++i) Hr(i, jj - 1) = Hw(i, jj - 1);
+                    const int c = jj - 1;
+                    for (int i = 0; i < c; ++i) {
+                        const cd h1 = Hr(i, c), h2 = Hr(i + 1, c);
+                        Hr(i, c) = J->cs[i] * h1 + J->sn[i] * h2;
+                        Hr(i + 1, c) = -std::conj(J->sn[i]) * h1 + J->cs[i] * h2;
+                    }
+                    double gc;
+                    cd gs;
+                    complex_givens(Hr(c, c), Hr(c + 1, c), gc, gs);
+                    J->cs[c] = gc;
+                    J->sn[c] = gs;
+                    const cd h1 = Hr(c, c), h2 = Hr(c + 1, c);
+                    Hr(c, c) = gc * h1 + gs * h2;
+                    Hr(c + 1, c) = 0;
+                    const cd g1 = J->g[c], g2 = J->g[c + 1];
+                    J->g[c] = gc * g1 + gs * g2;
+                    J->g[c + 1] = -std::conj(gs) * g1 + gc * g2;
+                    J->iterations++;
+                    est = std::abs(J->g[c + 1]);
+                    const double cyc = std::max(thr(), J->o.inner_rtol * J->beta0);
+                    end_cycle = est <= cyc || jj >= m || J->iterations >= J->o.maxiter || rho == 0.0 ||
+                                !std::isfinite(est);
+                } else if (!(rho > 0) || !std::isfinite(rho)) {
+                    J->error = "zero or non-finite start vector in a GMRES cycle";
+                    J->phase = PH_FINAL;
+                    J->candidate = false;
+                    break;
+                }
+                if (!end_cycle) {
+                    for (int i = 0; i < jj; ++i) {
+                        cd acc = T_(i);
+                        for (int c = 0; c < jj; ++c) acc -= Hw(i, c) * S_(c);
+                        J->cprov[(size_t)i] = acc / rho;
+                    }
+                    J->cprov[(size_t)jj] = e - (jj >= 1 ? S_(jj - 1) : cd(0, 0));
+                    J->j = jj + 1;
+                } else {
+                    const int kk = jj;
                     J->y.assign((size_t)kk, cd(0, 0));
                     bool singular = false;
                     for (int i = kk - 1; i >= 0; --i) {
                         cd acc = J->g[i];
-                        for (int q = i + 1; q < kk; ++q) acc -= Hij(i, q) * J->y[q];
-                        if (Hij(i, i) == cd(0, 0)) {
+                        for (int q = i + 1; q < kk; ++q) acc -= Hr(i, q) * J->y[q];
+                        if (Hr(i, i) == cd(0, 0)) {
                             singular = true;
                             break;
                         }
-                        J->y[i] = acc / Hij(i, i);
+                        J->y[i] = acc / Hr(i, i);
                     }
                     if (singular) {
                         J->error = "singular GMRES Hessenberg";
@@ -1053,10 +1252,8 @@ static void run_tick(pimi_batch *E) {
                         J->kend = kk;
                         J->phase = PH_END;
                     }
-                } else {
-                    J->j = jj + 1;
                 }
-                if (E->trace && J->iterations % 50 == 0)
+                if (E->trace && J->iterations % 50 == 0 && jj >= 1)
                     std::fprintf(stderr, "[pimi batch] job %lld iteration %lld estimate %.3e |x| %.3e\n",
                                  (long long)J->id, (long long)J->iterations, est, J->xnorm);
                 break;
@@ -1077,6 +1274,7 @@ static void run_tick(pimi_batch *E) {
                     J->j = 0;
                     J->beta0 = J->beta;
                     std::fill(J->H.begin(), J->H.end(), cd(0, 0));
+                    std::fill(J->Hraw.begin(), J->Hraw.end(), cd(0, 0));
                     std::fill(J->g.begin(), J->g.end(), cd(0, 0));
                     J->g[0] = J->beta;
                     J->result.clear();
@@ -1151,6 +1349,7 @@ pimi_status pimi_batch_create(pimi_context *ctx, const pimi_batch_opts *opts, pi
         }
         const char *tr = std::getenv("PIMI_TRACE");
         E->trace = tr && tr[0] == '1';
+        E->timing = (opts->flags & PIMI_BATCH_TIMING) != 0;
         HIPCHK(hipStreamCreate(&E->stream));
         E->arena.init(align_up(bytes, 256));
         *out = E.release();
@@ -1199,6 +1398,15 @@ pimi_status pimi_batch_submit(pimi_batch *E, pimi_plan *plan, const pimi_gmres_o
         J->kdim = o->krylovdim;
         J->nblk = blocks_for(J->n);
         J->bytes = job_bytes(J->n, J->nt, J->kdim, E->prec);
+        {
+            const double base = (double)(plan->st.col.size() * 4 + plan->st.terms.size() * sizeof(StencilTermDev) +
+                                         plan->st.term_base.size() * 8 + (plan->host.dims.size() + 1) * 12);
+            J->table_bytes64 = base + 16.0 * plan->st.val.size();
+            J->table_bytes32 = base + 8.0 * plan->st.val.size();
+            double m3 = 0;
+            for (auto m : plan->host.dims) m3 += (double)m * m * m;
+            J->gemm_flops = 4 * 8.0 * m3;  // four complex m x m products per sector, 8 flops per complex fma
+        }
         if (J->bytes > E->arena.size)
             throw Error(PIMI_ENOMEM, "the job's device state exceeds the engine's memory reservation");
         ensure_plan_preconditioner(plan);
@@ -1215,6 +1423,8 @@ pimi_status pimi_batch_submit(pimi_batch *E, pimi_plan *plan, const pimi_gmres_o
         }
         const int m = J->kdim;
         J->H.assign((size_t)(m + 1) * m, cd(0, 0));
+        J->Hraw.assign((size_t)(m + 1) * m, cd(0, 0));
+        J->cprov.assign((size_t)m + 2, cd(0, 0));
         J->g.assign((size_t)m + 1, cd(0, 0));
         J->sn.assign((size_t)m, cd(0, 0));
         J->cs.assign((size_t)m, 0.0);
@@ -1252,6 +1462,20 @@ pimi_status pimi_batch_counts(pimi_batch *E, int64_t *counts) {
         counts[3] = E->ticks;
         counts[4] = (int64_t)E->arena.used;
         counts[5] = (int64_t)E->arena.size;
+    });
+}
+
+pimi_status pimi_batch_stats(pimi_batch *E, double *out, int32_t reset) {
+    return guarded([&] {
+        if (!E || !out) throw Error(PIMI_EINVAL, "null argument");
+        BusyGuard guard(E->busy);
+        for (int c = 0; c < 8; ++c) {
+            out[c] = E->stat_ms[c];
+            out[8 + c] = E->stat_bytes[c];
+            out[16 + c] = E->stat_flops[c];
+        }
+        if (reset)
+            for (int c = 0; c < 8; ++c) E->stat_ms[c] = E->stat_bytes[c] = E->stat_flops[c] = 0;
     });
 }
 

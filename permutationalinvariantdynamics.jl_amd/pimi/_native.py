@@ -28,6 +28,7 @@ EXPORTS = [
     "pimi_sector_preconditioner_factors", "pimi_precond_apply",
     "pimi_batch_create", "pimi_batch_destroy", "pimi_batch_job_bytes", "pimi_batch_submit", "pimi_batch_run",
     "pimi_batch_counts", "pimi_batch_next_finished", "pimi_batch_fetch", "pimi_batch_apply_bench",
+    "pimi_batch_stats",
 ]
 
 
@@ -83,7 +84,12 @@ class ExpvOpts(C.Structure):
 
 
 class BatchOpts(C.Structure):
-    _fields_ = [("precision", C.c_int32), ("max_jobs", C.c_int32), ("memory_bytes", C.c_int64)]
+    _fields_ = [("precision", C.c_int32), ("max_jobs", C.c_int32), ("memory_bytes", C.c_int64),
+                ("flags", C.c_int32), ("reserved", C.c_int32)]
+
+
+BATCH_TIMING = 1
+OP_CLASSES = ("precond_gemm", "apply", "dots", "update", "cycle_end", "fp64", "other")
 
 
 class SolveInfo(C.Structure):

@@ -45,10 +45,10 @@ class BatchAlgorithm:
 
 
 class BatchEngine:
-    def __init__(self, precision=N.PIMI_C64, max_jobs=0, memory_bytes=0, device=0):
+    def __init__(self, precision=N.PIMI_C64, max_jobs=0, memory_bytes=0, device=0, timing=False):
         self.device = device
         self.precision = precision
-        opts = N.BatchOpts(int(precision), int(max_jobs), int(memory_bytes))
+        opts = N.BatchOpts(int(precision), int(max_jobs), int(memory_bytes), N.BATCH_TIMING if timing else 0, 0)
         h = C.c_void_p()
         N.check(N.lib().pimi_batch_create(_context(device), C.byref(opts), C.byref(h)))
         self._h = h
@@ -96,6 +96,14 @@ class BatchEngine:
         N.check(N.lib().pimi_batch_counts(self._h, N.ptr(c)))
         return dict(zip(("queued", "active", "finished", "ticks", "bytes_in_use", "bytes_reserved"),
                         (int(v) for v in c)))
+
+    def stats(self, reset=False) -> dict:
+        """Per operation class: device ms (timing engines only), algorithmic
+        bytes and flops accumulated over the ticks run so far."""
+        out = np.zeros(24)
+        N.check(N.lib().pimi_batch_stats(self._h, N.ptr(out), C.c_int32(1 if reset else 0)))
+        return {name: {"ms": out[c], "bytes": out[8 + c], "flops": out[16 + c]}
+                for c, name in enumerate(N.OP_CLASSES)}
 
     def next_finished(self):
         j = C.c_int64()
