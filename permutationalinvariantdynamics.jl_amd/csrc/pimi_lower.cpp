@@ -24,6 +24,9 @@
 #include <cstring>
 #include <functional>
 #include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -417,21 +420,43 @@ static std::vector<CSC> fuse(const std::vector<std::pair<cd, std::vector<CSC>>> 
     return out;
 }
 
-static CSC gram(const CSC &K) {  // K^H K, exact zeros dropped
-    std::vector<cd> A = csc_to_dense(K), Q((size_t)(K.n * K.n), cd(0, 0));
-    for (int64_t c = 0; c < K.n; ++c)
-        for (int64_t a = 0; a < K.n; ++a) {
-            cd s(0, 0);
-            for (int64_t k = 0; k < K.m; ++k) s += std::conj(A[(size_t)(k + a * K.m)]) * A[(size_t)(k + c * K.m)];
-            Q[(size_t)(a + c * K.n)] = s;
-        }
+// K^H K, exact zeros dropped.  Q[a,c] = sum_k conj(K[k,a]) K[k,c] accumulated
+// row by row in increasing k over the stored entries only: every nonzero
+// product is added in the same order as in the dense triple loop, and the
+// skipped exact-zero products cannot change a nonzero sum, so the blocks are
+// the dense loop's to the bit (an exactly zero sum is dropped either way).
+static CSC gram(const CSC &K) {
+    std::vector<std::vector<std::pair<int64_t, cd>>> rows((size_t)K.m);
+    for (int64_t col = 0; col < K.n; ++col)
+        for (int64_t p = K.colptr[(size_t)col]; p < K.colptr[(size_t)col + 1]; ++p)
+            rows[(size_t)K.rows[(size_t)p]].emplace_back(col, K.vals[(size_t)p]);
+    std::vector<cd> Q((size_t)(K.n * K.n), cd(0, 0));
+    for (int64_t k = 0; k < K.m; ++k)
+        for (const auto &ec : rows[(size_t)k])
+            for (const auto &ea : rows[(size_t)k])
+                Q[(size_t)(ea.first + ec.first * K.n)] += std::conj(ea.second) * ec.second;
     return dense_to_csc(Q, K.n, K.n);
+}
+
+// The geometry depends on (N, d) only: models of the same size (a parameter
+// scan, a stream of requests) share one build, copied out of this cache.
+static const Geometry &cached_geometry(int N, int d) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::unique_ptr<Geometry>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto &slot = cache[{N, d}];
+    if (!slot) {
+        std::unique_ptr<Geometry> g(new Geometry());
+        g->build(N, d);
+        slot = std::move(g);
+    }
+    return *slot;
 }
 
 void lower_model(int N, int d, const std::vector<TermDesc> &terms, FusedPlan &P) {
     if (N < 1 || d < 1) throw Error(PIMI_EINVAL, "N and d must be positive");
     if (d > 6) throw Error(PIMI_EINVAL, "local dimension d > 6 is not supported");
-    P.geom.build(N, d);
+    P.geom = cached_geometry(N, d);
     const Geometry &G = P.geom;
     P.ns = (int)G.sectors.size();
     P.dims.clear(); P.offs.clear();
@@ -571,8 +596,10 @@ void append_ell(Stencil &st, const RowLists &R, int64_t m_out, int32_t &width, i
 RowLists rows_of_dense(const std::vector<cd> &A, int64_t m, int64_t n) {
     RowLists R;
     R.rows.resize((size_t)m);
-    for (int64_t a = 0; a < m; ++a)
-        for (int64_t c = 0; c < n; ++c) {
+    // column order (contiguous in memory); each row still receives its entries
+    // in increasing column order
+    for (int64_t c = 0; c < n; ++c)
+        for (int64_t a = 0; a < m; ++a) {
             cd v = A[(size_t)(a + c * m)];
             if (v.real() != 0.0 || v.imag() != 0.0) R.rows[a].emplace_back((int32_t)c, v);
         }
