@@ -217,6 +217,12 @@ pimi_status pimi_plan_export(const pimi_plan *plan, int64_t *cp_base, int64_t *c
  * preconditioner factors: U (eigenvectors of each fused Hamiltonian block)
  * and inv_den (see pimi_preconditioner).  No counterpart in the reference. */
 pimi_status pimi_sector_preconditioner_factors(const pimi_plan *plan, double *U, double *inv_den);
+/* Build a device plan's stationary-solver tables ahead of use (the sector
+ * preconditioner factors and the sector-weighting tables), so that a later
+ * solve or batch submission does no host setup.  Not thread-safe for one
+ * plan; different plans may be prepared concurrently.  No counterpart in the
+ * reference. */
+pimi_status pimi_plan_prepare(pimi_plan *plan);
 void pimi_plan_destroy(pimi_plan *plan);
 
 /* One workspace per concurrent caller: owns a HIP stream and the Krylov basis. */

@@ -1580,6 +1580,16 @@ pimi_status pimi_plan_create_fused(pimi_context *ctx, const pimi_fused_desc *D, 
     });
 }
 
+pimi_status pimi_plan_prepare(pimi_plan *P) {
+    return guarded([&] {
+        if (!P) throw Error(PIMI_EINVAL, "null argument");
+        if (!P->ctx) throw Error(PIMI_EINVAL, "host-only plan");
+        HIPCHK(hipSetDevice(P->ctx->device));
+        ensure_plan_preconditioner(P);
+        ensure_plan_weighting(P);
+    });
+}
+
 pimi_status pimi_plan_info_get(const pimi_plan *P, pimi_plan_info *info) {
     return guarded([&] {
         if (!P || !info) throw Error(PIMI_EINVAL, "null argument");

@@ -367,6 +367,18 @@ class CompiledPIModel:
             self._ws[key] = ws
         return self._ws[key]
 
+    def release_workspaces(self):
+        """Free the device workspaces made so far (they are re-made on demand)."""
+        for ws in self._ws.values():
+            N.lib().pimi_workspace_destroy(ws)
+        self._ws.clear()
+
+    def prepare(self):
+        """Build the stationary-solver tables (sector preconditioner factors,
+        sector weighting) now rather than at the first solve."""
+        N.check(N.lib().pimi_plan_prepare(self._plan))
+        return self
+
     def trace_functional(self):
         nt = sum(self.basis.dims)
         idx = np.zeros(nt, dtype=np.int64)

@@ -1,0 +1,210 @@
+"""Device-resident request loop: continuous batching of stationary-state and
+dynamics requests on one device.  The reference has no serving layer; this
+module drives the reference's two solver entry points (stationary_state,
+src/highlevel.jl:321-357, and solve_dynamics, src/highlevel.jl:559-612) for a
+stream of requests.
+
+Stationary requests are continuously batched in one BatchEngine: a request
+that arrives while others are being solved is admitted between two ticks as
+soon as the engine's memory reservation holds its Krylov state (HBM-footprint
+admission), and finished jobs free their memory at once.  Dynamics requests
+run on a small pool of worker threads, each with its own device workspace and
+HIP stream, admitted while their workspaces fit a second memory budget; their
+kernels overlap the engine's on the device.  Model compilation and the
+stationary solver's host setup run on a thread pool off the loop thread.
+
+A request's result does not depend on the other requests in flight: the
+engine's arithmetic per job depends only on that job, and every dynamics
+request runs on its own workspace with deterministic reductions.
+"""
+from __future__ import annotations
+
+import queue
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from .api import ExpvAlgorithm, PIState, compile as pi_compile, computational_product_state, solve_dynamics
+from .batch import BatchAlgorithm, BatchEngine
+
+
+@dataclass
+class ServedResult:
+    index: int                 # position of the request in the submitted stream
+    kind: str                  # "steady" or "dynamics"
+    N: int
+    arrival: float             # seconds after the stream started
+    finished: float
+    latency: float
+    converged: bool
+    state: PIState | None = None          # stationary state (trace one)
+    values: dict | None = None            # dynamics: observable series
+    times: np.ndarray | None = None
+    info: dict = field(default_factory=dict)
+
+
+def dynamics_bytes(n, krylovdim):
+    """Device bytes of one dynamics workspace (basis + work vectors)."""
+    return (krylovdim + 10) * n * 16
+
+
+class Server:
+    def __init__(self, device=0, precision=N.PIMI_C32, steady_algorithm: BatchAlgorithm | None = None,
+                 dynamics_algorithm: ExpvAlgorithm | None = None, rtol=1e-8, memory_bytes=0,
+                 dynamics_workers=4, dynamics_memory_bytes=32 << 30, compile_workers=8, t_end=10.0, saveat=0.2):
+        self.device = device
+        self.rtol = rtol
+        self.steady_algorithm = steady_algorithm or BatchAlgorithm(krylovdim=200, inner_rtol=1e-4)
+        self.dynamics_algorithm = dynamics_algorithm or ExpvAlgorithm(krylovdim=30, atol=0.0, rtol=rtol)
+        self.engine = BatchEngine(precision=precision, memory_bytes=memory_bytes, device=device)
+        self.dynamics_workers = dynamics_workers
+        self.dynamics_memory_bytes = dynamics_memory_bytes
+        self.compile_workers = compile_workers
+        self.t_end = t_end
+        self.saveat = saveat
+
+    def close(self):
+        self.engine.close()
+
+    # -- request preparation (host, thread pool) --------------------------------
+    def _prepare(self, req):
+        comp = pi_compile(req.instance.model(), device=self.device)
+        if req.kind == "steady":
+            comp.prepare()
+        return comp
+
+    def _run_dynamics(self, comp, req):
+        b = comp.basis
+        rho0 = computational_product_state(b, 2)   # every particle in the upper level
+        res = solve_dynamics(comp, rho0, (0.0, self.t_end), saveat=self.saveat, algorithm=self.dynamics_algorithm,
+                             observables={"n": np.array([[0, 0], [0, 1]], dtype=complex)})
+        return res
+
+    def serve(self, requests, realtime=True):
+        """Run the stream; `requests` carry `arrival` (seconds), `kind` and
+        `instance`.  With realtime, a request is not touched before its
+        arrival time; otherwise all are available at once.  Returns the
+        results in request order."""
+        reqs = list(requests)
+        order = sorted(range(len(reqs)), key=lambda i: reqs[i].arrival)
+        results: list[ServedResult | None] = [None] * len(reqs)
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        lock = threading.Lock()
+        prepared_steady = queue.Queue()
+        dyn_queue = queue.Queue()
+        dyn_budget = [self.dynamics_memory_bytes]
+        budget_cv = threading.Condition()
+        errors = []
+
+        def prepare(i):
+            req = reqs[i]
+            try:
+                comp = self._prepare(req)
+            except Exception as e:  # reported by the loop
+                errors.append(e)
+                return
+            if req.kind == "steady":
+                prepared_steady.put((i, comp))
+            else:
+                dyn_queue.put((i, comp))
+
+        def dyn_worker():
+            while True:
+                item = dyn_queue.get()
+                if item is None:
+                    return
+                i, comp = item
+                need = dynamics_bytes(comp.info["n"], self.dynamics_algorithm.krylovdim)
+                with budget_cv:   # admission by device footprint
+                    while dyn_budget[0] < need and dyn_budget[0] < self.dynamics_memory_bytes:
+                        budget_cv.wait()
+                    dyn_budget[0] -= need
+                try:
+                    res = self._run_dynamics(comp, reqs[i])
+                    t = now()
+                    with lock:
+                        results[i] = ServedResult(i, "dynamics", comp.basis.N, reqs[i].arrival, t,
+                                                  t - reqs[i].arrival, bool(res.info.get("converged", 1)),
+                                                  values=res.values, times=res.t, info=res.info)
+                except Exception as e:
+                    errors.append(e)
+                finally:
+                    comp.release_workspaces()
+                    with budget_cv:
+                        dyn_budget[0] += need
+                        budget_cv.notify_all()
+
+        pool = ThreadPoolExecutor(max_workers=self.compile_workers)
+        workers = [threading.Thread(target=dyn_worker, daemon=True) for _ in range(self.dynamics_workers)]
+        for w in workers:
+            w.start()
+
+        def dispatch():   # hands each request to the preparation pool at its arrival time
+            for i in order:
+                if realtime:
+                    delay = reqs[i].arrival - now()
+                    if delay > 0:
+                        time.sleep(delay)
+                pool.submit(prepare, i)
+
+        dispatcher = threading.Thread(target=dispatch, daemon=True)
+        dispatcher.start()
+        n_steady = sum(1 for r in reqs if r.kind == "steady")
+        jobs = {}      # engine job id -> (request index, compiled model)
+        done_steady = 0
+        eng = self.engine
+        try:
+            while done_steady < n_steady:
+                if errors:
+                    raise errors[0]
+                # admit every prepared request that has arrived (between ticks)
+                while True:
+                    try:
+                        i, comp = prepared_steady.get_nowait()
+                    except queue.Empty:
+                        break
+                    jid = eng.submit(comp, self.steady_algorithm, rtol=self.rtol)
+                    jobs[jid] = (i, comp)
+                c = eng.counts()
+                if c["active"] + c["queued"] > 0:
+                    eng.run(max_ticks=1)
+                else:
+                    time.sleep(2e-4)
+                while (jid := eng.next_finished()) is not None:
+                    i, comp = jobs.pop(jid)
+                    r = eng.fetch(jid, check=False)
+                    t = now()
+                    results[i] = ServedResult(i, "steady", comp.basis.N, reqs[i].arrival, t, t - reqs[i].arrival,
+                                              bool(r.info["converged"]), state=r.state, info=r.info)
+                    done_steady += 1
+            dispatcher.join()
+            pool.shutdown(wait=True)
+            for _ in workers:
+                dyn_queue.put(None)
+            for w in workers:
+                w.join()
+            if errors:
+                raise errors[0]
+        finally:
+            pool.shutdown(wait=False)
+        return results
+
+
+def summarize(results, wall=None):
+    """p50/p99 latency (ms) per kind and overall throughput."""
+    out = {}
+    for kind in ("steady", "dynamics", "all"):
+        rs = [r for r in results if r is not None and (kind == "all" or r.kind == kind)]
+        if not rs:
+            continue
+        lat = np.array([r.latency for r in rs]) * 1e3
+        span = wall if wall is not None else max(r.finished for r in rs) - min(r.arrival for r in rs)
+        out[kind] = {"count": len(rs), "p50_ms": float(np.percentile(lat, 50)),
+                     "p99_ms": float(np.percentile(lat, 99)), "per_s": len(rs) / span if span > 0 else None,
+                     "converged": int(sum(r.converged for r in rs))}
+    return out

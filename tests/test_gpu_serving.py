@@ -1,0 +1,59 @@
+"""Serving loop: a Poisson stream of mixed stationary and dynamics requests.
+
+Each request's result is the same alone and in the mix (to the bit), the
+stationary states meet the relative residual and the closed form of the
+local-only model family, and the dynamics follow the closed form
+<sum_i n_i>(t) = N sigma_ee(t).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import pimi
+from pimi.serving import Server, summarize
+from pimi.workloads import Request, served_requests
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+import make_closed_form as CF  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _closed(inst, ts=None):
+    H = 2 * inst.omega * CF.jx + 1.0 * CF.jz
+    jumps = [(inst.gamma, CF.jm)] + ([(4 * inst.gamma_phi, CF.jz)] if inst.gamma_phi > 0 else [])
+    if ts is None:
+        return CF.steady(inst.N, H, jumps)[0]
+    e = np.array([[0, 0], [0, 1]], dtype=complex)
+    return CF.series(inst.N, H, jumps, e, ts)[:, 0]
+
+
+def test_serving_alone_equals_mix_and_closed_forms():
+    reqs = served_requests(rate=500.0, count=10, nmin=16, nmax=40)
+    print("serving kinds", [r.kind for r in reqs], "N", [r.instance.N for r in reqs], flush=True)
+    # rtol 1e-11: the observable error of a stationary state is the residual
+    # times the conditioning (up to ~1e3 on weakly driven members), so the
+    # closed-form comparison to 1e-6 needs a residual well below 1e-9
+    srv = Server(precision=pimi.PIMI_C32, rtol=1e-11, memory_bytes=4 << 30, dynamics_workers=2,
+                 dynamics_memory_bytes=2 << 30, compile_workers=4, t_end=2.0, saveat=0.2)
+    try:
+        mix = srv.serve(reqs, realtime=True)
+        print("summary", summarize(mix), flush=True)
+        for r, q in zip(mix, reqs):
+            assert r is not None and r.converged, (r, q)
+            alone = srv.serve([Request(0.0, q.kind, q.instance)], realtime=False)[0]
+            if q.kind == "steady":
+                assert r.info["relative_residual"] <= 1e-11
+                np.testing.assert_array_equal(alone.state.data, r.state.data)
+                comp = pimi.compile(q.instance.model())
+                val = pimi.expect(comp.collective_operator(np.array([[0, 0], [0, 1]], dtype=complex)), r.state).real
+                ref = _closed(q.instance)
+                assert abs(val - ref) <= 1e-6 * max(ref, 1e-3), (q.instance, val, ref)
+            else:
+                np.testing.assert_array_equal(alone.values["n"], r.values["n"])
+                ref = _closed(q.instance, list(r.times))
+                np.testing.assert_allclose(r.values["n"].real, ref, rtol=1e-6, atol=1e-9)
+    finally:
+        srv.close()

@@ -161,3 +161,19 @@ def test_sector_preconditioner_factors_agree_with_lapack(N_, d):
         return
     a, o = apply(Up, Ip, r), apply(U, inv, r)
     assert np.abs(a - o).max() <= 1e-12 * np.abs(o).max()
+
+
+def test_workload_generators_are_seeded():
+    from pimi.workloads import batched_instances, served_requests
+    a, b = batched_instances(64), batched_instances(64)
+    assert a == b and len(a) == 64
+    assert {i.N for i in batched_instances(512)} == {20, 40, 60, 80, 100, 120}
+    assert all(0 <= i.omega <= 2 and 1e-3 <= i.gamma_phi <= 1e-1 for i in a)
+    r = served_requests(200.0, 300)
+    assert r == served_requests(200.0, 300)
+    arr = np.array([q.arrival for q in r])
+    assert np.all(np.diff(arr) > 0)
+    assert abs(len(r) / arr[-1] - 200.0) < 40.0          # Poisson rate
+    frac = np.mean([q.kind == "steady" for q in r])
+    assert 0.5 < frac < 0.7
+    assert all(16 <= q.instance.N <= 160 for q in r)
