@@ -1,20 +1,31 @@
 """Benchmark: steady-state solves/s of the PI Liouvillian on one MI355X.
 
-Workload (one step = one stationary solve): the N=120 qubit ensemble,
-H = sum_i (jz + jx), local emission 0.1, local dephasing 0.2 (D[jz]), local
-pumping 0.02, collective emission 0.1; PI dimension C(123,3) = 302,621.
-Complex FP64 throughout: flexible GMRES on the device (outer Krylov
-dimension 200) whose right preconditioner is 8 steps of sector-preconditioned
-GMRES on the sector-block-diagonal part of L, solved from rho0 = t/(t.t) to
-relative residual |L rho|/(s |rho|) <= 1e-8.  One solve of plain
-sector-preconditioned GMRES (Krylov dimension 1000) and one with FP32 inner
-solves are timed beside it.
-Inputs are synthesized on the host and resident in HBM before the timed
-region; each solve's result is copied back (included in the timing).
+Headline workload (one step = one stationary solve): the N=120 qubit
+ensemble, H = sum_i (jz + jx), local emission 0.1, local dephasing 0.2
+(D[jz]), local pumping 0.02, collective emission 0.1; PI dimension
+C(123,3) = 302,621.  Complex FP64 throughout: flexible GMRES on the device
+(outer Krylov dimension 200) whose right preconditioner is 8 steps of
+sector-preconditioned GMRES on the sector-block-diagonal part of L, solved
+from rho0 = t/(t.t) to relative residual |L rho|/(s |rho|) <= 1e-8.  One solve
+of plain sector-preconditioned GMRES (Krylov dimension 1000) and one with FP32
+inner solves are timed beside it.  Inputs are synthesized on the host and
+resident in HBM before the timed region; each solve's result is copied back
+(included in the timing).
 
-Also reported: p50/p99 solve latency, the apply kernel against the HBM
-roofline (algorithmic bytes per apply / event-timed average apply), and the
-oracle's OpenMP CPU restatement timed on a bounded sample on the host.
+Also reported:
+  roofline      the headline solve's dominant kernel, the sector
+                preconditioner's FP64 MFMA GEMMs, against the FP64 matrix peak
+                (event-timed average over back-to-back applications);
+  spmv          the stencil apply (the SpMV of the metric) against the HBM
+                peak: one N=120 vector, and all 512 batched-workload models in
+                one launch;
+  batched       512 seeded local-only qubit models (N in {20..120}) solved by
+                the batch engine (FP32 Arnoldi in FP64 refinement) to relative
+                residual <= 1e-8, against the plain FP64 single-solve path on
+                a sample of the same models;
+  serving       a Poisson stream (200 requests/s, 60 requests, 60 % stationary
+                states, 40 % dynamics to t=10) through the serving loop;
+  cpu_baseline  the oracle's OpenMP CPU restatement on a bounded sample.
 """
 from __future__ import annotations
 
@@ -33,11 +44,14 @@ import ctypes as C  # noqa: E402
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFS = 78.6
 N_QUBITS = 120
 RTOL = 1e-8
 KRYLOV = 200          # outer (flexible) Krylov dimension of the headline solver
 INNER_STEPS = 8
 KRYLOV_PLAIN = 1000   # plain sector-preconditioned GMRES, no restart needed
+BATCH_COUNT = 512
+BATCH_PLAIN_SAMPLE = 32
 
 
 def build_model(pimi):
@@ -49,13 +63,14 @@ def build_model(pimi):
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE runs (calibrated by a device copy), or None."""
-    path = os.path.join(ROOT, "profiles", "k_apply_traffic_n120_fp64.json")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_n120.json")
     try:
         with open(path) as fh:
             d = json.load(fh)
     except OSError:
         return None
-    return d["traffic_bytes"] if d.get("kernel") == kernel else None
+    k = d.get("kernels", {}).get(kernel)
+    return None if k is None else k["traffic_bytes"]
 
 
 def apply_roofline(pimi, comp, reps=200):
@@ -63,10 +78,18 @@ def apply_roofline(pimi, comp, reps=200):
     ws = comp.workspace(pimi.PIMI_C64, 1)
     ms = C.c_double()
     N.check(N.lib().pimi_apply_bench(ws, C.c_int64(1), C.c_int32(reps), C.byref(ms)))
-    n = comp.info["n"]
-    algorithmic = 2 * n * 16 + comp.info["table_bytes_c64"]
-    achieved = algorithmic / (ms.value * 1e-3) / 1e9
-    return ms.value, algorithmic, achieved
+    algorithmic = 2 * comp.info["n"] * 16 + comp.info["table_bytes_c64"]
+    return ms.value, algorithmic, algorithmic / (ms.value * 1e-3) / 1e9
+
+
+def gemm_roofline(pimi, comp, reps=50):
+    """Average of one sector GEMM launch (a preconditioner application is four)."""
+    N = pimi._native
+    ws = comp.workspace(pimi.PIMI_C64, 1)
+    ms, flops = C.c_double(), C.c_double()
+    N.check(N.lib().pimi_precond_bench(ws, C.c_int32(reps), C.byref(ms), C.byref(flops)))
+    per_launch_ms, per_launch_flops = ms.value / 4, flops.value / 4
+    return per_launch_ms, per_launch_flops, per_launch_flops / (per_launch_ms * 1e-3) / 1e12
 
 
 def cpu_baseline(steady_iters, budget_s):
@@ -74,7 +97,7 @@ def cpu_baseline(steady_iters, budget_s):
     same sector preconditioner, MGS2 GMRES) on a bounded sample of the same
     solve: runs of k1 and k2 Arnoldi steps fit t(k) = a k + b k^2 (apply and
     preconditioner per step, Gram-Schmidt growing with k), evaluated at the
-    device solve's iteration count."""
+    plain device solve's iteration count."""
     from oracle import lowering as L
     from oracle import oracle as O
     threads = min(os.cpu_count() or 1, 64)
@@ -103,14 +126,85 @@ def cpu_baseline(steady_iters, budget_s):
                        f"the plain device solve's {steady_iters} steps = {est:.1f} s per solve")}
 
 
+def batched(pimi, args):
+    """512 seeded models through the batch engine; plain single-solve path on a sample."""
+    from pimi.workloads import batched_instances
+    inst = batched_instances(BATCH_COUNT)
+    comps = [pimi.compile(i.model()) for i in inst]
+    for c in comps:
+        c.prepare()
+    eng = pimi.BatchEngine(precision=pimi.PIMI_C32, timing=True)
+    # SpMV over every model in one launch
+    ms_spmv, bytes_spmv = eng.apply_bench(comps, reps=20)
+    eng.close()
+    eng = pimi.BatchEngine(precision=pimi.PIMI_C32, timing=True)
+    alg = pimi.BatchAlgorithm(krylovdim=300, inner_rtol=1e-4)
+    t0 = time.perf_counter()
+    ids = [eng.submit(c, alg, rtol=RTOL) for c in comps]
+    out = {}
+    while len(out) < len(ids):
+        eng.run(stop_when_finished=True)
+        while (j := eng.next_finished()) is not None:
+            out[j] = eng.fetch(j)
+    wall = time.perf_counter() - t0
+    st = eng.stats()
+    eng.close()
+    res = [out[j] for j in ids]
+    job_s = np.array([r.info["seconds"] for r in res]) * 1e3
+    ops = {k: {"ms": round(v["ms"], 2), "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None,
+               "TFps": round(v["flops"] / (v["ms"] * 1e-3) / 1e12, 2) if v["flops"] and v["ms"] else None}
+           for k, v in st.items()}
+    print(f"# batched: {len(res)} solves in {wall:.2f} s", file=sys.stderr, flush=True)
+    # the plain FP64 single-solve path (sector-preconditioned GMRES, one model at a time) on a sample
+    plain = pimi.GMRESAlgorithm(krylovdim=KRYLOV_PLAIN, maxiter=KRYLOV_PLAIN, preconditioner="sector")
+    t0 = time.perf_counter()
+    for c in comps[:BATCH_PLAIN_SAMPLE]:
+        pimi.stationary_state(c, algorithm=plain, atol=0.0, rtol=RTOL, criterion="backward")
+    plain_rate = BATCH_PLAIN_SAMPLE / (time.perf_counter() - t0)
+    rate = len(res) / wall
+    return {
+        "workload": f"{BATCH_COUNT} seeded driven-qubit models (N in 20..120, mean PI dim "
+                    f"{int(np.mean([c.info['n'] for c in comps]))}), FP32 Arnoldi in FP64 refinement, sector "
+                    f"preconditioner and weighting, to |L rho|/(s|rho|) <= 1e-8",
+        "solves_per_s": rate, "wall_s": wall,
+        "job_p50_ms": float(np.percentile(job_s, 50)), "job_p99_ms": float(np.percentile(job_s, 99)),
+        "max_relative_residual": max(r.info["relative_residual"] for r in res),
+        "converged": int(sum(r.info["converged"] for r in res)),
+        "plain_fp64_single_solves_per_s": plain_rate,
+        "plain_sample": f"first {BATCH_PLAIN_SAMPLE} of the same models, one at a time",
+        "speedup_vs_plain_fp64": rate / plain_rate,
+        "ops": ops,
+    }, {"achieved": bytes_spmv / (ms_spmv * 1e-3) / 1e9, "avg_ms": ms_spmv, "algorithmic_bytes": bytes_spmv,
+        "models": len(comps)}
+
+
+def serving(pimi):
+    from pimi.serving import Server, summarize
+    from pimi.workloads import served_requests
+    reqs = served_requests(200.0, 60)
+    srv = Server(precision=pimi.PIMI_C32)
+    t0 = time.perf_counter()
+    res = srv.serve(reqs)
+    wall = time.perf_counter() - t0
+    srv.close()
+    s = summarize(res, wall)
+    s["rate_per_s"] = 200.0
+    s["wall_s"] = wall
+    s["max_steady_relative_residual"] = max(r.info["relative_residual"] for r in res if r.kind == "steady")
+    return s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--profile", action="store_true", help="apply + copy calibration only (counter runs)")
+    ap.add_argument("--profile", action="store_true",
+                    help="apply, preconditioner and copy calibration only (counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -118,14 +212,16 @@ def main():
     import pimi
     comp = pimi.compile(build_model(pimi), backend="mi355x", device=0)
     if args.profile:
-        # counter runs: 20 event-timed applies, then 20 device copies of 64 MiB
-        # (the calibration kernel for the byte counters)
+        # counter runs: 20 event-timed applies, 5 preconditioner applications (20
+        # GEMM launches), then 20 device copies of 64 MiB (the counters' calibration)
         N = pimi._native
         ms_apply, bytes_apply, achieved = apply_roofline(pimi, comp, reps=20)
+        ms_gemm, flops_gemm, tfs = gemm_roofline(pimi, comp, reps=5)
         ms = C.c_double()
         N.check(N.lib().pimi_copy_bench(comp.workspace(pimi.PIMI_C64, 1), C.c_int64(64 << 20),
                                         C.c_int32(20), C.byref(ms)))
         print(json.dumps({"apply_ms": ms_apply, "apply_algorithmic_bytes": bytes_apply, "apply_GBps": achieved,
+                          "gemm_ms": ms_gemm, "gemm_flops": flops_gemm, "gemm_TFps": tfs,
                           "copy_bytes": 64 << 20, "copy_ms": ms.value,
                           "copy_GBps": 2 * (64 << 20) / (ms.value * 1e-3) / 1e9}))
         return
@@ -160,11 +256,20 @@ def main():
         side[name] = {"ms": (time.perf_counter() - s) * 1e3, "iterations": int(r.info["iterations"]),
                       "relative_residual": r.info["relative_residual"]}
         print(f"# {name}: {side[name]}", file=sys.stderr, flush=True)
-    ms_apply, bytes_apply, achieved = apply_roofline(pimi, comp)
+    ms_apply, bytes_apply, achieved_apply = apply_roofline(pimi, comp)
+    ms_gemm, flops_gemm, tfs_gemm = gemm_roofline(pimi, comp)
+    batch, spmv_batch = (None, None) if args.no_batch else batched(pimi, args)
+    serve = None if args.no_serve else serving(pimi)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(side["plain_fp64"]["iterations"], args.cpu_seconds)
     lat_ms = np.array(lat) * 1e3
+    spmv = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+            "single_n120_fp64": {"achieved": achieved_apply, "frac": achieved_apply / HBM_PEAK_GBS,
+                                 "avg_ms": ms_apply, "algorithmic_bytes": bytes_apply,
+                                 "traffic": measured_traffic("k_apply<double>")}}
+    if spmv_batch:
+        spmv["batched_512_fp32"] = dict(spmv_batch, frac=spmv_batch["achieved"] / HBM_PEAK_GBS)
     out = {
         "metric": "steady-state solves/s & p99 latency at resid 1e-8; SpMV GB/s vs HBM peak",
         "value": args.steps / total,
@@ -189,9 +294,13 @@ def main():
         "trace_error": max(i["trace_error"] for i in infos),
         "plain_fp64_sector_gmres": side["plain_fp64"],
         "fp32_inner": side["fp32_inner"],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("pimi::k_apply<double>"),
-                     "kernel": "k_apply<double>", "avg_ms": ms_apply, "algorithmic_bytes": bytes_apply},
+        "roofline": {"bound": "mfma", "achieved": tfs_gemm, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                     "frac": tfs_gemm / FP64_MFMA_PEAK_TFS,
+                     "traffic": measured_traffic("k_sector_gemm_mfma<double>"),
+                     "kernel": "k_sector_gemm_mfma<double>", "avg_ms": ms_gemm, "algorithmic_flops": flops_gemm},
+        "spmv": spmv,
+        "batched": batch,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -234,6 +234,12 @@ void pimi_workspace_destroy(pimi_workspace *ws);
  * precision); 0: host pointers (complex double). */
 pimi_status pimi_apply(pimi_workspace *ws, const void *x, void *y, int64_t ncols, int32_t on_device);
 
+/* y = L^dagger x, the adjoint under the PI (Hilbert-Schmidt) inner product
+ * (apply_adjoint!, src/liouvillian.jl:3275-3293; fused kernel 3107-3143):
+ * i[H, X] - {Q, X}/2 + sum conj(c) K^H X K + sum_branches conj(s) C^H X C.
+ * Same arguments as pimi_apply; the adjoint stencil is built on first use. */
+pimi_status pimi_apply_adjoint(pimi_workspace *ws, const void *x, void *y, int64_t ncols, int32_t on_device);
+
 /* Stationary state: solve (L/s + v t^H) rho = v from rho0 = v.  rho_out is a
  * host buffer of n complex doubles. */
 pimi_status pimi_steady_gmres(pimi_workspace *ws, const pimi_gmres_opts *opts, double *rho_out,
@@ -259,6 +265,13 @@ pimi_status pimi_copy_bench(pimi_workspace *ws, int64_t bytes, int32_t reps, dou
  * buffers, complex double; computed in the workspace precision).  For tests;
  * no counterpart in the reference. */
 pimi_status pimi_precond_apply(pimi_workspace *ws, const double *x, double *y, double scale);
+
+/* Throughput probe of the sector preconditioner (four grouped MFMA GEMMs per
+ * application, in the workspace precision) on a synthetic device vector,
+ * `reps` times, event-timed on the workspace stream: average milliseconds
+ * per application and its algorithmic flops (4 x 8 sum_nu m_nu^3).
+ * Measurement only. */
+pimi_status pimi_precond_bench(pimi_workspace *ws, int32_t reps, double *ms_avg, double *flops);
 
 /* Time of the last apply kernel launch sequence, in milliseconds (HIP events
  * recorded on the workspace stream around pimi_apply). */

@@ -49,7 +49,8 @@ struct BApply {
     void *y;
     const int64_t *offs;
     const int32_t *dims;
-    const int32_t *tile_sector;
+    const int32_t *tile_sector;  // 256-coordinate tiles (kb_sector_scale)
+    const ApplyTile *tiles;      // apply tiles (kb_apply)
     const int64_t *term_base;
     const StencilTermDev *terms;
     const int32_t *col;
@@ -116,25 +117,17 @@ __device__ __forceinline__ int seg_of(const T *__restrict__ seg, int nseg, int64
 // ---------------------------------------------------------------------------
 // batched kernels
 // ---------------------------------------------------------------------------
-// y = out_scale * L x for every job; tiles of all jobs laid end to end, the
-// global tile space split into one contiguous slab per XCD.
+// y = out_scale * L x for every job; apply tiles of all jobs laid end to end,
+// one wave each, the global tile space split into one contiguous slab per XCD.
 template <typename R>
-__global__ __launch_bounds__(kApplyBlock) void kb_apply(const BApply *__restrict__ J, int nj, int64_t total,
-                                                        int64_t per_xcd) {
+__global__ __launch_bounds__(kApplyWave) void kb_apply(const BApply *__restrict__ J, int nj, int64_t total,
+                                                       int64_t per_xcd) {
     const int64_t bid = blockIdx.x;
     const int64_t g = (bid % kNumXcd) * per_xcd + bid / kNumXcd;
     if (g >= total) return;
     const BApply A = J[seg_of(J, nj, g)];
-    const int64_t lt = g - A.first;
-    const int64_t i = lt * kApplyBlock + threadIdx.x;
-    if (i >= A.n) return;
-    int s = A.tile_sector[lt];
-    while (i >= A.offs[s + 1]) ++s;
-    const int64_t m = A.dims[s], loc = i - A.offs[s];
-    const cx<R> acc = apply_coordinate<R>((const cx<R> *)A.x, s, loc % m, loc / m, m, A.offs, A.dims, A.term_base,
-                                          A.terms, A.col, (const cx<R> *)A.val);
-    const R sc = (R)A.out_scale;
-    ((cx<R> *)A.y)[i] = cx<R>{sc * acc.x, sc * acc.y};
+    apply_tile<R, kApplyCols>((const cx<R> *)A.x, (cx<R> *)A.y, (R)A.out_scale, A.tiles[g - A.first], A.offs,
+                              A.dims, A.term_base, A.terms, A.col, (const cx<R> *)A.val);
 }
 
 // y = x * w[sector] (w = 1/sigma: back from the weighted coordinates), one
@@ -717,9 +710,9 @@ struct Tick {
             pimi_plan *P = J->plan;
             const void *val = sizeof(R) == 8 ? (const void *)P->val64.p : (const void *)P->val32.p;
             const StencilTermDev *terms = (!unit && J->weighted) ? P->terms_w.p : P->terms.p;
-            v.push_back(BApply{tot, in(J), out(J), P->offs.p, P->dims.p, P->tile_sector.p, P->term_base.p, terms,
-                               P->col.p, val, J->n, unit ? 1.0 : 1.0 / J->scale});
-            tot += P->ntiles;
+            v.push_back(BApply{tot, in(J), out(J), P->offs.p, P->dims.p, P->tile_sector.p, P->atiles4.p,
+                               P->term_base.p, terms, P->col.p, val, J->n, unit ? 1.0 : 1.0 / J->scale});
+            tot += P->natiles4;
             bytes[cls] += 2.0 * J->n * sizeof(cx<R>) + (sizeof(R) == 8 ? J->table_bytes64 : J->table_bytes32);
         }
         const size_t off = put(v);
@@ -727,7 +720,7 @@ struct Tick {
         const int64_t per_xcd = (tot + kNumXcd - 1) / kNumXcd;
         hipStream_t st = E->stream;
         push([=](const char *d) {
-            hipLaunchKernelGGL(kb_apply<R>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyBlock), 0, st,
+            hipLaunchKernelGGL(kb_apply<R>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyWave), 0, st,
                                (const BApply *)(d + off), nj, tot, per_xcd);
         });
     }
@@ -1035,7 +1028,7 @@ static void run_tick(pimi_batch *E) {
             if (J->weighted) {
                 pimi_plan *P = J->plan;
                 v.push_back(BApply{tot, J->x, J->xo, P->offs.p, P->dims.p, P->tile_sector.p, nullptr, nullptr,
-                                   nullptr, P->d_inv_sigma.p, J->n, 1.0});
+                                   nullptr, nullptr, P->d_inv_sigma.p, J->n, 1.0});
                 tot += P->ntiles;
             }
         if (!v.empty()) {
@@ -1539,9 +1532,9 @@ pimi_status pimi_batch_apply_bench(pimi_batch *E, pimi_plan *const *plans, int32
             ys[k].alloc(es * (size_t)P->n);
             HIPCHK(hipMemsetAsync(xs[k].p, 0x3f, es * (size_t)P->n, E->stream));
             const void *val = E->prec == PIMI_C32 ? (const void *)P->val32.p : (const void *)P->val64.p;
-            v.push_back(BApply{tot, xs[k].p, ys[k].p, P->offs.p, P->dims.p, P->tile_sector.p, P->term_base.p,
-                               P->terms.p, P->col.p, val, P->n, 1.0});
-            tot += P->ntiles;
+            v.push_back(BApply{tot, xs[k].p, ys[k].p, P->offs.p, P->dims.p, P->tile_sector.p, P->atiles4.p,
+                               P->term_base.p, P->terms.p, P->col.p, val, P->n, 1.0});
+            tot += P->natiles4;
             const int64_t base = (int64_t)(P->st.col.size() * 4 + P->st.terms.size() * sizeof(StencilTermDev) +
                                            P->st.term_base.size() * 8 + (P->host.dims.size() + 1) * 12);
             bytes += 2 * P->n * (int64_t)es + base + (int64_t)P->st.val.size() * (int64_t)es;
@@ -1552,10 +1545,10 @@ pimi_status pimi_batch_apply_bench(pimi_batch *E, pimi_plan *const *plans, int32
         const int64_t per_xcd = (tot + kNumXcd - 1) / kNumXcd;
         auto launch = [&] {
             if (E->prec == PIMI_C32)
-                hipLaunchKernelGGL(kb_apply<float>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyBlock), 0,
+                hipLaunchKernelGGL(kb_apply<float>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyWave), 0,
                                    E->stream, dv.p, nplans, tot, per_xcd);
             else
-                hipLaunchKernelGGL(kb_apply<double>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyBlock), 0,
+                hipLaunchKernelGGL(kb_apply<double>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyWave), 0,
                                    E->stream, dv.p, nplans, tot, per_xcd);
         };
         launch();  // warm-up

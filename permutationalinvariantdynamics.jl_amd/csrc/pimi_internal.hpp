@@ -65,11 +65,14 @@ struct pimi_plan {
     bool compiled = false;  // true: geometry available (collective operators)
     int64_t n = 0;
     int ns = 0;
-    int64_t ntiles = 0, tiles_per_xcd = 0;
+    int64_t ntiles = 0, tiles_per_xcd = 0;        // 256-coordinate tiles (sector scaling)
+    int64_t natiles = 0, atiles_per_xcd = 0;      // single-column apply tiles (one wave each)
+    int64_t natiles4 = 0;                         // kApplyCols-column apply tiles (batched launches)
     std::vector<int64_t> tidx;
     std::vector<double> tval;
     pimi::DevBuf<int64_t> offs, term_base, d_tidx;
     pimi::DevBuf<int32_t> dims, tile_sector, col;
+    pimi::DevBuf<pimi::ApplyTile> atiles, atiles4;
     pimi::DevBuf<pimi::StencilTermDev> terms;
     pimi::DevBuf<pimi::cx<double>> val64;
     pimi::DevBuf<pimi::cx<float>> val32;
@@ -89,6 +92,13 @@ struct pimi_plan {
     pimi::DevBuf<int32_t> dg_col;
     pimi::DevBuf<pimi::cx<double>> dg_val64;
     pimi::DevBuf<pimi::cx<float>> dg_val32;
+    // stencil of the adjoint Liouvillian (built on first use)
+    bool adj_ready = false;
+    pimi::DevBuf<int64_t> adj_term_base;
+    pimi::DevBuf<pimi::StencilTermDev> adj_terms;
+    pimi::DevBuf<int32_t> adj_col;
+    pimi::DevBuf<pimi::cx<double>> adj_val64;
+    pimi::DevBuf<pimi::cx<float>> adj_val32;
     // sector weighting u = S x, S = sqrt(f^nu) on sector nu (built on first use):
     // stencil terms of S L S^{-1} (scale times sigma_out / sigma_in), the
     // transformed trace functional (ones on the block diagonals) and 1/sigma

@@ -80,29 +80,110 @@ __device__ __forceinline__ cx<R> apply_coordinate(const cx<R> *__restrict__ xc, 
     return acc;
 }
 
-// y[i] = out_scale * (L x)[i] for one PI vector; one output coordinate per lane.
-// Tiles of kApplyBlock consecutive coordinates are spread so that XCD k
-// (workgroup id mod 8) handles one contiguous slab of tiles.
-template <typename R>
-__global__ __launch_bounds__(kApplyBlock) void k_apply(
-    const cx<R> *__restrict__ x, cx<R> *__restrict__ y, int64_t n, const int64_t *__restrict__ offs,
-    const int32_t *__restrict__ dims, const int32_t *__restrict__ tile_sector, int64_t ntiles,
-    int64_t tiles_per_xcd, const int64_t *__restrict__ term_base, const StencilTermDev *__restrict__ terms,
-    const int32_t *__restrict__ col, const cx<R> *__restrict__ val, R out_scale, int64_t ncols) {
-    const int64_t bid = blockIdx.x;
-    const int64_t tile = (bid % kNumXcd) * tiles_per_xcd + bid / kNumXcd;
-    if (tile >= ntiles) return;
-    const int64_t i = tile * kApplyBlock + threadIdx.x;
-    if (i >= n) return;
-    int s = tile_sector[tile];
-    while (i >= offs[s + 1]) ++s;
+// An apply tile: rows a0..a0+63 of columns b0..b0+nb-1 (nb <= kApplyCols) of
+// sector `sector`'s output block, one wave; each lane computes its row in the
+// nb columns.  Sector, columns and stencil terms are wave-uniform, so term
+// descriptors and the B-side (column) tables are scalar loads; the A-side
+// (row) tables are loaded once per term for all nb outputs of a lane, whose
+// input gathers are independent (memory-level parallelism).  Every output
+// sums its terms and ELL entries in the same order as apply_coordinate.
+struct ApplyTile {
+    int32_t sector, b0, a0, nb;
+};
+constexpr int kApplyWave = 64;
+constexpr int kApplyCols = 4;  // columns per lane of the batched apply (one for a single vector)
+
+template <typename R, int CB>
+__device__ __forceinline__ void apply_tile(const cx<R> *__restrict__ xc, cx<R> *__restrict__ yc, R out_scale,
+                                           ApplyTile tile, const int64_t *__restrict__ offs,
+                                           const int32_t *__restrict__ dims, const int64_t *__restrict__ term_base,
+                                           const StencilTermDev *__restrict__ terms,
+                                           const int32_t *__restrict__ col, const cx<R> *__restrict__ val) {
+    const int s = __builtin_amdgcn_readfirstlane(tile.sector);
+    const int64_t b0 = __builtin_amdgcn_readfirstlane(tile.b0);
+    const int nb = __builtin_amdgcn_readfirstlane(tile.nb);
     const int64_t m = dims[s];
-    const int64_t loc = i - offs[s];
-    const int64_t a = loc % m, b = loc / m;
-    for (int64_t cidx = blockIdx.y; cidx < ncols; cidx += gridDim.y) {
-        const cx<R> acc = apply_coordinate<R>(x + cidx * n, s, a, b, m, offs, dims, term_base, terms, col, val);
-        y[cidx * n + i] = cx<R>{out_scale * acc.x, out_scale * acc.y};
+    const int64_t arow = tile.a0 + (int64_t)(threadIdx.x & (kApplyWave - 1));
+    const bool active = arow < m;
+    const int64_t a = active ? arow : 0;  // inactive lanes evaluate row 0 (in bounds) and store nothing
+    cx<R> acc[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) acc[j] = cx<R>{0, 0};
+    const int64_t t0 = term_base[s], t1 = term_base[s + 1];
+    for (int64_t t = t0; t < t1; ++t) {
+        const StencilTermDev T = terms[t];
+        const cx<R> *Xn = xc + offs[T.in_sector];
+        const int64_t mn = dims[T.in_sector];
+        const cx<R> sc = {(R)T.s_re, (R)T.s_im};
+        cx<R> tacc[CB];
+#pragma unroll
+        for (int j = 0; j < CB; ++j) tacc[j] = cx<R>{0, 0};
+        if (T.flags & 1) {  // A identity: sum_kb conj(B[b, kb]) X[a, colB]
+            for (int kb = 0; kb < T.wb; ++kb) {
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    if (j < nb) {
+                        const int64_t q = T.b_base + kb * m + b0 + j;
+                        tacc[j] = cadd(tacc[j], cjmul(val[q], Xn[a + (int64_t)col[q] * mn]));
+                    }
+            }
+        } else if (T.flags & 2) {  // B identity: sum_ka A[a, ka] X[colA, b]
+            for (int ka = 0; ka < T.wa; ++ka) {
+                const int64_t q = T.a_base + ka * m + a;
+                const cx<R> v = val[q];
+                const int64_t c = col[q];
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    if (j < nb) tacc[j] = cadd(tacc[j], cmul(v, Xn[c + (b0 + j) * mn]));
+            }
+        } else {  // sum_kb conj(B[b, kb]) (sum_ka A[a, ka] X[colA, colB])
+            for (int kb = 0; kb < T.wb; ++kb) {
+                int64_t dcol[CB];
+                cx<R> inner[CB];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    dcol[j] = j < nb ? (int64_t)col[T.b_base + kb * m + b0 + j] * mn : 0;
+                    inner[j] = cx<R>{0, 0};
+                }
+                for (int ka = 0; ka < T.wa; ++ka) {
+                    const int64_t qa = T.a_base + ka * m + a;
+                    const cx<R> v = val[qa];
+                    const int64_t c = col[qa];
+#pragma unroll
+                    for (int j = 0; j < CB; ++j)
+                        if (j < nb) inner[j] = cadd(inner[j], cmul(v, Xn[c + dcol[j]]));
+                }
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    if (j < nb) tacc[j] = cadd(tacc[j], cjmul(val[T.b_base + kb * m + b0 + j], inner[j]));
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < CB; ++j) acc[j] = cadd(acc[j], cmul(sc, tacc[j]));
     }
+    if (active) {
+        const int64_t o = offs[s] + a;
+#pragma unroll
+        for (int j = 0; j < CB; ++j)
+            if (j < nb) yc[o + (b0 + j) * m] = cx<R>{out_scale * acc[j].x, out_scale * acc[j].y};
+    }
+}
+
+// y = out_scale * L x for ncols column vectors; one wave per single-column
+// tile (the most waves for one vector's latency hiding), tiles spread so
+// that XCD k (workgroup id mod 8) handles one contiguous slab.
+template <typename R>
+__global__ __launch_bounds__(kApplyWave) void k_apply(
+    const cx<R> *__restrict__ x, cx<R> *__restrict__ y, int64_t n, const ApplyTile *__restrict__ tiles, int64_t ntiles,
+    int64_t tiles_per_xcd, const int64_t *__restrict__ offs, const int32_t *__restrict__ dims,
+    const int64_t *__restrict__ term_base, const StencilTermDev *__restrict__ terms, const int32_t *__restrict__ col,
+    const cx<R> *__restrict__ val, R out_scale, int64_t ncols) {
+    const int64_t bid = blockIdx.x;
+    const int64_t t = (bid % kNumXcd) * tiles_per_xcd + bid / kNumXcd;
+    if (t >= ntiles) return;
+    const ApplyTile tile = tiles[t];
+    for (int64_t c = blockIdx.y; c < ncols; c += gridDim.y)
+        apply_tile<R, 1>(x + c * n, y + c * n, out_scale, tile, offs, dims, term_base, terms, col, val);
 }
 
 // y[i] = x[i] * w[sector of i], one coordinate per lane over the plan's tiles

@@ -673,6 +673,53 @@ void build_stencil(const FusedPlan &P, Stencil &st, bool sector_diagonal_only) {
         }
 }
 
+// The adjoint Liouvillian under the PI inner product (src/liouvillian.jl:3275-3293):
+// L X = -i[H, X] - {Q, X}/2 + sum_g c_g K X K^H + sum_branches s C X_nu C^H has
+// L^dagger Y = i[H, Y] - {Q, Y}/2 + sum_g conj(c_g) K^H Y K
+//              + sum_branches conj(s) C^H Y_lambda C  (into sector nu),
+// i.e. the fused plan with H -> -H, K -> K^H, each branch lambda <- nu turned
+// into nu <- lambda with C -> C^H, and conjugated scales.
+static CSC conj_transpose(const CSC &A) {
+    std::vector<cd> d = csc_to_dense(A), t((size_t)(A.m * A.n));
+    for (int64_t c = 0; c < A.n; ++c)
+        for (int64_t r = 0; r < A.m; ++r) t[(size_t)(c + r * A.n)] = std::conj(d[(size_t)(r + c * A.m)]);
+    return dense_to_csc(t, A.n, A.m);
+}
+
+void adjoint_plan(const FusedPlan &P, FusedPlan &A) {
+    A.ns = P.ns;
+    A.n = P.n;
+    A.dims = P.dims;
+    A.offs = P.offs;
+    A.has_h = P.has_h;
+    A.has_q = P.has_q;
+    A.H = P.H;
+    for (CSC &h : A.H)
+        for (cd &v : h.vals) v = -v;
+    A.Q = P.Q;
+    A.cg_scale.clear();
+    for (cd c : P.cg_scale) A.cg_scale.push_back(std::conj(c));
+    A.K.assign(P.K.size(), {});
+    for (size_t g = 0; g < P.K.size(); ++g)
+        for (const CSC &k : P.K[g]) A.K[g].push_back(conj_transpose(k));
+    A.ob_scale.clear();
+    for (cd c : P.ob_scale) A.ob_scale.push_back(std::conj(c));
+    A.ob.assign(P.ob.size(), {});
+    for (size_t g = 0; g < P.ob.size(); ++g)
+        for (const Branch &b : P.ob[g]) {
+            Branch c;
+            c.out = b.in;
+            c.in = b.out;
+            c.r = b.r;
+            const int64_t mo = P.dims[(size_t)b.out], mi = P.dims[(size_t)b.in];
+            c.dense.assign((size_t)(mi * mo), cd(0, 0));
+            for (int64_t j = 0; j < mi; ++j)
+                for (int64_t i = 0; i < mo; ++i)
+                    c.dense[(size_t)(j + i * mi)] = std::conj(b.dense[(size_t)(i + j * mo)]);
+            A.ob[g].push_back(std::move(c));
+        }
+}
+
 // ---------------------------------------------------------------------------
 // stationary-solve preconditioner factors
 // ---------------------------------------------------------------------------

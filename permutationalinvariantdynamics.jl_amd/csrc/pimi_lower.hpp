@@ -111,6 +111,10 @@ struct Stencil {
     int64_t entries_per_apply = 0;   // sum over outputs of wa*wb
 };
 
+// Fused plan of the adjoint Liouvillian under the PI inner product (geometry
+// not copied; enough for build_stencil).
+void adjoint_plan(const FusedPlan &plan, FusedPlan &adjoint);
+
 // sector_diagonal_only: keep only terms whose input sector is the output
 // sector (the sector-block-diagonal part of L, used by the inner solver).
 void build_stencil(const FusedPlan &plan, Stencil &st, bool sector_diagonal_only = false);

@@ -350,6 +350,21 @@ static void finalize_plan(pimi_plan *P) {
         while (i >= offs[s + 1]) ++s;
         tsec[(size_t)t] = s;
     }
+    // apply tiles: sector-major, then column groups, then 64-row chunks
+    auto make_tiles = [&](int cols) {
+        std::vector<ApplyTile> t;
+        for (int q = 0; q < F.ns; ++q) {
+            const int32_t m = (int32_t)F.dims[q];
+            for (int32_t b0 = 0; b0 < m; b0 += cols)
+                for (int32_t a0 = 0; a0 < m; a0 += kApplyWave)
+                    t.push_back(ApplyTile{q, b0, a0, std::min<int32_t>(cols, m - b0)});
+        }
+        return t;
+    };
+    std::vector<ApplyTile> at = make_tiles(1), at4 = make_tiles(kApplyCols);
+    P->natiles = (int64_t)at.size();
+    P->natiles4 = (int64_t)at4.size();
+    P->atiles_per_xcd = (P->natiles + kNumXcd - 1) / kNumXcd;
     std::vector<StencilTermDev> terms(P->st.terms.size());
     for (size_t k = 0; k < terms.size(); ++k) {
         const StencilTerm &a = P->st.terms[k];
@@ -377,6 +392,8 @@ static void finalize_plan(pimi_plan *P) {
     P->offs.upload(offs.data(), offs.size());
     P->dims.upload(dims.data(), dims.size());
     P->tile_sector.upload(tsec.data(), tsec.size());
+    P->atiles.upload(at.data(), at.size());
+    P->atiles4.upload(at4.data(), at4.size());
     P->term_base.upload(P->st.term_base.data(), P->st.term_base.size());
     P->terms.upload(terms.data(), terms.size());
     P->col.upload(P->st.col.data(), P->st.col.size());
@@ -413,20 +430,69 @@ static void ensure_diag_stencil(pimi_plan *P) {
 }
 
 // y = out_scale * L x (diag = false) or out_scale * D x, D the sector-block-diagonal part of L
+static void ensure_adjoint_stencil(pimi_plan *P) {
+    if (P->adj_ready) return;
+    FusedPlan A;
+    adjoint_plan(P->host, A);
+    Stencil st;
+    build_stencil(A, st);
+    std::vector<StencilTermDev> terms(st.terms.size());
+    for (size_t k = 0; k < terms.size(); ++k) {
+        const StencilTerm &a = st.terms[k];
+        terms[k] = StencilTermDev{a.in_sector, a.flags, a.wa, a.wb, a.a_base, a.b_base, a.s_re, a.s_im};
+    }
+    std::vector<cx<double>> v64(st.val.size());
+    std::vector<cx<float>> v32(st.val.size());
+    for (size_t k = 0; k < v64.size(); ++k) {
+        v64[k] = cx<double>{st.val[k].real(), st.val[k].imag()};
+        v32[k] = cx<float>{(float)st.val[k].real(), (float)st.val[k].imag()};
+    }
+    P->adj_term_base.upload(st.term_base.data(), st.term_base.size());
+    P->adj_terms.upload(terms.data(), terms.size());
+    P->adj_col.upload(st.col.data(), st.col.size());
+    P->adj_val64.upload(v64.data(), v64.size());
+    P->adj_val32.upload(v32.data(), v32.size());
+    P->adj_ready = true;
+}
+
+enum class ApplyOp { Full, Diag, Adjoint };
+
 // `weighted`: the sector-weighted operator S L S^{-1} (full stencil only; the
 // sector-diagonal part D is unchanged by a sector-constant S)
 template <typename R>
 static void launch_apply(pimi_workspace *W, const cx<R> *x, cx<R> *y, double out_scale, int64_t ncols = 1,
-                         bool diag = false, bool weighted = false) {
+                         bool diag = false, bool weighted = false, bool adjoint = false) {
     pimi_plan *P = W->plan;
+    const ApplyOp op = adjoint ? ApplyOp::Adjoint : (diag ? ApplyOp::Diag : ApplyOp::Full);
     const cx<R> *val;
-    if constexpr (sizeof(R) == 8) val = diag ? P->dg_val64.p : P->val64.p;
-    else val = diag ? P->dg_val32.p : P->val32.p;
-    const StencilTermDev *terms = diag ? P->dg_terms.p : (weighted ? P->terms_w.p : P->terms.p);
-    dim3 grid((unsigned)(P->tiles_per_xcd * kNumXcd), (unsigned)std::min<int64_t>(ncols, 65535));
-    hipLaunchKernelGGL(k_apply<R>, grid, dim3(kApplyBlock), 0, W->stream, x, y, P->n, P->offs.p, P->dims.p,
-                       P->tile_sector.p, P->ntiles, P->tiles_per_xcd, diag ? P->dg_term_base.p : P->term_base.p,
-                       terms, diag ? P->dg_col.p : P->col.p, val, (R)out_scale, ncols);
+    const int64_t *term_base;
+    const StencilTermDev *terms;
+    const int32_t *col;
+    switch (op) {
+        case ApplyOp::Diag:
+            if constexpr (sizeof(R) == 8) val = P->dg_val64.p;
+            else val = P->dg_val32.p;
+            term_base = P->dg_term_base.p;
+            terms = P->dg_terms.p;
+            col = P->dg_col.p;
+            break;
+        case ApplyOp::Adjoint:
+            if constexpr (sizeof(R) == 8) val = P->adj_val64.p;
+            else val = P->adj_val32.p;
+            term_base = P->adj_term_base.p;
+            terms = P->adj_terms.p;
+            col = P->adj_col.p;
+            break;
+        default:
+            if constexpr (sizeof(R) == 8) val = P->val64.p;
+            else val = P->val32.p;
+            term_base = P->term_base.p;
+            terms = weighted ? P->terms_w.p : P->terms.p;
+            col = P->col.p;
+    }
+    dim3 grid((unsigned)(P->atiles_per_xcd * kNumXcd), (unsigned)std::min<int64_t>(ncols, 65535));
+    hipLaunchKernelGGL(k_apply<R>, grid, dim3(kApplyWave), 0, W->stream, x, y, P->n, P->atiles.p, P->natiles,
+                       P->atiles_per_xcd, P->offs.p, P->dims.p, term_base, terms, col, val, (R)out_scale, ncols);
     HIPCHK(hipGetLastError());
 }
 
@@ -1771,20 +1837,22 @@ void pimi_workspace_destroy(pimi_workspace *W) {
     delete W;
 }
 
-pimi_status pimi_apply(pimi_workspace *W, const void *x, void *y, int64_t ncols, int32_t on_device) {
+static pimi_status apply_entry(pimi_workspace *W, const void *x, void *y, int64_t ncols, int32_t on_device,
+                               bool adjoint) {
     return guarded([&] {
         if (!W || !x || !y) throw Error(PIMI_EINVAL, "null argument");
         if (ncols < 1) throw Error(PIMI_EINVAL, "ncols must be positive");
         if (x == y) throw Error(PIMI_EINVAL, "Liouvillian source and destination must not alias");
         BusyGuard guard(W->busy);
         HIPCHK(hipSetDevice(W->plan->ctx->device));
+        if (adjoint) ensure_adjoint_stencil(W->plan);
         const int64_t n = W->plan->n;
         if (on_device) {
             HIPCHK(hipEventRecord(W->ev0, W->stream));
             if (W->prec == PIMI_C64)
-                launch_apply<double>(W, (const cx<double> *)x, (cx<double> *)y, 1.0, ncols);
+                launch_apply<double>(W, (const cx<double> *)x, (cx<double> *)y, 1.0, ncols, false, false, adjoint);
             else
-                launch_apply<float>(W, (const cx<float> *)x, (cx<float> *)y, 1.0, ncols);
+                launch_apply<float>(W, (const cx<float> *)x, (cx<float> *)y, 1.0, ncols, false, false, adjoint);
             HIPCHK(hipEventRecord(W->ev1, W->stream));
             sync(W);
         } else {
@@ -1794,7 +1862,7 @@ pimi_status pimi_apply(pimi_workspace *W, const void *x, void *y, int64_t ncols,
             HIPCHK(hipMemcpyAsync(dx.p, x, sizeof(cx<double>) * n * ncols, hipMemcpyHostToDevice, W->stream));
             if (W->prec == PIMI_C64) {
                 HIPCHK(hipEventRecord(W->ev0, W->stream));
-                launch_apply<double>(W, dx.p, dy.p, 1.0, ncols);
+                launch_apply<double>(W, dx.p, dy.p, 1.0, ncols, false, false, adjoint);
                 HIPCHK(hipEventRecord(W->ev1, W->stream));
             } else {
                 DevBuf<cx<float>> fx, fy;
@@ -1803,7 +1871,7 @@ pimi_status pimi_apply(pimi_workspace *W, const void *x, void *y, int64_t ncols,
                 hipLaunchKernelGGL((k_convert<float, double>), dim3(1024), dim3(256), 0, W->stream, fx.p, dx.p,
                                    n * ncols);
                 HIPCHK(hipEventRecord(W->ev0, W->stream));
-                launch_apply<float>(W, fx.p, fy.p, 1.0, ncols);
+                launch_apply<float>(W, fx.p, fy.p, 1.0, ncols, false, false, adjoint);
                 HIPCHK(hipEventRecord(W->ev1, W->stream));
                 hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, W->stream, dy.p, fy.p,
                                    n * ncols);
@@ -1816,6 +1884,14 @@ pimi_status pimi_apply(pimi_workspace *W, const void *x, void *y, int64_t ncols,
         HIPCHK(hipEventElapsedTime(&ms, W->ev0, W->ev1));
         W->last_apply_ms = ms;
     });
+}
+
+pimi_status pimi_apply(pimi_workspace *W, const void *x, void *y, int64_t ncols, int32_t on_device) {
+    return apply_entry(W, x, y, ncols, on_device, false);
+}
+
+pimi_status pimi_apply_adjoint(pimi_workspace *W, const void *x, void *y, int64_t ncols, int32_t on_device) {
+    return apply_entry(W, x, y, ncols, on_device, true);
 }
 
 pimi_status pimi_apply_bench(pimi_workspace *W, int64_t ncols, int32_t reps, double *ms_avg) {
@@ -1895,6 +1971,35 @@ pimi_status pimi_precond_apply(pimi_workspace *W, const double *x, double *y, do
             apply_precond(W, W->tmp.p, W->w.p, scale);
         }
         d2h(W, y, W->w.p, sizeof(cx<double>) * n);
+    });
+}
+
+pimi_status pimi_precond_bench(pimi_workspace *W, int32_t reps, double *ms_avg, double *flops) {
+    return guarded([&] {
+        if (!W || reps < 1 || !ms_avg || !flops) throw Error(PIMI_EINVAL, "invalid argument");
+        BusyGuard guard(W->busy);
+        HIPCHK(hipSetDevice(W->plan->ctx->device));
+        ensure_preconditioner(W);
+        const int64_t n = W->plan->n;
+        HIPCHK(hipMemsetAsync(W->w.p, 0x3f, sizeof(cx<double>) * n, W->stream));
+        auto once = [&] {
+            if (W->prec == PIMI_C32)
+                apply_precond_t<float>(W, (const cx<float> *)W->w.p, (cx<float> *)W->zbuf.p, 1.0,
+                                       (cx<float> *)W->pt1.p, (cx<float> *)W->pt2.p);
+            else
+                apply_precond_t<double>(W, W->w.p, W->zbuf.p, 1.0, W->pt1.p, W->pt2.p);
+        };
+        once();  // warm-up
+        HIPCHK(hipEventRecord(W->ev0, W->stream));
+        for (int32_t r = 0; r < reps; ++r) once();
+        HIPCHK(hipEventRecord(W->ev1, W->stream));
+        HIPCHK(hipEventSynchronize(W->ev1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, W->ev0, W->ev1));
+        *ms_avg = ms / reps;
+        double m3 = 0;
+        for (auto m : W->plan->host.dims) m3 += (double)m * m * m;
+        *flops = 4 * 8.0 * m3;
     });
 }
 

@@ -392,14 +392,16 @@ class CompiledPIModel:
         N.check(N.lib().pimi_collective_operator(self._plan, N.ptr(X), N.ptr(out)))
         return PIOperator(self.basis, out)
 
-    def apply(self, x, precision=N.PIMI_C64):
-        """y = L x on the device (host vectors in and out; one or more columns)."""
+    def apply(self, x, precision=N.PIMI_C64, adjoint=False):
+        """y = L x (or L^dagger x, the apply_adjoint! of src/liouvillian.jl:3275)
+        on the device (host vectors in and out; one or more columns)."""
         x = np.asarray(x, dtype=np.complex128)
         cols = 1 if x.ndim == 1 else x.shape[1]
         xf = np.ascontiguousarray(x.reshape(len(self.basis), cols).T)  # column-major blocks
         y = np.empty_like(xf)
         ws = self.workspace(precision)
-        N.check(N.lib().pimi_apply(ws, N.ptr(xf), N.ptr(y), C.c_int64(cols), C.c_int32(0)))
+        fn = N.lib().pimi_apply_adjoint if adjoint else N.lib().pimi_apply
+        N.check(fn(ws, N.ptr(xf), N.ptr(y), C.c_int64(cols), C.c_int32(0)))
         y = y.T
         return y.reshape(x.shape)
 
