@@ -23,7 +23,8 @@ struct PimiTerm;  kind::Int32; reserved::Int32; op::Ptr{ComplexF64}; op2::Ptr{Co
 struct PimiModel; N::Int32; d::Int32; nterms::Int32; reserved::Int32; terms::Ptr{PimiTerm}; end
 struct PimiGmresOpts
     krylovdim::Int32; preconditioner::Int32; maxiter::Int64; atol::Float64; rtol::Float64
-    operator_scale::Float64; inner_rtol::Float64; max_refinements::Int64; criterion::Int32; reserved2::Int32
+    operator_scale::Float64; inner_rtol::Float64; max_refinements::Int64; criterion::Int32; inner_steps::Int32
+    weighting::Int32; reserved::Int32
 end
 struct PimiExpvOpts; krylovdim::Int32; reserved::Int32; atol::Float64; rtol::Float64; safety::Float64; max_steps::Int64; end
 mutable struct PimiInfo
