@@ -104,7 +104,7 @@ function PID.steady_state(A::MI355XOperator; krylovdim=30, maxiter=500, atol=1e-
                           operator_scale=nothing, preconditioner=nothing, return_info=false, kwargs...)
     ws = _workspace(A, Int32(0), krylovdim)
     opts = PimiGmresOpts(krylovdim, preconditioner === :sector ? 1 : 0, maxiter, atol, rtol,
-                         something(operator_scale, 0.0), 1e-5, 50, 0, 0)
+                         something(operator_scale, 0.0), 1e-5, 50, 0, 0, 0, 0)
     rho = zeros(ComplexF64, length(A.basis)); info = PimiInfo()
     _check(ccall((:pimi_steady_gmres, libpimi), Int32, (Ptr{Cvoid}, Ref{PimiGmresOpts}, Ptr{ComplexF64}, Ref{PimiInfo}),
                  ws, opts, rho, info))
