@@ -278,6 +278,28 @@ pimi_status pimi_precond_bench(pimi_workspace *ws, int32_t reps, double *ms_avg,
 pimi_status pimi_workspace_last_apply_ms(const pimi_workspace *ws, double *ms);
 
 /* ---------------------------------------------------------------------------
+ * Conditional (no-jump) trajectory cohorts (src/batched_trajectories.jl:266-598):
+ * ncols PI states in one n x ncols column-major matrix (host, or device with
+ * on_device = 1), complex FP64 workspace.  Jump channels are the model's jump
+ * terms in term order, numbered 1..count.  Intensities follow the reference's
+ * sign check (nonfinite or below -1e-11: PIMI_EINVAL) and are clamped at zero.
+ * ------------------------------------------------------------------------- */
+pimi_status pimi_trajectory_channels(const pimi_plan *plan, int32_t *count);
+/* batched_conditional_action!: Y = -i[H, X] - {E, X}/2 + lambda X per column,
+ * E = sum_k rate_k L_k^dagger L_k, intensities[c] = lambda of column c. */
+pimi_status pimi_conditional_action(pimi_workspace *ws, const void *X, void *Y, int64_t ncols, double *intensities,
+                                    int32_t on_device);
+/* batched_conditional_rk4!: one fixed RK4 step of size h for every column,
+ * then each column trace-normalized (X in place). */
+pimi_status pimi_conditional_rk4(pimi_workspace *ws, void *X, int64_t ncols, double h, int32_t on_device);
+/* batched_channel_intensities!: rates[k + c*count] = intensity of channel k+1 in column c. */
+pimi_status pimi_channel_intensities(pimi_workspace *ws, const void *X, int64_t ncols, double *rates,
+                                     int32_t on_device);
+/* batched_apply_jumps!: column c with channels[c] = k > 0 becomes gain_k(X_c) / (t . gain_k(X_c));
+ * channels[c] = 0 leaves it unchanged.  Columns of one channel go through one multi-column apply. */
+pimi_status pimi_apply_jumps(pimi_workspace *ws, void *X, int64_t ncols, const int32_t *channels, int32_t on_device);
+
+/* ---------------------------------------------------------------------------
  * Batch engine: many independent stationary solves advanced together on one
  * device, one kernel launch per operation for all of them.  No counterpart in
  * the reference, which solves one model at a time; each job runs the

@@ -6,6 +6,7 @@
 
 #include <atomic>
 #include <complex>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -52,6 +53,33 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// A stencil's device tables (FP64 and FP32 values).
+struct DevStencil {
+    DevBuf<int64_t> term_base;
+    DevBuf<StencilTermDev> terms;
+    DevBuf<int32_t> col;
+    DevBuf<cx<double>> val64;
+    DevBuf<cx<float>> val32;
+    void upload(const Stencil &st) {
+        std::vector<StencilTermDev> t(st.terms.size());
+        for (size_t k = 0; k < t.size(); ++k) {
+            const StencilTerm &a = st.terms[k];
+            t[k] = StencilTermDev{a.in_sector, a.flags, a.wa, a.wb, a.a_base, a.b_base, a.s_re, a.s_im};
+        }
+        std::vector<cx<double>> v64(st.val.size());
+        std::vector<cx<float>> v32(st.val.size());
+        for (size_t k = 0; k < v64.size(); ++k) {
+            v64[k] = cx<double>{st.val[k].real(), st.val[k].imag()};
+            v32[k] = cx<float>{(float)st.val[k].real(), (float)st.val[k].imag()};
+        }
+        term_base.upload(st.term_base.data(), st.term_base.size());
+        terms.upload(t.data(), t.size());
+        col.upload(st.col.data(), st.col.size());
+        val64.upload(v64.data(), v64.size());
+        val32.upload(v32.data(), v32.size());
+    }
+};
+
 }  // namespace pimi
 
 struct pimi_context {
@@ -92,6 +120,13 @@ struct pimi_plan {
     pimi::DevBuf<int32_t> dg_col;
     pimi::DevBuf<pimi::cx<double>> dg_val64;
     pimi::DevBuf<pimi::cx<float>> dg_val32;
+    // conditional (no-jump) trajectory tables (built on first use): the
+    // H/Q-only stencil, one gain stencil per jump channel, and conj of the
+    // intensity weights (n complex each; the total first, then per channel)
+    bool traj_ready = false;
+    pimi::DevStencil cond;
+    std::vector<std::unique_ptr<pimi::DevStencil>> gain;
+    pimi::DevBuf<pimi::cx<double>> ebar;
     // stencil of the adjoint Liouvillian (built on first use)
     bool adj_ready = false;
     pimi::DevBuf<int64_t> adj_term_base;

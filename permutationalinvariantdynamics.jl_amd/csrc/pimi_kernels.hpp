@@ -341,21 +341,40 @@ __device__ __forceinline__ void gemm_tile_mfma(const cx<R> *__restrict__ A, cons
     const int li = lane & 15, lk = lane >> 4;
     const int64_t gi = row0 + li, gj = col0 + li;
     typename Ops::acc_t cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-    for (int64_t k0 = 0; k0 < m; k0 += 4) {
-        const int64_t gk = k0 + lk;
-        cx<R> a = {0, 0}, b = {0, 0};
+    auto load_a = [&](int64_t gk) {
+        cx<R> a = {0, 0};
         if (gi < m && gk < m) {
             if (transA) { a = A[base + gk + gi * m]; a.y = -a.y; }
             else a = A[base + gi + gk * m];
         }
+        return a;
+    };
+    auto load_b = [&](int64_t gk) {
+        cx<R> b = {0, 0};
         if (gk < m && gj < m) {
             if (transB) { b = B[base + gj + gk * m]; b.y = -b.y; }
             else b = B[base + gk + gj * m];
         }
-        cr = Ops::mac(a.x, b.x, cr);
-        cr = Ops::mac(-a.y, b.y, cr);
-        ci = Ops::mac(a.x, b.y, ci);
-        ci = Ops::mac(a.y, b.x, ci);
+        return b;
+    };
+    // K in groups of kUnroll MFMA steps: the operands of a whole group are
+    // loaded before its first MFMA, so kUnroll load pairs are in flight per
+    // wait instead of one (the k-loop is latency-bound at these sizes).
+    constexpr int kUnroll = 4;
+    for (int64_t k0 = 0; k0 < m; k0 += 4 * kUnroll) {
+        cx<R> a[kUnroll], b[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            a[u] = load_a(k0 + 4 * u + lk);
+            b[u] = load_b(k0 + 4 * u + lk);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            cr = Ops::mac(a[u].x, b[u].x, cr);
+            cr = Ops::mac(-a[u].y, b[u].y, cr);
+            ci = Ops::mac(a[u].x, b[u].y, ci);
+            ci = Ops::mac(a[u].y, b[u].x, ci);
+        }
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {

@@ -508,6 +508,7 @@ void lower_model(int N, int d, const std::vector<TermDesc> &terms, FusedPlan &P)
                 blocks[s] = G.collective_block(s, t.op.data());
                 Q[s] = gram(blocks[s]);
             }
+            P.channels.push_back(JumpChannel{0, (int32_t)P.K.size(), t.rate, Q});
             losses.emplace_back(cd(t.rate, 0.0), std::move(Q));
             P.cg_scale.push_back(cd(t.rate, 0.0));
             P.K.push_back(std::move(blocks));
@@ -523,6 +524,7 @@ void lower_model(int N, int d, const std::vector<TermDesc> &terms, FusedPlan &P)
                 }
             std::vector<CSC> Q(P.ns);
             for (int s = 0; s < P.ns; ++s) Q[s] = G.collective_block(s, LdL.data());
+            P.channels.push_back(JumpChannel{1, (int32_t)P.ob.size(), t.rate, Q});
             losses.emplace_back(cd(t.rate, 0.0), std::move(Q));
             std::vector<Branch> brs;
             std::vector<cd> tmp;

@@ -68,8 +68,18 @@ struct Geometry {
     CSC collective_block(int s, const cd *X) const;
 };
 
+// One jump channel of the model, in term order (src/trajectories.jl): its gain
+// (kind 0: collective gain K[index]; 1: one-body gain ob[index]), rate and
+// unscaled loss blocks L^dagger L per sector.
+struct JumpChannel {
+    int32_t kind = 0, index = 0;
+    double rate = 0.0;
+    std::vector<CSC> q;
+};
+
 struct FusedPlan {
     Geometry geom;
+    std::vector<JumpChannel> channels;
     int ns = 0;
     int64_t n = 0;
     std::vector<int64_t> dims, offs;

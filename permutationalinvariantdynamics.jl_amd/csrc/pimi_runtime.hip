@@ -328,6 +328,9 @@ struct pimi_workspace {
     double *hpin = nullptr;  // pinned host staging (small results)
     bool trace_progress = false;  // PIMI_TRACE=1: solver progress lines on stderr
     bool weighted = false;        // current stationary solve iterates on u = S x
+    // conditional-trajectory buffers (n x capacity each) and per-column scalars
+    int64_t traj_cap = 0;
+    pimi::DevBuf<pimi::cx<double>> tX, tY, tTmp, tK1, tK2, tPartial, tH, tZ, tLam;
 };
 
 namespace pimi {
@@ -1514,6 +1517,202 @@ static void expv_series(pimi_workspace *W, const double *rho0, const double *ts,
 // ---------------------------------------------------------------------------
 using namespace pimi;
 
+// ---------------------------------------------------------------------------
+// conditional (no-jump) trajectories: batched_conditional_action!,
+// batched_conditional_rk4!, batched_channel_intensities!, batched_apply_jumps!
+// (src/batched_trajectories.jl:266-598).  The conditional drift of a state X is
+//   -i[H, X] - {E, X}/2 + lambda X,   E = sum_k rate_k L_k^dagger L_k,
+//   lambda = sum_nu sqrt(f^nu) Re tr(E_nu X_nu)  (total jump intensity),
+// i.e. the left/right (H, Q) part of the stencil plus lambda X; a jump in
+// channel k maps X to gain_k(X) / (t . gain_k(X)).
+// ---------------------------------------------------------------------------
+__global__ void k_cols_axpy_lam(cx<double> *__restrict__ Y, const cx<double> *__restrict__ X,
+                                const cx<double> *__restrict__ h, cx<double> *__restrict__ lam_out, int64_t n,
+                                int64_t ncols) {
+    const int64_t total = n * ncols;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = k / n;
+        const double lam = fmax(h[c].x, 0.0);  // intensities are clamped at zero after the sign check
+        Y[k] = cx<double>{Y[k].x + lam * X[k].x, Y[k].y + lam * X[k].y};
+        if (k % n == 0 && lam_out) lam_out[c] = h[c];
+    }
+}
+
+// out = a + s b over len entries (RK4 stage combinations)
+__global__ void k_lincomb(cx<double> *__restrict__ out, const cx<double> *a, const cx<double> *b, double s,
+                          int64_t len) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < len; k += (int64_t)gridDim.x * blockDim.x) {
+        const cx<double> av = a[k], bv = b[k];
+        out[k] = cx<double>{av.x + s * bv.x, av.y + s * bv.y};
+    }
+}
+
+// z[c] = t . X[:, c] on the trace support, one block per column
+__global__ __launch_bounds__(256) void k_cols_trace(const cx<double> *__restrict__ X, int64_t n,
+                                                    const int64_t *__restrict__ idx, const double *__restrict__ t,
+                                                    int64_t nt, cx<double> *__restrict__ z) {
+    __shared__ double sh[256 / 64];
+    const cx<double> *x = X + (int64_t)blockIdx.x * n;
+    double re = 0, im = 0;
+    for (int64_t k = threadIdx.x; k < nt; k += 256) {
+        re += t[k] * x[idx[k]].x;
+        im += t[k] * x[idx[k]].y;
+    }
+    re = block_sum(re, sh);
+    __syncthreads();
+    im = block_sum(im, sh);
+    if (threadIdx.x == 0) z[blockIdx.x] = cx<double>{re, im};
+}
+
+// X[:, c] /= z[c]
+__global__ void k_cols_div(cx<double> *__restrict__ X, const cx<double> *__restrict__ z, int64_t n, int64_t ncols) {
+    const int64_t total = n * ncols;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
+        const cx<double> d = z[k / n], v = X[k];
+        const double den = d.x * d.x + d.y * d.y;
+        X[k] = cx<double>{(v.x * d.x + v.y * d.y) / den, (v.y * d.x - v.x * d.y) / den};
+    }
+}
+
+static void ensure_trajectory_tables(pimi_plan *P) {
+    if (P->traj_ready) return;
+    const FusedPlan &F = P->host;
+    if (!P->compiled || F.geom.sqrt_f.empty())
+        throw Error(PIMI_EINVAL, "trajectory operations need a plan compiled from a model");
+    auto skeleton = [&](FusedPlan &G) {
+        G.ns = F.ns;
+        G.n = F.n;
+        G.dims = F.dims;
+        G.offs = F.offs;
+    };
+    {   // the H/Q-only (conditional) stencil
+        FusedPlan C;
+        skeleton(C);
+        C.has_h = F.has_h;
+        C.has_q = F.has_q;
+        C.H = F.H;
+        C.Q = F.Q;
+        Stencil st;
+        build_stencil(C, st);
+        P->cond.upload(st);
+    }
+    P->gain.clear();
+    for (const JumpChannel &ch : F.channels) {  // one gain stencil per channel, scaled by its rate
+        FusedPlan G;
+        skeleton(G);
+        if (ch.kind == 0) {
+            G.cg_scale = {F.cg_scale[(size_t)ch.index]};
+            G.K = {F.K[(size_t)ch.index]};
+        } else {
+            G.ob_scale = {F.ob_scale[(size_t)ch.index]};
+            G.ob = {F.ob[(size_t)ch.index]};
+        }
+        Stencil st;
+        build_stencil(G, st);
+        std::unique_ptr<DevStencil> ds(new DevStencil());
+        ds->upload(st);
+        P->gain.push_back(std::move(ds));
+    }
+    // intensity weights: tr(E_nu X_nu) = sum_{r,c} E[r,c] X[c,r], X[c,r] at offs + c + r m
+    const size_t nch = F.channels.size(), n = (size_t)F.n;
+    std::vector<cx<double>> eb((nch + 1) * n, cx<double>{0, 0});
+    auto fill = [&](size_t slot, const std::vector<CSC> &Qs, double scale) {
+        for (int s = 0; s < F.ns; ++s) {
+            const double w = F.geom.sqrt_f[(size_t)s] * scale;
+            const int64_t m = F.dims[(size_t)s];
+            const CSC &q = Qs[(size_t)s];
+            for (int64_t c = 0; c < q.n; ++c)
+                for (int64_t p = q.colptr[(size_t)c]; p < q.colptr[(size_t)c + 1]; ++p) {
+                    const int64_t rr = q.rows[(size_t)p];
+                    const cd e = w * q.vals[(size_t)p];
+                    eb[slot * n + (size_t)(F.offs[(size_t)s] + c + rr * m)] = cx<double>{e.real(), -e.imag()};
+                }
+        }
+    };
+    if (F.has_q) fill(0, F.Q, 1.0);
+    for (size_t k = 0; k < nch; ++k) fill(k + 1, F.channels[k].q, F.channels[k].rate);
+    P->ebar.upload(eb.data(), eb.size());
+    P->traj_ready = true;
+}
+
+static void ensure_traj_ws(pimi_workspace *W, int64_t ncols) {
+    if (W->traj_cap >= ncols) return;
+    const size_t len = (size_t)W->plan->n * (size_t)ncols;
+    W->tX.alloc(len);
+    W->tY.alloc(len);
+    W->tTmp.alloc(len);
+    W->tK1.alloc(len);
+    W->tK2.alloc(len);
+    W->tPartial.alloc((size_t)kDotBlocks * (size_t)ncols);
+    W->tH.alloc((size_t)ncols);
+    W->tZ.alloc((size_t)ncols);
+    W->tLam.alloc((size_t)4 * ncols);
+    W->traj_cap = ncols;
+}
+
+static void launch_stencil(pimi_workspace *W, const DevStencil &S, const cx<double> *x, cx<double> *y,
+                           int64_t ncols) {
+    pimi_plan *P = W->plan;
+    dim3 grid((unsigned)(P->atiles_per_xcd * kNumXcd), (unsigned)std::min<int64_t>(ncols, 65535));
+    hipLaunchKernelGGL(k_apply<double>, grid, dim3(kApplyWave), 0, W->stream, x, y, P->n, P->atiles.p, P->natiles,
+                       P->atiles_per_xcd, P->offs.p, P->dims.p, S.term_base.p, S.terms.p, S.col.p, S.val64.p, 1.0,
+                       ncols);
+    HIPCHK(hipGetLastError());
+}
+
+// h[c] = sum_i conj(X[i, c]) ebar[i] = conj(sum_i X e), whose real part is the intensity
+static void cols_intensity(pimi_workspace *W, const cx<double> *X, int64_t ncols, const cx<double> *ebar,
+                           cx<double> *h) {
+    const int64_t n = W->plan->n;
+    hipLaunchKernelGGL(k_dots_partial<double>, dim3(kDotBlocks, (unsigned)ncols), dim3(kRedBlock), 0, W->stream, X,
+                       n, ebar, n, W->tPartial.p);
+    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)ncols), dim3(kRedBlock), 0, W->stream, W->tPartial.p,
+                       kDotBlocks, h, (cx<double> *)nullptr);
+}
+
+// Y = conditional drift of X; lam (ncols) receives the unclamped total intensities
+static void conditional_action(pimi_workspace *W, const cx<double> *X, cx<double> *Y, int64_t ncols,
+                               cx<double> *lam) {
+    pimi_plan *P = W->plan;
+    const int64_t n = P->n;
+    launch_stencil(W, P->cond, X, Y, ncols);
+    cols_intensity(W, X, ncols, P->ebar.p, W->tH.p);
+    hipLaunchKernelGGL(k_cols_axpy_lam, dim3(grid_for(n * ncols)), dim3(kRedBlock), 0, W->stream, Y, X, W->tH.p, lam,
+                       n, ncols);
+}
+
+// Intensities are finite and not below -max(1e-11, 100 eps) (src/trajectories.jl:755)
+static void check_intensities(const std::vector<cx<double>> &h, const char *what) {
+    for (size_t c = 0; c < h.size(); ++c) {
+        if (!std::isfinite(h[c].x)) throw Error(PIMI_EINVAL, std::string(what) + " is nonfinite");
+        if (h[c].x < -1e-11) throw Error(PIMI_EINVAL, std::string(what) + " has a negative trace");
+    }
+}
+
+// host (on_device = 0) or device input/output of n x ncols complex doubles
+struct TrajIO {
+    pimi_workspace *W;
+    int64_t count;
+    bool on_device;
+    cx<double> *dev(const void *p, DevBuf<cx<double>> &buf) {
+        if (on_device) return (cx<double> *)p;
+        HIPCHK(hipMemcpyAsync(buf.p, p, sizeof(cx<double>) * count, hipMemcpyHostToDevice, W->stream));
+        return buf.p;
+    }
+    void back(void *p, const cx<double> *d) {
+        if (on_device) return;
+        HIPCHK(hipMemcpyAsync(p, d, sizeof(cx<double>) * count, hipMemcpyDeviceToHost, W->stream));
+    }
+};
+
+static void traj_prologue(pimi_workspace *W, int64_t ncols) {
+    if (W->prec != PIMI_C64) throw Error(PIMI_EINVAL, "trajectory operations need a complex FP64 workspace");
+    if (ncols < 1 || ncols > 65535) throw Error(PIMI_EINVAL, "ncols must lie in 1..65535");
+    HIPCHK(hipSetDevice(W->plan->ctx->device));
+    ensure_trajectory_tables(W->plan);
+    ensure_traj_ws(W, ncols);
+}
+
 extern "C" {
 
 const char *pimi_last_error(void) { return g_last_error.c_str(); }
@@ -2031,6 +2230,136 @@ pimi_status pimi_expv_series(pimi_workspace *W, const double *rho0, const double
         BusyGuard guard(W->busy);
         HIPCHK(hipSetDevice(W->plan->ctx->device));
         expv_series(W, rho0, ts, nts, opts, nobs, obs, obs_out, rho_final, info);
+    });
+}
+
+pimi_status pimi_trajectory_channels(const pimi_plan *P, int32_t *count) {
+    return guarded([&] {
+        if (!P || !count) throw Error(PIMI_EINVAL, "null argument");
+        *count = (int32_t)P->host.channels.size();
+    });
+}
+
+pimi_status pimi_conditional_action(pimi_workspace *W, const void *X, void *Y, int64_t ncols, double *intensities,
+                                    int32_t on_device) {
+    return guarded([&] {
+        if (!W || !X || !Y || !intensities) throw Error(PIMI_EINVAL, "null argument");
+        if (X == Y) throw Error(PIMI_EINVAL, "batched trajectory source and destination must not alias");
+        BusyGuard guard(W->busy);
+        traj_prologue(W, ncols);
+        TrajIO io{W, W->plan->n * ncols, on_device != 0};
+        const cx<double> *dx = io.dev(X, W->tX);
+        cx<double> *dy = on_device ? (cx<double> *)Y : W->tY.p;
+        conditional_action(W, dx, dy, ncols, W->tLam.p);
+        std::vector<cx<double>> h((size_t)ncols);
+        HIPCHK(hipMemcpyAsync(h.data(), W->tLam.p, sizeof(cx<double>) * ncols, hipMemcpyDeviceToHost, W->stream));
+        io.back(Y, dy);
+        sync(W);
+        check_intensities(h, "total jump intensity");
+        for (int64_t c = 0; c < ncols; ++c) intensities[c] = std::max(0.0, h[(size_t)c].x);
+    });
+}
+
+pimi_status pimi_conditional_rk4(pimi_workspace *W, void *X, int64_t ncols, double h, int32_t on_device) {
+    return guarded([&] {
+        if (!W || !X) throw Error(PIMI_EINVAL, "null argument");
+        if (!(h > 0) || !std::isfinite(h)) throw Error(PIMI_EINVAL, "batched trajectory step must be positive");
+        BusyGuard guard(W->busy);
+        traj_prologue(W, ncols);
+        pimi_plan *P = W->plan;
+        const int64_t len = P->n * ncols;
+        TrajIO io{W, len, on_device != 0};
+        cx<double> *x = io.dev(X, W->tX);
+        cx<double> *tmp = W->tTmp.p, *k1 = W->tK1.p, *k2 = W->tK2.p, *lam = W->tLam.p;
+        const dim3 g(grid_for(len)), b(kRedBlock);
+        // the stage sequence of batched_conditional_rk4! (src/batched_trajectories.jl:335-368)
+        conditional_action(W, x, k1, ncols, lam);
+        HIPCHK(hipMemcpyAsync(k2, k1, sizeof(cx<double>) * len, hipMemcpyDeviceToDevice, W->stream));
+        hipLaunchKernelGGL(k_lincomb, g, b, 0, W->stream, tmp, x, k1, h / 2, len);
+        conditional_action(W, tmp, k1, ncols, lam + ncols);
+        hipLaunchKernelGGL(k_lincomb, g, b, 0, W->stream, k2, k2, k1, 2.0, len);
+        hipLaunchKernelGGL(k_lincomb, g, b, 0, W->stream, tmp, x, k1, h / 2, len);
+        conditional_action(W, tmp, k1, ncols, lam + 2 * ncols);
+        hipLaunchKernelGGL(k_lincomb, g, b, 0, W->stream, k2, k2, k1, 2.0, len);
+        hipLaunchKernelGGL(k_lincomb, g, b, 0, W->stream, tmp, x, k1, h, len);
+        conditional_action(W, tmp, k1, ncols, lam + 3 * ncols);
+        hipLaunchKernelGGL(k_lincomb, g, b, 0, W->stream, k2, k2, k1, 1.0, len);
+        hipLaunchKernelGGL(k_lincomb, g, b, 0, W->stream, x, x, k2, h / 6, len);
+        hipLaunchKernelGGL(k_cols_trace, dim3((unsigned)ncols), dim3(256), 0, W->stream, x, P->n, P->d_tidx.p,
+                           P->d_tval.p, (int64_t)P->tidx.size(), W->tZ.p);
+        std::vector<cx<double>> z((size_t)ncols), hl((size_t)(4 * ncols));
+        HIPCHK(hipMemcpyAsync(z.data(), W->tZ.p, sizeof(cx<double>) * ncols, hipMemcpyDeviceToHost, W->stream));
+        HIPCHK(hipMemcpyAsync(hl.data(), lam, sizeof(cx<double>) * 4 * ncols, hipMemcpyDeviceToHost, W->stream));
+        sync(W);
+        check_intensities(hl, "total jump intensity");
+        for (auto &v : z)
+            if (!(std::abs(cd(v.x, v.y)) > 2.220446049250313e-16))
+                throw Error(PIMI_EINVAL, "conditional state acquired zero trace");
+        hipLaunchKernelGGL(k_cols_div, g, b, 0, W->stream, x, W->tZ.p, P->n, ncols);
+        io.back(X, x);
+        sync(W);
+    });
+}
+
+pimi_status pimi_channel_intensities(pimi_workspace *W, const void *X, int64_t ncols, double *rates,
+                                     int32_t on_device) {
+    return guarded([&] {
+        if (!W || !X || !rates) throw Error(PIMI_EINVAL, "null argument");
+        BusyGuard guard(W->busy);
+        traj_prologue(W, ncols);
+        pimi_plan *P = W->plan;
+        const int64_t n = P->n, nch = (int64_t)P->host.channels.size();
+        TrajIO io{W, n * ncols, on_device != 0};
+        const cx<double> *x = io.dev(X, W->tX);
+        std::vector<cx<double>> h((size_t)ncols);
+        for (int64_t k = 0; k < nch; ++k) {
+            cols_intensity(W, x, ncols, P->ebar.p + (size_t)(k + 1) * (size_t)n, W->tH.p);
+            HIPCHK(hipMemcpyAsync(h.data(), W->tH.p, sizeof(cx<double>) * ncols, hipMemcpyDeviceToHost, W->stream));
+            sync(W);
+            check_intensities(h, "jump intensity");
+            for (int64_t c = 0; c < ncols; ++c) rates[k + c * nch] = std::max(0.0, h[(size_t)c].x);
+        }
+    });
+}
+
+pimi_status pimi_apply_jumps(pimi_workspace *W, void *X, int64_t ncols, const int32_t *channels, int32_t on_device) {
+    return guarded([&] {
+        if (!W || !X || !channels) throw Error(PIMI_EINVAL, "null argument");
+        BusyGuard guard(W->busy);
+        traj_prologue(W, ncols);
+        pimi_plan *P = W->plan;
+        const int64_t n = P->n, nch = (int64_t)P->host.channels.size();
+        for (int64_t c = 0; c < ncols; ++c)
+            if (channels[c] < 0 || channels[c] > nch) throw Error(PIMI_EINVAL, "selected jump channel out of range");
+        TrajIO io{W, n * ncols, on_device != 0};
+        cx<double> *x = io.dev(X, W->tX);
+        const size_t col = sizeof(cx<double>) * (size_t)n;
+        for (int64_t k = 1; k <= nch; ++k) {
+            std::vector<int64_t> sel;
+            for (int64_t c = 0; c < ncols; ++c)
+                if (channels[c] == k) sel.push_back(c);
+            if (sel.empty()) continue;
+            const int64_t cnt = (int64_t)sel.size();
+            for (int64_t q = 0; q < cnt; ++q)  // gather the cohort
+                HIPCHK(hipMemcpyAsync(W->tTmp.p + q * n, x + sel[(size_t)q] * n, col, hipMemcpyDeviceToDevice,
+                                      W->stream));
+            launch_stencil(W, *P->gain[(size_t)(k - 1)], W->tTmp.p, W->tK1.p, cnt);
+            hipLaunchKernelGGL(k_cols_trace, dim3((unsigned)cnt), dim3(256), 0, W->stream, W->tK1.p, n,
+                               P->d_tidx.p, P->d_tval.p, (int64_t)P->tidx.size(), W->tZ.p);
+            std::vector<cx<double>> z((size_t)cnt);
+            HIPCHK(hipMemcpyAsync(z.data(), W->tZ.p, sizeof(cx<double>) * cnt, hipMemcpyDeviceToHost, W->stream));
+            sync(W);
+            for (auto &v : z)
+                if (!(std::abs(cd(v.x, v.y)) > 2.220446049250313e-16))
+                    throw Error(PIMI_EINVAL, "selected jump has zero probability");
+            hipLaunchKernelGGL(k_cols_div, dim3(grid_for(n * cnt)), dim3(kRedBlock), 0, W->stream, W->tK1.p, W->tZ.p,
+                               n, cnt);
+            for (int64_t q = 0; q < cnt; ++q)  // scatter back
+                HIPCHK(hipMemcpyAsync(x + sel[(size_t)q] * n, W->tK1.p + q * n, col, hipMemcpyDeviceToDevice,
+                                      W->stream));
+        }
+        io.back(X, x);
+        sync(W);
     });
 }
 

@@ -12,3 +12,7 @@ from .api import (  # noqa: F401
     stationary_state,
 )
 from .batch import BatchAlgorithm, BatchEngine, batched_stationary_states  # noqa: F401
+from .trajectories import (  # noqa: F401
+    batched_apply_jumps, batched_channel_intensities, batched_conditional_action, batched_conditional_rk4,
+    trajectory_channels,
+)
