@@ -113,6 +113,8 @@ struct pimi_plan {
     int64_t ngtiles = 0;
     pimi::DevBuf<pimi::GemmTile> mtiles;  // 16x16 tiles for the matrix-core GEMM
     int64_t nmtiles = 0;
+    pimi::DevBuf<pimi::MTile> xtiles;  // the same tiles grouped by XCD, padded with m = 0
+    int64_t nxtiles = 0;
     // sector-block-diagonal part of L (inner solver), built on first use
     bool diag_ready = false;
     pimi::DevBuf<int64_t> dg_term_base;

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace pimi {
 
 template <typename R>
@@ -390,18 +392,123 @@ __device__ __forceinline__ void gemm_tile_mfma(const cx<R> *__restrict__ A, cons
     }
 }
 
+// Raw buffer loads of one complex entry: the resource covers one sector block,
+// offsets are 32-bit byte offsets, and an offset at or past the block's end
+// reads zero (the hardware range check), which is how out-of-range tile lanes
+// get their zero operands without a branch or a 64-bit address per load.
+// 0x00020000: the gfx9-family word 3 of a raw (stride 0) buffer resource.
+constexpr uint32_t kBufOut = 0x80000000u;  // an offset past every block (blocks < 2 GiB)
 template <typename R>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t block_rsrc(const cx<R> *p, int64_t entries) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)(entries * (int64_t)sizeof(cx<R>)), 0x00020000);
+}
+__device__ __forceinline__ cx<double> buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, cx<double> *) {
+    return __builtin_bit_cast(cx<double>, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ cx<float> buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, cx<float> *) {
+    return __builtin_bit_cast(cx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
+// A split-K GEMM tile: the sector's size and offset travel in the descriptor,
+// so a workgroup's operand loads wait for one descriptor load only.
+// m = 0 marks padding.
+struct MTile {
+    int64_t base;
+    int32_t m, ti, tj, pad;
+};
+
+// One 16x16 output tile per workgroup of four waves, K split four ways: wave w
+// accumulates k-steps [w K/4, (w+1) K/4), the partials meet in LDS and wave 0
+// adds them in the order w = 0..3 (deterministic).  Four times the waves and a
+// quarter of each wave's dependent k-chain, for sector sizes (m <= a few
+// hundred) whose GEMMs have too few tiles to fill the chip.  A wave loads the
+// operands of two k-steps before their MFMAs, and wave 0 loads its D entries
+// before the k-loop, so the loads overlap.
+template <typename R, bool transA, bool transB>
 __global__ __launch_bounds__(256) void k_sector_gemm_mfma(const cx<R> *__restrict__ A, const cx<R> *__restrict__ B,
-                                                          cx<R> *__restrict__ Cm, const int64_t *__restrict__ offs,
-                                                          const int32_t *__restrict__ dims,
-                                                          const GemmTile *__restrict__ tiles, int64_t ntiles,
-                                                          int transA, int transB, const cx<R> *__restrict__ D,
-                                                          R dscale) {
+                                                          cx<R> *__restrict__ Cm, const MTile *__restrict__ tiles,
+                                                          int64_t ntiles, const cx<R> *__restrict__ D, R dscale) {
+    typedef MfmaOps<R> Ops;
+    __shared__ R part[3][2][4][64];
     const int lane = threadIdx.x & 63;
-    const int64_t tid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (tid >= ntiles) return;  // whole wave leaves together; no block barriers below
-    const GemmTile t = tiles[tid];
-    gemm_tile_mfma<R>(A, B, Cm, dims[t.sector], offs[t.sector], t, transA, transB, D, dscale, lane);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t tid = blockIdx.x;
+    if (tid >= ntiles) return;  // the whole workgroup leaves together
+    const MTile t = tiles[tid];
+    if (t.m <= 0) return;       // padding tile
+    const int m = t.m;
+    const int64_t base = t.base;
+    const int row0 = t.ti * kMfmaTile, col0 = t.tj * kMfmaTile;
+    const int li = lane & 15, lk = lane >> 4;
+    const int gi = row0 + li, gj = col0 + li;
+    constexpr uint32_t E = sizeof(cx<R>);
+    cx<R> *const tag = nullptr;
+    const __amdgpu_buffer_rsrc_t ra = block_rsrc<R>(A + base, (int64_t)m * m);
+    const __amdgpu_buffer_rsrc_t rb = block_rsrc<R>(B + base, (int64_t)m * m);
+    cx<R> dd[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dd[r] = cx<R>{0, 0};
+    if (D && w == 0) {
+        const __amdgpu_buffer_rsrc_t rd = block_rsrc<R>(D + base, (int64_t)m * m);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = row0 + Ops::row(lane, r);
+            dd[r] = buf_load(rd, (i < m && gj < m) ? (uint32_t)(i + gj * m) * E : kBufOut, tag);
+        }
+    }
+    constexpr int kUnroll = 4, kStep = 4 * kUnroll;
+    const int nsteps = (m + kStep - 1) / kStep;
+    const int s0 = nsteps * w / 4, s1 = nsteps * (w + 1) / 4;
+    typename Ops::acc_t cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    // G k-steps (4 G MFMA k-slices): all operand loads first, then the MFMAs.
+    auto group = [&](int k0, auto G) {
+        constexpr int NU = decltype(G)::value * kUnroll;
+        cx<R> a[NU], b[NU];
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const int gk = k0 + 4 * u + lk;
+            const uint32_t ia = (uint32_t)(transA ? gk + gi * m : gi + gk * m) * E;
+            const uint32_t ib = (uint32_t)(transB ? gj + gk * m : gk + gj * m) * E;
+            a[u] = buf_load(ra, (gi < m && gk < m) ? ia : kBufOut, tag);
+            b[u] = buf_load(rb, (gk < m && gj < m) ? ib : kBufOut, tag);
+            if (transA) a[u].y = -a[u].y;
+            if (transB) b[u].y = -b[u].y;
+        }
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            cr = Ops::mac(a[u].x, b[u].x, cr);
+            cr = Ops::mac(-a[u].y, b[u].y, cr);
+            ci = Ops::mac(a[u].x, b[u].y, ci);
+            ci = Ops::mac(a[u].y, b[u].x, ci);
+        }
+    };
+    int st = s0;
+    for (; st + 1 < s1; st += 2) group(st * kStep, std::integral_constant<int, 2>());
+    if (st < s1) group(st * kStep, std::integral_constant<int, 1>());
+    if (w > 0)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            part[w - 1][0][r][lane] = cr[r];
+            part[w - 1][1][r][lane] = ci[r];
+        }
+    __syncthreads();
+    if (w != 0) return;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cr[r] += part[q][0][r][lane];
+            ci[r] += part[q][1][r][lane];
+        }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = row0 + Ops::row(lane, r);
+        if (i >= m || gj >= m) continue;
+        const int64_t o = base + i + (int64_t)gj * m;
+        cx<R> v = {cr[r], ci[r]};
+        if (D) v = cmul(v, cx<R>{dd[r].x * dscale, dd[r].y * dscale});
+        Cm[o] = v;
+    }
 }
 
 }  // namespace pimi

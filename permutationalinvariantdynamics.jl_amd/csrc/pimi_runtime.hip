@@ -735,6 +735,43 @@ void ensure_plan_preconditioner(pimi_plan *P) {
         }
         P->mtiles.upload(mt.data(), mt.size());
         P->nmtiles = (int64_t)mt.size();
+        // The single-plan launch keeps each sector on one XCD (workgroup w runs
+        // on XCD w mod 8, each XCD has its own L2): sectors go to XCDs by
+        // longest-processing-time on their m^3 cost, and XCD x's tiles take
+        // workgroup ids x, x + 8, x + 16, ...; shorter lists are padded with
+        // empty tiles (m = 0).  Each operand block then enters one L2
+        // instead of all eight.
+        {
+            for (int s = 0; s < P->ns; ++s)  // the GEMM's buffer loads address a block with 32-bit byte offsets
+                if (P->host.dims[s] > 11585)
+                    throw Error(PIMI_EINVAL, "sector block too large for the preconditioner (m > 11585)");
+            std::vector<int> order((size_t)P->ns);
+            for (int s = 0; s < P->ns; ++s) order[(size_t)s] = s;
+            auto cost = [&](int s) {
+                const int64_t nt = (P->host.dims[s] + kMfmaTile - 1) / kMfmaTile;
+                return nt * nt * nt;
+            };
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+            std::vector<std::vector<MTile>> per(kNumXcd);
+            int64_t load[kNumXcd] = {0};
+            for (int s : order) {
+                int x = 0;
+                for (int k = 1; k < kNumXcd; ++k)
+                    if (load[k] < load[x]) x = k;
+                load[x] += cost(s);
+                const int nt = (int)((P->host.dims[s] + kMfmaTile - 1) / kMfmaTile);
+                for (int tj = 0; tj < nt; ++tj)
+                    for (int ti = 0; ti < nt; ++ti)
+                        per[(size_t)x].push_back(MTile{P->host.offs[(size_t)s], (int32_t)P->host.dims[s], ti, tj, 0});
+            }
+            size_t len = 0;
+            for (auto &v : per) len = std::max(len, v.size());
+            std::vector<MTile> xt(len * kNumXcd, MTile{0, 0, 0, 0, 0});
+            for (int x = 0; x < kNumXcd; ++x)
+                for (size_t k = 0; k < per[(size_t)x].size(); ++k) xt[k * kNumXcd + (size_t)x] = per[(size_t)x][k];
+            P->xtiles.upload(xt.data(), xt.size());
+            P->nxtiles = (int64_t)xt.size();
+        }
         P->prec_ready = true;
     }
 }
@@ -774,20 +811,17 @@ static void ensure_preconditioner(pimi_workspace *W) {
 template <typename R>
 static void apply_precond_t(pimi_workspace *W, const cx<R> *r, cx<R> *out, double scale, cx<R> *t1, cx<R> *t2) {
     pimi_plan *P = W->plan;
-    dim3 g((unsigned)((P->nmtiles + 3) / 4)), b(256);
+    dim3 g((unsigned)P->nxtiles), b(256);  // one 16x16 tile per workgroup (split K), XCD-grouped order
     const cx<R> *nul = nullptr;
     const cx<R> *U, *Inv;
     if constexpr (sizeof(R) == 8) { U = P->pU64.p; Inv = P->pInv64.p; }
     else { U = P->pU32.p; Inv = P->pInv32.p; }
-    const int64_t nt = P->nmtiles;
-    hipLaunchKernelGGL(k_sector_gemm_mfma<R>, g, b, 0, W->stream, U, r, t1, P->offs.p, P->dims.p, P->mtiles.p, nt, 1,
-                       0, nul, (R)1);
-    hipLaunchKernelGGL(k_sector_gemm_mfma<R>, g, b, 0, W->stream, t1, U, t2, P->offs.p, P->dims.p, P->mtiles.p, nt, 0,
-                       0, Inv, (R)scale);
-    hipLaunchKernelGGL(k_sector_gemm_mfma<R>, g, b, 0, W->stream, U, t2, t1, P->offs.p, P->dims.p, P->mtiles.p, nt, 0,
-                       0, nul, (R)1);
-    hipLaunchKernelGGL(k_sector_gemm_mfma<R>, g, b, 0, W->stream, t1, U, out, P->offs.p, P->dims.p, P->mtiles.p, nt,
-                       0, 1, nul, (R)1);
+    const int64_t nt = P->nxtiles;
+    const MTile *xt = P->xtiles.p;
+    hipLaunchKernelGGL((k_sector_gemm_mfma<R, true, false>), g, b, 0, W->stream, U, r, t1, xt, nt, nul, (R)1);
+    hipLaunchKernelGGL((k_sector_gemm_mfma<R, false, false>), g, b, 0, W->stream, t1, U, t2, xt, nt, Inv, (R)scale);
+    hipLaunchKernelGGL((k_sector_gemm_mfma<R, false, false>), g, b, 0, W->stream, U, t2, t1, xt, nt, nul, (R)1);
+    hipLaunchKernelGGL((k_sector_gemm_mfma<R, false, true>), g, b, 0, W->stream, t1, U, out, xt, nt, nul, (R)1);
     HIPCHK(hipGetLastError());
 }
 
