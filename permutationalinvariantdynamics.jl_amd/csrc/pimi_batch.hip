@@ -51,8 +51,8 @@ struct BApply {
     const int32_t *dims;
     const int32_t *tile_sector;  // 256-coordinate tiles (kb_sector_scale)
     const ApplyTile *tiles;      // apply tiles (kb_apply)
-    const int64_t *term_base;
-    const StencilTermDev *terms;
+    const int32_t *entry_base;
+    const ApplyEntry *entries;
     const int32_t *col;
     const void *val;
     int64_t n;
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kApplyWave) void kb_apply(const BApply *__restrict_
     if (g >= total) return;
     const BApply A = J[seg_of(J, nj, g)];
     apply_tile<R, kApplyCols>((const cx<R> *)A.x, (cx<R> *)A.y, (R)A.out_scale, A.tiles[g - A.first], A.offs,
-                              A.dims, A.term_base, A.terms, A.col, (const cx<R> *)A.val);
+                              A.dims, A.entry_base, A.entries, A.col, (const cx<R> *)A.val);
 }
 
 // y = x * w[sector] (w = 1/sigma: back from the weighted coordinates), one
@@ -708,10 +708,11 @@ struct Tick {
         int64_t tot = 0;
         for (BJob *J : js) {
             pimi_plan *P = J->plan;
-            const void *val = sizeof(R) == 8 ? (const void *)P->val64.p : (const void *)P->val32.p;
-            const StencilTermDev *terms = (!unit && J->weighted) ? P->terms_w.p : P->terms.p;
+            const void *val = sizeof(R) == 8 ? (const void *)P->main.val64.p : (const void *)P->main.val32.p;
+            const ApplyEntry *entries = (!unit && J->weighted) ? P->main.entries_w.p : P->main.entries.p;
             v.push_back(BApply{tot, in(J), out(J), P->offs.p, P->dims.p, P->tile_sector.p, P->atiles4.p,
-                               P->term_base.p, terms, P->col.p, val, J->n, unit ? 1.0 : 1.0 / J->scale});
+                               P->main.entry_base.p, entries, P->main.col.p, val, J->n,
+                               unit ? 1.0 : 1.0 / J->scale});
             tot += P->natiles4;
             bytes[cls] += 2.0 * J->n * sizeof(cx<R>) + (sizeof(R) == 8 ? J->table_bytes64 : J->table_bytes32);
         }
@@ -1392,10 +1393,8 @@ pimi_status pimi_batch_submit(pimi_batch *E, pimi_plan *plan, const pimi_gmres_o
         J->nblk = blocks_for(J->n);
         J->bytes = job_bytes(J->n, J->nt, J->kdim, E->prec);
         {
-            const double base = (double)(plan->st.col.size() * 4 + plan->st.terms.size() * sizeof(StencilTermDev) +
-                                         plan->st.term_base.size() * 8 + (plan->host.dims.size() + 1) * 12);
-            J->table_bytes64 = base + 16.0 * plan->st.val.size();
-            J->table_bytes32 = base + 8.0 * plan->st.val.size();
+            J->table_bytes64 = (double)plan->main.table_bytes(16);
+            J->table_bytes32 = (double)plan->main.table_bytes(8);
             double m3 = 0;
             for (auto m : plan->host.dims) m3 += (double)m * m * m;
             J->gemm_flops = 4 * 8.0 * m3;  // four complex m x m products per sector, 8 flops per complex fma
@@ -1531,13 +1530,11 @@ pimi_status pimi_batch_apply_bench(pimi_batch *E, pimi_plan *const *plans, int32
             xs[k].alloc(es * (size_t)P->n);
             ys[k].alloc(es * (size_t)P->n);
             HIPCHK(hipMemsetAsync(xs[k].p, 0x3f, es * (size_t)P->n, E->stream));
-            const void *val = E->prec == PIMI_C32 ? (const void *)P->val32.p : (const void *)P->val64.p;
+            const void *val = E->prec == PIMI_C32 ? (const void *)P->main.val32.p : (const void *)P->main.val64.p;
             v.push_back(BApply{tot, xs[k].p, ys[k].p, P->offs.p, P->dims.p, P->tile_sector.p, P->atiles4.p,
-                               P->term_base.p, P->terms.p, P->col.p, val, P->n, 1.0});
+                               P->main.entry_base.p, P->main.entries.p, P->main.col.p, val, P->n, 1.0});
             tot += P->natiles4;
-            const int64_t base = (int64_t)(P->st.col.size() * 4 + P->st.terms.size() * sizeof(StencilTermDev) +
-                                           P->st.term_base.size() * 8 + (P->host.dims.size() + 1) * 12);
-            bytes += 2 * P->n * (int64_t)es + base + (int64_t)P->st.val.size() * (int64_t)es;
+            bytes += 2 * P->n * (int64_t)es + P->main.table_bytes((int)es);
         }
         DevBuf<BApply> dv;
         dv.alloc(v.size());

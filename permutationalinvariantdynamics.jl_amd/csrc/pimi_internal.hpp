@@ -4,7 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
+#include <climits>
 #include <complex>
 #include <memory>
 #include <string>
@@ -53,30 +55,91 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
-// A stencil's device tables (FP64 and FP32 values).
+// A stencil's apply tables: per output sector a list of ApplyEntry (padded to
+// whole chunks), and the ELL slot tables with the identity slot appended
+// (FP64 and FP32 values).  build() lowers a host Stencil; push() uploads.
+// entries_w holds the same entries with the sector-weighted scales
+// s * sigma_out / sigma_in (built by push_weighted when first needed).
 struct DevStencil {
-    DevBuf<int64_t> term_base;
-    DevBuf<StencilTermDev> terms;
+    std::vector<int32_t> h_entry_base;  // ns + 1
+    std::vector<ApplyEntry> h_entries;
+    std::vector<int32_t> h_col;
+    std::vector<cd> h_val;
+    DevBuf<int32_t> entry_base;
+    DevBuf<ApplyEntry> entries, entries_w;
     DevBuf<int32_t> col;
     DevBuf<cx<double>> val64;
     DevBuf<cx<float>> val32;
-    void upload(const Stencil &st) {
-        std::vector<StencilTermDev> t(st.terms.size());
-        for (size_t k = 0; k < t.size(); ++k) {
-            const StencilTerm &a = st.terms[k];
-            t[k] = StencilTermDev{a.in_sector, a.flags, a.wa, a.wb, a.a_base, a.b_base, a.s_re, a.s_im};
+
+    void build(const Stencil &st, const std::vector<int64_t> &dims, const std::vector<int64_t> &offs) {
+        int64_t maxm = 1;
+        for (int64_t m : dims) maxm = std::max(maxm, m);
+        const int64_t id = (int64_t)st.col.size();
+        if (id + maxm >= (int64_t)INT32_MAX) throw Error(PIMI_EINVAL, "stencil tables exceed 32-bit indexing");
+        h_col = st.col;
+        h_val = st.val;
+        for (int64_t r = 0; r < maxm; ++r) {
+            h_col.push_back((int32_t)r);
+            h_val.push_back(cd(1.0, 0.0));
         }
-        std::vector<cx<double>> v64(st.val.size());
-        std::vector<cx<float>> v32(st.val.size());
-        for (size_t k = 0; k < v64.size(); ++k) {
-            v64[k] = cx<double>{st.val[k].real(), st.val[k].imag()};
-            v32[k] = cx<float>{(float)st.val[k].real(), (float)st.val[k].imag()};
+        h_entries.clear();
+        h_entry_base.assign(1, 0);
+        const int ns = (int)dims.size();
+        for (int q = 0; q < ns; ++q) {
+            const int64_t m = dims[(size_t)q];
+            for (int64_t t = st.term_base[(size_t)q]; t < st.term_base[(size_t)q + 1]; ++t) {
+                const StencilTerm &T = st.terms[(size_t)t];
+                std::vector<int32_t> qa, qb;
+                if (T.flags & 1) qa.push_back((int32_t)id);
+                else
+                    for (int k = 0; k < T.wa; ++k) qa.push_back((int32_t)(T.a_base + k * m));
+                if (T.flags & 2) qb.push_back((int32_t)id);
+                else
+                    for (int k = 0; k < T.wb; ++k) qb.push_back((int32_t)(T.b_base + k * m));
+                for (int32_t b : qb)
+                    for (int32_t a : qa)
+                        h_entries.push_back(ApplyEntry{offs[(size_t)T.in_sector], a, b,
+                                                       (int32_t)dims[(size_t)T.in_sector], T.in_sector, T.s_re,
+                                                       T.s_im});
+            }
+            while ((h_entries.size() - (size_t)h_entry_base.back()) % kEntryChunk)
+                h_entries.push_back(ApplyEntry{offs[(size_t)q], (int32_t)id, (int32_t)id, (int32_t)m, q, 0.0, 0.0});
+            if (h_entries.size() >= (size_t)INT32_MAX) throw Error(PIMI_EINVAL, "stencil entry list too long");
+            h_entry_base.push_back((int32_t)h_entries.size());
         }
-        term_base.upload(st.term_base.data(), st.term_base.size());
-        terms.upload(t.data(), t.size());
-        col.upload(st.col.data(), st.col.size());
+    }
+    void push() {
+        std::vector<cx<double>> v64(h_val.size());
+        std::vector<cx<float>> v32(h_val.size());
+        for (size_t k = 0; k < h_val.size(); ++k) {
+            v64[k] = cx<double>{h_val[k].real(), h_val[k].imag()};
+            v32[k] = cx<float>{(float)h_val[k].real(), (float)h_val[k].imag()};
+        }
+        entry_base.upload(h_entry_base.data(), h_entry_base.size());
+        entries.upload(h_entries.data(), h_entries.size());
+        col.upload(h_col.data(), h_col.size());
         val64.upload(v64.data(), v64.size());
         val32.upload(v32.data(), v32.size());
+    }
+    void upload(const Stencil &st, const std::vector<int64_t> &dims, const std::vector<int64_t> &offs) {
+        build(st, dims, offs);
+        push();
+    }
+    void push_weighted(const std::vector<double> &sigma) {
+        std::vector<ApplyEntry> w(h_entries);
+        for (size_t q = 0; q + 1 < h_entry_base.size(); ++q)
+            for (int32_t k = h_entry_base[q]; k < h_entry_base[q + 1]; ++k) {
+                const double r = sigma[q] / sigma[(size_t)w[(size_t)k].in_sector];
+                w[(size_t)k].s_re *= r;
+                w[(size_t)k].s_im *= r;
+            }
+        entries_w.upload(w.data(), w.size());
+    }
+    // bytes of the tables one apply reads (entries, slot tables, sector sizes)
+    int64_t table_bytes(int elem) const {
+        return (int64_t)(h_entries.size() * sizeof(ApplyEntry) + h_entry_base.size() * 4 + h_col.size() * 4 +
+                         (h_entry_base.size() - 1) * 4) +
+               (int64_t)h_val.size() * elem;
     }
 };
 
@@ -98,12 +161,10 @@ struct pimi_plan {
     int64_t natiles4 = 0;                         // kApplyCols-column apply tiles (batched launches)
     std::vector<int64_t> tidx;
     std::vector<double> tval;
-    pimi::DevBuf<int64_t> offs, term_base, d_tidx;
-    pimi::DevBuf<int32_t> dims, tile_sector, col;
+    pimi::DevBuf<int64_t> offs, d_tidx;
+    pimi::DevBuf<int32_t> dims, tile_sector;
     pimi::DevBuf<pimi::ApplyTile> atiles, atiles4;
-    pimi::DevBuf<pimi::StencilTermDev> terms;
-    pimi::DevBuf<pimi::cx<double>> val64;
-    pimi::DevBuf<pimi::cx<float>> val32;
+    pimi::DevStencil main;  // the Liouvillian's apply tables
     pimi::DevBuf<double> d_tval;
     // stationary-solve preconditioner factors (built on first use)
     bool prec_ready = false;
@@ -117,11 +178,7 @@ struct pimi_plan {
     int64_t nxtiles = 0;
     // sector-block-diagonal part of L (inner solver), built on first use
     bool diag_ready = false;
-    pimi::DevBuf<int64_t> dg_term_base;
-    pimi::DevBuf<pimi::StencilTermDev> dg_terms;
-    pimi::DevBuf<int32_t> dg_col;
-    pimi::DevBuf<pimi::cx<double>> dg_val64;
-    pimi::DevBuf<pimi::cx<float>> dg_val32;
+    pimi::DevStencil diag;
     // conditional (no-jump) trajectory tables (built on first use): the
     // H/Q-only stencil, one gain stencil per jump channel, and conj of the
     // intensity weights (n complex each; the total first, then per channel)
@@ -131,17 +188,12 @@ struct pimi_plan {
     pimi::DevBuf<pimi::cx<double>> ebar;
     // stencil of the adjoint Liouvillian (built on first use)
     bool adj_ready = false;
-    pimi::DevBuf<int64_t> adj_term_base;
-    pimi::DevBuf<pimi::StencilTermDev> adj_terms;
-    pimi::DevBuf<int32_t> adj_col;
-    pimi::DevBuf<pimi::cx<double>> adj_val64;
-    pimi::DevBuf<pimi::cx<float>> adj_val32;
+    pimi::DevStencil adj;
     // sector weighting u = S x, S = sqrt(f^nu) on sector nu (built on first use):
-    // stencil terms of S L S^{-1} (scale times sigma_out / sigma_in), the
-    // transformed trace functional (ones on the block diagonals) and 1/sigma
+    // entries of S L S^{-1} (main.entries_w: scale times sigma_out / sigma_in),
+    // the transformed trace functional (ones on the block diagonals) and 1/sigma
     bool weight_ready = false;
     std::vector<double> sigma;  // per sector
-    pimi::DevBuf<pimi::StencilTermDev> terms_w;
     pimi::DevBuf<double> d_ones, d_inv_sigma;
 };
 

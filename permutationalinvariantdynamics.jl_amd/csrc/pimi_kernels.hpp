@@ -27,68 +27,37 @@ __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) { return {a.x * b.x - a.
 template <typename R>
 __device__ __forceinline__ cx<R> cjmul(cx<R> a, cx<R> b) { return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }
 
-struct StencilTermDev {
-    int32_t in_sector;
-    int32_t flags;  // bit0: A identity, bit1: B identity
-    int32_t wa, wb;
-    int64_t a_base, b_base;
-    double s_re, s_im;
-};
-
 constexpr int kApplyBlock = 256;
 constexpr int kRedBlock = 256;
 constexpr int kNumXcd = 8;
 
-// (L x)[i] for output coordinate i, sector s, position (a, b) in the m x m
-// block: the sum over the sector's stencil terms of s_t * (A X_in B^H)[a, b].
-template <typename R>
-__device__ __forceinline__ cx<R> apply_coordinate(const cx<R> *__restrict__ xc, int s, int64_t a, int64_t b,
-                                                  int64_t m, const int64_t *__restrict__ offs,
-                                                  const int32_t *__restrict__ dims,
-                                                  const int64_t *__restrict__ term_base,
-                                                  const StencilTermDev *__restrict__ terms,
-                                                  const int32_t *__restrict__ col, const cx<R> *__restrict__ val) {
-    const int64_t t0 = term_base[s], t1 = term_base[s + 1];
-    cx<R> acc = {0, 0};
-    for (int64_t t = t0; t < t1; ++t) {
-        const StencilTermDev T = terms[t];
-        const cx<R> *Xn = xc + offs[T.in_sector];
-        const int64_t mn = dims[T.in_sector];
-        cx<R> tacc = {0, 0};
-        if (T.flags & 1) {
-            for (int kb = 0; kb < T.wb; ++kb) {
-                const int64_t q = T.b_base + kb * m + b;
-                tacc = cadd(tacc, cjmul(val[q], Xn[a + (int64_t)col[q] * mn]));
-            }
-        } else if (T.flags & 2) {
-            for (int ka = 0; ka < T.wa; ++ka) {
-                const int64_t q = T.a_base + ka * m + a;
-                tacc = cadd(tacc, cmul(val[q], Xn[col[q] + b * mn]));
-            }
-        } else {
-            for (int kb = 0; kb < T.wb; ++kb) {
-                const int64_t qb = T.b_base + kb * m + b;
-                const int64_t dcol = (int64_t)col[qb] * mn;
-                cx<R> inner = {0, 0};
-                for (int ka = 0; ka < T.wa; ++ka) {
-                    const int64_t qa = T.a_base + ka * m + a;
-                    inner = cadd(inner, cmul(val[qa], Xn[col[qa] + dcol]));
-                }
-                tacc = cadd(tacc, cjmul(val[qb], inner));
-            }
-        }
-        acc = cadd(acc, cmul(cx<R>{(R)T.s_re, (R)T.s_im}, tacc));
-    }
-    return acc;
-}
+// One stencil entry of an output sector: its contribution to Y[a, b] is
+//   s * A[a] * conj(B[b]) * X_in[colA(a), colB(b)],
+// where the A and B factors are ELL slots: row r of the slot starting at q
+// is (col[q + r], val[q + r]).  An identity factor points at the identity
+// slot (col[id + r] = r, val[id + r] = 1), so every entry has the same form:
+// a stencil term A X_in B^H with ELL widths wa, wb becomes wa * wb entries.
+// Each sector's list is padded with zero-scale entries to a multiple of
+// kEntryChunk, so a wave walks it in whole chunks.
+struct ApplyEntry {
+    int64_t in_off;     // offset of the input block
+    int32_t qa, qb;     // A-side and B-side slot starts
+    int32_t mn;         // input block size
+    int32_t in_sector;
+    double s_re, s_im;  // term scale
+};
+constexpr int kEntryChunk = 8;
 
 // An apply tile: rows a0..a0+63 of columns b0..b0+nb-1 (nb <= kApplyCols) of
 // sector `sector`'s output block, one wave; each lane computes its row in the
-// nb columns.  Sector, columns and stencil terms are wave-uniform, so term
-// descriptors and the B-side (column) tables are scalar loads; the A-side
-// (row) tables are loaded once per term for all nb outputs of a lane, whose
-// input gathers are independent (memory-level parallelism).  Every output
-// sums its terms and ELL entries in the same order as apply_coordinate.
+// nb columns.  Entries are processed a chunk at a time: all slot-table loads
+// of the chunk, then all input gathers, then the arithmetic, so a wave keeps
+// a chunk's worth of independent loads in flight instead of walking a chain
+// of dependent loads term by term (the single-vector apply is latency-bound).
+// Entry descriptors and the B-side (column) slots are wave-uniform (scalar
+// loads); the A-side (row) slots are loaded once per entry for all nb
+// columns of a lane.  Every output sums its entries in list order, as
+// (s * conj(B)) * A * X.
 struct ApplyTile {
     int32_t sector, b0, a0, nb;
 };
@@ -98,76 +67,59 @@ constexpr int kApplyCols = 4;  // columns per lane of the batched apply (one for
 template <typename R, int CB>
 __device__ __forceinline__ void apply_tile(const cx<R> *__restrict__ xc, cx<R> *__restrict__ yc, R out_scale,
                                            ApplyTile tile, const int64_t *__restrict__ offs,
-                                           const int32_t *__restrict__ dims, const int64_t *__restrict__ term_base,
-                                           const StencilTermDev *__restrict__ terms,
+                                           const int32_t *__restrict__ dims,
+                                           const int32_t *__restrict__ entry_base,
+                                           const ApplyEntry *__restrict__ entries,
                                            const int32_t *__restrict__ col, const cx<R> *__restrict__ val) {
+    constexpr int CH = CB == 1 ? kEntryChunk : kEntryChunk / 2;
     const int s = __builtin_amdgcn_readfirstlane(tile.sector);
-    const int64_t b0 = __builtin_amdgcn_readfirstlane(tile.b0);
+    const int b0 = __builtin_amdgcn_readfirstlane(tile.b0);
     const int nb = __builtin_amdgcn_readfirstlane(tile.nb);
-    const int64_t m = dims[s];
-    const int64_t arow = tile.a0 + (int64_t)(threadIdx.x & (kApplyWave - 1));
+    const int m = dims[s];
+    const int arow = tile.a0 + (int)(threadIdx.x & (kApplyWave - 1));
     const bool active = arow < m;
-    const int64_t a = active ? arow : 0;  // inactive lanes evaluate row 0 (in bounds) and store nothing
+    const int a = active ? arow : 0;  // inactive lanes evaluate row 0 (in bounds) and store nothing
     cx<R> acc[CB];
 #pragma unroll
     for (int j = 0; j < CB; ++j) acc[j] = cx<R>{0, 0};
-    const int64_t t0 = term_base[s], t1 = term_base[s + 1];
-    for (int64_t t = t0; t < t1; ++t) {
-        const StencilTermDev T = terms[t];
-        const cx<R> *Xn = xc + offs[T.in_sector];
-        const int64_t mn = dims[T.in_sector];
-        const cx<R> sc = {(R)T.s_re, (R)T.s_im};
-        cx<R> tacc[CB];
+    const int e0 = entry_base[s], e1 = entry_base[s + 1];
+    for (int e = e0; e < e1; e += CH) {
+        int ca[CH];
+        cx<R> va[CH];
+        cx<R> xv[CH][CB];
 #pragma unroll
-        for (int j = 0; j < CB; ++j) tacc[j] = cx<R>{0, 0};
-        if (T.flags & 1) {  // A identity: sum_kb conj(B[b, kb]) X[a, colB]
-            for (int kb = 0; kb < T.wb; ++kb) {
+        for (int k = 0; k < CH; ++k) {
+            const int q = entries[e + k].qa + a;
+            ca[k] = col[q];
+            va[k] = val[q];
+        }
 #pragma unroll
-                for (int j = 0; j < CB; ++j)
-                    if (j < nb) {
-                        const int64_t q = T.b_base + kb * m + b0 + j;
-                        tacc[j] = cadd(tacc[j], cjmul(val[q], Xn[a + (int64_t)col[q] * mn]));
-                    }
-            }
-        } else if (T.flags & 2) {  // B identity: sum_ka A[a, ka] X[colA, b]
-            for (int ka = 0; ka < T.wa; ++ka) {
-                const int64_t q = T.a_base + ka * m + a;
-                const cx<R> v = val[q];
-                const int64_t c = col[q];
+        for (int k = 0; k < CH; ++k) {
+            const ApplyEntry &E = entries[e + k];
+            const cx<R> *Xn = xc + E.in_off;
 #pragma unroll
-                for (int j = 0; j < CB; ++j)
-                    if (j < nb) tacc[j] = cadd(tacc[j], cmul(v, Xn[c + (b0 + j) * mn]));
-            }
-        } else {  // sum_kb conj(B[b, kb]) (sum_ka A[a, ka] X[colA, colB])
-            for (int kb = 0; kb < T.wb; ++kb) {
-                int64_t dcol[CB];
-                cx<R> inner[CB];
-#pragma unroll
-                for (int j = 0; j < CB; ++j) {
-                    dcol[j] = j < nb ? (int64_t)col[T.b_base + kb * m + b0 + j] * mn : 0;
-                    inner[j] = cx<R>{0, 0};
-                }
-                for (int ka = 0; ka < T.wa; ++ka) {
-                    const int64_t qa = T.a_base + ka * m + a;
-                    const cx<R> v = val[qa];
-                    const int64_t c = col[qa];
-#pragma unroll
-                    for (int j = 0; j < CB; ++j)
-                        if (j < nb) inner[j] = cadd(inner[j], cmul(v, Xn[c + dcol[j]]));
-                }
-#pragma unroll
-                for (int j = 0; j < CB; ++j)
-                    if (j < nb) tacc[j] = cadd(tacc[j], cjmul(val[T.b_base + kb * m + b0 + j], inner[j]));
+            for (int j = 0; j < CB; ++j) {
+                const int cb = col[E.qb + b0 + (j < nb ? j : 0)];
+                xv[k][j] = Xn[(uint32_t)(ca[k] + cb * E.mn)];
             }
         }
 #pragma unroll
-        for (int j = 0; j < CB; ++j) acc[j] = cadd(acc[j], cmul(sc, tacc[j]));
+        for (int k = 0; k < CH; ++k) {
+            const ApplyEntry &E = entries[e + k];
+            const cx<R> sc = {(R)E.s_re, (R)E.s_im};
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                const cx<R> vb = val[E.qb + b0 + (j < nb ? j : 0)];
+                const cx<R> coef = cmul(cjmul(vb, sc), va[k]);
+                acc[j] = cadd(acc[j], cmul(coef, xv[k][j]));
+            }
+        }
     }
     if (active) {
         const int64_t o = offs[s] + a;
 #pragma unroll
         for (int j = 0; j < CB; ++j)
-            if (j < nb) yc[o + (b0 + j) * m] = cx<R>{out_scale * acc[j].x, out_scale * acc[j].y};
+            if (j < nb) yc[o + (int64_t)(b0 + j) * m] = cx<R>{out_scale * acc[j].x, out_scale * acc[j].y};
     }
 }
 
@@ -178,14 +130,14 @@ template <typename R>
 __global__ __launch_bounds__(kApplyWave) void k_apply(
     const cx<R> *__restrict__ x, cx<R> *__restrict__ y, int64_t n, const ApplyTile *__restrict__ tiles, int64_t ntiles,
     int64_t tiles_per_xcd, const int64_t *__restrict__ offs, const int32_t *__restrict__ dims,
-    const int64_t *__restrict__ term_base, const StencilTermDev *__restrict__ terms, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ entry_base, const ApplyEntry *__restrict__ entries, const int32_t *__restrict__ col,
     const cx<R> *__restrict__ val, R out_scale, int64_t ncols) {
     const int64_t bid = blockIdx.x;
     const int64_t t = (bid % kNumXcd) * tiles_per_xcd + bid / kNumXcd;
     if (t >= ntiles) return;
     const ApplyTile tile = tiles[t];
     for (int64_t c = blockIdx.y; c < ncols; c += gridDim.y)
-        apply_tile<R, 1>(x + c * n, y + c * n, out_scale, tile, offs, dims, term_base, terms, col, val);
+        apply_tile<R, 1>(x + c * n, y + c * n, out_scale, tile, offs, dims, entry_base, entries, col, val);
 }
 
 // y[i] = x[i] * w[sector of i], one coordinate per lane over the plan's tiles

@@ -368,17 +368,7 @@ static void finalize_plan(pimi_plan *P) {
     P->natiles = (int64_t)at.size();
     P->natiles4 = (int64_t)at4.size();
     P->atiles_per_xcd = (P->natiles + kNumXcd - 1) / kNumXcd;
-    std::vector<StencilTermDev> terms(P->st.terms.size());
-    for (size_t k = 0; k < terms.size(); ++k) {
-        const StencilTerm &a = P->st.terms[k];
-        terms[k] = StencilTermDev{a.in_sector, a.flags, a.wa, a.wb, a.a_base, a.b_base, a.s_re, a.s_im};
-    }
-    std::vector<cx<double>> v64(P->st.val.size());
-    std::vector<cx<float>> v32(P->st.val.size());
-    for (size_t k = 0; k < v64.size(); ++k) {
-        v64[k] = cx<double>{P->st.val[k].real(), P->st.val[k].imag()};
-        v32[k] = cx<float>{(float)P->st.val[k].real(), (float)P->st.val[k].imag()};
-    }
+    P->main.build(P->st, F.dims, F.offs);
     // trace functional (src/source_protocol.jl:15-34)
     P->tidx.clear();
     P->tval.clear();
@@ -397,11 +387,7 @@ static void finalize_plan(pimi_plan *P) {
     P->tile_sector.upload(tsec.data(), tsec.size());
     P->atiles.upload(at.data(), at.size());
     P->atiles4.upload(at4.data(), at4.size());
-    P->term_base.upload(P->st.term_base.data(), P->st.term_base.size());
-    P->terms.upload(terms.data(), terms.size());
-    P->col.upload(P->st.col.data(), P->st.col.size());
-    P->val64.upload(v64.data(), v64.size());
-    P->val32.upload(v32.data(), v32.size());
+    P->main.push();
     P->d_tidx.upload(P->tidx.data(), P->tidx.size());
     P->d_tval.upload(P->tval.data(), P->tval.size());
 }
@@ -413,48 +399,17 @@ static void ensure_diag_stencil(pimi_plan *P) {
     if (P->diag_ready) return;
     Stencil st;
     build_stencil(P->host, st, true);
-    std::vector<StencilTermDev> terms(st.terms.size());
-    for (size_t k = 0; k < terms.size(); ++k) {
-        const StencilTerm &a = st.terms[k];
-        terms[k] = StencilTermDev{a.in_sector, a.flags, a.wa, a.wb, a.a_base, a.b_base, a.s_re, a.s_im};
-    }
-    std::vector<cx<double>> v64(st.val.size());
-    std::vector<cx<float>> v32(st.val.size());
-    for (size_t k = 0; k < v64.size(); ++k) {
-        v64[k] = cx<double>{st.val[k].real(), st.val[k].imag()};
-        v32[k] = cx<float>{(float)st.val[k].real(), (float)st.val[k].imag()};
-    }
-    P->dg_term_base.upload(st.term_base.data(), st.term_base.size());
-    P->dg_terms.upload(terms.data(), terms.size());
-    P->dg_col.upload(st.col.data(), st.col.size());
-    P->dg_val64.upload(v64.data(), v64.size());
-    P->dg_val32.upload(v32.data(), v32.size());
+    P->diag.upload(st, P->host.dims, P->host.offs);
     P->diag_ready = true;
 }
 
-// y = out_scale * L x (diag = false) or out_scale * D x, D the sector-block-diagonal part of L
 static void ensure_adjoint_stencil(pimi_plan *P) {
     if (P->adj_ready) return;
     FusedPlan A;
     adjoint_plan(P->host, A);
     Stencil st;
     build_stencil(A, st);
-    std::vector<StencilTermDev> terms(st.terms.size());
-    for (size_t k = 0; k < terms.size(); ++k) {
-        const StencilTerm &a = st.terms[k];
-        terms[k] = StencilTermDev{a.in_sector, a.flags, a.wa, a.wb, a.a_base, a.b_base, a.s_re, a.s_im};
-    }
-    std::vector<cx<double>> v64(st.val.size());
-    std::vector<cx<float>> v32(st.val.size());
-    for (size_t k = 0; k < v64.size(); ++k) {
-        v64[k] = cx<double>{st.val[k].real(), st.val[k].imag()};
-        v32[k] = cx<float>{(float)st.val[k].real(), (float)st.val[k].imag()};
-    }
-    P->adj_term_base.upload(st.term_base.data(), st.term_base.size());
-    P->adj_terms.upload(terms.data(), terms.size());
-    P->adj_col.upload(st.col.data(), st.col.size());
-    P->adj_val64.upload(v64.data(), v64.size());
-    P->adj_val32.upload(v32.data(), v32.size());
+    P->adj.upload(st, P->host.dims, P->host.offs);
     P->adj_ready = true;
 }
 
@@ -467,35 +422,15 @@ static void launch_apply(pimi_workspace *W, const cx<R> *x, cx<R> *y, double out
                          bool diag = false, bool weighted = false, bool adjoint = false) {
     pimi_plan *P = W->plan;
     const ApplyOp op = adjoint ? ApplyOp::Adjoint : (diag ? ApplyOp::Diag : ApplyOp::Full);
+    const DevStencil &S = op == ApplyOp::Diag ? P->diag : (op == ApplyOp::Adjoint ? P->adj : P->main);
+    const ApplyEntry *entries = (op == ApplyOp::Full && weighted) ? S.entries_w.p : S.entries.p;
     const cx<R> *val;
-    const int64_t *term_base;
-    const StencilTermDev *terms;
-    const int32_t *col;
-    switch (op) {
-        case ApplyOp::Diag:
-            if constexpr (sizeof(R) == 8) val = P->dg_val64.p;
-            else val = P->dg_val32.p;
-            term_base = P->dg_term_base.p;
-            terms = P->dg_terms.p;
-            col = P->dg_col.p;
-            break;
-        case ApplyOp::Adjoint:
-            if constexpr (sizeof(R) == 8) val = P->adj_val64.p;
-            else val = P->adj_val32.p;
-            term_base = P->adj_term_base.p;
-            terms = P->adj_terms.p;
-            col = P->adj_col.p;
-            break;
-        default:
-            if constexpr (sizeof(R) == 8) val = P->val64.p;
-            else val = P->val32.p;
-            term_base = P->term_base.p;
-            terms = weighted ? P->terms_w.p : P->terms.p;
-            col = P->col.p;
-    }
+    if constexpr (sizeof(R) == 8) val = S.val64.p;
+    else val = S.val32.p;
     dim3 grid((unsigned)(P->atiles_per_xcd * kNumXcd), (unsigned)std::min<int64_t>(ncols, 65535));
     hipLaunchKernelGGL(k_apply<R>, grid, dim3(kApplyWave), 0, W->stream, x, y, P->n, P->atiles.p, P->natiles,
-                       P->atiles_per_xcd, P->offs.p, P->dims.p, term_base, terms, col, val, (R)out_scale, ncols);
+                       P->atiles_per_xcd, P->offs.p, P->dims.p, S.entry_base.p, entries, S.col.p, val, (R)out_scale,
+                       ncols);
     HIPCHK(hipGetLastError());
 }
 
@@ -782,16 +717,9 @@ void ensure_plan_weighting(pimi_plan *P) {
     P->sigma.assign((size_t)F.ns, 1.0);
     if (!F.geom.sqrt_f.empty())
         for (int q = 0; q < F.ns; ++q) P->sigma[(size_t)q] = F.geom.sqrt_f[(size_t)q];
-    std::vector<StencilTermDev> tw(P->st.terms.size());
-    for (int q = 0; q < F.ns; ++q)
-        for (int64_t k = P->st.term_base[(size_t)q]; k < P->st.term_base[(size_t)q + 1]; ++k) {
-            const StencilTerm &a = P->st.terms[(size_t)k];
-            const double r = P->sigma[(size_t)q] / P->sigma[(size_t)a.in_sector];
-            tw[(size_t)k] = StencilTermDev{a.in_sector, a.flags, a.wa, a.wb, a.a_base, a.b_base, a.s_re * r, a.s_im * r};
-        }
     std::vector<double> ones(P->tidx.size(), 1.0), inv((size_t)F.ns);
     for (int q = 0; q < F.ns; ++q) inv[(size_t)q] = 1.0 / P->sigma[(size_t)q];
-    P->terms_w.upload(tw.data(), tw.size());
+    P->main.push_weighted(P->sigma);
     P->d_ones.upload(ones.data(), ones.size());
     P->d_inv_sigma.upload(inv.data(), inv.size());
     P->weight_ready = true;
@@ -1628,7 +1556,7 @@ static void ensure_trajectory_tables(pimi_plan *P) {
         C.Q = F.Q;
         Stencil st;
         build_stencil(C, st);
-        P->cond.upload(st);
+        P->cond.upload(st, F.dims, F.offs);
     }
     P->gain.clear();
     for (const JumpChannel &ch : F.channels) {  // one gain stencil per channel, scaled by its rate
@@ -1644,7 +1572,7 @@ static void ensure_trajectory_tables(pimi_plan *P) {
         Stencil st;
         build_stencil(G, st);
         std::unique_ptr<DevStencil> ds(new DevStencil());
-        ds->upload(st);
+        ds->upload(st, F.dims, F.offs);
         P->gain.push_back(std::move(ds));
     }
     // intensity weights: tr(E_nu X_nu) = sum_{r,c} E[r,c] X[c,r], X[c,r] at offs + c + r m
@@ -1689,7 +1617,7 @@ static void launch_stencil(pimi_workspace *W, const DevStencil &S, const cx<doub
     pimi_plan *P = W->plan;
     dim3 grid((unsigned)(P->atiles_per_xcd * kNumXcd), (unsigned)std::min<int64_t>(ncols, 65535));
     hipLaunchKernelGGL(k_apply<double>, grid, dim3(kApplyWave), 0, W->stream, x, y, P->n, P->atiles.p, P->natiles,
-                       P->atiles_per_xcd, P->offs.p, P->dims.p, S.term_base.p, S.terms.p, S.col.p, S.val64.p, 1.0,
+                       P->atiles_per_xcd, P->offs.p, P->dims.p, S.entry_base.p, S.entries.p, S.col.p, S.val64.p, 1.0,
                        ncols);
     HIPCHK(hipGetLastError());
 }
@@ -1898,10 +1826,8 @@ pimi_status pimi_plan_info_get(const pimi_plan *P, pimi_plan_info *info) {
         for (auto m : P->host.dims) info->max_block = std::max<int32_t>(info->max_block, (int32_t)m);
         info->nterms = (int64_t)P->st.terms.size();
         info->stencil_entries = P->st.entries_per_apply;
-        int64_t base = (int64_t)(P->st.col.size() * 4 + P->st.terms.size() * sizeof(StencilTermDev) +
-                                 P->st.term_base.size() * 8 + (P->host.dims.size() + 1) * 12);
-        info->table_bytes_c64 = base + (int64_t)P->st.val.size() * 16;
-        info->table_bytes_c32 = base + (int64_t)P->st.val.size() * 8;
+        info->table_bytes_c64 = P->main.table_bytes(16);
+        info->table_bytes_c32 = P->main.table_bytes(8);
     });
 }
 
