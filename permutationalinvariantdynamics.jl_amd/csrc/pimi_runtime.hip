@@ -104,6 +104,82 @@ __global__ __launch_bounds__(kRedBlock) void k_gs_update(cx<R> *__restrict__ w, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Reductions finished by their consumer.  A producer kernel leaves per-
+// workgroup partial sums; the next kernel, which needs the total anyway, has
+// every workgroup add them up itself in one fixed order (lane-strided, then
+// across the wave), so all workgroups hold bitwise the same total and one
+// launch per reduction disappears without atomics or device-wide fences.
+// ---------------------------------------------------------------------------
+constexpr int kFusedDotCols = 16;  // widest basis whose dot partials a consumer reduces
+constexpr int kNormBlocks = 512;   // grid of kernels that leave norm partials
+
+// sum over q < G of p[q * stride] by one wave
+__device__ __forceinline__ double wave_total(const double *p, int G, int stride = 1) {
+    const int lane = threadIdx.x & 63;
+    double t = 0;
+    for (int q = lane; q < G; q += 64) t += p[(int64_t)q * stride];
+    return wave_sum(t);
+}
+
+// w -= V[:, 0:ncol] h with h the reduction of the dot partials
+// dpart[c * G + q] (q < G) of k_dots_partial (ncol <= kFusedDotCols);
+// workgroup 0 adds h into hacc.  With npart != null, each workgroup leaves
+// its share of ||w||^2 (of the stored w) in npart[blockIdx.x].
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void k_gs_update_r(cx<R> *__restrict__ w, const cx<R> *__restrict__ V,
+                                                           int64_t ldv, int ncol, const cx<double> *__restrict__ dpart,
+                                                           int G, cx<double> *hacc, int64_t n,
+                                                           double *__restrict__ npart) {
+    __shared__ cx<double> hs[kFusedDotCols];
+    __shared__ double sh[kRedBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int c = wid; c < ncol; c += kRedBlock / 64) {
+        const double re = wave_total(&dpart[(int64_t)c * G].x, G, 2);
+        const double im = wave_total(&dpart[(int64_t)c * G].y, G, 2);
+        if (lane == 0) hs[c] = cx<double>{re, im};
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && hacc && threadIdx.x < ncol)
+        hacc[threadIdx.x] = cx<double>{hacc[threadIdx.x].x + hs[threadIdx.x].x, hacc[threadIdx.x].y + hs[threadIdx.x].y};
+    double s = 0;
+    for (int64_t k = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kRedBlock) {
+        double re = w[k].x, im = w[k].y;
+        for (int c = 0; c < ncol; ++c) {
+            const cx<R> v = V[k + (int64_t)c * ldv];
+            re -= (double)v.x * hs[c].x - (double)v.y * hs[c].y;
+            im -= (double)v.x * hs[c].y + (double)v.y * hs[c].x;
+        }
+        const cx<R> o = {(R)re, (R)im};
+        w[k] = o;
+        s += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+    }
+    if (npart) {
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) npart[blockIdx.x] = s;
+    }
+}
+
+// dst = src / ||src|| with ||src||^2 the reduction of npart[0:G]; workgroup 0
+// writes ||src|| to *nrm.  A zero norm (exact breakdown) gives dst = 0.
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void k_scale_norm_r(cx<R> *dst, const cx<R> *src, int64_t n,
+                                                            const double *__restrict__ npart, int G,
+                                                            double *__restrict__ nrm) {
+    __shared__ double d;
+    if (threadIdx.x < 64) {
+        const double t = wave_total(npart, G);
+        if (threadIdx.x == 0) {
+            d = sqrt(t);
+            if (blockIdx.x == 0) *nrm = d;
+        }
+    }
+    __syncthreads();
+    const double dd = d;
+    for (int64_t k = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kRedBlock)
+        dst[k] = dd == 0.0 ? cx<R>{0, 0} : cx<R>{(R)(src[k].x / dd), (R)(src[k].y / dd)};
+}
+
 // x += V[:, 0:ncol] c   (or, with beta_scale, x = beta * V c when overwrite)
 template <typename R>
 __global__ __launch_bounds__(kRedBlock) void k_gemv(cx<R> *__restrict__ x, const cx<R> *__restrict__ V, int64_t ldv,
@@ -449,14 +525,36 @@ static void dev_dots(pimi_workspace *W, const cx<R> *V, int64_t ldv, int ncol, c
     hipLaunchKernelGGL(k_reduce_cols, dim3(ncol), dim3(kRedBlock), 0, W->stream, W->partial.p, kDotBlocks, h, hacc);
 }
 
+// Two passes of classical Gram-Schmidt of w against V[:, 0:ncol], the
+// coefficients accumulated into hcol.  Narrow bases (ncol <= kFusedDotCols)
+// take two launches per pass: the dot partials, then the update, which
+// reduces them itself; after the second pass W->npartial holds the partials
+// of ||w||^2 for dev_scale_normalize.  Wider bases reduce the dots in a
+// launch of their own and the norm partials follow the second update.
 template <typename R>
 static void dev_cgs2(pimi_workspace *W, const cx<R> *V, int64_t ldv, int ncol, cx<R> *w, int64_t n,
                      cx<double> *hcol) {
     for (int pass = 0; pass < 2; ++pass) {
-        dev_dots<R>(W, V, ldv, ncol, w, n, W->hvec.p, hcol);
-        hipLaunchKernelGGL(k_gs_update<R>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, w, V, ldv, ncol,
-                           W->hvec.p, n);
+        if (ncol <= kFusedDotCols) {
+            hipLaunchKernelGGL(k_dots_partial<R>, dim3(kDotBlocks, ncol), dim3(kRedBlock), 0, W->stream, V, ldv, w,
+                               n, W->partial.p);
+            hipLaunchKernelGGL(k_gs_update_r<R>, dim3(kNormBlocks), dim3(kRedBlock), 0, W->stream, w, V, ldv, ncol,
+                               W->partial.p, kDotBlocks, hcol, n, pass == 1 ? W->npartial.p : nullptr);
+        } else {
+            dev_dots<R>(W, V, ldv, ncol, w, n, W->hvec.p, hcol);
+            hipLaunchKernelGGL(k_gs_update<R>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, w, V, ldv, ncol,
+                               W->hvec.p, n);
+        }
     }
+    if (ncol > kFusedDotCols)
+        hipLaunchKernelGGL(k_norm_partial<R>, dim3(kNormBlocks), dim3(kRedBlock), 0, W->stream, w, n, W->npartial.p);
+}
+
+// dst = w / ||w|| and *nrm = ||w||, from the norm partials dev_cgs2 left
+template <typename R>
+static void dev_scale_normalize(pimi_workspace *W, cx<R> *dst, const cx<R> *w, int64_t n, double *nrm) {
+    hipLaunchKernelGGL(k_scale_norm_r<R>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, dst, w, n,
+                       W->npartial.p, kNormBlocks, nrm);
 }
 
 static void sync(pimi_workspace *W) { HIPCHK(hipStreamSynchronize(W->stream)); }
@@ -611,7 +709,7 @@ static void ensure_workspace(pimi_workspace *W) {
     W->hvec.alloc(maxcols);
     W->hacc.alloc((size_t)(m + 2));
     W->coef.alloc(m + 2);
-    W->npartial.alloc(kDotBlocks);
+    W->npartial.alloc(std::max(kDotBlocks, kNormBlocks));
     W->nscal.alloc(8);
     if (W->prec == PIMI_C32) {
         W->V32.alloc((size_t)n * (m + 1));
@@ -845,9 +943,7 @@ static GmresResult gmres_device(pimi_workspace *W, double invscale, double atol,
             applications++;
             HIPCHK(hipMemsetAsync(W->hacc.p, 0, sizeof(cx<double>) * (j + 1), W->stream));
             dev_cgs2<double>(W, V, n, j + 1, W->w.p, n, W->hacc.p);
-            dev_norm<double>(W, W->w.p, n, W->nscal.p);
-            hipLaunchKernelGGL(k_scale_div<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream,
-                               V + (int64_t)(j + 1) * n, W->w.p, n, W->nscal.p);
+            dev_scale_normalize<double>(W, V + (int64_t)(j + 1) * n, W->w.p, n, W->nscal.p);
             HIPCHK(hipMemcpyAsync(hcol.data(), W->hacc.p, sizeof(cx<double>) * (j + 1), hipMemcpyDeviceToHost,
                                   W->stream));
             HIPCHK(hipMemcpyAsync(W->hpin, W->nscal.p, sizeof(double), hipMemcpyDeviceToHost, W->stream));
@@ -965,8 +1061,7 @@ static void inner_solve_dev(pimi_workspace *W, int kin, double scale) {
         apply_precond_t<R>(W, Vi + (size_t)i * n, zt, scale, t1, t2);
         launch_apply<R>(W, zt, wt, 1.0 / scale, 1, true);
         dev_cgs2<R>(W, Vi, n, i + 1, wt, n, W->Hin.p + (size_t)i * (kin + 1));
-        dev_norm<R>(W, wt, n, nrm + i);
-        hipLaunchKernelGGL(k_scale_div<R>, gv, bv, 0, W->stream, Vi + (size_t)(i + 1) * n, wt, n, nrm + i);
+        dev_scale_normalize<R>(W, Vi + (size_t)(i + 1) * n, wt, n, nrm + i);
     }
     hipLaunchKernelGGL(k_inner_lsq, dim3(1), dim3(64), 0, W->stream, W->Hin.p, nrm, kin, W->inner_y.p);
     hipLaunchKernelGGL(k_gemv<R>, gv, bv, 0, W->stream, wt, Vi, n, kin, W->inner_y.p, n, 1.0, 1);
@@ -1068,9 +1163,7 @@ static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol
             applications++;
             HIPCHK(hipMemsetAsync(W->hacc.p, 0, sizeof(cx<double>) * (j + 1), W->stream));
             dev_cgs2<double>(W, V, n, j + 1, W->w.p, n, W->hacc.p);
-            dev_norm<double>(W, W->w.p, n, W->nscal.p);
-            hipLaunchKernelGGL(k_scale_div<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream,
-                               V + (int64_t)(j + 1) * n, W->w.p, n, W->nscal.p);
+            dev_scale_normalize<double>(W, V + (int64_t)(j + 1) * n, W->w.p, n, W->nscal.p);
             HIPCHK(hipMemcpyAsync(hcol.data(), W->hacc.p, sizeof(cx<double>) * (j + 1), hipMemcpyDeviceToHost,
                                   W->stream));
             HIPCHK(hipMemcpyAsync(W->hpin, W->nscal.p, sizeof(double), hipMemcpyDeviceToHost, W->stream));
@@ -1304,9 +1397,7 @@ static void arnoldi(pimi_workspace *W, Factorization &F) {
         dev_norm<double>(W, W->w.p, n, W->nscal.p + 2);
         HIPCHK(hipMemsetAsync(W->hacc.p, 0, sizeof(cx<double>) * (j + 1), W->stream));
         dev_cgs2<double>(W, V, n, j + 1, W->w.p, n, W->hacc.p);
-        dev_norm<double>(W, W->w.p, n, W->nscal.p + 3);
-        hipLaunchKernelGGL(k_scale_div<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream,
-                           V + (int64_t)(j + 1) * n, W->w.p, n, W->nscal.p + 3);
+        dev_scale_normalize<double>(W, V + (int64_t)(j + 1) * n, W->w.p, n, W->nscal.p + 3);
         HIPCHK(hipMemcpyAsync(hcol.data(), W->hacc.p, sizeof(cx<double>) * (j + 1), hipMemcpyDeviceToHost,
                               W->stream));
         HIPCHK(hipMemcpyAsync(W->hpin, W->nscal.p + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, W->stream));
@@ -1329,7 +1420,8 @@ static void arnoldi(pimi_workspace *W, Factorization &F) {
         launch_apply<double>(W, V + (int64_t)(q - 1) * n, W->w.p, 1.0);
         HIPCHK(hipMemsetAsync(W->hacc.p, 0, sizeof(cx<double>) * q, W->stream));
         dev_cgs2<double>(W, V, n, q, W->w.p, n, W->hacc.p);
-        dev_norm<double>(W, W->w.p, n, W->nscal.p + 3);
+        hipLaunchKernelGGL(k_reduce_norm, dim3(1), dim3(kRedBlock), 0, W->stream, W->npartial.p, kNormBlocks,
+                           W->nscal.p + 3);
         HIPCHK(hipMemcpyAsync(hcol.data(), W->hacc.p, sizeof(cx<double>) * q, hipMemcpyDeviceToHost, W->stream));
         HIPCHK(hipMemcpyAsync(W->hpin, W->nscal.p + 3, sizeof(double), hipMemcpyDeviceToHost, W->stream));
         sync(W);
