@@ -299,6 +299,29 @@ pimi_status pimi_channel_intensities(pimi_workspace *ws, const void *X, int64_t 
  * channels[c] = 0 leaves it unchanged.  Columns of one channel go through one multi-column apply. */
 pimi_status pimi_apply_jumps(pimi_workspace *ws, void *X, int64_t ncols, const int32_t *channels, int32_t on_device);
 
+/* ---- Arnoldi spectrum (src/krylov.jl:914-961, krylov_liouvillian_spectrum) ----
+ * Arnoldi factorization L V_k = V_{k+1} H_k started from q0 (host, n complex;
+ * normalized here) with two classical Gram-Schmidt passes per step; at most m
+ * steps (m <= the workspace krylovdim), stopping early once
+ * H[j+1, j] <= breakfactor * max(|L v_j|, DBL_MIN) (the reference's breakdown
+ * test, krylov.jl:941-945).  H: (m+1) x m column-major complex host array,
+ * zeroed first; *k_out = steps taken.  The basis stays in the workspace for
+ * pimi_arnoldi_vectors.  FP64 workspaces only.  PIMI_EINVAL for a zero or
+ * non-finite q0. */
+pimi_status pimi_arnoldi(pimi_workspace *ws, const double *q0, int32_t m, double breakfactor, double *H,
+                         int32_t *k_out);
+/* Ritz vectors of the last pimi_arnoldi call: out[:, c] = V[:, 0:k] Y[:, c]
+ * for c < ncols (Y: k x ncols column-major complex host; out: n x ncols). */
+pimi_status pimi_arnoldi_vectors(pimi_workspace *ws, int32_t k, const double *Y, int32_t ncols, double *out);
+
+/* ---- fixed-step RK4 (src/evolution.jl:75-118, evolve!) ----
+ * x_out = RK4 propagation of x0 over [t0, t1] in `steps` equal steps, with the
+ * reference's stage arithmetic (k2 accumulates k1 + 2 k2' + 2 k3' + k4',
+ * x += (h/6) k2).  Host vectors (n complex); all stages on the device.  FP64
+ * workspaces only; PIMI_EINVAL for steps <= 0 or a non-finite tspan. */
+pimi_status pimi_evolve_rk4(pimi_workspace *ws, const double *x0, double t0, double t1, int64_t steps,
+                            double *x_out);
+
 /* ---------------------------------------------------------------------------
  * Batch engine: many independent stationary solves advanced together on one
  * device, one kernel launch per operation for all of them.  No counterpart in

@@ -264,3 +264,37 @@ def compile_plan(N, d, terms):
     geom = low.OneBodyGeometry(basis)
     plan = low.compile_model(basis, terms, geom)
     return OraclePlan(plan), geom
+
+
+def dense_liouvillian(plan: OraclePlan):
+    """L as a dense n x n matrix, column k = L e_k from the oracle's apply
+    (small n only)."""
+    n = plan.n
+    L = np.zeros((n, n), dtype=np.complex128)
+    e = np.zeros(n, dtype=np.complex128)
+    for k in range(n):
+        e[k] = 1.0
+        L[:, k] = plan.apply(e)
+        e[k] = 0.0
+    return L
+
+
+def rk4(plan: OraclePlan, x, t0, t1, steps):
+    """Fixed-step RK4, restating evolve! (src/evolution.jl:75-118): three
+    scratch vectors, k2 accumulates k1 + 2k2 + 2k3 + k4 and x += (h/6) k2."""
+    h = (t1 - t0) / steps
+    x = np.array(x, dtype=np.complex128)
+    for _ in range(steps):
+        k1 = plan.apply(x)
+        k2 = k1.copy()
+        tmp = x + (h / 2) * k1
+        k1 = plan.apply(tmp)
+        k2 = k2 + 2 * k1
+        tmp = x + (h / 2) * k1
+        k1 = plan.apply(tmp)
+        k2 = k2 + 2 * k1
+        tmp = x + h * k1
+        k1 = plan.apply(tmp)
+        k2 = k2 + k1
+        x = x + (h / 6) * k2
+    return x

@@ -28,6 +28,7 @@
 #include <memory>
 #include <string>
 #include <vector>
+#include <cfloat>
 
 #include "pimi_internal.hpp"
 
@@ -256,6 +257,31 @@ __global__ __launch_bounds__(kRedBlock) void k_axpy(cx<R> *__restrict__ y, const
     for (int64_t k = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kRedBlock) {
         const cx<R> v = x[k];
         y[k] = cx<R>{(R)(y[k].x + are * v.x - aim * v.y), (R)(y[k].y + are * v.y + aim * v.x)};
+    }
+}
+
+// Fixed-step RK4 stage updates (src/evolution.jl:75-118): with k1 the latest
+// derivative, k2 the weighted stage accumulator and tmp the next stage state,
+//   first: k2 = k1;         tmp = x + c k1
+//   mid:   k2 = k2 + 2 k1;  tmp = x + c k1
+//   last:  k2 = k2 + k1;    x = x + c k2
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void k_rk4_stage(int stage, cx<R> *__restrict__ x, cx<R> *__restrict__ tmp,
+                                                         cx<R> *__restrict__ k2, const cx<R> *__restrict__ k1,
+                                                         int64_t n, double c) {
+    for (int64_t k = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kRedBlock) {
+        const cx<R> a = k1[k];
+        if (stage == 0) {
+            k2[k] = a;
+            tmp[k] = cx<R>{(R)(x[k].x + c * a.x), (R)(x[k].y + c * a.y)};
+        } else if (stage == 1) {
+            const cx<R> s = k2[k];
+            k2[k] = cx<R>{s.x + 2 * a.x, s.y + 2 * a.y};
+            tmp[k] = cx<R>{(R)(x[k].x + c * a.x), (R)(x[k].y + c * a.y)};
+        } else {
+            const cx<R> s = {k2[k].x + a.x, k2[k].y + a.y};
+            x[k] = cx<R>{(R)(x[k].x + c * s.x), (R)(x[k].y + c * s.y)};
+        }
     }
 }
 
@@ -2412,6 +2438,103 @@ pimi_status pimi_apply_jumps(pimi_workspace *W, void *X, int64_t ncols, const in
         }
         io.back(X, x);
         sync(W);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Arnoldi spectrum (src/krylov.jl:914-961) and fixed-step RK4 evolution
+// (src/evolution.jl:75-118)
+// ---------------------------------------------------------------------------
+pimi_status pimi_arnoldi(pimi_workspace *W, const double *q0, int32_t m, double breakfactor, double *H,
+                         int32_t *k_out) {
+    return guarded([&] {
+        if (!W || !q0 || !H || !k_out) throw Error(PIMI_EINVAL, "null argument");
+        if (W->prec != PIMI_C64) throw Error(PIMI_EINVAL, "the Arnoldi spectrum needs an FP64 workspace");
+        if (m < 1 || m > W->m) throw Error(PIMI_EINVAL, "Arnoldi dimension must lie in 1..workspace krylovdim");
+        if (!(breakfactor >= 0)) throw Error(PIMI_EINVAL, "breakfactor must be nonnegative");
+        BusyGuard guard(W->busy);
+        HIPCHK(hipSetDevice(W->plan->ctx->device));
+        const int64_t n = W->plan->n;
+        cx<double> *V = W->V.p;
+        h2d(W, W->tmp.p, q0, sizeof(cx<double>) * n);
+        dev_norm<double>(W, W->tmp.p, n, W->nscal.p);
+        const double nq = read_scalar(W, W->nscal.p);
+        if (!(nq > 0) || !std::isfinite(nq)) throw Error(PIMI_EINVAL, "initial_vector must be nonzero and finite");
+        hipLaunchKernelGGL(k_scale_div<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, V, W->tmp.p, n,
+                           W->nscal.p);
+        std::memset(H, 0, sizeof(cx<double>) * (size_t)(m + 1) * m);
+        std::vector<cx<double>> hcol((size_t)m + 1);
+        int k = 0;
+        for (int j = 0; j < m; ++j) {
+            launch_apply<double>(W, V + (int64_t)j * n, W->w.p, 1.0);
+            dev_norm<double>(W, W->w.p, n, W->nscal.p + 1);  // image norm |L v_j|
+            HIPCHK(hipMemsetAsync(W->hacc.p, 0, sizeof(cx<double>) * (j + 1), W->stream));
+            dev_cgs2<double>(W, V, n, j + 1, W->w.p, n, W->hacc.p);
+            hipLaunchKernelGGL(k_reduce_norm, dim3(1), dim3(kRedBlock), 0, W->stream, W->npartial.p, kNormBlocks,
+                               W->nscal.p + 2);
+            HIPCHK(hipMemcpyAsync(hcol.data(), W->hacc.p, sizeof(cx<double>) * (j + 1), hipMemcpyDeviceToHost,
+                                  W->stream));
+            HIPCHK(hipMemcpyAsync(W->hpin, W->nscal.p + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, W->stream));
+            sync(W);
+            const double image = W->hpin[0], hnext = W->hpin[1];
+            for (int i = 0; i <= j; ++i) {
+                H[2 * ((size_t)i + (size_t)j * (m + 1))] = hcol[(size_t)i].x;
+                H[2 * ((size_t)i + (size_t)j * (m + 1)) + 1] = hcol[(size_t)i].y;
+            }
+            H[2 * ((size_t)(j + 1) + (size_t)j * (m + 1))] = hnext;
+            k = j + 1;
+            if (j + 1 == m || std::abs(hnext) <= breakfactor * std::max(image, DBL_MIN)) break;
+            hipLaunchKernelGGL(k_scale_div<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream,
+                               V + (int64_t)(j + 1) * n, W->w.p, n, W->nscal.p + 2);
+        }
+        sync(W);
+        *k_out = k;
+    });
+}
+
+pimi_status pimi_arnoldi_vectors(pimi_workspace *W, int32_t k, const double *Y, int32_t ncols, double *out) {
+    return guarded([&] {
+        if (!W || !Y || !out) throw Error(PIMI_EINVAL, "null argument");
+        if (W->prec != PIMI_C64) throw Error(PIMI_EINVAL, "the Arnoldi spectrum needs an FP64 workspace");
+        if (k < 1 || k > W->m) throw Error(PIMI_EINVAL, "k must lie in 1..workspace krylovdim");
+        if (ncols < 0) throw Error(PIMI_EINVAL, "ncols must be nonnegative");
+        BusyGuard guard(W->busy);
+        HIPCHK(hipSetDevice(W->plan->ctx->device));
+        const int64_t n = W->plan->n;
+        for (int32_t c = 0; c < ncols; ++c) {
+            h2d(W, W->coef.p, Y + 2 * (size_t)c * k, sizeof(cx<double>) * k);
+            hipLaunchKernelGGL(k_gemv<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, W->tmp.p, W->V.p, n,
+                               k, W->coef.p, n, 1.0, 1);
+            d2h(W, out + 2 * (size_t)c * n, W->tmp.p, sizeof(cx<double>) * n);
+        }
+    });
+}
+
+pimi_status pimi_evolve_rk4(pimi_workspace *W, const double *x0, double t0, double t1, int64_t steps, double *x_out) {
+    return guarded([&] {
+        if (!W || !x0 || !x_out) throw Error(PIMI_EINVAL, "null argument");
+        if (W->prec != PIMI_C64) throw Error(PIMI_EINVAL, "RK4 evolution needs an FP64 workspace");
+        if (steps <= 0) throw Error(PIMI_EINVAL, "steps must be positive");
+        if (!std::isfinite(t0) || !std::isfinite(t1)) throw Error(PIMI_EINVAL, "tspan must be finite");
+        BusyGuard guard(W->busy);
+        HIPCHK(hipSetDevice(W->plan->ctx->device));
+        const int64_t n = W->plan->n;
+        const double h = (t1 - t0) / (double)steps;
+        cx<double> *x = W->cur.p, *tmp = W->tmp.p, *k1 = W->w.p, *k2 = W->trial.p;
+        h2d(W, x, x0, sizeof(cx<double>) * n);
+        const dim3 g(grid_for(n)), b(kRedBlock);
+        for (int64_t s = 0; s < steps; ++s) {
+            launch_apply<double>(W, x, k1, 1.0);
+            hipLaunchKernelGGL(k_rk4_stage<double>, g, b, 0, W->stream, 0, x, tmp, k2, k1, n, h / 2);
+            launch_apply<double>(W, tmp, k1, 1.0);
+            hipLaunchKernelGGL(k_rk4_stage<double>, g, b, 0, W->stream, 1, x, tmp, k2, k1, n, h / 2);
+            launch_apply<double>(W, tmp, k1, 1.0);
+            hipLaunchKernelGGL(k_rk4_stage<double>, g, b, 0, W->stream, 1, x, tmp, k2, k1, n, h);
+            launch_apply<double>(W, tmp, k1, 1.0);
+            hipLaunchKernelGGL(k_rk4_stage<double>, g, b, 0, W->stream, 2, x, tmp, k2, k1, n, h / 6);
+        }
+        HIPCHK(hipGetLastError());
+        d2h(W, x_out, x, sizeof(cx<double>) * n);
     });
 }
 

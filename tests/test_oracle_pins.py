@@ -269,3 +269,33 @@ def test_sector_weighting_pins_weakly_driven_observable():
         errs[w] = abs(np.vdot(nop, x).real - exact) / exact
     assert errs["sector"] <= 1e-7, errs
     assert errs[None] >= 100 * errs["sector"], errs
+
+
+def _small_model():
+    sp = L.spin_matrices(2)
+    return O.compile_plan(4, 2, L.qubit_ensemble_terms(hamiltonian=sp["jz"] + 0.7 * sp["jx"], emission=0.3,
+                                                        dephasing=0.1, pumping=0.05, collective_emission=0.2))[0]
+
+
+def test_rk4_matches_dense_exponential_at_fourth_order():
+    """The oracle's RK4 (evolve!, src/evolution.jl:75-118) against expm(L t):
+    accurate at fine steps, and the error falls 16x per halving of h (a wrong
+    stage weight or stage point drops the order and fails the ratio)."""
+    plan = _small_model()
+    Ld = O.dense_liouvillian(plan)
+    rng = np.random.default_rng(5)
+    x0 = rng.standard_normal(plan.n) + 1j * rng.standard_normal(plan.n)
+    exact = sla.expm(Ld * 1.5) @ x0
+    errs = [np.linalg.norm(O.rk4(plan, x0, 0.0, 1.5, s) - exact) / np.linalg.norm(exact) for s in (40, 80, 640)]
+    assert errs[2] <= 3e-10, errs
+    assert 13.0 <= errs[0] / errs[1] <= 19.0, errs
+
+
+def test_dense_liouvillian_spectrum_is_a_lindblad_spectrum():
+    """Trace preservation gives one eigenvalue 0 (the steady state) and the
+    rest decays: max real part 0, all others negative."""
+    Ld = O.dense_liouvillian(_small_model())
+    ev = np.linalg.eigvals(Ld)
+    ev = ev[np.argsort(-ev.real)]
+    assert abs(ev[0]) <= 1e-12 * np.abs(ev).max()
+    assert ev[1].real < -1e-6
