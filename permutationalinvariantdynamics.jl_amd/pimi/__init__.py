@@ -16,3 +16,11 @@ from .trajectories import (  # noqa: F401
     batched_apply_jumps, batched_channel_intensities, batched_conditional_action, batched_conditional_rk4,
     trajectory_channels,
 )
+from .checkpoints import (  # noqa: F401
+    PI_CHECKPOINT_VERSION, PIStateCheckpoint, checkpoint_state, load_checkpoint, make_checkpoint, save_checkpoint,
+)
+from .families import (  # noqa: F401
+    CompiledPIModelFamily, ParameterScanPlan, ParameterScanPoint, ParameterScanResult, SpecializedPIModel,
+    compile_family, parameter_scan, specialize,
+)
+from .spectral import evolve, krylov_liouvillian_spectrum, time_evolve  # noqa: F401

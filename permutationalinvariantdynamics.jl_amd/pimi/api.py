@@ -72,16 +72,31 @@ def _weyl_dimension(p):
 
 
 class PIBasis:
-    """Complete PI operator basis for N identical d-level systems: sectors in
-    descending lexicographic order, block dimension = Weyl dimension."""
+    """PI operator basis for N identical d-level systems (src/basis.jl:11-35):
+    sectors in descending lexicographic order, block dimension = Weyl
+    dimension.  `sectors` retains a subset of the complete list (kept in that
+    order); the device backend compiles complete bases only."""
 
-    def __init__(self, N: int, d: int):
+    def __init__(self, N: int, d: int, sectors=None):
         if N < 0:
             raise ValueError("N must be nonnegative")
         if d < 1:
             raise ValueError("d must be positive")
         self.N, self.d = int(N), int(d)
-        self.sectors = _partitions(self.N, self.d)
+        full = _partitions(self.N, self.d)
+        if sectors is None:
+            self.sectors = full
+        else:
+            want = [tuple(int(v) for v in p) for p in sectors]
+            if not want:
+                raise ValueError("a PI basis needs at least one sector")
+            if len(set(want)) != len(want):
+                raise ValueError("sectors must be unique")
+            bad = [p for p in want if p not in full]
+            if bad:
+                raise ValueError(f"{bad[0]} is not a partition of N={self.N} into {self.d} parts")
+            self.sectors = [p for p in full if p in want]
+        self.complete = len(self.sectors) == len(full)
         self.dims = [_weyl_dimension(p) for p in self.sectors]
         self.offsets = np.concatenate([[0], np.cumsum([m * m for m in self.dims])]).astype(np.int64)
 
@@ -89,10 +104,11 @@ class PIBasis:
         return int(self.offsets[-1])
 
     def __eq__(self, other):
-        return isinstance(other, PIBasis) and (self.N, self.d) == (other.N, other.d)
+        return isinstance(other, PIBasis) and (self.N, self.d) == (other.N, other.d) and \
+            self.sectors == other.sectors
 
     def __hash__(self):
-        return hash((self.N, self.d))
+        return hash((self.N, self.d, tuple(self.sectors)))
 
     def __repr__(self):
         return f"PIBasis(N={self.N}, d={self.d}, sectors={len(self.sectors)}, dimension={len(self)})"
@@ -214,7 +230,8 @@ class Models:
 
 class PIState:
     def __init__(self, basis: PIBasis, data):
-        data = np.asarray(data, dtype=complex)
+        data = np.asarray(data)
+        data = data.astype(np.complex64 if data.dtype in (np.complex64, np.float32) else np.complex128, copy=False)
         if data.shape != (len(basis),):
             raise ValueError("coefficient vector has wrong length")
         self.basis = basis
@@ -237,16 +254,21 @@ def _symmetric_occupation_index(counts):
     return zb + 1
 
 
-def computational_product_state(basis: PIBasis, level: int) -> PIState:
-    """|level><level|^N (one-based level) in the fully symmetric sector."""
+def computational_product_state(basis: PIBasis, level: int, T=np.float64) -> PIState:
+    """|level><level|^N (one-based level) in the fully symmetric sector
+    (src/spin.jl:79-86); T is the real scalar type (float64 or float32)."""
     if not (1 <= level <= basis.d):
         raise ValueError(f"level must be a one-based local-state index in 1:{basis.d}")
+    sym = tuple([basis.N] + [0] * (basis.d - 1))
+    if sym not in basis.sectors:
+        raise ValueError("the basis does not retain the fully symmetric sector")
+    s = basis.sectors.index(sym)
     counts = [0] * basis.d
     counts[level - 1] = basis.N
     a = _symmetric_occupation_index(counts) - 1
-    m = basis.dims[0]
-    v = np.zeros(len(basis), dtype=complex)
-    v[a + a * m] = 1.0
+    m = basis.dims[s]
+    v = np.zeros(len(basis), dtype=np.complex64 if np.dtype(T) == np.float32 else np.complex128)
+    v[basis.offsets[s] + a + a * m] = 1.0
     return PIState(basis, v)
 
 
@@ -332,6 +354,8 @@ class CompiledPIModel:
     on demand per (precision, krylovdim)."""
 
     def __init__(self, model: PIModel, device=0, host_only=False):
+        if not model.basis.complete:
+            raise ValueError("the mi355x backend compiles complete PI bases only")
         self.model = model
         self.basis = model.basis
         self.device = device
