@@ -551,21 +551,22 @@ static void dev_dots(pimi_workspace *W, const cx<R> *V, int64_t ldv, int ncol, c
     hipLaunchKernelGGL(k_reduce_cols, dim3(ncol), dim3(kRedBlock), 0, W->stream, W->partial.p, kDotBlocks, h, hacc);
 }
 
-// Two passes of classical Gram-Schmidt of w against V[:, 0:ncol], the
-// coefficients accumulated into hcol.  Narrow bases (ncol <= kFusedDotCols)
-// take two launches per pass: the dot partials, then the update, which
-// reduces them itself; after the second pass W->npartial holds the partials
-// of ||w||^2 for dev_scale_normalize.  Wider bases reduce the dots in a
-// launch of their own and the norm partials follow the second update.
+// `passes` (two by default) passes of classical Gram-Schmidt of w against
+// V[:, 0:ncol], the coefficients accumulated into hcol.  Narrow bases
+// (ncol <= kFusedDotCols) take two launches per pass: the dot partials, then
+// the update, which reduces them itself; after the last pass W->npartial
+// holds the partials of ||w||^2 for dev_scale_normalize.  Wider bases reduce
+// the dots in a launch of their own and the norm partials follow the last
+// update.
 template <typename R>
 static void dev_cgs2(pimi_workspace *W, const cx<R> *V, int64_t ldv, int ncol, cx<R> *w, int64_t n,
-                     cx<double> *hcol) {
-    for (int pass = 0; pass < 2; ++pass) {
+                     cx<double> *hcol, int passes = 2) {
+    for (int pass = 0; pass < passes; ++pass) {
         if (ncol <= kFusedDotCols) {
             hipLaunchKernelGGL(k_dots_partial<R>, dim3(kDotBlocks, ncol), dim3(kRedBlock), 0, W->stream, V, ldv, w,
                                n, W->partial.p);
             hipLaunchKernelGGL(k_gs_update_r<R>, dim3(kNormBlocks), dim3(kRedBlock), 0, W->stream, w, V, ldv, ncol,
-                               W->partial.p, kDotBlocks, hcol, n, pass == 1 ? W->npartial.p : nullptr);
+                               W->partial.p, kDotBlocks, hcol, n, pass == passes - 1 ? W->npartial.p : nullptr);
         } else {
             dev_dots<R>(W, V, ldv, ncol, w, n, W->hvec.p, hcol);
             hipLaunchKernelGGL(k_gs_update<R>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, w, V, ldv, ncol,
@@ -1086,7 +1087,10 @@ static void inner_solve_dev(pimi_workspace *W, int kin, double scale) {
     for (int i = 0; i < kin; ++i) {
         apply_precond_t<R>(W, Vi + (size_t)i * n, zt, scale, t1, t2);
         launch_apply<R>(W, zt, wt, 1.0 / scale, 1, true);
-        dev_cgs2<R>(W, Vi, n, i + 1, wt, n, W->Hin.p + (size_t)i * (kin + 1));
+        // one Gram-Schmidt pass: the inner solve only preconditions the
+        // flexible outer iteration, and at <= 32 vectors one classical pass
+        // keeps the inner basis orthogonal to far below its tolerance
+        dev_cgs2<R>(W, Vi, n, i + 1, wt, n, W->Hin.p + (size_t)i * (kin + 1), 1);
         dev_scale_normalize<R>(W, Vi + (size_t)(i + 1) * n, wt, n, nrm + i);
     }
     hipLaunchKernelGGL(k_inner_lsq, dim3(1), dim3(64), 0, W->stream, W->Hin.p, nrm, kin, W->inner_y.p);
