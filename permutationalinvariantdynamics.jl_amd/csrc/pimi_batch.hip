@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -55,6 +56,21 @@ struct BApply {
     const ApplyEntry *entries;
     const int32_t *col;
     const void *val;
+    int64_t n;
+    double out_scale;
+};
+
+struct BDia {  // banded apply of one job (k_apply_dia's tables)
+    int64_t first;
+    const void *x;
+    void *y;
+    const int64_t *offs;
+    const int32_t *dims;
+    const ApplyTile *tiles;
+    const int32_t *gbase;
+    const DiaGroup *groups;
+    const DiaEntry *ents;
+    const void *coef;
     int64_t n;
     double out_scale;
 };
@@ -128,6 +144,51 @@ __global__ __launch_bounds__(kApplyWave) void kb_apply(const BApply *__restrict_
     const BApply A = J[seg_of(J, nj, g)];
     apply_tile<R, kApplyCols>((const cx<R> *)A.x, (cx<R> *)A.y, (R)A.out_scale, A.tiles[g - A.first], A.offs,
                               A.dims, A.entry_base, A.entries, A.col, (const cx<R> *)A.val);
+}
+
+// The same through the banded form: kDiaWaves tiles (waves) per workgroup,
+// workgroups split into one contiguous slab per XCD; each wave finds its job.
+template <typename R, int TB>
+__global__ __launch_bounds__(kApplyWave *kDiaWaves) void kb_apply_dia(const BDia *__restrict__ J, int nj,
+                                                                      int64_t total, int64_t per_xcd) {
+    const int64_t bid = blockIdx.x;
+    const int64_t blk = (bid % kNumXcd) * per_xcd + bid / kNumXcd;
+    const int64_t g = blk * kDiaWaves + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (g >= total) return;
+    const PIMI_CONST BDia *Jc = cptr(J);
+    int lo = 0, hi = nj - 1;  // the wave's job: binary search on first (scalar)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (Jc[mid].first <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const BDia A = Jc[lo];
+    dia_tile<R, TB>((const cx<R> *)A.x, (cx<R> *)A.y, (R)A.out_scale, cptr(A.tiles)[g - A.first], A.offs, A.dims,
+                    A.gbase, A.groups, A.ents, (const cx<R> *)A.coef);
+}
+
+template <typename R>
+static void launch_kb_dia(hipStream_t st, const BDia *d, int nj, int64_t tot, int w) {
+    const int64_t nblk = (tot + kDiaWaves - 1) / kDiaWaves;
+    const int64_t per_xcd = (nblk + kNumXcd - 1) / kNumXcd;
+    const dim3 grid((unsigned)(per_xcd * kNumXcd)), block(kApplyWave * kDiaWaves);
+    switch (kDiaWidths[w]) {
+        case 4: hipLaunchKernelGGL((kb_apply_dia<R, 4>), grid, block, 0, st, d, nj, tot, per_xcd); break;
+        case 8: hipLaunchKernelGGL((kb_apply_dia<R, 8>), grid, block, 0, st, d, nj, tot, per_xcd); break;
+        default: hipLaunchKernelGGL((kb_apply_dia<R, 16>), grid, block, 0, st, d, nj, tot, per_xcd); break;
+    }
+}
+
+// banded-apply descriptor of one plan (coefficients in precision R, weighted or not)
+template <typename R>
+static BDia dia_desc(int64_t first, const pimi_plan *P, const void *x, void *y, bool weighted, double out_scale,
+                     int w) {
+    const DevStencil &S = P->main;
+    const void *coef;
+    if constexpr (sizeof(R) == 8) coef = weighted ? (const void *)S.coefw64.p : (const void *)S.coef64.p;
+    else coef = weighted ? (const void *)S.coefw32.p : (const void *)S.coef32.p;
+    return BDia{first, x, y, P->offs.p, P->dims.p, P->dtiles[w].p, S.gbase.p, S.groups.p, S.dents.p, coef, P->n,
+                out_scale};
 }
 
 // y = x * w[sector] (w = 1/sigma: back from the weighted coordinates), one
@@ -632,6 +693,7 @@ struct pimi_batch {
     pimi::DevBuf<pimi::cx<double>> dmail;
     std::atomic<int> busy{0};
     int64_t ticks = 0;
+    int dia_w = 1;  // banded-apply tile width index (kDiaWidths) of the engine's launches
     bool trace = false;
     bool timing = false;  // PIMI_BATCH_TIMING: HIP events between launches, per operation class
     std::vector<hipEvent_t> events;
@@ -705,9 +767,18 @@ struct Tick {
                bool unit) {
         if (js.empty()) return;
         std::vector<BApply> v;
-        int64_t tot = 0;
+        std::vector<BDia> vd;
+        int64_t tot = 0, totd = 0;
+        const int w = E->dia_w;
         for (BJob *J : js) {
             pimi_plan *P = J->plan;
+            if (P->main.dia_ok) {
+                vd.push_back(dia_desc<R>(totd, P, in(J), out(J), !unit && J->weighted, unit ? 1.0 : 1.0 / J->scale,
+                                         w));
+                totd += P->ndtiles[w];
+                bytes[cls] += 2.0 * J->n * sizeof(cx<R>) + (sizeof(R) == 8 ? J->table_bytes64 : J->table_bytes32);
+                continue;
+            }
             const void *val = sizeof(R) == 8 ? (const void *)P->main.val64.p : (const void *)P->main.val32.p;
             const ApplyEntry *entries = (!unit && J->weighted) ? P->main.entries_w.p : P->main.entries.p;
             v.push_back(BApply{tot, in(J), out(J), P->offs.p, P->dims.p, P->tile_sector.p, P->atiles4.p,
@@ -716,10 +787,16 @@ struct Tick {
             tot += P->natiles4;
             bytes[cls] += 2.0 * J->n * sizeof(cx<R>) + (sizeof(R) == 8 ? J->table_bytes64 : J->table_bytes32);
         }
+        hipStream_t st = E->stream;
+        if (!vd.empty()) {
+            const size_t offd = put(vd);
+            const int njd = (int)vd.size();
+            push([=](const char *d) { launch_kb_dia<R>(st, (const BDia *)(d + offd), njd, totd, w); });
+        }
+        if (v.empty()) return;
         const size_t off = put(v);
         const int nj = (int)v.size();
         const int64_t per_xcd = (tot + kNumXcd - 1) / kNumXcd;
-        hipStream_t st = E->stream;
         push([=](const char *d) {
             hipLaunchKernelGGL(kb_apply<R>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyWave), 0, st,
                                (const BApply *)(d + off), nj, tot, per_xcd);
@@ -1343,6 +1420,10 @@ pimi_status pimi_batch_create(pimi_context *ctx, const pimi_batch_opts *opts, pi
         }
         const char *tr = std::getenv("PIMI_TRACE");
         E->trace = tr && tr[0] == '1';
+        E->dia_w = E->prec == PIMI_C32 ? 1 : 0;  // measured best at 512 qubit models: 8 columns FP32, 4 FP64
+        if (const char *tb = std::getenv("PIMI_DIA_TB_BATCH"))
+            for (int w = 0; w < kNumDiaWidths; ++w)
+                if (kDiaWidths[w] == std::atoi(tb)) E->dia_w = w;
         E->timing = (opts->flags & PIMI_BATCH_TIMING) != 0;
         HIPCHK(hipStreamCreate(&E->stream));
         E->arena.init(align_up(bytes, 256));
@@ -1522,7 +1603,8 @@ pimi_status pimi_batch_apply_bench(pimi_batch *E, pimi_plan *const *plans, int32
         const size_t es = E->prec == PIMI_C32 ? sizeof(cx<float>) : sizeof(cx<double>);
         std::vector<DevBuf<char>> xs((size_t)nplans), ys((size_t)nplans);
         std::vector<BApply> v;
-        int64_t tot = 0, bytes = 0;
+        std::vector<BDia> vd;
+        int64_t tot = 0, totd = 0, bytes = 0;
         for (int32_t k = 0; k < nplans; ++k) {
             pimi_plan *P = plans[k];
             if (!P || !P->ctx || P->ctx->device != E->ctx->device)
@@ -1530,23 +1612,42 @@ pimi_status pimi_batch_apply_bench(pimi_batch *E, pimi_plan *const *plans, int32
             xs[k].alloc(es * (size_t)P->n);
             ys[k].alloc(es * (size_t)P->n);
             HIPCHK(hipMemsetAsync(xs[k].p, 0x3f, es * (size_t)P->n, E->stream));
+            bytes += 2 * P->n * (int64_t)es + P->main.table_bytes((int)es);
+            if (P->main.dia_ok) {
+                vd.push_back(E->prec == PIMI_C32 ? dia_desc<float>(totd, P, xs[k].p, ys[k].p, false, 1.0, E->dia_w)
+                                                 : dia_desc<double>(totd, P, xs[k].p, ys[k].p, false, 1.0, E->dia_w));
+                totd += P->ndtiles[E->dia_w];
+                continue;
+            }
             const void *val = E->prec == PIMI_C32 ? (const void *)P->main.val32.p : (const void *)P->main.val64.p;
             v.push_back(BApply{tot, xs[k].p, ys[k].p, P->offs.p, P->dims.p, P->tile_sector.p, P->atiles4.p,
                                P->main.entry_base.p, P->main.entries.p, P->main.col.p, val, P->n, 1.0});
             tot += P->natiles4;
-            bytes += 2 * P->n * (int64_t)es + P->main.table_bytes((int)es);
         }
         DevBuf<BApply> dv;
-        dv.alloc(v.size());
-        HIPCHK(hipMemcpyAsync(dv.p, v.data(), sizeof(BApply) * v.size(), hipMemcpyHostToDevice, E->stream));
+        DevBuf<BDia> dvd;
+        if (!v.empty()) {
+            dv.alloc(v.size());
+            HIPCHK(hipMemcpyAsync(dv.p, v.data(), sizeof(BApply) * v.size(), hipMemcpyHostToDevice, E->stream));
+        }
+        if (!vd.empty()) {
+            dvd.alloc(vd.size());
+            HIPCHK(hipMemcpyAsync(dvd.p, vd.data(), sizeof(BDia) * vd.size(), hipMemcpyHostToDevice, E->stream));
+        }
         const int64_t per_xcd = (tot + kNumXcd - 1) / kNumXcd;
+        const int nv = (int)v.size(), nvd = (int)vd.size();
         auto launch = [&] {
+            if (nvd) {
+                if (E->prec == PIMI_C32) launch_kb_dia<float>(E->stream, dvd.p, nvd, totd, E->dia_w);
+                else launch_kb_dia<double>(E->stream, dvd.p, nvd, totd, E->dia_w);
+            }
+            if (!nv) return;
             if (E->prec == PIMI_C32)
                 hipLaunchKernelGGL(kb_apply<float>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyWave), 0,
-                                   E->stream, dv.p, nplans, tot, per_xcd);
+                                   E->stream, dv.p, nv, tot, per_xcd);
             else
                 hipLaunchKernelGGL(kb_apply<double>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyWave), 0,
-                                   E->stream, dv.p, nplans, tot, per_xcd);
+                                   E->stream, dv.p, nv, tot, per_xcd);
         };
         launch();  // warm-up
         hipEvent_t e0, e1;

@@ -8,6 +8,7 @@
 #include <atomic>
 #include <climits>
 #include <complex>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -107,7 +108,183 @@ struct DevStencil {
             if (h_entries.size() >= (size_t)INT32_MAX) throw Error(PIMI_EINVAL, "stencil entry list too long");
             h_entry_base.push_back((int32_t)h_entries.size());
         }
+        build_dia(st, dims, offs);
     }
+    // Banded form (k_apply_dia): built when every factor is a sum of
+    // diagonals and each (input sector, row shift) group's column shifts
+    // span at most three diagonals; dia_ok false otherwise (the ELL entries
+    // above are then the only form).
+    bool dia_ok = false;
+    std::vector<int32_t> h_gbase;  // ns + 1
+    std::vector<DiaGroup> h_groups;
+    std::vector<DiaEntry> h_dents;
+    std::vector<cd> h_coef;
+    std::vector<int32_t> h_dent_sector, h_dent_in;  // per entry: output and input sector (weighting)
+    DevBuf<int32_t> gbase;
+    DevBuf<DiaGroup> groups;
+    DevBuf<DiaEntry> dents;
+    DevBuf<cx<double>> coef64, coefw64;
+    DevBuf<cx<float>> coef32, coefw32;
+    bool dia_w_ready = false;
+
+    void build_dia(const Stencil &st, const std::vector<int64_t> &dims, const std::vector<int64_t> &offs) {
+        dia_ok = false;
+        h_gbase.assign(1, 0);
+        h_groups.clear();
+        h_dents.clear();
+        h_coef.clear();
+        h_dent_sector.clear();
+        h_dent_in.clear();
+        h_dims = dims;
+        const int ns = (int)dims.size();
+        // diagonals of one factor: d -> coefficient per output row (m rows)
+        auto diagonals = [&](bool identity, int64_t base, int32_t w, int64_t m) {
+            std::map<int, std::vector<cd>> D;
+            if (identity) {
+                D[0].assign((size_t)m, cd(1.0, 0.0));
+                return D;
+            }
+            for (int64_t a = 0; a < m; ++a)
+                for (int k = 0; k < w; ++k) {
+                    const size_t q = (size_t)(base + (int64_t)k * m + a);
+                    if (st.val[q] == cd(0.0, 0.0)) continue;
+                    auto &v = D[st.col[q] - (int)a];
+                    if (v.empty()) v.assign((size_t)m, cd(0.0, 0.0));
+                    v[(size_t)a] += st.val[q];
+                }
+            return D;
+        };
+        struct PendingEntry {
+            int kind, dB;
+            std::vector<cd> ca, wb;
+        };
+        for (int q = 0; q < ns; ++q) {
+            const int64_t m = dims[(size_t)q];
+            // groups of this sector in first-appearance order
+            std::vector<std::pair<int, int>> keys;
+            std::vector<std::vector<PendingEntry>> pend;
+            for (int64_t t = st.term_base[(size_t)q]; t < st.term_base[(size_t)q + 1]; ++t) {
+                const StencilTerm &T = st.terms[(size_t)t];
+                const cd s(T.s_re, T.s_im);
+                const bool ia = T.flags & 1, ib = T.flags & 2;
+                auto DA = diagonals(ia, T.a_base, T.wa, m);
+                auto DB = diagonals(ib, T.b_base, T.wb, m);
+                for (auto &pa : DA)
+                    for (auto &pb : DB) {
+                        PendingEntry E;
+                        E.dB = pb.first;
+                        E.ca.assign((size_t)m, cd(0.0, 0.0));
+                        E.wb.assign((size_t)m, cd(0.0, 0.0));
+                        if (ib) {  // left (both identity included): scale on the row side
+                            E.kind = 0;
+                            for (int64_t a = 0; a < m; ++a) E.ca[(size_t)a] = s * pa.second[(size_t)a];
+                        } else if (ia) {
+                            E.kind = 1;
+                            for (int64_t b = 0; b < m; ++b) E.wb[(size_t)b] = s * std::conj(pb.second[(size_t)b]);
+                        } else {
+                            E.kind = 2;
+                            bool real = true;
+                            for (int64_t a = 0; a < m; ++a) {
+                                E.ca[(size_t)a] = s * pa.second[(size_t)a];
+                                E.wb[(size_t)a] = std::conj(pb.second[(size_t)a]);
+                                real = real && E.ca[(size_t)a].imag() == 0.0 && E.wb[(size_t)a].imag() == 0.0;
+                            }
+                            if (real) E.kind = 3;
+                        }
+                        const std::pair<int, int> key(T.in_sector, pa.first);
+                        size_t g = 0;
+                        while (g < keys.size() && keys[g] != key) ++g;
+                        if (g == keys.size()) {
+                            keys.push_back(key);
+                            pend.emplace_back();
+                        }
+                        pend[g].push_back(std::move(E));
+                    }
+            }
+            for (size_t g = 0; g < keys.size(); ++g) {
+                int lo = INT32_MAX, hi = INT32_MIN;
+                for (auto &E : pend[g]) {
+                    lo = std::min(lo, E.dB);
+                    hi = std::max(hi, E.dB);
+                }
+                if (hi - lo > 2) return;  // not banded enough: ELL form only
+                const int nu = keys[g].first;
+                DiaGroup G{offs[(size_t)nu], (int32_t)dims[(size_t)nu], keys[g].second, lo, hi - lo,
+                           (int32_t)h_dents.size(), 0};
+                // every coefficient vector starts on a kDiaCoefPad boundary and
+                // is zero-padded to a multiple of it, so a tile reads its TB
+                // column coefficients as one unclamped contiguous (scalar) load
+                auto put = [&](const std::vector<cd> &v) {
+                    const int32_t q = (int32_t)h_coef.size();
+                    h_coef.insert(h_coef.end(), v.begin(), v.end());
+                    h_coef.resize((size_t)q + (v.size() + kDiaCoefPad - 1) / kDiaCoefPad * kDiaCoefPad,
+                                  cd(0.0, 0.0));
+                    return q;
+                };
+                for (auto &E : pend[g]) {
+                    DiaEntry D{E.kind, E.dB - lo, 0, 0};
+                    if (E.kind != 1) D.qa = put(E.ca);
+                    if (E.kind != 0) D.qb = put(E.wb);
+                    if (h_coef.size() >= (size_t)INT32_MAX) return;
+                    h_dents.push_back(D);
+                    h_dent_sector.push_back(q);
+                    h_dent_in.push_back(nu);
+                }
+                G.e1 = (int32_t)h_dents.size();
+                h_groups.push_back(G);
+            }
+            h_gbase.push_back((int32_t)h_groups.size());
+        }
+        if (h_coef.empty()) h_coef.push_back(cd(0.0, 0.0));
+        dia_ok = true;
+    }
+    void push_dia() {
+        if (!dia_ok) return;
+        std::vector<cx<double>> c64(h_coef.size());
+        std::vector<cx<float>> c32(h_coef.size());
+        for (size_t k = 0; k < h_coef.size(); ++k) {
+            c64[k] = cx<double>{h_coef[k].real(), h_coef[k].imag()};
+            c32[k] = cx<float>{(float)h_coef[k].real(), (float)h_coef[k].imag()};
+        }
+        gbase.upload(h_gbase.data(), h_gbase.size());
+        if (h_groups.empty()) groups.alloc(1);  // no entries: the kernel reads no group
+        else groups.upload(h_groups.data(), h_groups.size());
+        if (h_dents.empty()) dents.alloc(1);
+        else dents.upload(h_dents.data(), h_dents.size());
+        coef64.upload(c64.data(), c64.size());
+        coef32.upload(c32.data(), c32.size());
+    }
+    // coefficients of S L S^{-1}: every entry's scaled side (rows for left
+    // and sandwich, columns for right) times sigma_out / sigma_in
+    void push_dia_weighted(const std::vector<double> &sigma) {
+        if (!dia_ok || dia_w_ready) return;
+        std::vector<cd> w(h_coef);
+        for (size_t e = 0; e < h_dents.size(); ++e) {
+            const double r = sigma[(size_t)h_dent_sector[e]] / sigma[(size_t)h_dent_in[e]];
+            if (r == 1.0) continue;
+            const DiaEntry &D = h_dents[e];
+            const int32_t q = D.kind == 1 ? D.qb : D.qa;
+            const size_t len = (size_t)dims_of_entry(e);
+            for (size_t k = 0; k < len; ++k) w[(size_t)q + k] *= r;
+        }
+        std::vector<cx<double>> c64(w.size());
+        std::vector<cx<float>> c32(w.size());
+        for (size_t k = 0; k < w.size(); ++k) {
+            c64[k] = cx<double>{w[k].real(), w[k].imag()};
+            c32[k] = cx<float>{(float)w[k].real(), (float)w[k].imag()};
+        }
+        coefw64.upload(c64.data(), c64.size());
+        coefw32.upload(c32.data(), c32.size());
+        dia_w_ready = true;
+    }
+    std::vector<int64_t> h_dims;  // sector sizes (banded coefficient vector lengths)
+    int64_t dims_of_entry(size_t e) const { return h_dims[(size_t)h_dent_sector[e]]; }
+    int64_t dia_table_bytes(int elem) const {
+        return (int64_t)(h_groups.size() * sizeof(DiaGroup) + h_dents.size() * sizeof(DiaEntry) +
+                         h_gbase.size() * 4) +
+               (int64_t)h_coef.size() * elem;
+    }
+
     void push() {
         std::vector<cx<double>> v64(h_val.size());
         std::vector<cx<float>> v32(h_val.size());
@@ -115,6 +292,7 @@ struct DevStencil {
             v64[k] = cx<double>{h_val[k].real(), h_val[k].imag()};
             v32[k] = cx<float>{(float)h_val[k].real(), (float)h_val[k].imag()};
         }
+        push_dia();
         entry_base.upload(h_entry_base.data(), h_entry_base.size());
         entries.upload(h_entries.data(), h_entries.size());
         col.upload(h_col.data(), h_col.size());
@@ -126,6 +304,7 @@ struct DevStencil {
         push();
     }
     void push_weighted(const std::vector<double> &sigma) {
+        push_dia_weighted(sigma);
         std::vector<ApplyEntry> w(h_entries);
         for (size_t q = 0; q + 1 < h_entry_base.size(); ++q)
             for (int32_t k = h_entry_base[q]; k < h_entry_base[q + 1]; ++k) {
@@ -137,6 +316,7 @@ struct DevStencil {
     }
     // bytes of the tables one apply reads (entries, slot tables, sector sizes)
     int64_t table_bytes(int elem) const {
+        if (dia_ok) return dia_table_bytes(elem);
         return (int64_t)(h_entries.size() * sizeof(ApplyEntry) + h_entry_base.size() * 4 + h_col.size() * 4 +
                          (h_entry_base.size() - 1) * 4) +
                (int64_t)h_val.size() * elem;
@@ -164,6 +344,12 @@ struct pimi_plan {
     pimi::DevBuf<int64_t> offs, d_tidx;
     pimi::DevBuf<int32_t> dims, tile_sector;
     pimi::DevBuf<pimi::ApplyTile> atiles, atiles4;
+    // banded-apply tiles (k_apply_dia) of kDiaWidths[w] columns, and the
+    // width index chosen for single-vector and for multi-column/batched
+    // launches (set by pimi_plan_autotune_apply, or from the defaults)
+    pimi::DevBuf<pimi::ApplyTile> dtiles[pimi::kNumDiaWidths];
+    int64_t ndtiles[pimi::kNumDiaWidths] = {};
+    int dia_w_single = 1, dia_w_multi = 1;
     pimi::DevStencil main;  // the Liouvillian's apply tables
     pimi::DevBuf<double> d_tval;
     // stationary-solve preconditioner factors (built on first use)

@@ -140,6 +140,209 @@ __global__ __launch_bounds__(kApplyWave) void k_apply(
         apply_tile<R, 1>(x + c * n, y + c * n, out_scale, tile, offs, dims, entry_base, entries, col, val);
 }
 
+// ---------------------------------------------------------------------------
+// Banded (diagonal) form of the stencil apply.
+//
+// When every factor of every term is a sum of a few diagonals,
+//   A = sum_dA diag-shift(cA_dA),  A[a, a + dA] = cA_dA[a],
+// a term s A X_in B^H adds to Y[a, b] the products
+//   s cA_dA[a] conj(cB_dB[b]) X_in[a + dA, b + dB].
+// The qubit sectors are of this form (H, Q and J- tridiagonal or narrower,
+// every one-body contraction one diagonal).  Entries are grouped by (input
+// sector, dA); a group's lanes read one strip X_in[a + dA, b0 + c0 + k],
+// k = 0 .. TB + span - 1, with plain coalesced loads (no column tables), and
+// every entry of the group combines the strip at its own column offset
+// dB - c0 in 0..2.  Coefficient vectors are zero wherever a + dA or b + dB
+// leaves the input block, so clamped load indices never change a result.
+// Kinds:  left (B = I): Y += cA'[a] X;  right (A = I): Y += w[b] X;
+//         sandwich: Y += (cA'[a] w[b]) X;  real sandwich: the same with real
+//         cA', w (two real FMAs per product instead of four complex ones);
+// cA' carries the term scale, w[b] = conj(cB[b]) (times the scale for right).
+// Every output sums its groups in list order and each group's entries in
+// list order with explicit FMAs, so the result of an output does not depend
+// on the tile width TB.
+struct DiaGroup {
+    int64_t in_off;     // offset of the input block
+    int32_t mn;         // input block size
+    int32_t dA;         // row shift of the strip
+    int32_t c0;         // first strip column relative to the tile's first column
+    int32_t span;       // strip columns beyond TB (0..2)
+    int32_t e0, e1;     // entry range
+};
+struct DiaEntry {
+    int32_t kind;       // 0 left, 1 right, 2 sandwich, 3 real sandwich
+    int32_t dB;         // strip column offset, 0..2
+    int32_t qa, qb;     // coefficient vectors: per output row (A side), per output column (B side)
+};
+constexpr int kDiaWaves = 4;  // tiles (waves) per workgroup
+constexpr int kNumDiaWidths = 3;
+constexpr int kDiaWidths[kNumDiaWidths] = {4, 8, 16};  // tile columns TB
+constexpr int kDiaCoefPad = 16;  // coefficient vectors: aligned and zero-padded to this many elements
+
+__device__ __forceinline__ float rfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double rfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <typename R>
+__device__ __forceinline__ cx<R> cfma(cx<R> a, cx<R> b, cx<R> c) {
+    c.x = rfma(a.x, b.x, c.x);
+    c.x = rfma(-a.y, b.y, c.x);
+    c.y = rfma(a.x, b.y, c.y);
+    c.y = rfma(a.y, b.x, c.y);
+    return c;
+}
+template <typename R>
+__device__ __forceinline__ cx<R> cmulf(cx<R> a, cx<R> b) {
+    return cx<R>{rfma(a.x, b.x, -a.y * b.y), rfma(a.x, b.y, a.y * b.x)};
+}
+
+// Address spaces: tables that no kernel writes while it runs are read through
+// the constant address space (wave-uniform reads become scalar loads, also
+// when the table pointer itself was loaded from a descriptor); vectors
+// through the global one (global_load, not flat_load).
+// (The attributes apply to the device pass only; the host pass sees plain
+// pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PIMI_CONST __attribute__((address_space(4)))
+#define PIMI_GLOBAL __attribute__((address_space(1)))
+#else
+#define PIMI_CONST
+#define PIMI_GLOBAL
+#endif
+template <typename T>
+__device__ __forceinline__ const PIMI_CONST T *cptr(const T *p) { return (const PIMI_CONST T *)p; }
+template <typename T>
+__device__ __forceinline__ const PIMI_GLOBAL T *gptr(const T *p) { return (const PIMI_GLOBAL T *)p; }
+template <typename T>
+__device__ __forceinline__ PIMI_GLOBAL T *gptr_w(T *p) { return (PIMI_GLOBAL T *)p; }
+
+// TB consecutive column coefficients, read as one block
+template <typename R, int TB>
+struct DiaCols {
+    cx<R> v[TB];
+};
+
+template <typename R, int TB, int KIND, int DB>
+__device__ __forceinline__ void dia_entry(cx<R> (&acc)[TB], const cx<R> (&st)[TB + 2], const DiaEntry &E,
+                                          const PIMI_GLOBAL cx<R> *coefv, const PIMI_CONST cx<R> *coefs, int a,
+                                          int b0) {
+    if constexpr (KIND == 0) {
+        const cx<R> ca = coefv[E.qa + a];
+#pragma unroll
+        for (int j = 0; j < TB; ++j) acc[j] = cfma(ca, st[j + DB], acc[j]);
+    } else {
+        const DiaCols<R, TB> w = *(const PIMI_CONST DiaCols<R, TB> *)(coefs + E.qb + b0);
+        if constexpr (KIND == 1) {
+#pragma unroll
+            for (int j = 0; j < TB; ++j) acc[j] = cfma(w.v[j], st[j + DB], acc[j]);
+        } else if constexpr (KIND == 2) {
+            const cx<R> ca = coefv[E.qa + a];
+#pragma unroll
+            for (int j = 0; j < TB; ++j) acc[j] = cfma(cmulf(ca, w.v[j]), st[j + DB], acc[j]);
+        } else {
+            const R ca = coefv[E.qa + a].x;
+#pragma unroll
+            for (int j = 0; j < TB; ++j) {
+                const R c = ca * w.v[j].x;
+                acc[j].x = rfma(c, st[j + DB].x, acc[j].x);
+                acc[j].y = rfma(c, st[j + DB].y, acc[j].y);
+            }
+        }
+    }
+}
+
+// the strip X_in[r, cfirst + k], k < ncol, of one group (clamped indices)
+template <typename R, int TB>
+__device__ __forceinline__ void dia_strip(cx<R> (&st)[TB + 2], const PIMI_GLOBAL cx<R> *Xn, int r, int mn,
+                                          int cfirst, int ncol) {
+#pragma unroll
+    for (int k = 0; k < TB + 2; ++k) {
+        if (k < ncol) {
+            const int c = min(max(cfirst + k, 0), mn - 1);
+            st[k] = Xn[(uint32_t)(r + c * mn)];
+        } else {
+            st[k] = cx<R>{0, 0};
+        }
+    }
+}
+
+// One banded tile: rows a0..a0+63 (one per lane) of columns b0..b0+nb-1
+// (nb <= TB) of sector s's output block.  Everything but the lane's row is
+// wave-uniform: descriptors and column coefficients are scalar loads, the
+// entry switch a scalar branch.
+template <typename R, int TB>
+__device__ __forceinline__ void dia_tile(const cx<R> *xc_, cx<R> *yc_, R out_scale, ApplyTile tile,
+                                         const int64_t *offs_, const int32_t *dims_, const int32_t *gbase_,
+                                         const DiaGroup *groups_, const DiaEntry *ents_, const cx<R> *coef_) {
+    const PIMI_GLOBAL cx<R> *xc = gptr(xc_);
+    PIMI_GLOBAL cx<R> *yc = gptr_w(yc_);
+    const PIMI_CONST int64_t *offs = cptr(offs_);
+    const PIMI_CONST int32_t *dims = cptr(dims_);
+    const PIMI_CONST int32_t *gbase = cptr(gbase_);
+    const PIMI_CONST DiaGroup *groups = cptr(groups_);
+    const PIMI_CONST DiaEntry *ents = cptr(ents_);
+    const PIMI_GLOBAL cx<R> *coefv = gptr(coef_);
+    const PIMI_CONST cx<R> *coefs = cptr(coef_);
+    const int s = __builtin_amdgcn_readfirstlane(tile.sector);
+    const int b0 = __builtin_amdgcn_readfirstlane(tile.b0);
+    const int a0 = __builtin_amdgcn_readfirstlane(tile.a0);
+    const int nb = __builtin_amdgcn_readfirstlane(tile.nb);
+    const int m = dims[s];
+    const int arow = a0 + (int)(threadIdx.x & (kApplyWave - 1));
+    const bool active = arow < m;
+    const int a = active ? arow : m - 1;  // inactive lanes evaluate the last row and store nothing
+    cx<R> acc[TB];
+#pragma unroll
+    for (int j = 0; j < TB; ++j) acc[j] = cx<R>{0, 0};
+    const int g0 = gbase[s], g1 = gbase[s + 1];
+    for (int g = g0; g < g1; ++g) {
+        const DiaGroup G = groups[g];
+        cx<R> st[TB + 2];
+        dia_strip<R, TB>(st, xc + G.in_off, min(max(a + G.dA, 0), G.mn - 1), G.mn, b0 + G.c0, TB + G.span);
+        for (int e = G.e0; e < G.e1; ++e) {
+            const DiaEntry E = ents[e];
+            switch (E.kind * 4 + E.dB) {
+                case 0: dia_entry<R, TB, 0, 0>(acc, st, E, coefv, coefs, a, b0); break;
+                case 1: dia_entry<R, TB, 0, 1>(acc, st, E, coefv, coefs, a, b0); break;
+                case 2: dia_entry<R, TB, 0, 2>(acc, st, E, coefv, coefs, a, b0); break;
+                case 4: dia_entry<R, TB, 1, 0>(acc, st, E, coefv, coefs, a, b0); break;
+                case 5: dia_entry<R, TB, 1, 1>(acc, st, E, coefv, coefs, a, b0); break;
+                case 6: dia_entry<R, TB, 1, 2>(acc, st, E, coefv, coefs, a, b0); break;
+                case 8: dia_entry<R, TB, 2, 0>(acc, st, E, coefv, coefs, a, b0); break;
+                case 9: dia_entry<R, TB, 2, 1>(acc, st, E, coefv, coefs, a, b0); break;
+                case 10: dia_entry<R, TB, 2, 2>(acc, st, E, coefv, coefs, a, b0); break;
+                case 12: dia_entry<R, TB, 3, 0>(acc, st, E, coefv, coefs, a, b0); break;
+                case 13: dia_entry<R, TB, 3, 1>(acc, st, E, coefv, coefs, a, b0); break;
+                case 14: dia_entry<R, TB, 3, 2>(acc, st, E, coefv, coefs, a, b0); break;
+                default: break;
+            }
+        }
+    }
+    if (active) {
+        const int64_t o = offs[s] + a;
+#pragma unroll
+        for (int j = 0; j < TB; ++j)
+            if (j < nb) yc[o + (int64_t)(b0 + j) * m] = cx<R>{out_scale * acc[j].x, out_scale * acc[j].y};
+    }
+}
+
+// y = out_scale * L x for ncols column vectors through the banded form;
+// kDiaWaves tiles per workgroup, tiles spread so that XCD k (workgroup id
+// mod 8) handles one contiguous slab of the sector-major tile list.
+template <typename R, int TB>
+__global__ __launch_bounds__(kApplyWave *kDiaWaves) void k_apply_dia(
+    const cx<R> *__restrict__ x, cx<R> *__restrict__ y, int64_t n, const ApplyTile *__restrict__ tiles,
+    int64_t ntiles, int64_t blocks_per_xcd, const int64_t *__restrict__ offs, const int32_t *__restrict__ dims,
+    const int32_t *__restrict__ gbase, const DiaGroup *__restrict__ groups, const DiaEntry *__restrict__ ents,
+    const cx<R> *__restrict__ coef, R out_scale, int64_t ncols) {
+    const int64_t bid = blockIdx.x;
+    const int64_t blk = (bid % kNumXcd) * blocks_per_xcd + bid / kNumXcd;
+    const int64_t t = blk * kDiaWaves + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (t >= ntiles) return;
+    const ApplyTile tile = cptr(tiles)[t];
+    for (int64_t c = blockIdx.y; c < ncols; c += gridDim.y)
+        dia_tile<R, TB>(x + c * n, y + c * n, out_scale, tile, offs, dims, gbase, groups, ents, coef);
+}
+
 // y[i] = x[i] * w[sector of i], one coordinate per lane over the plan's tiles
 template <typename R>
 __global__ __launch_bounds__(kApplyBlock) void k_sector_scale(const cx<R> *__restrict__ x, cx<R> *__restrict__ y,

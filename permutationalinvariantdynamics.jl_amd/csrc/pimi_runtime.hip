@@ -437,6 +437,17 @@ struct pimi_workspace {
 
 namespace pimi {
 
+// banded-apply tile width index: from an environment override (4, 8 or 16
+// columns) or the default
+static int dia_width_default(const char *env, int dflt) {
+    const char *v = std::getenv(env);
+    if (!v) return dflt;
+    const int tb = std::atoi(v);
+    for (int w = 0; w < kNumDiaWidths; ++w)
+        if (kDiaWidths[w] == tb) return w;
+    return dflt;
+}
+
 static void finalize_plan(pimi_plan *P) {
     const FusedPlan &F = P->host;
     build_stencil(F, P->st);
@@ -489,6 +500,13 @@ static void finalize_plan(pimi_plan *P) {
     P->tile_sector.upload(tsec.data(), tsec.size());
     P->atiles.upload(at.data(), at.size());
     P->atiles4.upload(at4.data(), at4.size());
+    for (int w = 0; w < kNumDiaWidths; ++w) {
+        std::vector<ApplyTile> dt = make_tiles(kDiaWidths[w]);
+        P->ndtiles[w] = (int64_t)dt.size();
+        P->dtiles[w].upload(dt.data(), dt.size());
+    }
+    P->dia_w_single = dia_width_default("PIMI_DIA_TB_SINGLE", 0);
+    P->dia_w_multi = dia_width_default("PIMI_DIA_TB_MULTI", 1);
     P->main.push();
     P->d_tidx.upload(P->tidx.data(), P->tidx.size());
     P->d_tval.upload(P->tval.data(), P->tval.size());
@@ -519,12 +537,42 @@ enum class ApplyOp { Full, Diag, Adjoint };
 
 // `weighted`: the sector-weighted operator S L S^{-1} (full stencil only; the
 // sector-diagonal part D is unchanged by a sector-constant S)
+// y = out_scale * (stencil) x for ncols vectors through the banded kernel,
+// tile width kDiaWidths[w]
+template <typename R>
+static void launch_dia(hipStream_t stream, const pimi_plan *P, const DevStencil &S, bool weighted, const cx<R> *x,
+                       cx<R> *y, double out_scale, int64_t ncols, int w) {
+    const cx<R> *coef;
+    if constexpr (sizeof(R) == 8) coef = weighted ? S.coefw64.p : S.coef64.p;
+    else coef = weighted ? S.coefw32.p : S.coef32.p;
+    const int64_t nt = P->ndtiles[w];
+    const int64_t nblk = (nt + kDiaWaves - 1) / kDiaWaves;
+    const int64_t per_xcd = (nblk + kNumXcd - 1) / kNumXcd;
+    dim3 grid((unsigned)(per_xcd * kNumXcd), (unsigned)std::min<int64_t>(ncols, 65535));
+    dim3 block(kApplyWave * kDiaWaves);
+#define PIMI_DIA_LAUNCH(TB)                                                                                   \
+    hipLaunchKernelGGL((k_apply_dia<R, TB>), grid, block, 0, stream, x, y, P->n, P->dtiles[w].p, nt, per_xcd, \
+                       P->offs.p, P->dims.p, S.gbase.p, S.groups.p, S.dents.p, coef, (R)out_scale, ncols)
+    switch (kDiaWidths[w]) {
+        case 4: PIMI_DIA_LAUNCH(4); break;
+        case 8: PIMI_DIA_LAUNCH(8); break;
+        default: PIMI_DIA_LAUNCH(16); break;
+    }
+#undef PIMI_DIA_LAUNCH
+    HIPCHK(hipGetLastError());
+}
+
 template <typename R>
 static void launch_apply(pimi_workspace *W, const cx<R> *x, cx<R> *y, double out_scale, int64_t ncols = 1,
                          bool diag = false, bool weighted = false, bool adjoint = false) {
     pimi_plan *P = W->plan;
     const ApplyOp op = adjoint ? ApplyOp::Adjoint : (diag ? ApplyOp::Diag : ApplyOp::Full);
     const DevStencil &S = op == ApplyOp::Diag ? P->diag : (op == ApplyOp::Adjoint ? P->adj : P->main);
+    if (S.dia_ok) {
+        launch_dia<R>(W->stream, P, S, op == ApplyOp::Full && weighted, x, y, out_scale, ncols,
+                      ncols > 1 ? P->dia_w_multi : P->dia_w_single);
+        return;
+    }
     const ApplyEntry *entries = (op == ApplyOp::Full && weighted) ? S.entries_w.p : S.entries.p;
     const cx<R> *val;
     if constexpr (sizeof(R) == 8) val = S.val64.p;
@@ -1737,6 +1785,10 @@ static void ensure_traj_ws(pimi_workspace *W, int64_t ncols) {
 static void launch_stencil(pimi_workspace *W, const DevStencil &S, const cx<double> *x, cx<double> *y,
                            int64_t ncols) {
     pimi_plan *P = W->plan;
+    if (S.dia_ok) {
+        launch_dia<double>(W->stream, P, S, false, x, y, 1.0, ncols, ncols > 1 ? P->dia_w_multi : P->dia_w_single);
+        return;
+    }
     dim3 grid((unsigned)(P->atiles_per_xcd * kNumXcd), (unsigned)std::min<int64_t>(ncols, 65535));
     hipLaunchKernelGGL(k_apply<double>, grid, dim3(kApplyWave), 0, W->stream, x, y, P->n, P->atiles.p, P->natiles,
                        P->atiles_per_xcd, P->offs.p, P->dims.p, S.entry_base.p, S.entries.p, S.col.p, S.val64.p, 1.0,
