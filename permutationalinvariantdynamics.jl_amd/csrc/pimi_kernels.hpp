@@ -250,17 +250,28 @@ __device__ __forceinline__ void dia_entry(cx<R> (&acc)[TB], const cx<R> (&st)[TB
     }
 }
 
-// the strip X_in[r, cfirst + k], k < ncol, of one group (clamped indices)
-template <typename R, int TB>
-__device__ __forceinline__ void dia_strip(cx<R> (&st)[TB + 2], const PIMI_GLOBAL cx<R> *Xn, int r, int mn,
-                                          int cfirst, int ncol) {
+// the strip X_in[r, cfirst + k], k < NK, of one group through a raw buffer
+// over the input block: the lane's (clamped) row offset in voffset, the
+// column offset in soffset.  A column outside the block gives an offset
+// outside the resource (negative offsets wrap past its end), which the
+// hardware range check reads as zero -- exactly the zero the coefficient
+// vectors hold there.  0x00020000: the gfx9-family word 3 of a raw buffer.
+template <typename R, int TB, int NK>
+__device__ __forceinline__ void dia_strip(cx<R> (&st)[TB + 2], const cx<R> *Xn, int r, int mn, int cfirst) {
+    constexpr int es = (int)sizeof(cx<R>);
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)Xn, (short)0, mn * mn * es, 0x00020000);
+    const int vr = r * es;
+    const int step = mn * es;
+    const int s0 = cfirst * step;
 #pragma unroll
     for (int k = 0; k < TB + 2; ++k) {
-        if (k < ncol) {
-            const int c = min(max(cfirst + k, 0), mn - 1);
-            st[k] = Xn[(uint32_t)(r + c * mn)];
+        if constexpr (sizeof(R) == 8) {
+            st[k] = k < NK ? __builtin_bit_cast(cx<R>, __builtin_amdgcn_raw_buffer_load_b128(rs, vr, s0 + k * step, 0))
+                           : cx<R>{0, 0};
         } else {
-            st[k] = cx<R>{0, 0};
+            st[k] = k < NK ? __builtin_bit_cast(cx<R>, __builtin_amdgcn_raw_buffer_load_b64(rs, vr, s0 + k * step, 0))
+                           : cx<R>{0, 0};
         }
     }
 }
@@ -297,7 +308,9 @@ __device__ __forceinline__ void dia_tile(const cx<R> *xc_, cx<R> *yc_, R out_sca
     for (int g = g0; g < g1; ++g) {
         const DiaGroup G = groups[g];
         cx<R> st[TB + 2];
-        dia_strip<R, TB>(st, xc + G.in_off, min(max(a + G.dA, 0), G.mn - 1), G.mn, b0 + G.c0, TB + G.span);
+        const int r = min(max(a + G.dA, 0), G.mn - 1);
+        if (G.span == 0) dia_strip<R, TB, TB>(st, xc_ + G.in_off, r, G.mn, b0 + G.c0);
+        else dia_strip<R, TB, TB + 2>(st, xc_ + G.in_off, r, G.mn, b0 + G.c0);
         for (int e = G.e0; e < G.e1; ++e) {
             const DiaEntry E = ents[e];
             switch (E.kind * 4 + E.dB) {
