@@ -141,8 +141,15 @@ __global__ __launch_bounds__(kApplyWave) void kb_apply(const BApply *__restrict_
     const int64_t bid = blockIdx.x;
     const int64_t g = (bid % kNumXcd) * per_xcd + bid / kNumXcd;
     if (g >= total) return;
-    const BApply A = J[seg_of(J, nj, g)];
-    apply_tile<R, kApplyCols>((const cx<R> *)A.x, (cx<R> *)A.y, (R)A.out_scale, A.tiles[g - A.first], A.offs,
+    const PIMI_CONST BApply *Jc = cptr(J);
+    int lo = 0, hi = nj - 1;  // the wave's job: binary search on first (scalar)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (Jc[mid].first <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const BApply A = Jc[lo];
+    apply_tile<R, kApplyCols>((const cx<R> *)A.x, (cx<R> *)A.y, (R)A.out_scale, cptr(A.tiles)[g - A.first], A.offs,
                               A.dims, A.entry_base, A.entries, A.col, (const cx<R> *)A.val);
 }
 

@@ -27,6 +27,26 @@ __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) { return {a.x * b.x - a.
 template <typename R>
 __device__ __forceinline__ cx<R> cjmul(cx<R> a, cx<R> b) { return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }
 
+// Address spaces: tables that no kernel writes while it runs are read through
+// the constant address space (wave-uniform reads become scalar loads, also
+// when the table pointer itself was loaded from a descriptor); vectors
+// through the global one (global_load, not flat_load).
+// (The attributes apply to the device pass only; the host pass sees plain
+// pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PIMI_CONST __attribute__((address_space(4)))
+#define PIMI_GLOBAL __attribute__((address_space(1)))
+#else
+#define PIMI_CONST
+#define PIMI_GLOBAL
+#endif
+template <typename T>
+__device__ __forceinline__ const PIMI_CONST T *cptr(const T *p) { return (const PIMI_CONST T *)p; }
+template <typename T>
+__device__ __forceinline__ const PIMI_GLOBAL T *gptr(const T *p) { return (const PIMI_GLOBAL T *)p; }
+template <typename T>
+__device__ __forceinline__ PIMI_GLOBAL T *gptr_w(T *p) { return (PIMI_GLOBAL T *)p; }
+
 constexpr int kApplyBlock = 256;
 constexpr int kRedBlock = 256;
 constexpr int kNumXcd = 8;
@@ -65,13 +85,21 @@ constexpr int kApplyWave = 64;
 constexpr int kApplyCols = 4;  // columns per lane of the batched apply (one for a single vector)
 
 template <typename R, int CB>
-__device__ __forceinline__ void apply_tile(const cx<R> *__restrict__ xc, cx<R> *__restrict__ yc, R out_scale,
-                                           ApplyTile tile, const int64_t *__restrict__ offs,
-                                           const int32_t *__restrict__ dims,
-                                           const int32_t *__restrict__ entry_base,
-                                           const ApplyEntry *__restrict__ entries,
-                                           const int32_t *__restrict__ col, const cx<R> *__restrict__ val) {
+__device__ __forceinline__ void apply_tile(const cx<R> *xc_, cx<R> *__restrict__ yc, R out_scale, ApplyTile tile,
+                                           const int64_t *__restrict__ offs, const int32_t *dims_,
+                                           const int32_t *entry_base_, const ApplyEntry *entries_,
+                                           const int32_t *col_, const cx<R> *val_) {
     constexpr int CH = CB == 1 ? kEntryChunk : kEntryChunk / 2;
+    // wave-uniform tables (entries, B-side slots) as scalar loads, per-lane
+    // ones (A-side slots, input gathers) as global loads
+    const PIMI_CONST ApplyEntry *entries = cptr(entries_);
+    const PIMI_CONST int32_t *colS = cptr(col_);
+    const PIMI_CONST cx<R> *valS = cptr(val_);
+    const PIMI_GLOBAL int32_t *col = gptr(col_);
+    const PIMI_GLOBAL cx<R> *val = gptr(val_);
+    const PIMI_GLOBAL cx<R> *xc = gptr(xc_);
+    const PIMI_CONST int32_t *entry_base = cptr(entry_base_);
+    const PIMI_CONST int32_t *dims = cptr(dims_);
     const int s = __builtin_amdgcn_readfirstlane(tile.sector);
     const int b0 = __builtin_amdgcn_readfirstlane(tile.b0);
     const int nb = __builtin_amdgcn_readfirstlane(tile.nb);
@@ -95,21 +123,21 @@ __device__ __forceinline__ void apply_tile(const cx<R> *__restrict__ xc, cx<R> *
         }
 #pragma unroll
         for (int k = 0; k < CH; ++k) {
-            const ApplyEntry &E = entries[e + k];
-            const cx<R> *Xn = xc + E.in_off;
+            const ApplyEntry E = entries[e + k];
+            const PIMI_GLOBAL cx<R> *Xn = xc + E.in_off;
 #pragma unroll
             for (int j = 0; j < CB; ++j) {
-                const int cb = col[E.qb + b0 + (j < nb ? j : 0)];
+                const int cb = colS[E.qb + b0 + (j < nb ? j : 0)];
                 xv[k][j] = Xn[(uint32_t)(ca[k] + cb * E.mn)];
             }
         }
 #pragma unroll
         for (int k = 0; k < CH; ++k) {
-            const ApplyEntry &E = entries[e + k];
+            const ApplyEntry E = entries[e + k];
             const cx<R> sc = {(R)E.s_re, (R)E.s_im};
 #pragma unroll
             for (int j = 0; j < CB; ++j) {
-                const cx<R> vb = val[E.qb + b0 + (j < nb ? j : 0)];
+                const cx<R> vb = valS[E.qb + b0 + (j < nb ? j : 0)];
                 const cx<R> coef = cmul(cjmul(vb, sc), va[k]);
                 acc[j] = cadd(acc[j], cmul(coef, xv[k][j]));
             }
@@ -135,7 +163,7 @@ __global__ __launch_bounds__(kApplyWave) void k_apply(
     const int64_t bid = blockIdx.x;
     const int64_t t = (bid % kNumXcd) * tiles_per_xcd + bid / kNumXcd;
     if (t >= ntiles) return;
-    const ApplyTile tile = tiles[t];
+    const ApplyTile tile = cptr(tiles)[t];
     for (int64_t c = blockIdx.y; c < ncols; c += gridDim.y)
         apply_tile<R, 1>(x + c * n, y + c * n, out_scale, tile, offs, dims, entry_base, entries, col, val);
 }
@@ -195,25 +223,6 @@ __device__ __forceinline__ cx<R> cmulf(cx<R> a, cx<R> b) {
     return cx<R>{rfma(a.x, b.x, -a.y * b.y), rfma(a.x, b.y, a.y * b.x)};
 }
 
-// Address spaces: tables that no kernel writes while it runs are read through
-// the constant address space (wave-uniform reads become scalar loads, also
-// when the table pointer itself was loaded from a descriptor); vectors
-// through the global one (global_load, not flat_load).
-// (The attributes apply to the device pass only; the host pass sees plain
-// pointers.)
-#if defined(__HIP_DEVICE_COMPILE__)
-#define PIMI_CONST __attribute__((address_space(4)))
-#define PIMI_GLOBAL __attribute__((address_space(1)))
-#else
-#define PIMI_CONST
-#define PIMI_GLOBAL
-#endif
-template <typename T>
-__device__ __forceinline__ const PIMI_CONST T *cptr(const T *p) { return (const PIMI_CONST T *)p; }
-template <typename T>
-__device__ __forceinline__ const PIMI_GLOBAL T *gptr(const T *p) { return (const PIMI_GLOBAL T *)p; }
-template <typename T>
-__device__ __forceinline__ PIMI_GLOBAL T *gptr_w(T *p) { return (PIMI_GLOBAL T *)p; }
 
 // TB consecutive column coefficients, read as one block
 template <typename R, int TB>
