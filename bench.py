@@ -23,6 +23,8 @@ Also reported:
                 the batch engine (FP32 Arnoldi in FP64 refinement) to relative
                 residual <= 1e-8, against the plain FP64 single-solve path on
                 a sample of the same models;
+  batched_n120  the same family at N=120 only (64 models), against the plain
+                FP64 single-solve path on a sample of them;
   serving       a Poisson stream (200 requests/s, 60 requests, 60 % stationary
                 states, 40 % dynamics to t=10) through the serving loop;
   cpu_baseline  the oracle's OpenMP CPU restatement on a bounded sample.
@@ -52,6 +54,8 @@ INNER_STEPS = 8
 KRYLOV_PLAIN = 1000   # plain sector-preconditioned GMRES, no restart needed
 BATCH_COUNT = 512
 BATCH_PLAIN_SAMPLE = 32
+APPLY_KERNEL = "k_apply_dia<double, 4>"        # single-vector banded apply (4-column tiles)
+BATCH_APPLY_KERNEL = "kb_apply_dia<float, 8>"  # batch engine's banded apply, FP32 (8-column tiles)
 
 
 def build_model(pimi):
@@ -60,10 +64,10 @@ def build_model(pimi):
                                      pumping=0.02, collective_emission=0.1)
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, record="hbm_traffic_n120.json"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE runs (calibrated by a device copy), or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_n120.json")
+    path = os.path.join(ROOT, "profiles", record)
     try:
         with open(path) as fh:
             d = json.load(fh)
@@ -178,6 +182,44 @@ def batched(pimi, args):
         "models": len(comps)}
 
 
+def batched_n120(pimi, count=64, plain_sample=4):
+    """`count` seeded N=120 models of the batched family (one size) through
+    the batch engine (FP32 Arnoldi in FP64 refinement), against the plain FP64
+    single-solve path (sector-preconditioned GMRES) on a sample of them."""
+    from pimi.workloads import batched_instances
+    inst = batched_instances(count, seed=20261205, sizes=(120,))
+    comps = [pimi.compile(i.model()) for i in inst]
+    for c in comps:
+        c.prepare()
+    eng = pimi.BatchEngine(precision=pimi.PIMI_C32)
+    alg = pimi.BatchAlgorithm(krylovdim=300, inner_rtol=1e-4)
+    t0 = time.perf_counter()
+    ids = [eng.submit(c, alg, rtol=RTOL) for c in comps]
+    out = {}
+    while len(out) < len(ids):
+        eng.run(stop_when_finished=True)
+        while (j := eng.next_finished()) is not None:
+            out[j] = eng.fetch(j)
+    wall = time.perf_counter() - t0
+    eng.close()
+    res = [out[j] for j in ids]
+    plain = pimi.GMRESAlgorithm(krylovdim=KRYLOV_PLAIN, maxiter=KRYLOV_PLAIN, preconditioner="sector")
+    t0 = time.perf_counter()
+    for c in comps[:plain_sample]:
+        pimi.stationary_state(c, algorithm=plain, atol=0.0, rtol=RTOL, criterion="backward")
+    plain_rate = plain_sample / (time.perf_counter() - t0)
+    rate = len(res) / wall
+    print(f"# batched N=120: {len(res)} solves in {wall:.2f} s", file=sys.stderr, flush=True)
+    return {"workload": f"{count} seeded driven-qubit models at N=120 (PI dim 302621), Omega ~ U[0,2], gamma_phi "
+                        f"log-uniform [1e-3,1e-1], FP32 Arnoldi in FP64 refinement, to |L rho|/(s|rho|) <= 1e-8",
+            "solves_per_s": rate, "wall_s": wall,
+            "max_relative_residual": max(r.info["relative_residual"] for r in res),
+            "converged": int(sum(r.info["converged"] for r in res)),
+            "plain_fp64_single_solves_per_s": plain_rate,
+            "plain_sample": f"first {plain_sample} of the same models, one at a time",
+            "speedup_vs_plain_fp64": rate / plain_rate}
+
+
 def serving(pimi):
     from pimi.serving import Server, summarize
     from pimi.workloads import served_requests
@@ -212,15 +254,24 @@ def main():
     import pimi
     comp = pimi.compile(build_model(pimi), backend="mi355x", device=0)
     if args.profile:
-        # counter runs: 20 event-timed applies, 5 preconditioner applications (20
-        # GEMM launches), then 20 device copies of 64 MiB (the counters' calibration)
+        # counter runs: 200 event-timed applies, 5 preconditioner applications (20
+        # GEMM launches), 5 batched applies, then 20 device copies of 64 MiB (the
+        # counters' calibration)
         N = pimi._native
-        ms_apply, bytes_apply, achieved = apply_roofline(pimi, comp, reps=20)
+        ms_apply, bytes_apply, achieved = apply_roofline(pimi, comp, reps=200)
         ms_gemm, flops_gemm, tfs = gemm_roofline(pimi, comp, reps=5)
+        # the batched-workload apply: 512 models in one launch (FP32), 5 launches
+        from pimi.workloads import batched_instances
+        comps = [pimi.compile(i.model()) for i in batched_instances(BATCH_COUNT)]
+        eng = pimi.BatchEngine(precision=pimi.PIMI_C32)
+        ms_b, bytes_b = eng.apply_bench(comps, reps=5)
+        eng.close()
         ms = C.c_double()
         N.check(N.lib().pimi_copy_bench(comp.workspace(pimi.PIMI_C64, 1), C.c_int64(64 << 20),
                                         C.c_int32(20), C.byref(ms)))
         print(json.dumps({"apply_ms": ms_apply, "apply_algorithmic_bytes": bytes_apply, "apply_GBps": achieved,
+                          "batched_apply_ms": ms_b, "batched_apply_algorithmic_bytes": bytes_b,
+                          "batched_apply_GBps": bytes_b / (ms_b * 1e-3) / 1e9,
                           "gemm_ms": ms_gemm, "gemm_flops": flops_gemm, "gemm_TFps": tfs,
                           "copy_bytes": 64 << 20, "copy_ms": ms.value,
                           "copy_GBps": 2 * (64 << 20) / (ms.value * 1e-3) / 1e9}))
@@ -259,6 +310,7 @@ def main():
     ms_apply, bytes_apply, achieved_apply = apply_roofline(pimi, comp)
     ms_gemm, flops_gemm, tfs_gemm = gemm_roofline(pimi, comp)
     batch, spmv_batch = (None, None) if args.no_batch else batched(pimi, args)
+    batch120 = None if args.no_batch else batched_n120(pimi)
     serve = None if args.no_serve else serving(pimi)
     cpu = None
     if not args.no_cpu:
@@ -267,9 +319,11 @@ def main():
     spmv = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
             "single_n120_fp64": {"achieved": achieved_apply, "frac": achieved_apply / HBM_PEAK_GBS,
                                  "avg_ms": ms_apply, "algorithmic_bytes": bytes_apply,
-                                 "traffic": measured_traffic("k_apply<double>")}}
+                                 "traffic": measured_traffic(APPLY_KERNEL), "kernel": APPLY_KERNEL}}
     if spmv_batch:
-        spmv["batched_512_fp32"] = dict(spmv_batch, frac=spmv_batch["achieved"] / HBM_PEAK_GBS)
+        spmv["batched_512_fp32"] = dict(spmv_batch, frac=spmv_batch["achieved"] / HBM_PEAK_GBS,
+                                        traffic=measured_traffic(BATCH_APPLY_KERNEL, "hbm_traffic_batch512.json"),
+                                        kernel=BATCH_APPLY_KERNEL)
     out = {
         "metric": "steady-state solves/s & p99 latency at resid 1e-8; SpMV GB/s vs HBM peak",
         "value": args.steps / total,
@@ -300,6 +354,7 @@ def main():
                      "kernel": "k_sector_gemm_mfma<double>", "avg_ms": ms_gemm, "algorithmic_flops": flops_gemm},
         "spmv": spmv,
         "batched": batch,
+        "batched_n120": batch120,
         "serving": serve,
         "cpu_baseline": cpu,
     }

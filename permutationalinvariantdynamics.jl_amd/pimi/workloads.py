@@ -32,11 +32,13 @@ class Instance:
                                     dephasing=4 * self.gamma_phi, pumping=0.0)
 
 
-def batched_instances(count=512, seed=BATCH_SEED):
+def batched_instances(count=512, seed=BATCH_SEED, sizes=(20, 40, 60, 80, 100, 120)):
+    """`count` seeded instances; N drawn uniformly from `sizes` (a single size
+    gives a same-N batch with the same parameter distribution)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = []
     for _ in range(count):
-        N = int(rng.choice([20, 40, 60, 80, 100, 120]))
+        N = int(rng.choice(list(sizes)))
         omega = float(rng.uniform(0.0, 2.0))
         gphi = float(np.exp(rng.uniform(np.log(1e-3), np.log(1e-1))))
         out.append(Instance(N, omega, 0.1, gphi))
