@@ -244,6 +244,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-side", action="store_true", help="skip the plain and FP32-inner side solves")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile", action="store_true",
                     help="apply, preconditioner and copy calibration only (counter runs)")
@@ -298,6 +299,10 @@ def main():
     total = time.perf_counter() - t0
     iters = int(np.median([i["iterations"] for i in infos]))
     side = {}
+    if args.no_side:
+        print(json.dumps({"solves_per_s": args.steps / total, "ms_per_step": total / args.steps * 1e3,
+                          "iterations": iters}))
+        return
     for name, algorithm, prec in (
             ("plain_fp64", pimi.GMRESAlgorithm(krylovdim=KRYLOV_PLAIN, maxiter=KRYLOV_PLAIN,
                                                preconditioner="sector"), pimi.PIMI_C64),
