@@ -679,37 +679,69 @@ static size_t job_bytes(int64_t n, int64_t nt, int kdim, int prec) {
     return b;
 }
 
+// A pipeline lane: its own stream, descriptor staging, mailbox and share of
+// the active jobs.  The engine runs two lanes: while the device executes one
+// lane's tick, the host reads back the other lane's results and stages its
+// next tick, so host work overlaps device work (and the two lanes' kernels
+// may overlap on the device).  A job stays in the lane that admitted it.
+struct BLane {
+    hipStream_t stream = nullptr;
+    char *hstage = nullptr;
+    size_t hstage_cap = 0;
+    DevBuf<char> dstage;
+    cx<double> *hmail = nullptr;
+    size_t hmail_cap = 0;
+    DevBuf<cx<double>> dmail;
+    std::vector<hipEvent_t> events;
+    std::vector<BJob *> active;
+    // the tick in flight (launched, not yet read back)
+    bool inflight = false;
+    std::vector<size_t> moff;       // mailbox offset of each active job
+    std::vector<int> op_class;      // per launch (timing)
+    double bytes[8] = {}, flops[8] = {};
+    ~BLane() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto ev : events) (void)hipEventDestroy(ev);
+        if (hstage) (void)hipHostFree(hstage);
+        if (hmail) (void)hipHostFree(hmail);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+constexpr int kLanes = 2;
+
 }  // namespace pimi
 
 struct pimi_batch {
     pimi_context *ctx = nullptr;
     int prec = PIMI_C64;
     int max_jobs = 1024;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // engine-level work outside ticks (apply bench)
     pimi::Arena arena;
     int64_t next_id = 1;
     std::deque<pimi::BJob *> queue;
-    std::vector<pimi::BJob *> active;
+    pimi::BLane lanes[pimi::kLanes];
     std::map<int64_t, std::unique_ptr<pimi::BJob>> jobs;
     std::deque<int64_t> finished;
-    char *hstage = nullptr;
-    size_t hstage_cap = 0;
-    pimi::DevBuf<char> dstage;
-    pimi::cx<double> *hmail = nullptr;
-    size_t hmail_cap = 0;
-    pimi::DevBuf<pimi::cx<double>> dmail;
     std::atomic<int> busy{0};
     int64_t ticks = 0;
     int dia_w = 1;  // banded-apply tile width index (kDiaWidths) of the engine's launches
     bool trace = false;
     bool timing = false;  // PIMI_BATCH_TIMING: HIP events between launches, per operation class
-    std::vector<hipEvent_t> events;
     double stat_ms[8] = {}, stat_bytes[8] = {}, stat_flops[8] = {};
+    size_t nactive() const {
+        size_t k = 0;
+        for (const auto &L : lanes) k += L.active.size();
+        return k;
+    }
+    bool any_inflight() const {
+        for (const auto &L : lanes)
+            if (L.inflight) return true;
+        return false;
+    }
     ~pimi_batch() {
+        for (auto &L : lanes)
+            if (L.stream) (void)hipStreamSynchronize(L.stream);
         if (stream) (void)hipStreamSynchronize(stream);
-        for (auto ev : events) (void)hipEventDestroy(ev);
-        if (hstage) (void)hipHostFree(hstage);
-        if (hmail) (void)hipHostFree(hmail);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -727,6 +759,7 @@ enum OpClass { OC_GEMM = 0, OC_APPLY, OC_DOTS, OC_UPDATE, OC_CYCLE_END, OC_FP64,
 
 struct Tick {
     pimi_batch *E;
+    hipStream_t stream;
     std::vector<char> buf;
     std::vector<std::function<void(const char *)>> ops;
     std::vector<int> op_class;
@@ -760,7 +793,7 @@ struct Tick {
         const size_t off = put(v);
         const int ns = (int)v.size();
         const unsigned grid = (unsigned)std::min<int64_t>(tot, kMaxGrid);
-        hipStream_t st = E->stream;
+        hipStream_t st = stream;
         push([=](const char *d) {
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(bdim), 0, st, (const BVec *)(d + off), ns, tot);
         });
@@ -794,7 +827,7 @@ struct Tick {
             tot += P->natiles4;
             bytes[cls] += 2.0 * J->n * sizeof(cx<R>) + (sizeof(R) == 8 ? J->table_bytes64 : J->table_bytes32);
         }
-        hipStream_t st = E->stream;
+        hipStream_t st = stream;
         if (!vd.empty()) {
             const size_t offd = put(vd);
             const int njd = (int)vd.size();
@@ -821,7 +854,7 @@ struct Tick {
                                bordered ? 1 : 0, 0});
         const size_t off = put(v);
         const unsigned nj = (unsigned)v.size();
-        hipStream_t st = E->stream;
+        hipStream_t st = stream;
         push([=](const char *d) {
             hipLaunchKernelGGL(kb_trace<R>, dim3(nj), dim3(256), 0, st, (const BTrace *)(d + off));
         });
@@ -859,7 +892,7 @@ struct Tick {
             const size_t off = put(v);
             const int ng = (int)v.size();
             const int ta = shapes[stage].ta, tb = shapes[stage].tb;
-            hipStream_t st = E->stream;
+            hipStream_t st = stream;
             push([=](const char *d) {
                 hipLaunchKernelGGL(kb_gemm<R>, dim3((unsigned)((tot + 3) / 4)), dim3(256), 0, st,
                                    (const BGemm *)(d + off), ng, tot, ta, tb);
@@ -880,7 +913,7 @@ struct Tick {
         vec(v, kb_norm_partial<R>, nblk_of);
         const size_t off = put(v);
         const unsigned nj = (unsigned)v.size();
-        hipStream_t st = E->stream;
+        hipStream_t st = stream;
         push([=](const char *d) {
             hipLaunchKernelGGL(kb_reduce_norm, dim3(nj), dim3(64), 0, st, (const BVec *)(d + off));
         });
@@ -910,7 +943,7 @@ struct Tick {
         {
             const size_t off = put(c);
             const unsigned nj = (unsigned)c.size();
-            hipStream_t st = E->stream;
+            hipStream_t st = stream;
             push([=](const char *dd) {
                 hipLaunchKernelGGL(kb_dcgs_coef, dim3(nj), dim3(64), 0, st, (const BVec *)(dd + off));
             });
@@ -931,7 +964,7 @@ struct Tick {
         const size_t off = put(u);
         const int ns = (int)u.size();
         const unsigned grid = (unsigned)std::min<int64_t>(tot, kMaxGrid);
-        hipStream_t st = E->stream;
+        hipStream_t st = stream;
         push([=](const char *d) {
             hipLaunchKernelGGL(kb_gemv<R>, dim3(grid), dim3(kRedBlock), 0, st, (const BVec *)(d + off), ns, tot,
                                mode);
@@ -972,20 +1005,25 @@ static void carve(pimi_batch *E, BJob *J) {
     if ((size_t)(p - (E->arena.base + J->off)) > J->bytes) throw Error(PIMI_EINVAL, "job layout exceeds its footprint");
 }
 
-static void admit(pimi_batch *E) {
-    while (!E->queue.empty() && (int)E->active.size() < E->max_jobs) {
+// Admit queued jobs into lane L (FIFO) while memory and slots allow, up to
+// L's even share of all active and queued jobs (so the lanes stay balanced);
+// called when L stages a tick, so a job's first work is ordered on its lane's
+// stream.
+static void admit(pimi_batch *E, BLane &L) {
+    const size_t share = (E->nactive() + E->queue.size() + kLanes - 1) / kLanes;
+    while (!E->queue.empty() && (int)E->nactive() < E->max_jobs && L.active.size() < share) {
         BJob *J = E->queue.front();
         size_t off;
         if (!E->arena.alloc(J->bytes, off)) break;  // FIFO: wait for memory to free up
         E->queue.pop_front();
         J->off = off;
         carve(E, J);
-        HIPCHK(hipMemcpyAsync(J->vval, J->vv.data(), sizeof(double) * J->nt, hipMemcpyHostToDevice, E->stream));
+        HIPCHK(hipMemcpyAsync(J->vval, J->vv.data(), sizeof(double) * J->nt, hipMemcpyHostToDevice, L.stream));
         J->state = JS_ACTIVE;
         J->phase = J->o.operator_scale > 0 ? PH_INIT : PH_PROBE;
         J->scale = J->o.operator_scale;
         J->t_admit = std::chrono::steady_clock::now();
-        E->active.push_back(J);
+        L.active.push_back(J);
     }
 }
 
@@ -1004,12 +1042,13 @@ static void finish(pimi_batch *E, BJob *J, bool ok) {
     E->finished.push_back(J->id);
 }
 
+// Stage and launch lane L's tick (no wait); complete_tick reads it back.
 template <typename R>
-static void run_tick(pimi_batch *E) {
-    admit(E);
-    if (E->active.empty()) return;
+static void launch_tick(pimi_batch *E, BLane &L) {
+    admit(E, L);
+    if (L.active.empty()) return;
     std::vector<BJob *> probe, init, arn, begin, end, fin;
-    for (BJob *J : E->active) switch (J->phase) {
+    for (BJob *J : L.active) switch (J->phase) {
             case PH_PROBE: probe.push_back(J); break;
             case PH_INIT: init.push_back(J); break;
             case PH_ARNOLDI:
@@ -1019,7 +1058,7 @@ static void run_tick(pimi_batch *E) {
             case PH_END: end.push_back(J); break;
             default: fin.push_back(J); break;
         }
-    Tick T{E, {}, {}};
+    Tick T{E, L.stream, {}, {}};
     auto W64 = [](BJob *J) -> void * { return J->w64; };
     auto cX = [](BJob *J) -> const void * { return J->x; };
     auto cW64 = [](BJob *J) -> const void * { return J->w64; };
@@ -1052,7 +1091,7 @@ static void run_tick(pimi_batch *E) {
         for (BJob *J : init) sc.push_back(BScatter{J->b, J->x, J->plan->d_tidx.p, J->vval, J->nt});
         const size_t off = T.put(sc);
         const unsigned nj = (unsigned)sc.size();
-        hipStream_t st = E->stream;
+        hipStream_t st = L.stream;
         T.push([=](const char *d) {
             hipLaunchKernelGGL(kb_scatter, dim3(nj), dim3(256), 0, st, (const BScatter *)(d + off));
         });
@@ -1119,7 +1158,7 @@ static void run_tick(pimi_batch *E) {
         if (!v.empty()) {
             const size_t off = T.put(v);
             const int nj = (int)v.size();
-            hipStream_t st = E->stream;
+            hipStream_t st = L.stream;
             T.push([=](const char *d) {
                 hipLaunchKernelGGL(kb_sector_scale, dim3((unsigned)tot), dim3(kApplyBlock), 0, st,
                                    (const BApply *)(d + off), nj, tot);
@@ -1134,79 +1173,94 @@ static void run_tick(pimi_batch *E) {
     T.cls = OC_OTHER;
     std::vector<BCopy> g;
     size_t mtot = 0;
-    std::vector<size_t> moff(E->active.size());
-    for (size_t k = 0; k < E->active.size(); ++k) {
-        BJob *J = E->active[k];
+    std::vector<size_t> &moff = L.moff;
+    moff.assign(L.active.size(), 0);
+    for (size_t k = 0; k < L.active.size(); ++k) {
+        BJob *J = L.active[k];
         const int64_t cnt = kMailHacc + (J->phase == PH_ARNOLDI ? 2 * (J->j + 1) : 0);
         moff[k] = mtot;
         g.push_back(BCopy{J->mail, nullptr, cnt});
         mtot += (size_t)cnt;
     }
-    if (E->dmail.count < mtot) E->dmail.alloc(std::max<size_t>(mtot, 2 * E->dmail.count));
-    for (size_t k = 0; k < g.size(); ++k) g[k].dst = E->dmail.p + moff[k];
+    if (L.dmail.count < mtot) L.dmail.alloc(std::max<size_t>(mtot, 2 * L.dmail.count));
+    for (size_t k = 0; k < g.size(); ++k) g[k].dst = L.dmail.p + moff[k];
     {
         const size_t off = T.put(g);
         const unsigned ng = (unsigned)g.size();
-        hipStream_t st = E->stream;
+        hipStream_t st = L.stream;
         T.push([=](const char *d) {
             hipLaunchKernelGGL(kb_gather, dim3(ng), dim3(256), 0, st, (const BCopy *)(d + off));
         });
     }
     // -- upload, launch, download
-    if (E->hstage_cap < T.buf.size()) {
-        if (E->hstage) HIPCHK(hipHostFree(E->hstage));
-        E->hstage_cap = std::max<size_t>(T.buf.size(), 2 * E->hstage_cap);
-        HIPCHK(hipHostMalloc((void **)&E->hstage, E->hstage_cap, hipHostMallocDefault));
+    if (L.hstage_cap < T.buf.size()) {
+        if (L.hstage) HIPCHK(hipHostFree(L.hstage));
+        L.hstage_cap = std::max<size_t>(T.buf.size(), 2 * L.hstage_cap);
+        HIPCHK(hipHostMalloc((void **)&L.hstage, L.hstage_cap, hipHostMallocDefault));
     }
-    if (E->dstage.count < T.buf.size()) E->dstage.alloc(std::max<size_t>(T.buf.size(), 2 * E->dstage.count));
-    const char *dbase = E->dstage.p;
+    if (L.dstage.count < T.buf.size()) L.dstage.alloc(std::max<size_t>(T.buf.size(), 2 * L.dstage.count));
+    const char *dbase = L.dstage.p;
     const bool timing = E->timing;
     if (timing)
-        while (E->events.size() < T.ops.size() + 1) {
+        while (L.events.size() < T.ops.size() + 1) {
             hipEvent_t ev;
             HIPCHK(hipEventCreate(&ev));
-            E->events.push_back(ev);
+            L.events.push_back(ev);
         }
     for (auto &rl : T.relocs) {
         const void *addr = dbase + rl.second;
         std::memcpy(T.buf.data() + rl.first, &addr, sizeof addr);
     }
-    std::memcpy(E->hstage, T.buf.data(), T.buf.size());
-    HIPCHK(hipMemcpyAsync(E->dstage.p, E->hstage, T.buf.size(), hipMemcpyHostToDevice, E->stream));
-    if (timing) HIPCHK(hipEventRecord(E->events[0], E->stream));
+    std::memcpy(L.hstage, T.buf.data(), T.buf.size());
+    HIPCHK(hipMemcpyAsync(L.dstage.p, L.hstage, T.buf.size(), hipMemcpyHostToDevice, L.stream));
+    if (timing) HIPCHK(hipEventRecord(L.events[0], L.stream));
     for (size_t k = 0; k < T.ops.size(); ++k) {
         T.ops[k](dbase);
-        if (timing) HIPCHK(hipEventRecord(E->events[k + 1], E->stream));
+        if (timing) HIPCHK(hipEventRecord(L.events[k + 1], L.stream));
     }
     HIPCHK(hipGetLastError());
-    if (E->hmail_cap < mtot) {
-        if (E->hmail) HIPCHK(hipHostFree(E->hmail));
-        E->hmail_cap = std::max<size_t>(mtot, 2 * E->hmail_cap);
-        HIPCHK(hipHostMalloc((void **)&E->hmail, sizeof(cx<double>) * E->hmail_cap, hipHostMallocDefault));
+    if (L.hmail_cap < mtot) {
+        if (L.hmail) HIPCHK(hipHostFree(L.hmail));
+        L.hmail_cap = std::max<size_t>(mtot, 2 * L.hmail_cap);
+        HIPCHK(hipHostMalloc((void **)&L.hmail, sizeof(cx<double>) * L.hmail_cap, hipHostMallocDefault));
     }
-    HIPCHK(hipMemcpyAsync(E->hmail, E->dmail.p, sizeof(cx<double>) * mtot, hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipMemcpyAsync(L.hmail, L.dmail.p, sizeof(cx<double>) * mtot, hipMemcpyDeviceToHost, L.stream));
     for (BJob *J : fin) {
         J->result.resize((size_t)J->n);
         HIPCHK(hipMemcpyAsync(J->result.data(), J->weighted ? J->xo : J->x, sizeof(cx<double>) * J->n,
-                              hipMemcpyDeviceToHost, E->stream));
+                              hipMemcpyDeviceToHost, L.stream));
     }
-    HIPCHK(hipStreamSynchronize(E->stream));
+    L.op_class = T.op_class;
+    for (int c = 0; c < OC_COUNT; ++c) {
+        L.bytes[c] = T.bytes[c];
+        L.flops[c] = T.flops[c];
+    }
+    L.inflight = true;
+}
+
+// Wait for lane L's tick and advance every one of its jobs' state machines.
+template <typename R>
+static void complete_tick(pimi_batch *E, BLane &L) {
+    if (!L.inflight) return;
+    HIPCHK(hipStreamSynchronize(L.stream));
+    L.inflight = false;
     E->ticks++;
     for (int c = 0; c < OC_COUNT; ++c) {
-        E->stat_bytes[c] += T.bytes[c];
-        E->stat_flops[c] += T.flops[c];
+        E->stat_bytes[c] += L.bytes[c];
+        E->stat_flops[c] += L.flops[c];
     }
-    if (timing)
-        for (size_t k = 0; k < T.ops.size(); ++k) {
+    if (E->timing)
+        for (size_t k = 0; k < L.op_class.size(); ++k) {
             float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, E->events[k], E->events[k + 1]));
-            E->stat_ms[T.op_class[k]] += ms;
+            HIPCHK(hipEventElapsedTime(&ms, L.events[k], L.events[k + 1]));
+            E->stat_ms[L.op_class[k]] += ms;
         }
+    const std::vector<size_t> &moff = L.moff;
     // -- host updates
     std::vector<BJob *> still;
-    for (size_t k = 0; k < E->active.size(); ++k) {
-        BJob *J = E->active[k];
-        const cx<double> *mb = E->hmail + moff[k];
+    for (size_t k = 0; k < L.active.size(); ++k) {
+        BJob *J = L.active[k];
+        const cx<double> *mb = L.hmail + moff[k];
         const double *sc = (const double *)mb;
         auto thr = [&]() { return 0.5 * J->tight * (J->o.atol + J->o.rtol * J->xnorm); };
         bool done = false, ok = false;
@@ -1397,7 +1451,16 @@ static void run_tick(pimi_batch *E) {
         if (done) finish(E, J, ok);
         else still.push_back(J);
     }
-    E->active.swap(still);
+    L.active.swap(still);
+}
+
+// Complete every lane's tick in flight (before anything reads engine-wide
+// state that the ticks update).
+static void drain(pimi_batch *E) {
+    for (auto &L : E->lanes) {
+        if (E->prec == PIMI_C32) complete_tick<float>(E, L);
+        else complete_tick<double>(E, L);
+    }
 }
 
 }  // namespace pimi
@@ -1433,6 +1496,7 @@ pimi_status pimi_batch_create(pimi_context *ctx, const pimi_batch_opts *opts, pi
                 if (kDiaWidths[w] == std::atoi(tb)) E->dia_w = w;
         E->timing = (opts->flags & PIMI_BATCH_TIMING) != 0;
         HIPCHK(hipStreamCreate(&E->stream));
+        for (auto &L : E->lanes) HIPCHK(hipStreamCreate(&L.stream));
         E->arena.init(align_up(bytes, 256));
         *out = E.release();
     });
@@ -1521,12 +1585,24 @@ pimi_status pimi_batch_run(pimi_batch *E, int32_t max_ticks, int32_t stop_when_f
         if (!E) throw Error(PIMI_EINVAL, "null argument");
         BusyGuard guard(E->busy);
         HIPCHK(hipSetDevice(E->ctx->device));
+        // One round: each lane reads back its tick in flight (while the other
+        // lane's tick runs) and launches its next.  A lane's tick may still
+        // be in flight when this returns; the next run (or stats, or destroy)
+        // completes it.  Jobs appear in the finished list only once their
+        // last tick is complete, so fetch never waits on the device.
         const size_t before = E->finished.size();
         for (int32_t t = 0; t < max_ticks; ++t) {
-            if (E->active.empty() && E->queue.empty()) break;
+            if (E->nactive() == 0 && E->queue.empty() && !E->any_inflight()) break;
             const size_t f0 = E->finished.size();
-            if (E->prec == PIMI_C32) run_tick<float>(E);
-            else run_tick<double>(E);
+            for (auto &L : E->lanes) {
+                if (E->prec == PIMI_C32) {
+                    complete_tick<float>(E, L);
+                    launch_tick<float>(E, L);
+                } else {
+                    complete_tick<double>(E, L);
+                    launch_tick<double>(E, L);
+                }
+            }
             if (stop_when_finished && E->finished.size() > f0) break;
         }
         if (finished) *finished = (int64_t)(E->finished.size() - before);
@@ -1537,7 +1613,7 @@ pimi_status pimi_batch_counts(pimi_batch *E, int64_t *counts) {
     return guarded([&] {
         if (!E || !counts) throw Error(PIMI_EINVAL, "null argument");
         counts[0] = (int64_t)E->queue.size();
-        counts[1] = (int64_t)E->active.size();
+        counts[1] = (int64_t)E->nactive();
         counts[2] = (int64_t)E->finished.size();
         counts[3] = E->ticks;
         counts[4] = (int64_t)E->arena.used;
@@ -1549,6 +1625,8 @@ pimi_status pimi_batch_stats(pimi_batch *E, double *out, int32_t reset) {
     return guarded([&] {
         if (!E || !out) throw Error(PIMI_EINVAL, "null argument");
         BusyGuard guard(E->busy);
+        HIPCHK(hipSetDevice(E->ctx->device));
+        drain(E);  // statistics of every launched tick
         for (int c = 0; c < 8; ++c) {
             out[c] = E->stat_ms[c];
             out[8 + c] = E->stat_bytes[c];
