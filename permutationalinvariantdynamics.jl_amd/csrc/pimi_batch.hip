@@ -660,8 +660,11 @@ struct BJob {
     std::string error;
 };
 
+// Workgroups of a job's BLAS-1 passes and reduction partials: one per 2048
+// coordinates (so a lone large job still fills the chip), at most 1024; fixed
+// by the job's dimension alone, so its reductions do not depend on the batch.
 static int blocks_for(int64_t n) {
-    return (int)std::min<int64_t>(std::max<int64_t>((n + 2047) / 2048, 1), 128);
+    return (int)std::min<int64_t>(std::max<int64_t>((n + 2047) / 2048, 1), 1024);
 }
 
 // Device bytes of one job's state: FP64 x, b, r, w64, xo; the Krylov basis and four
