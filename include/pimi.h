@@ -119,6 +119,8 @@ typedef struct {
     int64_t stencil_entries;/* sum over outputs of A-row x B-row widths */
     int64_t table_bytes_c64;/* device stencil table bytes (complex FP64 tables) */
     int64_t table_bytes_c32;
+    int32_t apply_form;     /* 1: banded (diagonal) apply tables, 0: ELL tables (see PIMI_APPLY_FORM) */
+    int32_t banded_groups;  /* banded form: (input sector, row shift) groups over all sectors */
 } pimi_plan_info;
 
 typedef enum {

@@ -8,6 +8,7 @@
 #include <atomic>
 #include <climits>
 #include <complex>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <string>
@@ -129,6 +130,10 @@ struct DevStencil {
 
     void build_dia(const Stencil &st, const std::vector<int64_t> &dims, const std::vector<int64_t> &offs) {
         dia_ok = false;
+        // PIMI_APPLY_FORM=ell keeps every plan on the ELL tables (a second,
+        // independent kernel for the same operator)
+        if (const char *f = std::getenv("PIMI_APPLY_FORM"))
+            if (std::string(f) == "ell") return;
         h_gbase.assign(1, 0);
         h_groups.clear();
         h_dents.clear();

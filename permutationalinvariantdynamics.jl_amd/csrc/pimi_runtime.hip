@@ -2002,6 +2002,8 @@ pimi_status pimi_plan_info_get(const pimi_plan *P, pimi_plan_info *info) {
         info->stencil_entries = P->st.entries_per_apply;
         info->table_bytes_c64 = P->main.table_bytes(16);
         info->table_bytes_c32 = P->main.table_bytes(8);
+        info->apply_form = P->main.dia_ok ? 1 : 0;
+        info->banded_groups = P->main.dia_ok ? (int32_t)P->main.h_groups.size() : 0;
     });
 }
 

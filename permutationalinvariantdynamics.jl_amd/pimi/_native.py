@@ -70,7 +70,8 @@ class FusedDesc(C.Structure):
 
 class PlanInfo(C.Structure):
     _fields_ = [("n", C.c_int64), ("nsectors", C.c_int32), ("max_block", C.c_int32), ("nterms", C.c_int64),
-                ("stencil_entries", C.c_int64), ("table_bytes_c64", C.c_int64), ("table_bytes_c32", C.c_int64)]
+                ("stencil_entries", C.c_int64), ("table_bytes_c64", C.c_int64), ("table_bytes_c32", C.c_int64),
+                ("apply_form", C.c_int32), ("banded_groups", C.c_int32)]
 
 
 class GmresOpts(C.Structure):

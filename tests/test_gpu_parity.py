@@ -5,6 +5,7 @@ Tolerances: complex FP64 apply <= 1e-13 relative (max-norm); FP32 apply
 solvers converge to rtol 1e-12 with atol 0.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -395,3 +396,46 @@ def test_adjoint_apply_is_conjugate_transpose_and_annihilates_trace():
     y64 = comp8.apply(t + _rand(comp8.info["n"], 4), adjoint=True)
     y32 = comp8.apply(t + _rand(comp8.info["n"], 4), precision=1, adjoint=True)
     assert _relmax(y32, y64) <= 2e-6
+
+
+def _compile_with_env(model, **env):
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        return pimi.compile(model)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def test_banded_form_is_used_for_qubits_and_not_for_qutrits():
+    comp = pimi.compile(_c2_model(20))
+    assert comp.info["apply_form"] == 1 and comp.info["banded_groups"] > 0
+    model, _ = _qutrit(4)
+    assert pimi.compile(model).info["apply_form"] == 0
+
+
+@pytest.mark.parametrize("N", [8, 61, 120])
+def test_banded_apply_matches_ell_apply(N):
+    """Two independent kernels for the same operator: the banded tables and
+    the ELL tables (PIMI_APPLY_FORM=ell), forward and adjoint, FP64."""
+    print(f"banded vs ELL N={N}", flush=True)
+    banded = pimi.compile(_c2_model(N))
+    ell = _compile_with_env(_c2_model(N), PIMI_APPLY_FORM="ell")
+    assert banded.info["apply_form"] == 1 and ell.info["apply_form"] == 0
+    x = _rand(banded.info["n"], 40 + N)
+    for adj in (False, True):
+        assert _relmax(banded.apply(x, adjoint=adj), ell.apply(x, adjoint=adj)) <= 1e-14
+
+
+def test_banded_apply_bitwise_independent_of_tile_width():
+    """Every output sums its entries in the same order whatever the tile
+    width, so 4-, 8- and 16-column tiles give bitwise the same vector."""
+    model = _c2_model(61)
+    x = _rand(pimi.compile(model).info["n"], 77)
+    ys = [_compile_with_env(model, PIMI_DIA_TB_SINGLE=tb).apply(x) for tb in (4, 8, 16)]
+    np.testing.assert_array_equal(ys[0], ys[1])
+    np.testing.assert_array_equal(ys[0], ys[2])
