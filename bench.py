@@ -54,8 +54,9 @@ INNER_STEPS = 8
 KRYLOV_PLAIN = 1000   # plain sector-preconditioned GMRES, no restart needed
 BATCH_COUNT = 512
 BATCH_PLAIN_SAMPLE = 32
-APPLY_KERNEL = "k_apply_dia<double, 4>"        # single-vector banded apply (4-column tiles)
-BATCH_APPLY_KERNEL = "kb_apply_dia<float, 8>"  # batch engine's banded apply, FP32 (8-column tiles)
+# the banded apply kernels (tile width picked by the autotuner at run time)
+APPLY_KERNEL = "k_apply_dia<double, {}>"        # single-vector banded apply, FP64
+BATCH_APPLY_KERNEL = "kb_apply_dia<float, {}>"  # batch engine's banded apply, FP32
 
 
 def build_model(pimi):
@@ -138,10 +139,12 @@ def batched(pimi, args):
     for c in comps:
         c.prepare()
     eng = pimi.BatchEngine(precision=pimi.PIMI_C32, timing=True)
-    # SpMV over every model in one launch
+    # SpMV over every model in one launch, at the autotuned tile width
+    width = eng.autotune(comps)
     ms_spmv, bytes_spmv = eng.apply_bench(comps, reps=20)
     eng.close()
     eng = pimi.BatchEngine(precision=pimi.PIMI_C32, timing=True)
+    eng.autotune(comps)
     alg = pimi.BatchAlgorithm(krylovdim=300, inner_rtol=1e-4)
     t0 = time.perf_counter()
     ids = [eng.submit(c, alg, rtol=RTOL) for c in comps]
@@ -179,7 +182,7 @@ def batched(pimi, args):
         "speedup_vs_plain_fp64": rate / plain_rate,
         "ops": ops,
     }, {"achieved": bytes_spmv / (ms_spmv * 1e-3) / 1e9, "avg_ms": ms_spmv, "algorithmic_bytes": bytes_spmv,
-        "models": len(comps)}
+        "models": len(comps), "tile_width": width}
 
 
 def batched_n120(pimi, count=64, plain_sample=4):
@@ -192,6 +195,7 @@ def batched_n120(pimi, count=64, plain_sample=4):
     for c in comps:
         c.prepare()
     eng = pimi.BatchEngine(precision=pimi.PIMI_C32)
+    eng.autotune(comps)
     alg = pimi.BatchAlgorithm(krylovdim=300, inner_rtol=1e-4)
     t0 = time.perf_counter()
     ids = [eng.submit(c, alg, rtol=RTOL) for c in comps]
@@ -254,6 +258,8 @@ def main():
         sys.exit(2)
     import pimi
     comp = pimi.compile(build_model(pimi), backend="mi355x", device=0)
+    # tile widths of the banded apply, picked on this device (FP64 and FP32 workspaces)
+    widths = {"c64": comp.autotune(pimi.PIMI_C64), "c32": comp.autotune(pimi.PIMI_C32)}
     if args.profile:
         # counter runs: 200 event-timed applies, 5 preconditioner applications (20
         # GEMM launches), 5 batched applies, then 20 device copies of 64 MiB (the
@@ -265,12 +271,15 @@ def main():
         from pimi.workloads import batched_instances
         comps = [pimi.compile(i.model()) for i in batched_instances(BATCH_COUNT)]
         eng = pimi.BatchEngine(precision=pimi.PIMI_C32)
+        wb = eng.autotune(comps)
         ms_b, bytes_b = eng.apply_bench(comps, reps=5)
         eng.close()
         ms = C.c_double()
         N.check(N.lib().pimi_copy_bench(comp.workspace(pimi.PIMI_C64, 1), C.c_int64(64 << 20),
                                         C.c_int32(20), C.byref(ms)))
-        print(json.dumps({"apply_ms": ms_apply, "apply_algorithmic_bytes": bytes_apply, "apply_GBps": achieved,
+        print(json.dumps({"apply_kernel": APPLY_KERNEL.format(widths["c64"][0]),
+                          "batched_apply_kernel": BATCH_APPLY_KERNEL.format(wb),
+                          "apply_ms": ms_apply, "apply_algorithmic_bytes": bytes_apply, "apply_GBps": achieved,
                           "batched_apply_ms": ms_b, "batched_apply_algorithmic_bytes": bytes_b,
                           "batched_apply_GBps": bytes_b / (ms_b * 1e-3) / 1e9,
                           "gemm_ms": ms_gemm, "gemm_flops": flops_gemm, "gemm_TFps": tfs,
@@ -324,11 +333,13 @@ def main():
     spmv = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
             "single_n120_fp64": {"achieved": achieved_apply, "frac": achieved_apply / HBM_PEAK_GBS,
                                  "avg_ms": ms_apply, "algorithmic_bytes": bytes_apply,
-                                 "traffic": measured_traffic(APPLY_KERNEL), "kernel": APPLY_KERNEL}}
+                                 "traffic": measured_traffic(APPLY_KERNEL.format(widths["c64"][0])),
+                                 "kernel": APPLY_KERNEL.format(widths["c64"][0])},
+            "autotuned_tile_widths": widths}
     if spmv_batch:
+        bk = BATCH_APPLY_KERNEL.format(spmv_batch["tile_width"])
         spmv["batched_512_fp32"] = dict(spmv_batch, frac=spmv_batch["achieved"] / HBM_PEAK_GBS,
-                                        traffic=measured_traffic(BATCH_APPLY_KERNEL, "hbm_traffic_batch512.json"),
-                                        kernel=BATCH_APPLY_KERNEL)
+                                        traffic=measured_traffic(bk, "hbm_traffic_batch512.json"), kernel=bk)
     out = {
         "metric": "steady-state solves/s & p99 latency at resid 1e-8; SpMV GB/s vs HBM peak",
         "value": args.steps / total,

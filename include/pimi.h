@@ -259,6 +259,16 @@ pimi_status pimi_expv_series(pimi_workspace *ws, const double *rho0, const doubl
  * reference (measurement only). */
 pimi_status pimi_apply_bench(pimi_workspace *ws, int64_t ncols, int32_t reps, double *ms_avg);
 
+/* Autotuner of the banded apply's tile schedule: times `reps` applies of this
+ * workspace's plan at each tile width (4, 8 and 16 columns per lane) for one
+ * vector and for `ncols` vectors, in the workspace precision, on synthetic
+ * device vectors, and keeps the fastest width of each for every later apply
+ * of the plan in that precision.  widths_out[0], widths_out[1]: the chosen
+ * single-vector and multi-column widths (0 when the plan uses ELL tables,
+ * which have no width to tune).  No counterpart in the reference (the
+ * schedule of a device kernel). */
+pimi_status pimi_workspace_autotune_apply(pimi_workspace *ws, int64_t ncols, int32_t reps, int32_t *widths_out);
+
 /* Device-to-device copy of `bytes` bytes, `reps` times, event-timed: the
  * calibration kernel for HBM byte counters.  Measurement only. */
 pimi_status pimi_copy_bench(pimi_workspace *ws, int64_t bytes, int32_t reps, double *ms_avg);
@@ -390,6 +400,13 @@ pimi_status pimi_batch_fetch(pimi_batch *batch, int64_t job_id, double *rho_out,
  * = sum over plans of 2 n s + stencil tables.  Measurement only. */
 pimi_status pimi_batch_apply_bench(pimi_batch *batch, pimi_plan *const *plans, int32_t nplans, int32_t reps,
                                    double *ms_avg, int64_t *algorithmic_bytes);
+/* Autotuner of the engine's banded-apply tile width: times one launch over a
+ * synthetic vector of every plan (as pimi_batch_apply_bench) at each width
+ * (4, 8, 16 columns), `reps` times each, and keeps the fastest for the
+ * engine's later launches; *width_out = the chosen width.  No counterpart in
+ * the reference. */
+pimi_status pimi_batch_autotune_apply(pimi_batch *batch, pimi_plan *const *plans, int32_t nplans, int32_t reps,
+                                      int32_t *width_out);
 
 #ifdef __cplusplus
 }

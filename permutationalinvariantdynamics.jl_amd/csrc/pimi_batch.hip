@@ -1681,6 +1681,68 @@ pimi_status pimi_batch_fetch(pimi_batch *E, int64_t job_id, double *rho_out, pim
     return st;
 }
 
+pimi_status pimi_batch_autotune_apply(pimi_batch *E, pimi_plan *const *plans, int32_t nplans, int32_t reps,
+                                      int32_t *width_out) {
+    return guarded([&] {
+        if (!E || !plans || nplans < 1 || reps < 1 || !width_out) throw Error(PIMI_EINVAL, "invalid argument");
+        BusyGuard guard(E->busy);
+        HIPCHK(hipSetDevice(E->ctx->device));
+        const size_t es = E->prec == PIMI_C32 ? sizeof(cx<float>) : sizeof(cx<double>);
+        std::vector<DevBuf<char>> xs((size_t)nplans), ys((size_t)nplans);
+        std::vector<pimi_plan *> banded;
+        for (int32_t k = 0; k < nplans; ++k) {
+            pimi_plan *P = plans[k];
+            if (!P || !P->ctx || P->ctx->device != E->ctx->device)
+                throw Error(PIMI_EINVAL, "every plan must live on the engine's device");
+            if (!P->main.dia_ok) continue;
+            xs[banded.size()].alloc(es * (size_t)P->n);
+            ys[banded.size()].alloc(es * (size_t)P->n);
+            HIPCHK(hipMemsetAsync(xs[banded.size()].p, 0x3f, es * (size_t)P->n, E->stream));
+            banded.push_back(P);
+        }
+        *width_out = kDiaWidths[E->dia_w];
+        if (banded.empty()) return;  // nothing banded: the width is not used
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        int best = E->dia_w;
+        float best_ms = INFINITY;
+        DevBuf<BDia> dvd;
+        dvd.alloc(banded.size());
+        for (int w = 0; w < kNumDiaWidths; ++w) {
+            std::vector<BDia> vd;
+            int64_t totd = 0;
+            for (size_t k = 0; k < banded.size(); ++k) {
+                vd.push_back(E->prec == PIMI_C32 ? dia_desc<float>(totd, banded[k], xs[k].p, ys[k].p, false, 1.0, w)
+                                                 : dia_desc<double>(totd, banded[k], xs[k].p, ys[k].p, false, 1.0, w));
+                totd += banded[k]->ndtiles[w];
+            }
+            HIPCHK(hipMemcpyAsync(dvd.p, vd.data(), sizeof(BDia) * vd.size(), hipMemcpyHostToDevice, E->stream));
+            const int nvd = (int)vd.size();
+            auto launch = [&] {
+                if (E->prec == PIMI_C32) launch_kb_dia<float>(E->stream, dvd.p, nvd, totd, w);
+                else launch_kb_dia<double>(E->stream, dvd.p, nvd, totd, w);
+            };
+            launch();  // warm
+            HIPCHK(hipEventRecord(e0, E->stream));
+            for (int32_t r = 0; r < reps; ++r) launch();
+            HIPCHK(hipEventRecord(e1, E->stream));
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = w;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        E->dia_w = best;
+        *width_out = kDiaWidths[best];
+    });
+}
+
 pimi_status pimi_batch_apply_bench(pimi_batch *E, pimi_plan *const *plans, int32_t nplans, int32_t reps,
                                    double *ms_avg, int64_t *algorithmic_bytes) {
     return guarded([&] {

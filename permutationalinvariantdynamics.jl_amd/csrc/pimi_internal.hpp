@@ -350,11 +350,11 @@ struct pimi_plan {
     pimi::DevBuf<int32_t> dims, tile_sector;
     pimi::DevBuf<pimi::ApplyTile> atiles, atiles4;
     // banded-apply tiles (k_apply_dia) of kDiaWidths[w] columns, and the
-    // width index chosen for single-vector and for multi-column/batched
-    // launches (set by pimi_plan_autotune_apply, or from the defaults)
+    // width index chosen for single-vector and for multi-column launches,
+    // per precision (set by pimi_workspace_autotune_apply, or the defaults)
     pimi::DevBuf<pimi::ApplyTile> dtiles[pimi::kNumDiaWidths];
     int64_t ndtiles[pimi::kNumDiaWidths] = {};
-    int dia_w_single = 1, dia_w_multi = 1;
+    int dia_w_single[2] = {0, 0}, dia_w_multi[2] = {1, 1};  // [0]: FP64 launches, [1]: FP32
     pimi::DevStencil main;  // the Liouvillian's apply tables
     pimi::DevBuf<double> d_tval;
     // stationary-solve preconditioner factors (built on first use)

@@ -505,8 +505,8 @@ static void finalize_plan(pimi_plan *P) {
         P->ndtiles[w] = (int64_t)dt.size();
         P->dtiles[w].upload(dt.data(), dt.size());
     }
-    P->dia_w_single = dia_width_default("PIMI_DIA_TB_SINGLE", 0);
-    P->dia_w_multi = dia_width_default("PIMI_DIA_TB_MULTI", 1);
+    P->dia_w_single[0] = P->dia_w_single[1] = dia_width_default("PIMI_DIA_TB_SINGLE", 0);
+    P->dia_w_multi[0] = P->dia_w_multi[1] = dia_width_default("PIMI_DIA_TB_MULTI", 1);
     P->main.push();
     P->d_tidx.upload(P->tidx.data(), P->tidx.size());
     P->d_tval.upload(P->tval.data(), P->tval.size());
@@ -569,8 +569,9 @@ static void launch_apply(pimi_workspace *W, const cx<R> *x, cx<R> *y, double out
     const ApplyOp op = adjoint ? ApplyOp::Adjoint : (diag ? ApplyOp::Diag : ApplyOp::Full);
     const DevStencil &S = op == ApplyOp::Diag ? P->diag : (op == ApplyOp::Adjoint ? P->adj : P->main);
     if (S.dia_ok) {
+        const int pi = sizeof(R) == 8 ? 0 : 1;
         launch_dia<R>(W->stream, P, S, op == ApplyOp::Full && weighted, x, y, out_scale, ncols,
-                      ncols > 1 ? P->dia_w_multi : P->dia_w_single);
+                      ncols > 1 ? P->dia_w_multi[pi] : P->dia_w_single[pi]);
         return;
     }
     const ApplyEntry *entries = (op == ApplyOp::Full && weighted) ? S.entries_w.p : S.entries.p;
@@ -1786,7 +1787,8 @@ static void launch_stencil(pimi_workspace *W, const DevStencil &S, const cx<doub
                            int64_t ncols) {
     pimi_plan *P = W->plan;
     if (S.dia_ok) {
-        launch_dia<double>(W->stream, P, S, false, x, y, 1.0, ncols, ncols > 1 ? P->dia_w_multi : P->dia_w_single);
+        launch_dia<double>(W->stream, P, S, false, x, y, 1.0, ncols,
+                           ncols > 1 ? P->dia_w_multi[0] : P->dia_w_single[0]);
         return;
     }
     dim3 grid((unsigned)(P->atiles_per_xcd * kNumXcd), (unsigned)std::min<int64_t>(ncols, 65535));
@@ -2227,6 +2229,58 @@ pimi_status pimi_apply(pimi_workspace *W, const void *x, void *y, int64_t ncols,
 
 pimi_status pimi_apply_adjoint(pimi_workspace *W, const void *x, void *y, int64_t ncols, int32_t on_device) {
     return apply_entry(W, x, y, ncols, on_device, true);
+}
+
+pimi_status pimi_workspace_autotune_apply(pimi_workspace *W, int64_t ncols, int32_t reps, int32_t *widths_out) {
+    return guarded([&] {
+        if (!W || !widths_out || ncols < 1 || reps < 1) throw Error(PIMI_EINVAL, "bad argument");
+        BusyGuard guard(W->busy);
+        pimi_plan *P = W->plan;
+        HIPCHK(hipSetDevice(P->ctx->device));
+        widths_out[0] = widths_out[1] = 0;
+        if (!P->main.dia_ok) return;
+        const int pi = W->prec == PIMI_C64 ? 0 : 1;
+        const size_t es = W->prec == PIMI_C64 ? sizeof(cx<double>) : sizeof(cx<float>);
+        const int64_t n = P->n;
+        DevBuf<char> dx, dy;
+        dx.alloc(es * (size_t)(n * ncols));
+        dy.alloc(es * (size_t)(n * ncols));
+        // finite synthetic input: every byte 0x3f (FP64 ~0.49, FP32 ~0.75)
+        HIPCHK(hipMemsetAsync(dx.p, 0x3f, es * (size_t)(n * ncols), W->stream));
+        auto time_width = [&](int w, int64_t nc) {
+            auto run = [&]() {
+                if (W->prec == PIMI_C64)
+                    launch_dia<double>(W->stream, P, P->main, false, (const cx<double> *)dx.p, (cx<double> *)dy.p,
+                                       1.0, nc, w);
+                else
+                    launch_dia<float>(W->stream, P, P->main, false, (const cx<float> *)dx.p, (cx<float> *)dy.p, 1.0,
+                                      nc, w);
+            };
+            run();  // warm
+            HIPCHK(hipEventRecord(W->ev0, W->stream));
+            for (int r = 0; r < reps; ++r) run();
+            HIPCHK(hipEventRecord(W->ev1, W->stream));
+            sync(W);
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, W->ev0, W->ev1));
+            return ms;
+        };
+        for (int pass = 0; pass < 2; ++pass) {
+            const int64_t nc = pass == 0 ? 1 : ncols;
+            int best = 0;
+            float best_ms = INFINITY;
+            for (int w = 0; w < kNumDiaWidths; ++w) {
+                const float ms = time_width(w, nc);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = w;
+                }
+            }
+            if (pass == 0) P->dia_w_single[pi] = best;
+            else P->dia_w_multi[pi] = best;
+            widths_out[pass] = kDiaWidths[best];
+        }
+    });
 }
 
 pimi_status pimi_apply_bench(pimi_workspace *W, int64_t ncols, int32_t reps, double *ms_avg) {

@@ -31,6 +31,7 @@ EXPORTS = [
     "pimi_batch_stats", "pimi_plan_prepare", "pimi_precond_bench",
     "pimi_trajectory_channels", "pimi_conditional_action", "pimi_conditional_rk4", "pimi_channel_intensities",
     "pimi_apply_jumps", "pimi_arnoldi", "pimi_arnoldi_vectors", "pimi_evolve_rk4",
+    "pimi_workspace_autotune_apply", "pimi_batch_autotune_apply",
 ]
 
 

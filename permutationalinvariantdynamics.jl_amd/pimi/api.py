@@ -429,6 +429,16 @@ class CompiledPIModel:
         y = y.T
         return y.reshape(x.shape)
 
+    def autotune(self, precision=N.PIMI_C64, ncols=8, reps=50):
+        """Pick the banded apply's tile widths for this plan by timing 4-, 8-
+        and 16-column tiles on the device (one vector and `ncols` vectors);
+        the choice holds for every later apply in `precision`.  Returns the
+        (single-vector, multi-column) widths, (0, 0) for ELL-form plans."""
+        out = (C.c_int32 * 2)()
+        N.check(N.lib().pimi_workspace_autotune_apply(self.workspace(precision), C.c_int64(ncols), C.c_int32(reps),
+                                                       out))
+        return int(out[0]), int(out[1])
+
     def last_apply_ms(self, precision=N.PIMI_C64):
         ms = C.c_double()
         N.check(N.lib().pimi_workspace_last_apply_ms(self.workspace(precision), C.byref(ms)))

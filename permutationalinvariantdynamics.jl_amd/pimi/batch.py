@@ -125,6 +125,15 @@ class BatchEngine:
             N.check(st)
         return res
 
+    def autotune(self, compiled_list, reps=5):
+        """Pick the engine's banded-apply tile width by timing one launch over
+        a vector of every model at each width; returns the chosen width."""
+        arr = (C.c_void_p * len(compiled_list))(*[c._plan.value for c in compiled_list])
+        w = C.c_int32()
+        N.check(N.lib().pimi_batch_autotune_apply(self._h, arr, C.c_int32(len(compiled_list)), C.c_int32(reps),
+                                                  C.byref(w)))
+        return w.value
+
     def apply_bench(self, compiled_list, reps=20):
         """One launch applying L to a vector of every model, event-timed:
         (ms per launch, algorithmic bytes per launch)."""

@@ -174,3 +174,15 @@ def test_batch_apply_bench_counts_bytes():
     eng.close()
     assert ms > 0
     assert nbytes == sum(2 * c.info["n"] * 16 + c.info["table_bytes_c64"] for c in comps)
+
+
+def test_batch_autotune_keeps_results():
+    comps = [pimi.compile(_c4_model(n_, 0.7, 0.01)) for n_ in (16, 24)]
+    alg = pimi.BatchAlgorithm(krylovdim=60, inner_rtol=1e-4)
+    ref = pimi.batched_stationary_states(comps, alg, rtol=1e-9)
+    eng = pimi.BatchEngine()
+    assert eng.autotune(comps, reps=2) in (4, 8, 16)
+    out = pimi.batched_stationary_states(comps, alg, rtol=1e-9, engine=eng)
+    eng.close()
+    for a, b in zip(ref, out):
+        np.testing.assert_array_equal(a.state.data, b.state.data)

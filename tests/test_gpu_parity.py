@@ -439,3 +439,16 @@ def test_banded_apply_bitwise_independent_of_tile_width():
     ys = [_compile_with_env(model, PIMI_DIA_TB_SINGLE=tb).apply(x) for tb in (4, 8, 16)]
     np.testing.assert_array_equal(ys[0], ys[1])
     np.testing.assert_array_equal(ys[0], ys[2])
+
+
+def test_apply_autotune_picks_a_width_and_keeps_results():
+    comp = pimi.compile(_c2_model(40))
+    x = _rand(comp.info["n"], 5)
+    X = _rand(comp.info["n"], 6, cols=3)
+    y0, Y0 = comp.apply(x), comp.apply(X)
+    single, multi = comp.autotune(pimi.PIMI_C64, ncols=3, reps=5)
+    assert single in (4, 8, 16) and multi in (4, 8, 16)
+    np.testing.assert_array_equal(comp.apply(x), y0)
+    np.testing.assert_array_equal(comp.apply(X), Y0)
+    model, _ = _qutrit(2)
+    assert pimi.compile(model).autotune(reps=2) == (0, 0)
