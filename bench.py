@@ -262,8 +262,8 @@ def main():
     widths = {"c64": comp.autotune(pimi.PIMI_C64), "c32": comp.autotune(pimi.PIMI_C32)}
     if args.profile:
         # counter runs: 200 event-timed applies, 5 preconditioner applications (20
-        # GEMM launches), 5 batched applies, then 20 device copies of 64 MiB (the
-        # counters' calibration)
+        # GEMM launches), 5 batched applies at the autotuned width and 5 at each
+        # width, then 20 device copies of 64 MiB (the counters' calibration)
         N = pimi._native
         ms_apply, bytes_apply, achieved = apply_roofline(pimi, comp, reps=200)
         ms_gemm, flops_gemm, tfs = gemm_roofline(pimi, comp, reps=5)
@@ -274,6 +274,15 @@ def main():
         wb = eng.autotune(comps)
         ms_b, bytes_b = eng.apply_bench(comps, reps=5)
         eng.close()
+        # and 5 launches at each tile width, so every width the autotuner may
+        # pick has its counters (the engine reads the width override at creation)
+        per_width = {}
+        for tb in (4, 8, 16):
+            os.environ["PIMI_DIA_TB_BATCH"] = str(tb)
+            eng = pimi.BatchEngine(precision=pimi.PIMI_C32)
+            per_width[tb] = eng.apply_bench(comps, reps=5)[0]
+            eng.close()
+        del os.environ["PIMI_DIA_TB_BATCH"]
         ms = C.c_double()
         N.check(N.lib().pimi_copy_bench(comp.workspace(pimi.PIMI_C64, 1), C.c_int64(64 << 20),
                                         C.c_int32(20), C.byref(ms)))
@@ -281,6 +290,7 @@ def main():
                           "batched_apply_kernel": BATCH_APPLY_KERNEL.format(wb),
                           "apply_ms": ms_apply, "apply_algorithmic_bytes": bytes_apply, "apply_GBps": achieved,
                           "batched_apply_ms": ms_b, "batched_apply_algorithmic_bytes": bytes_b,
+                          "batched_apply_ms_per_width": per_width,
                           "batched_apply_GBps": bytes_b / (ms_b * 1e-3) / 1e9,
                           "gemm_ms": ms_gemm, "gemm_flops": flops_gemm, "gemm_TFps": tfs,
                           "copy_bytes": 64 << 20, "copy_ms": ms.value,

@@ -226,10 +226,18 @@ struct DevStencil {
                                   cd(0.0, 0.0));
                     return q;
                 };
+                // a real sandwich keeps real parts only, two per complex slot
+                // (element i of the real vector at real index i of the slot array)
+                auto packed = [](const std::vector<cd> &v) {
+                    std::vector<cd> p((v.size() + 1) / 2, cd(0.0, 0.0));
+                    for (size_t i = 0; i < v.size(); ++i)
+                        p[i / 2] = i % 2 ? cd(p[i / 2].real(), v[i].real()) : cd(v[i].real(), p[i / 2].imag());
+                    return p;
+                };
                 for (auto &E : pend[g]) {
                     DiaEntry D{E.kind, E.dB - lo, 0, 0};
-                    if (E.kind != 1) D.qa = put(E.ca);
-                    if (E.kind != 0) D.qb = put(E.wb);
+                    if (E.kind != 1) D.qa = put(E.kind == 3 ? packed(E.ca) : E.ca);
+                    if (E.kind != 0) D.qb = put(E.kind == 3 ? packed(E.wb) : E.wb);
                     if (h_coef.size() >= (size_t)INT32_MAX) return;
                     h_dents.push_back(D);
                     h_dent_sector.push_back(q);
@@ -269,7 +277,8 @@ struct DevStencil {
             if (r == 1.0) continue;
             const DiaEntry &D = h_dents[e];
             const int32_t q = D.kind == 1 ? D.qb : D.qa;
-            const size_t len = (size_t)dims_of_entry(e);
+            const size_t m = (size_t)dims_of_entry(e);
+            const size_t len = D.kind == 3 ? (m + 1) / 2 : m;  // packed reals: two per slot
             for (size_t k = 0; k < len; ++k) w[(size_t)q + k] *= r;
         }
         std::vector<cx<double>> c64(w.size());

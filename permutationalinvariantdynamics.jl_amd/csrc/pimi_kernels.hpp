@@ -224,37 +224,61 @@ __device__ __forceinline__ cx<R> cmulf(cx<R> a, cx<R> b) {
 }
 
 
-// TB consecutive column coefficients, read as one block
+// TB consecutive column coefficients, read as one block (complex, or the
+// packed reals of a real sandwich)
 template <typename R, int TB>
 struct DiaCols {
     cx<R> v[TB];
 };
+template <typename R, int TB>
+struct DiaColsR {
+    R v[TB];
+};
+
+// Per-lane (row) coefficients come through a raw buffer over the pool: the
+// lane's offset (a times the element size) is fixed for the tile, the
+// vector's start is a scalar offset, so a load needs no address arithmetic.
+template <typename R>
+__device__ __forceinline__ cx<R> dia_row(__amdgpu_buffer_rsrc_t pool, int voff, int q) {
+    if constexpr (sizeof(R) == 8)
+        return __builtin_bit_cast(cx<R>, __builtin_amdgcn_raw_buffer_load_b128(pool, voff, q * 16, 0));
+    else
+        return __builtin_bit_cast(cx<R>, __builtin_amdgcn_raw_buffer_load_b64(pool, voff, q * 8, 0));
+}
+template <typename R>
+__device__ __forceinline__ R dia_row_real(__amdgpu_buffer_rsrc_t pool, int voff_real, int q) {
+    if constexpr (sizeof(R) == 8)
+        return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(pool, voff_real, q * 16, 0));
+    else
+        return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(pool, voff_real, q * 8, 0));
+}
 
 template <typename R, int TB, int KIND, int DB>
 __device__ __forceinline__ void dia_entry(cx<R> (&acc)[TB], const cx<R> (&st)[TB + 2], const DiaEntry &E,
-                                          const PIMI_GLOBAL cx<R> *coefv, const PIMI_CONST cx<R> *coefs, int a,
-                                          int b0) {
+                                          __amdgpu_buffer_rsrc_t pool, const PIMI_CONST cx<R> *coefs, int va,
+                                          int vr, int b0) {
     if constexpr (KIND == 0) {
-        const cx<R> ca = coefv[E.qa + a];
+        const cx<R> ca = dia_row<R>(pool, va, E.qa);
 #pragma unroll
         for (int j = 0; j < TB; ++j) acc[j] = cfma(ca, st[j + DB], acc[j]);
+    } else if constexpr (KIND == 3) {
+        const DiaColsR<R, TB> w = *(const PIMI_CONST DiaColsR<R, TB> *)((const PIMI_CONST R *)(coefs + E.qb) + b0);
+        const R ca = dia_row_real<R>(pool, vr, E.qa);
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+            const R c = ca * w.v[j];
+            acc[j].x = rfma(c, st[j + DB].x, acc[j].x);
+            acc[j].y = rfma(c, st[j + DB].y, acc[j].y);
+        }
     } else {
         const DiaCols<R, TB> w = *(const PIMI_CONST DiaCols<R, TB> *)(coefs + E.qb + b0);
         if constexpr (KIND == 1) {
 #pragma unroll
             for (int j = 0; j < TB; ++j) acc[j] = cfma(w.v[j], st[j + DB], acc[j]);
-        } else if constexpr (KIND == 2) {
-            const cx<R> ca = coefv[E.qa + a];
+        } else {
+            const cx<R> ca = dia_row<R>(pool, va, E.qa);
 #pragma unroll
             for (int j = 0; j < TB; ++j) acc[j] = cfma(cmulf(ca, w.v[j]), st[j + DB], acc[j]);
-        } else {
-            const R ca = coefv[E.qa + a].x;
-#pragma unroll
-            for (int j = 0; j < TB; ++j) {
-                const R c = ca * w.v[j].x;
-                acc[j].x = rfma(c, st[j + DB].x, acc[j].x);
-                acc[j].y = rfma(c, st[j + DB].y, acc[j].y);
-            }
         }
     }
 }
@@ -300,7 +324,6 @@ __device__ __forceinline__ void dia_tile(const cx<R> *xc_, cx<R> *yc_, R out_sca
     const PIMI_CONST int32_t *gbase = cptr(gbase_);
     const PIMI_CONST DiaGroup *groups = cptr(groups_);
     const PIMI_CONST DiaEntry *ents = cptr(ents_);
-    const PIMI_GLOBAL cx<R> *coefv = gptr(coef_);
     const PIMI_CONST cx<R> *coefs = cptr(coef_);
     const int s = __builtin_amdgcn_readfirstlane(tile.sector);
     const int b0 = __builtin_amdgcn_readfirstlane(tile.b0);
@@ -310,6 +333,12 @@ __device__ __forceinline__ void dia_tile(const cx<R> *xc_, cx<R> *yc_, R out_sca
     const int arow = a0 + (int)(threadIdx.x & (kApplyWave - 1));
     const bool active = arow < m;
     const int a = active ? arow : m - 1;  // inactive lanes evaluate the last row and store nothing
+    // the coefficient pool as one raw buffer (every read is in range; the
+    // pool is far below 2 GiB), and the lane's byte offsets into a vector:
+    // complex element a, and real element a of a packed real vector
+    const __amdgpu_buffer_rsrc_t pool =
+        __builtin_amdgcn_make_buffer_rsrc((void *)coef_, (short)0, 0x7fffffff, 0x00020000);
+    const int va = a * (int)sizeof(cx<R>), vr = a * (int)sizeof(R);
     cx<R> acc[TB];
 #pragma unroll
     for (int j = 0; j < TB; ++j) acc[j] = cx<R>{0, 0};
@@ -323,18 +352,18 @@ __device__ __forceinline__ void dia_tile(const cx<R> *xc_, cx<R> *yc_, R out_sca
         for (int e = G.e0; e < G.e1; ++e) {
             const DiaEntry E = ents[e];
             switch (E.kind * 4 + E.dB) {
-                case 0: dia_entry<R, TB, 0, 0>(acc, st, E, coefv, coefs, a, b0); break;
-                case 1: dia_entry<R, TB, 0, 1>(acc, st, E, coefv, coefs, a, b0); break;
-                case 2: dia_entry<R, TB, 0, 2>(acc, st, E, coefv, coefs, a, b0); break;
-                case 4: dia_entry<R, TB, 1, 0>(acc, st, E, coefv, coefs, a, b0); break;
-                case 5: dia_entry<R, TB, 1, 1>(acc, st, E, coefv, coefs, a, b0); break;
-                case 6: dia_entry<R, TB, 1, 2>(acc, st, E, coefv, coefs, a, b0); break;
-                case 8: dia_entry<R, TB, 2, 0>(acc, st, E, coefv, coefs, a, b0); break;
-                case 9: dia_entry<R, TB, 2, 1>(acc, st, E, coefv, coefs, a, b0); break;
-                case 10: dia_entry<R, TB, 2, 2>(acc, st, E, coefv, coefs, a, b0); break;
-                case 12: dia_entry<R, TB, 3, 0>(acc, st, E, coefv, coefs, a, b0); break;
-                case 13: dia_entry<R, TB, 3, 1>(acc, st, E, coefv, coefs, a, b0); break;
-                case 14: dia_entry<R, TB, 3, 2>(acc, st, E, coefv, coefs, a, b0); break;
+                case 0: dia_entry<R, TB, 0, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 1: dia_entry<R, TB, 0, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 2: dia_entry<R, TB, 0, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 4: dia_entry<R, TB, 1, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 5: dia_entry<R, TB, 1, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 6: dia_entry<R, TB, 1, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 8: dia_entry<R, TB, 2, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 9: dia_entry<R, TB, 2, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 10: dia_entry<R, TB, 2, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 12: dia_entry<R, TB, 3, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 13: dia_entry<R, TB, 3, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 14: dia_entry<R, TB, 3, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
                 default: break;
             }
         }
