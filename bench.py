@@ -257,45 +257,47 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
     import pimi
-    comp = pimi.compile(build_model(pimi), backend="mi355x", device=0)
-    # tile widths of the banded apply, picked on this device (FP64 and FP32 workspaces)
-    widths = {"c64": comp.autotune(pimi.PIMI_C64), "c32": comp.autotune(pimi.PIMI_C32)}
     if args.profile:
-        # counter runs: 200 event-timed applies, 5 preconditioner applications (20
-        # GEMM launches), 5 batched applies at the autotuned width and 5 at each
-        # width, then 20 device copies of 64 MiB (the counters' calibration)
+        # counter runs, with fixed tile widths (no autotuning), so that every
+        # launch of a kernel name has the same shape: at each width (4, 8, 16
+        # columns) event-timed single-vector applies (200 at width 4, 50 at the
+        # others) and 5 batched launches over the 512 models; 5 preconditioner
+        # applications (20 GEMM launches); 20 device copies of 64 MiB (the
+        # counters' calibration)
         N = pimi._native
-        ms_apply, bytes_apply, achieved = apply_roofline(pimi, comp, reps=200)
+        single = {}
+        for tb in (4, 8, 16):
+            os.environ["PIMI_DIA_TB_SINGLE"] = str(tb)
+            c = pimi.compile(build_model(pimi), backend="mi355x", device=0)
+            single[tb] = apply_roofline(pimi, c, reps=200 if tb == 4 else 50)
+            c.release_workspaces()
+        del os.environ["PIMI_DIA_TB_SINGLE"]
+        comp = pimi.compile(build_model(pimi), backend="mi355x", device=0)
         ms_gemm, flops_gemm, tfs = gemm_roofline(pimi, comp, reps=5)
-        # the batched-workload apply: 512 models in one launch (FP32), 5 launches
         from pimi.workloads import batched_instances
         comps = [pimi.compile(i.model()) for i in batched_instances(BATCH_COUNT)]
-        eng = pimi.BatchEngine(precision=pimi.PIMI_C32)
-        wb = eng.autotune(comps)
-        ms_b, bytes_b = eng.apply_bench(comps, reps=5)
-        eng.close()
-        # and 5 launches at each tile width, so every width the autotuner may
-        # pick has its counters (the engine reads the width override at creation)
-        per_width = {}
+        batched_ms, bytes_b = {}, 0
         for tb in (4, 8, 16):
             os.environ["PIMI_DIA_TB_BATCH"] = str(tb)
             eng = pimi.BatchEngine(precision=pimi.PIMI_C32)
-            per_width[tb] = eng.apply_bench(comps, reps=5)[0]
+            batched_ms[tb], bytes_b = eng.apply_bench(comps, reps=5)
             eng.close()
         del os.environ["PIMI_DIA_TB_BATCH"]
         ms = C.c_double()
         N.check(N.lib().pimi_copy_bench(comp.workspace(pimi.PIMI_C64, 1), C.c_int64(64 << 20),
                                         C.c_int32(20), C.byref(ms)))
-        print(json.dumps({"apply_kernel": APPLY_KERNEL.format(widths["c64"][0]),
-                          "batched_apply_kernel": BATCH_APPLY_KERNEL.format(wb),
-                          "apply_ms": ms_apply, "apply_algorithmic_bytes": bytes_apply, "apply_GBps": achieved,
-                          "batched_apply_ms": ms_b, "batched_apply_algorithmic_bytes": bytes_b,
-                          "batched_apply_ms_per_width": per_width,
-                          "batched_apply_GBps": bytes_b / (ms_b * 1e-3) / 1e9,
+        print(json.dumps({"single_n120_fp64": {APPLY_KERNEL.format(tb): {"ms": v[0], "algorithmic_bytes": v[1],
+                                                                         "GBps": v[2]} for tb, v in single.items()},
+                          "batched_512_fp32": {BATCH_APPLY_KERNEL.format(tb): {"ms": v, "algorithmic_bytes": bytes_b,
+                                                                               "GBps": bytes_b / (v * 1e-3) / 1e9}
+                                               for tb, v in batched_ms.items()},
                           "gemm_ms": ms_gemm, "gemm_flops": flops_gemm, "gemm_TFps": tfs,
                           "copy_bytes": 64 << 20, "copy_ms": ms.value,
                           "copy_GBps": 2 * (64 << 20) / (ms.value * 1e-3) / 1e9}))
         return
+    comp = pimi.compile(build_model(pimi), backend="mi355x", device=0)
+    # tile widths of the banded apply, picked on this device (FP64 and FP32 workspaces)
+    widths = {"c64": comp.autotune(pimi.PIMI_C64), "c32": comp.autotune(pimi.PIMI_C32)}
     alg = pimi.GMRESAlgorithm(krylovdim=KRYLOV, maxiter=10 * KRYLOV, preconditioner="sector_inner",
                               inner_steps=INNER_STEPS)
 
