@@ -284,11 +284,13 @@ __device__ __forceinline__ void dia_entry(cx<R> (&acc)[TB], const cx<R> (&st)[TB
 }
 
 // the strip X_in[r, cfirst + k], k < NK, of one group through a raw buffer
-// over the input block: the lane's (clamped) row offset in voffset, the
-// column offset in soffset.  A column outside the block gives an offset
-// outside the resource (negative offsets wrap past its end), which the
-// hardware range check reads as zero -- exactly the zero the coefficient
-// vectors hold there.  0x00020000: the gfx9-family word 3 of a raw buffer.
+// over the input block, the whole offset ((clamped) row + column times the
+// block size) in voffset, which the hardware range check covers: a column
+// outside the block gives an offset outside the resource (a negative one
+// reads as a huge unsigned offset), which reads zero -- exactly the zero the
+// coefficient vectors hold there.  (The scalar offset operand is not used:
+// the range check need not cover it.)  0x00020000: the gfx9-family word 3 of
+// a raw buffer resource.
 template <typename R, int TB, int NK>
 __device__ __forceinline__ void dia_strip(cx<R> (&st)[TB + 2], const cx<R> *Xn, int r, int mn, int cfirst) {
     constexpr int es = (int)sizeof(cx<R>);
@@ -299,11 +301,12 @@ __device__ __forceinline__ void dia_strip(cx<R> (&st)[TB + 2], const cx<R> *Xn, 
     const int s0 = cfirst * step;
 #pragma unroll
     for (int k = 0; k < TB + 2; ++k) {
+        const int vo = vr + (s0 + k * step);
         if constexpr (sizeof(R) == 8) {
-            st[k] = k < NK ? __builtin_bit_cast(cx<R>, __builtin_amdgcn_raw_buffer_load_b128(rs, vr, s0 + k * step, 0))
+            st[k] = k < NK ? __builtin_bit_cast(cx<R>, __builtin_amdgcn_raw_buffer_load_b128(rs, vo, 0, 0))
                            : cx<R>{0, 0};
         } else {
-            st[k] = k < NK ? __builtin_bit_cast(cx<R>, __builtin_amdgcn_raw_buffer_load_b64(rs, vr, s0 + k * step, 0))
+            st[k] = k < NK ? __builtin_bit_cast(cx<R>, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, 0, 0))
                            : cx<R>{0, 0};
         }
     }
