@@ -316,7 +316,13 @@ __device__ __forceinline__ void dia_strip(cx<R> (&st)[TB + 2], const cx<R> *Xn, 
 // (nb <= TB) of sector s's output block.  Everything but the lane's row is
 // wave-uniform: descriptors and column coefficients are scalar loads, the
 // entry switch a scalar branch.
-template <typename R, int TB>
+//
+// SPLIT > 1: the tile's groups are dealt to the SPLIT waves of the workgroup
+// (wave w takes groups w, w + SPLIT, ...), which shortens each wave's chain of
+// dependent strip loads; the partial sums meet in LDS and wave 0 adds them in
+// the order w = 0 .. SPLIT-1 (deterministic).  Used for the single-solve
+// paths, where one vector has too few tiles to hide the chain's latency.
+template <typename R, int TB, int SPLIT = 1>
 __device__ __forceinline__ void dia_tile(const cx<R> *xc_, cx<R> *yc_, R out_scale, ApplyTile tile,
                                          const int64_t *offs_, const int32_t *dims_, const int32_t *gbase_,
                                          const DiaGroup *groups_, const DiaEntry *ents_, const cx<R> *coef_) {
@@ -345,8 +351,9 @@ __device__ __forceinline__ void dia_tile(const cx<R> *xc_, cx<R> *yc_, R out_sca
     cx<R> acc[TB];
 #pragma unroll
     for (int j = 0; j < TB; ++j) acc[j] = cx<R>{0, 0};
+    const int wave = SPLIT > 1 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
     const int g0 = gbase[s], g1 = gbase[s + 1];
-    for (int g = g0; g < g1; ++g) {
+    for (int g = g0 + wave; g < g1; g += SPLIT) {
         const DiaGroup G = groups[g];
         cx<R> st[TB + 2];
         const int r = min(max(a + G.dA, 0), G.mn - 1);
@@ -371,6 +378,21 @@ __device__ __forceinline__ void dia_tile(const cx<R> *xc_, cx<R> *yc_, R out_sca
             }
         }
     }
+    if constexpr (SPLIT > 1) {
+        __shared__ cx<R> part[SPLIT - 1][TB][kApplyWave];
+        const int lane = threadIdx.x & (kApplyWave - 1);
+        if (wave > 0)
+#pragma unroll
+            for (int j = 0; j < TB; ++j) part[wave - 1][j][lane] = acc[j];
+        __syncthreads();
+        if (wave == 0)
+#pragma unroll
+            for (int w = 0; w < SPLIT - 1; ++w)
+#pragma unroll
+                for (int j = 0; j < TB; ++j) acc[j] = cadd(acc[j], part[w][j][lane]);
+        __syncthreads();  // the LDS is free for the workgroup's next column vector
+        if (wave > 0) return;
+    }
     if (active) {
         const int64_t o = offs[s] + a;
 #pragma unroll
@@ -379,22 +401,23 @@ __device__ __forceinline__ void dia_tile(const cx<R> *xc_, cx<R> *yc_, R out_sca
     }
 }
 
-// y = out_scale * L x for ncols column vectors through the banded form;
-// kDiaWaves tiles per workgroup, tiles spread so that XCD k (workgroup id
-// mod 8) handles one contiguous slab of the sector-major tile list.
+// y = out_scale * L x for ncols column vectors through the banded form, one
+// tile per workgroup of kDiaSplit waves that share the tile's groups; tiles
+// spread so that XCD k (workgroup id mod 8) handles one contiguous slab of
+// the sector-major tile list.
+constexpr int kDiaSplit = 4;
 template <typename R, int TB>
-__global__ __launch_bounds__(kApplyWave *kDiaWaves) void k_apply_dia(
+__global__ __launch_bounds__(kApplyWave *kDiaSplit) void k_apply_dia_split(
     const cx<R> *__restrict__ x, cx<R> *__restrict__ y, int64_t n, const ApplyTile *__restrict__ tiles,
     int64_t ntiles, int64_t blocks_per_xcd, const int64_t *__restrict__ offs, const int32_t *__restrict__ dims,
     const int32_t *__restrict__ gbase, const DiaGroup *__restrict__ groups, const DiaEntry *__restrict__ ents,
     const cx<R> *__restrict__ coef, R out_scale, int64_t ncols) {
     const int64_t bid = blockIdx.x;
-    const int64_t blk = (bid % kNumXcd) * blocks_per_xcd + bid / kNumXcd;
-    const int64_t t = blk * kDiaWaves + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    if (t >= ntiles) return;
+    const int64_t t = (bid % kNumXcd) * blocks_per_xcd + bid / kNumXcd;
+    if (t >= ntiles) return;  // the whole workgroup leaves together
     const ApplyTile tile = cptr(tiles)[t];
     for (int64_t c = blockIdx.y; c < ncols; c += gridDim.y)
-        dia_tile<R, TB>(x + c * n, y + c * n, out_scale, tile, offs, dims, gbase, groups, ents, coef);
+        dia_tile<R, TB, kDiaSplit>(x + c * n, y + c * n, out_scale, tile, offs, dims, gbase, groups, ents, coef);
 }
 
 // y[i] = x[i] * w[sector of i], one coordinate per lane over the plan's tiles

@@ -545,14 +545,14 @@ static void launch_dia(hipStream_t stream, const pimi_plan *P, const DevStencil 
     const cx<R> *coef;
     if constexpr (sizeof(R) == 8) coef = weighted ? S.coefw64.p : S.coef64.p;
     else coef = weighted ? S.coefw32.p : S.coef32.p;
+    // one tile per workgroup, its groups split over kDiaSplit waves
     const int64_t nt = P->ndtiles[w];
-    const int64_t nblk = (nt + kDiaWaves - 1) / kDiaWaves;
-    const int64_t per_xcd = (nblk + kNumXcd - 1) / kNumXcd;
+    const int64_t per_xcd = (nt + kNumXcd - 1) / kNumXcd;
     dim3 grid((unsigned)(per_xcd * kNumXcd), (unsigned)std::min<int64_t>(ncols, 65535));
-    dim3 block(kApplyWave * kDiaWaves);
+    dim3 block(kApplyWave * kDiaSplit);
 #define PIMI_DIA_LAUNCH(TB)                                                                                   \
-    hipLaunchKernelGGL((k_apply_dia<R, TB>), grid, block, 0, stream, x, y, P->n, P->dtiles[w].p, nt, per_xcd, \
-                       P->offs.p, P->dims.p, S.gbase.p, S.groups.p, S.dents.p, coef, (R)out_scale, ncols)
+    hipLaunchKernelGGL((k_apply_dia_split<R, TB>), grid, block, 0, stream, x, y, P->n, P->dtiles[w].p, nt,     \
+                       per_xcd, P->offs.p, P->dims.p, S.gbase.p, S.groups.p, S.dents.p, coef, (R)out_scale, ncols)
     switch (kDiaWidths[w]) {
         case 4: PIMI_DIA_LAUNCH(4); break;
         case 8: PIMI_DIA_LAUNCH(8); break;
