@@ -150,7 +150,20 @@ def batched_stationary_states(compiled_models, algorithm: BatchAlgorithm | None 
     """Stationary states of many compiled models, solved together; results in
     input order."""
     own = engine is None
-    eng = engine or BatchEngine(precision=precision, device=compiled_models[0].device if compiled_models else 0)
+    if own:
+        # reserve what the jobs need (all resident at once), not the default
+        # three quarters of free HBM: a large reservation costs time to map
+        alg = algorithm or BatchAlgorithm()
+        need = 0
+        for c in compiled_models:
+            b = C.c_int64()
+            N.check(N.lib().pimi_batch_job_bytes(c._plan, C.c_int32(precision), C.c_int32(alg.krylovdim),
+                                                 C.byref(b)))
+            need += b.value + 256
+        eng = BatchEngine(precision=precision, memory_bytes=need + (1 << 20),
+                          device=compiled_models[0].device if compiled_models else 0)
+    else:
+        eng = engine
     try:
         ids = [eng.submit(c, algorithm, atol=atol, rtol=rtol) for c in compiled_models]
         out = {}
