@@ -25,6 +25,8 @@ Also reported:
                 a sample of the same models;
   batched_n120  the same family at N=120 only (64 models), against the plain
                 FP64 single-solve path on a sample of them;
+  precision_form  spin-1 N=18: FP32 Arnoldi in FP64 refinement against the plain
+                FP64 device solve, observables against the FP64 reference values;
   serving       a Poisson stream (200 requests/s, 60 requests, 60 % stationary
                 states, 40 % dynamics to t=10) through the serving loop;
   cpu_baseline  the oracle's OpenMP CPU restatement on a bounded sample.
@@ -224,6 +226,62 @@ def batched_n120(pimi, count=64, plain_sample=4):
             "speedup_vs_plain_fp64": rate / plain_rate}
 
 
+def precision_form(pimi):
+    """Spin-1 ensemble N=18 (PI dimension 1,562,275, blocks up to 442): H = sum(Sz + 0.7 Sx) + (0.3/N) Jz^2,
+    local S- jumps 0.1.  Complex FP32 Arnoldi/apply/preconditioner inside FP64 iterative refinement (inner
+    residual reduction 1e-5 per cycle, outer relative residual 1e-9) against the plain FP64 device solve
+    (sector-preconditioned GMRES); observables <Jz>/N and <J+J->/N^2 of each against the FP64 reference
+    values in tests/golden/c3_reference.json (written by the oracle's solve)."""
+    S3 = pimi.spin_matrices(3)
+    Nq = 18
+    B = pimi.PIBasis(Nq, 3)
+    Jz = pimi.collective_operator(B, S3.jz)
+    comp = pimi.compile(pimi.PIModel(B, [pimi.LocalHamiltonian(1.0 * S3.jz + 0.7 * S3.jx),
+                                         pimi.DirectPIHamiltonian(Jz * Jz, rate=0.3 / Nq),
+                                         pimi.LocalJump(S3.jm, rate=0.1)])).prepare()
+    comp.autotune(pimi.PIMI_C64)
+    comp.autotune(pimi.PIMI_C32)
+    jz = comp.collective_operator(S3.jz).data
+    jp = comp.collective_operator(S3.jp).data
+    jm = comp.collective_operator(S3.jm).data
+    _, tv = comp.trace_functional()
+    jj = np.zeros_like(jp)
+    k = 0
+    for sec, m in enumerate(comp.basis.dims):  # (A B) coefficient blocks = A_nu B_nu / sqrt(f)
+        o0, o1 = comp.basis.offsets[sec], comp.basis.offsets[sec + 1]
+        jj[o0:o1] = ((jp[o0:o1].reshape(m, m, order="F") @ jm[o0:o1].reshape(m, m, order="F")) / tv[k]).reshape(
+            -1, order="F")
+        k += m
+    with open(os.path.join(ROOT, "tests", "golden", "c3_reference.json")) as fh:
+        gold = json.load(fh)
+
+    def run(kind):
+        if kind == "fp32_refinement":
+            r = pimi.batched_stationary_states([comp], pimi.BatchAlgorithm(krylovdim=200, inner_rtol=1e-5),
+                                               rtol=1e-9, precision=pimi.PIMI_C32)[0]
+        else:
+            r = pimi.stationary_state(comp, algorithm=pimi.GMRESAlgorithm(400, 2000, "sector"), atol=0.0,
+                                      rtol=1e-9, return_info=True, criterion="backward", weighting="sector")
+        return r
+
+    out = {"workload": "spin-1 ensemble N=18 (PI dim 1562275): H = sum(Sz + 0.7 Sx) + (0.3/N) Jz^2, local S- 0.1; "
+                       "stationary state to |L rho|/(s|rho|) <= 1e-9", "n": comp.info["n"]}
+    for kind in ("fp32_refinement", "plain_fp64"):
+        run(kind)  # warm-up (workspace, graphs)
+        t0 = time.perf_counter()
+        r = run(kind)
+        secs = time.perf_counter() - t0
+        z = np.vdot(jz, r.state.data).real / Nq
+        q = np.vdot(jj, r.state.data).real / Nq ** 2
+        out[kind] = {"ms": secs * 1e3, "iterations": int(r.info["iterations"]),
+                     "relative_residual": r.info["relative_residual"],
+                     "jz_over_N_rel_err": abs(z - gold["jz_over_N"]) / abs(gold["jz_over_N"]),
+                     "jpjm_over_N2_rel_err": abs(q - gold["jpjm_over_N2"]) / abs(gold["jpjm_over_N2"])}
+        print(f"# precision form {kind}: {out[kind]}", file=sys.stderr, flush=True)
+    out["speedup_fp32_refinement_vs_plain_fp64"] = out["plain_fp64"]["ms"] / out["fp32_refinement"]["ms"]
+    return out
+
+
 def serving(pimi):
     from pimi.serving import Server, summarize
     from pimi.workloads import served_requests
@@ -338,6 +396,7 @@ def main():
     batch, spmv_batch = (None, None) if args.no_batch else batched(pimi, args)
     batch120 = None if args.no_batch else batched_n120(pimi)
     serve = None if args.no_serve else serving(pimi)
+    precision = None if args.no_batch else precision_form(pimi)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(side["plain_fp64"]["iterations"], args.cpu_seconds)
@@ -383,6 +442,7 @@ def main():
         "spmv": spmv,
         "batched": batch,
         "batched_n120": batch120,
+        "precision_form": precision,
         "serving": serve,
         "cpu_baseline": cpu,
     }
