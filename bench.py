@@ -145,6 +145,10 @@ def batched(pimi, args):
     width = eng.autotune(comps)
     ms_spmv, bytes_spmv = eng.apply_bench(comps, reps=20)
     eng.close()
+    eng = pimi.BatchEngine(precision=pimi.PIMI_C64, memory_bytes=1 << 30)
+    width64 = eng.autotune(comps)
+    ms_spmv64, bytes_spmv64 = eng.apply_bench(comps, reps=20)
+    eng.close()
     eng = pimi.BatchEngine(precision=pimi.PIMI_C32, timing=True)
     eng.autotune(comps)
     alg = pimi.BatchAlgorithm(krylovdim=300, inner_rtol=1e-4)
@@ -184,7 +188,10 @@ def batched(pimi, args):
         "speedup_vs_plain_fp64": rate / plain_rate,
         "ops": ops,
     }, {"achieved": bytes_spmv / (ms_spmv * 1e-3) / 1e9, "avg_ms": ms_spmv, "algorithmic_bytes": bytes_spmv,
-        "models": len(comps), "tile_width": width}
+        "models": len(comps), "tile_width": width,
+        "fp64": {"achieved": bytes_spmv64 / (ms_spmv64 * 1e-3) / 1e9, "avg_ms": ms_spmv64,
+                 "algorithmic_bytes": bytes_spmv64, "tile_width": width64,
+                 "frac": bytes_spmv64 / (ms_spmv64 * 1e-3) / 1e9 / HBM_PEAK_GBS}}
 
 
 def batched_n120(pimi, count=64, plain_sample=4):
