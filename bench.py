@@ -25,6 +25,8 @@ Also reported:
                 a sample of the same models;
   batched_n120  the same family at N=120 only (64 models), against the plain
                 FP64 single-solve path on a sample of them;
+  dynamics_n120 the C2 expv dynamics (t in [0, 10], 101 samples) and the
+                oracle's CPU expv on a bounded sample of it;
   precision_form  spin-1 N=18: FP32 Arnoldi in FP64 refinement against the plain
                 FP64 device solve, observables against the FP64 reference values;
   serving       a Poisson stream (200 requests/s, 60 requests, 60 % stationary
@@ -289,6 +291,45 @@ def precision_form(pimi):
     return out
 
 
+def dynamics_n120(pimi, comp, cpu=True):
+    """The C2 dynamics: the headline model from |e><e|^N over t in [0, 10], saved every 0.1 (101 samples), by
+    adaptive Krylov expv (Krylov dimension 40, atol 0, rtol 1e-12), observable sum_i n_i; timed warm.  With
+    `cpu`, the oracle's OpenMP restatement of the same expv on all host cores over the first two save
+    intervals, scaled to the 100 intervals."""
+    sp = pimi.spin_matrices(2)
+    rho0 = pimi.computational_product_state(comp.basis, 2)
+    alg = pimi.ExpvAlgorithm(40, 0.0, 1e-12)
+    obs = {"n": sp.jp @ sp.jm}
+    pimi.solve_dynamics(comp, rho0, (0.0, 10.0), saveat=0.1, algorithm=alg, observables=obs)
+    t0 = time.perf_counter()
+    res = pimi.solve_dynamics(comp, rho0, (0.0, 10.0), saveat=0.1, algorithm=alg, observables=obs)
+    secs = time.perf_counter() - t0
+    out = {"workload": "qubit ensemble N=120 (headline model) from |e><e|^N, t in [0, 10], saveat 0.1 (101 samples), "
+                       "expv Krylov 40, rtol 1e-12, observable sum_i n_i",
+           "ms": secs * 1e3, "samples": len(res.t), "applications": int(res.info["applications"]),
+           "accepted_steps": int(res.info["accepted"]), "rejected_steps": int(res.info["rejected"]),
+           "n_final": float(res.values["n"][-1].real)}
+    if cpu:
+        from oracle import lowering as L
+        from oracle import oracle as O
+        threads = min(os.cpu_count() or 1, 64)
+        lsp = L.spin_matrices(2)
+        ref, _ = O.compile_plan(N_QUBITS, 2, L.qubit_ensemble_terms(
+            hamiltonian=lsp["jz"] + lsp["jx"], emission=0.1, dephasing=0.2, pumping=0.02, collective_emission=0.1))
+        ts = np.array([0.0, 0.1, 0.2])
+        t0 = time.perf_counter()
+        ref.solve_dynamics(rho0.data, ts, [comp.collective_operator(sp.jp @ sp.jm).data], krylovdim=40, atol=0.0,
+                           rtol=1e-12, nthreads=threads)
+        t2 = time.perf_counter() - t0
+        est = t2 / 2 * 100
+        out["cpu_baseline"] = {"value_s": est, "cores": threads, "kind": "port",
+                               "sample": f"oracle OpenMP expv, first 2 of 100 save intervals in {t2:.2f} s on "
+                                         f"{threads} threads, scaled to 100 intervals"}
+        out["speedup_vs_cpu"] = est / secs
+    print(f"# dynamics N=120: {out['ms']:.0f} ms, {out['applications']} applications", file=sys.stderr, flush=True)
+    return out
+
+
 def serving(pimi):
     from pimi.serving import Server, summarize
     from pimi.workloads import served_requests
@@ -404,6 +445,7 @@ def main():
     batch120 = None if args.no_batch else batched_n120(pimi)
     serve = None if args.no_serve else serving(pimi)
     precision = None if args.no_batch else precision_form(pimi)
+    dynamics = dynamics_n120(pimi, comp, cpu=not args.no_cpu)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(side["plain_fp64"]["iterations"], args.cpu_seconds)
@@ -450,6 +492,7 @@ def main():
         "batched": batch,
         "batched_n120": batch120,
         "precision_form": precision,
+        "dynamics_n120": dynamics,
         "serving": serve,
         "cpu_baseline": cpu,
     }
