@@ -415,6 +415,8 @@ struct pimi_workspace {
     pimi::DevBuf<pimi::cx<double>> V, w, r, x, b, trial, cur, tmp, obs;
     pimi::DevBuf<pimi::cx<float>> V32, w32, x32, y32;
     pimi::DevBuf<pimi::cx<double>> partial, hvec, hacc, coef;
+    pimi::DevBuf<pimi::cx<double>> hcols;  // expv Arnoldi: Hessenberg columns of one factorization, (m+1) x m
+    pimi::DevBuf<double> hnorms;           // expv Arnoldi: per step |A v_j| and the remainder norm
     pimi::DevBuf<double> npartial, nscal, d_vval;
     pimi::DevBuf<pimi::cx<double>> zbuf, pt1, pt2;  // preconditioner work vectors
     // flexible GMRES: outer search directions Z and the inner solver's vectors
@@ -1471,18 +1473,34 @@ static void arnoldi(pimi_workspace *W, Factorization &F) {
     int k = 0;
     bool exact_breakdown = false, near_breakdown = false;
     const double breakfactor = std::sqrt(2.220446049250313e-16);
-    for (int j = 0; j < m; ++j) {
+    // All m steps run on the device without a host round trip: column j of
+    // the Hessenberg matrix and the step's norms land in device buffers, and
+    // the host reads them once and finds the first (exact or dimension)
+    // breakdown afterwards.  Steps after a breakdown divide by a zero norm
+    // and compute values that are never read (columns beyond k are unused).
+    const int steps = (int)std::min<int64_t>(m, n);
+    if (W->hcols.count < (size_t)(m + 1) * m) W->hcols.alloc((size_t)(m + 1) * m);
+    if (W->hnorms.count < (size_t)2 * m) W->hnorms.alloc((size_t)2 * m);
+    HIPCHK(hipMemsetAsync(W->hcols.p, 0, sizeof(cx<double>) * (size_t)(m + 1) * m, W->stream));
+    for (int j = 0; j < steps; ++j) {
         launch_apply<double>(W, V + (int64_t)j * n, W->w.p, 1.0);
-        dev_norm<double>(W, W->w.p, n, W->nscal.p + 2);
-        HIPCHK(hipMemsetAsync(W->hacc.p, 0, sizeof(cx<double>) * (j + 1), W->stream));
-        dev_cgs2<double>(W, V, n, j + 1, W->w.p, n, W->hacc.p);
-        dev_scale_normalize<double>(W, V + (int64_t)(j + 1) * n, W->w.p, n, W->nscal.p + 3);
-        HIPCHK(hipMemcpyAsync(hcol.data(), W->hacc.p, sizeof(cx<double>) * (j + 1), hipMemcpyDeviceToHost,
-                              W->stream));
-        HIPCHK(hipMemcpyAsync(W->hpin, W->nscal.p + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, W->stream));
-        sync(W);
-        double image_norm = W->hpin[0], rem = W->hpin[1];
-        for (int i = 0; i <= j; ++i) H[(size_t)i + (size_t)j * (m + 1)] = cd(hcol[i].x, hcol[i].y);
+        dev_norm<double>(W, W->w.p, n, W->hnorms.p + 2 * j);
+        dev_cgs2<double>(W, V, n, j + 1, W->w.p, n, W->hcols.p + (size_t)j * (m + 1));
+        dev_scale_normalize<double>(W, V + (int64_t)(j + 1) * n, W->w.p, n, W->hnorms.p + 2 * j + 1);
+    }
+    std::vector<cx<double>> hall((size_t)(m + 1) * steps);
+    std::vector<double> nall((size_t)2 * steps);
+    HIPCHK(hipMemcpyAsync(hall.data(), W->hcols.p, sizeof(cx<double>) * hall.size(), hipMemcpyDeviceToHost,
+                          W->stream));
+    HIPCHK(hipMemcpyAsync(nall.data(), W->hnorms.p, sizeof(double) * nall.size(), hipMemcpyDeviceToHost,
+                          W->stream));
+    sync(W);
+    for (int j = 0; j < steps; ++j) {
+        const double image_norm = nall[(size_t)2 * j], rem = nall[(size_t)2 * j + 1];
+        for (int i = 0; i <= j; ++i) {
+            const cx<double> h = hall[(size_t)i + (size_t)j * (m + 1)];
+            H[(size_t)i + (size_t)j * (m + 1)] = cd(h.x, h.y);
+        }
         H[(size_t)(j + 1) + (size_t)j * (m + 1)] = rem;
         k = j + 1;
         if (rem == 0 || (int64_t)(j + 1) == n) { exact_breakdown = true; break; }
