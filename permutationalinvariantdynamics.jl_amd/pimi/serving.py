@@ -55,7 +55,7 @@ def dynamics_bytes(n, krylovdim):
 class Server:
     def __init__(self, device=0, precision=N.PIMI_C32, steady_algorithm: BatchAlgorithm | None = None,
                  dynamics_algorithm: ExpvAlgorithm | None = None, rtol=1e-8, memory_bytes=0,
-                 dynamics_workers=4, dynamics_memory_bytes=32 << 30, compile_workers=8, t_end=10.0, saveat=0.2):
+                 dynamics_workers=2, dynamics_memory_bytes=32 << 30, compile_workers=8, t_end=10.0, saveat=0.2):
         self.device = device
         self.rtol = rtol
         self.steady_algorithm = steady_algorithm or BatchAlgorithm(krylovdim=200, inner_rtol=1e-4)
