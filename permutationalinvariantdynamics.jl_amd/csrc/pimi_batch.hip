@@ -244,12 +244,21 @@ template <typename R>
 __global__ __launch_bounds__(256) void kb_gemm(const BGemm *__restrict__ G, int ng, int64_t total, int transA,
                                                int transB) {
     const int lane = threadIdx.x & 63;
-    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the wave's tile and job are wave-uniform: scalar loads of the
+    // descriptors (constant address space), operands through raw buffers
+    const int64_t w = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (w >= total) return;  // whole wave leaves together; no block barriers below
-    const BGemm J = G[seg_of(G, ng, w)];
-    const GemmTile t = J.tiles[w - J.first];
-    gemm_tile_mfma<R>((const cx<R> *)J.A, (const cx<R> *)J.B, (cx<R> *)J.C, J.dims[t.sector], J.offs[t.sector], t,
-                      transA, transB, (const cx<R> *)J.D, (R)J.dscale, lane);
+    const PIMI_CONST BGemm *Gc = cptr(G);
+    int lo = 0, hi = ng - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (Gc[mid].first <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    const BGemm J = Gc[lo];
+    const GemmTile t = cptr(J.tiles)[w - J.first];
+    gemm_tile_mfma_buf<R>((const cx<R> *)J.A, (const cx<R> *)J.B, (cx<R> *)J.C, cptr(J.dims)[t.sector],
+                          cptr(J.offs)[t.sector], t, transA, transB, (const cx<R> *)J.D, (R)J.dscale, lane);
 }
 
 // The BLAS-1 kernels below run over "virtual blocks": entry e of the

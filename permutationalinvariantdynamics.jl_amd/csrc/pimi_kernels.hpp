@@ -641,6 +641,58 @@ __device__ __forceinline__ cx<float> buf_load(__amdgpu_buffer_rsrc_t r, uint32_t
     return __builtin_bit_cast(cx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
 }
 
+// The same 16x16 tile with its operands through raw buffers over the sector
+// block: an out-of-range row or k reads zero by the hardware range check (no
+// per-lane branch, 32-bit offsets).
+template <typename R>
+__device__ __forceinline__ void gemm_tile_mfma_buf(const cx<R> *A, const cx<R> *B, cx<R> *__restrict__ Cm,
+                                                   int64_t m, int64_t base, GemmTile t, int transA, int transB,
+                                                   const cx<R> *__restrict__ D, R dscale, int lane) {
+    typedef MfmaOps<R> Ops;
+    const int row0 = t.ti * kMfmaTile, col0 = t.tj * kMfmaTile;
+    const int mi = (int)m;
+    const int li = lane & 15, lk = lane >> 4;
+    const int gi = row0 + li, gj = col0 + li;
+    constexpr uint32_t E = sizeof(cx<R>);
+    cx<R> *const tag = nullptr;
+    const __amdgpu_buffer_rsrc_t ra = block_rsrc<R>(A + base, m * m);
+    const __amdgpu_buffer_rsrc_t rb = block_rsrc<R>(B + base, m * m);
+    typename Ops::acc_t cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    constexpr int kUnroll = 4;
+    for (int k0 = 0; k0 < mi; k0 += 4 * kUnroll) {
+        cx<R> a[kUnroll], b[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const int gk = k0 + 4 * u + lk;
+            const uint32_t ia = (uint32_t)(transA ? gk + gi * mi : gi + gk * mi) * E;
+            const uint32_t ib = (uint32_t)(transB ? gj + gk * mi : gk + gj * mi) * E;
+            a[u] = buf_load(ra, (gi < mi && gk < mi) ? ia : kBufOut, tag);
+            b[u] = buf_load(rb, (gk < mi && gj < mi) ? ib : kBufOut, tag);
+            if (transA) a[u].y = -a[u].y;
+            if (transB) b[u].y = -b[u].y;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            cr = Ops::mac(a[u].x, b[u].x, cr);
+            cr = Ops::mac(-a[u].y, b[u].y, cr);
+            ci = Ops::mac(a[u].x, b[u].y, ci);
+            ci = Ops::mac(a[u].y, b[u].x, ci);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = row0 + Ops::row(lane, r);
+        if (i >= mi || gj >= mi) continue;
+        const int64_t o = base + i + (int64_t)gj * mi;
+        cx<R> v = {cr[r], ci[r]};
+        if (D) {
+            const cx<R> dd = D[o];
+            v = cmul(v, cx<R>{dd.x * dscale, dd.y * dscale});
+        }
+        Cm[o] = v;
+    }
+}
+
 // A split-K GEMM tile: the sector's size and offset travel in the descriptor,
 // so a workgroup's operand loads wait for one descriptor load only.
 // m = 0 marks padding.
