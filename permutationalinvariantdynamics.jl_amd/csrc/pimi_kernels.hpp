@@ -537,7 +537,8 @@ __global__ __launch_bounds__(256) void k_sector_gemm(const cx<R> *__restrict__ A
 // The same grouped per-sector complex GEMM on the matrix cores: one wave per
 // 16x16 output tile (tiles of 16 here), four waves per workgroup, K stepped by
 // 4 with v_mfma_*_16x16x4.  Complex products use four real MFMAs:
-//   Cr += Ar Br - Ai Bi,   Ci += Ar Bi + Ai Br.
+//   Cr += Ar Br - Ai Bi,   Ci += Ar Bi + Ai Br
+// (the split-K sector kernel below uses three, by Gauss's rearrangement).
 // Operand lanes: A(i = lane%16, k = lane/16), B(k = lane/16, j = lane%16).
 // Accumulator lanes: f32 -> row 4*(lane/16) + r, f64 -> row lane/16 + 4 r;
 // column lane%16 (checked on gfx950 with a known product).
@@ -743,7 +744,9 @@ __global__ __launch_bounds__(256) void k_sector_gemm_mfma(const cx<R> *__restric
     constexpr int kUnroll = 4, kStep = 4 * kUnroll;
     const int nsteps = (m + kStep - 1) / kStep;
     const int s0 = nsteps * w / 4, s1 = nsteps * (w + 1) / 4;
-    typename Ops::acc_t cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    // Three real products per complex one (Gauss): t1 = Re a Re b, t2 = Im a Im b,
+    // t3 = (Re a + Im a)(Re b + Im b); Re = t1 - t2, Im = t3 - t1 - t2.
+    typename Ops::acc_t t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
     // G k-steps (4 G MFMA k-slices): all operand loads first, then the MFMAs.
     auto group = [&](int k0, auto G) {
         constexpr int NU = decltype(G)::value * kUnroll;
@@ -760,15 +763,20 @@ __global__ __launch_bounds__(256) void k_sector_gemm_mfma(const cx<R> *__restric
         }
 #pragma unroll
         for (int u = 0; u < NU; ++u) {
-            cr = Ops::mac(a[u].x, b[u].x, cr);
-            cr = Ops::mac(-a[u].y, b[u].y, cr);
-            ci = Ops::mac(a[u].x, b[u].y, ci);
-            ci = Ops::mac(a[u].y, b[u].x, ci);
+            t1 = Ops::mac(a[u].x, b[u].x, t1);
+            t2 = Ops::mac(a[u].y, b[u].y, t2);
+            t3 = Ops::mac(a[u].x + a[u].y, b[u].x + b[u].y, t3);
         }
     };
     int st = s0;
     for (; st + 1 < s1; st += 2) group(st * kStep, std::integral_constant<int, 2>());
     if (st < s1) group(st * kStep, std::integral_constant<int, 1>());
+    typename Ops::acc_t cr, ci;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        cr[r] = t1[r] - t2[r];
+        ci[r] = t3[r] - t1[r] - t2[r];
+    }
     if (w > 0)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
