@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kRedBlock) void kb_gemv(const BVec *__restrict__ S,
 // and conj(V_col) . w for col = 0..ncol-1, p = V[:, ncol-1]; a = V, b = w
 template <typename R>
 __global__ __launch_bounds__(kRedBlock) void kb_dots2_partial(const BVec *__restrict__ S, int ns, int64_t total) {
-    __shared__ double sh[kRedBlock / 64];
+    __shared__ double sh[4 * (kRedBlock / 64)];
     for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
         const BVec J = S[seg_of(S, ns, vb)];
         const int64_t lb = vb - J.first;
@@ -357,17 +357,12 @@ __global__ __launch_bounds__(kRedBlock) void kb_dots2_partial(const BVec *__rest
             r1 += (double)a.x * (double)d.x + (double)a.y * (double)d.y;
             i1 += (double)a.x * (double)d.y - (double)a.y * (double)d.x;
         }
-        r0 = block_sum(r0, sh);
-        __syncthreads();
-        i0 = block_sum(i0, sh);
-        __syncthreads();
-        r1 = block_sum(r1, sh);
-        __syncthreads();
-        i1 = block_sum(i1, sh);
+        double sums[4] = {r0, i0, r1, i1};
+        block_sums(sums, sh);
         if (threadIdx.x == 0) {
             cx<double> *out = (cx<double> *)J.c + (c * J.nblk + chunk) * 2;
-            out[0] = cx<double>{r0, i0};
-            out[1] = cx<double>{r1, i1};
+            out[0] = cx<double>{sums[0], sums[1]};
+            out[1] = cx<double>{sums[2], sums[3]};
         }
         __syncthreads();
     }

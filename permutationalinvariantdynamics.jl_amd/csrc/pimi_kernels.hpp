@@ -453,6 +453,25 @@ __device__ __forceinline__ T block_sum(T v, T *sh) {
     return r;
 }
 
+// block_sum of N values with one barrier (the same summation order, so the
+// same results); `sh` must hold N * blockDim.x/64 entries.
+template <int N, typename T>
+__device__ __forceinline__ void block_sums(T (&v)[N], T *sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < N; ++q) sh[q * nw + wid] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        T r = 0;
+        for (int k = 0; k < nw; ++k) r += sh[q * nw + k];
+        v[q] = r;
+    }
+}
+
 // Grouped per-sector complex GEMM over PI-layout blocks:
 //   C_s = op(A_s) op(B_s) [.* D_s * dscale]   for every tile of every sector,
 // op = identity (0) or conjugate transpose (1).  A tile is a TS x TS patch of

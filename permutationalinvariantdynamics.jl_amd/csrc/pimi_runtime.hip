@@ -44,7 +44,7 @@ template <typename R>
 __global__ __launch_bounds__(kRedBlock) void k_dots_partial(const cx<R> *__restrict__ V, int64_t ldv,
                                                             const cx<R> *__restrict__ w, int64_t n,
                                                             cx<double> *__restrict__ partial) {
-    __shared__ double sh[kRedBlock / 64];
+    __shared__ double sh[2 * (kRedBlock / 64)];
     const int c = blockIdx.y;
     const cx<R> *v = V + (int64_t)c * ldv;
     double re = 0, im = 0;
@@ -53,10 +53,9 @@ __global__ __launch_bounds__(kRedBlock) void k_dots_partial(const cx<R> *__restr
         re += (double)a.x * (double)b.x + (double)a.y * (double)b.y;
         im += (double)a.x * (double)b.y - (double)a.y * (double)b.x;
     }
-    re = block_sum(re, sh);
-    __syncthreads();
-    im = block_sum(im, sh);
-    if (threadIdx.x == 0) partial[(int64_t)c * gridDim.x + blockIdx.x] = cx<double>{re, im};
+    double sums[2] = {re, im};
+    block_sums(sums, sh);
+    if (threadIdx.x == 0) partial[(int64_t)c * gridDim.x + blockIdx.x] = cx<double>{sums[0], sums[1]};
 }
 
 // h[c] = sum of the partials of column c; if hacc != null, hacc[c] += h[c].
@@ -410,6 +409,11 @@ struct pimi_workspace {
     hipStream_t stream = nullptr;
     std::atomic<int> busy{0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Arnoldi lookahead: two pinned slots of (m + 2) entries, each holding one
+    // step's Gram-Schmidt column and, after it, the new basis vector's norm;
+    // an event per slot marks its copies done
+    pimi::cx<double> *hlook = nullptr;
+    hipEvent_t look_ev[2] = {nullptr, nullptr};
     double last_apply_ms = 0.0;
     // double-precision Krylov storage
     pimi::DevBuf<pimi::cx<double>> V, w, r, x, b, trial, cur, tmp, obs;
@@ -813,6 +817,28 @@ static double read_scalar(pimi_workspace *W, const double *dptr) {
     return W->hpin[0];
 }
 
+// Arnoldi lookahead.  Step j's device work ends with the Gram-Schmidt
+// coefficients in hacc[0:j+1] and the norm of the new basis vector in
+// nscal[0]; both are copied to pinned slot j % 2 and the slot's event
+// recorded.  Step j+1 needs only V[:, j+1], which step j leaves on the
+// device, so the solvers enqueue step j+1 before waiting for slot j: the
+// host's Givens update of column j overlaps the device's step j+1.  Every
+// buffer the steps share is reused in stream order, after the slot's copies.
+static void look_post(pimi_workspace *W, int j) {
+    cx<double> *slot = W->hlook + (size_t)(j & 1) * ((size_t)W->m + 2);
+    HIPCHK(hipMemcpyAsync(slot, W->hacc.p, sizeof(cx<double>) * (j + 1), hipMemcpyDeviceToHost, W->stream));
+    HIPCHK(hipMemcpyAsync(slot + W->m + 1, W->nscal.p, sizeof(double), hipMemcpyDeviceToHost, W->stream));
+    HIPCHK(hipEventRecord(W->look_ev[j & 1], W->stream));
+}
+
+// waits for step j's slot; returns its coefficients and the subdiagonal norm
+static const cx<double> *look_wait(pimi_workspace *W, int j, double &hnext) {
+    HIPCHK(hipEventSynchronize(W->look_ev[j & 1]));
+    const cx<double> *slot = W->hlook + (size_t)(j & 1) * ((size_t)W->m + 2);
+    hnext = *reinterpret_cast<const double *>(slot + W->m + 1);
+    return slot;
+}
+
 
 void ensure_plan_preconditioner(pimi_plan *P) {
     if (!P->prec_ready) {
@@ -999,7 +1025,6 @@ static GmresResult gmres_device(pimi_workspace *W, double invscale, double atol,
     dev_norm<double>(W, W->r.p, n, W->nscal.p);
     double beta = read_scalar(W, W->nscal.p);
     if (beta <= threshold()) return {true, 0, 0, beta};
-    std::vector<cx<double>> hcol((size_t)m + 2);
     while (iterations < maxiter) {
         std::fill(H.begin(), H.end(), cd(0, 0));
         std::fill(g.begin(), g.end(), cd(0, 0));
@@ -1011,23 +1036,26 @@ static GmresResult gmres_device(pimi_workspace *W, double invscale, double atol,
         g[0] = beta;
         int kdone = 0;
         int jmax = (int)std::min<int64_t>(m, maxiter - iterations);
-        for (int j = 0; j < jmax; ++j) {
+        auto step = [&](int j) {
             const cx<double> *vin = V + (int64_t)j * n;
             if (precond) {
                 apply_precond(W, vin, W->zbuf.p, 1.0 / invscale);
                 vin = W->zbuf.p;
             }
             steady_apply(W, vin, W->w.p, invscale);
-            applications++;
             HIPCHK(hipMemsetAsync(W->hacc.p, 0, sizeof(cx<double>) * (j + 1), W->stream));
             dev_cgs2<double>(W, V, n, j + 1, W->w.p, n, W->hacc.p);
             dev_scale_normalize<double>(W, V + (int64_t)(j + 1) * n, W->w.p, n, W->nscal.p);
-            HIPCHK(hipMemcpyAsync(hcol.data(), W->hacc.p, sizeof(cx<double>) * (j + 1), hipMemcpyDeviceToHost,
-                                  W->stream));
-            HIPCHK(hipMemcpyAsync(W->hpin, W->nscal.p, sizeof(double), hipMemcpyDeviceToHost, W->stream));
-            sync(W);
-            for (int i = 0; i <= j; ++i) Hij(i, j) = cd(hcol[i].x, hcol[i].y);
-            Hij(j + 1, j) = W->hpin[0];
+            look_post(W, j);
+        };
+        if (jmax > 0) step(0);
+        for (int j = 0; j < jmax; ++j) {
+            if (j + 1 < jmax) step(j + 1);  // lookahead (wasted only when column j converges)
+            applications++;
+            double hnext = 0.0;
+            const cx<double> *hc = look_wait(W, j, hnext);
+            for (int i = 0; i <= j; ++i) Hij(i, j) = cd(hc[i].x, hc[i].y);
+            Hij(j + 1, j) = hnext;
             for (int i = 0; i < j; ++i) {
                 cd h1 = Hij(i, j), h2 = Hij(i + 1, j);
                 Hij(i, j) = cs[i] * h1 + sn[i] * h2;
@@ -1195,7 +1223,6 @@ static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol
         HIPCHK(hipMemcpyAsync(W->inner_io.p, r, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
         run_inner(W, kin, scale);
         HIPCHK(hipMemcpyAsync(z, W->inner_io.p + n, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
-        applications += kin;
     };
     std::vector<cd> H((size_t)(m + 1) * m), g(m + 1), sn(m), yv(m);
     std::vector<double> cs(m);
@@ -1227,7 +1254,6 @@ static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol
     dev_norm<double>(W, W->r.p, n, W->nscal.p);
     double beta = read_scalar(W, W->nscal.p);
     if (beta <= threshold()) return {true, 0, 0, beta};
-    std::vector<cx<double>> hcol((size_t)m + 2);
     while (iterations < maxiter) {
         std::fill(H.begin(), H.end(), cd(0, 0));
         std::fill(g.begin(), g.end(), cd(0, 0));
@@ -1238,19 +1264,22 @@ static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol
         g[0] = beta;
         int kdone = 0;
         int jmax = (int)std::min<int64_t>(m, maxiter - iterations);
-        for (int j = 0; j < jmax; ++j) {
+        auto step = [&](int j) {
             inner(V + (int64_t)j * n, Z + (int64_t)j * n);
             steady_apply(W, Z + (int64_t)j * n, W->w.p, invscale);
-            applications++;
             HIPCHK(hipMemsetAsync(W->hacc.p, 0, sizeof(cx<double>) * (j + 1), W->stream));
             dev_cgs2<double>(W, V, n, j + 1, W->w.p, n, W->hacc.p);
             dev_scale_normalize<double>(W, V + (int64_t)(j + 1) * n, W->w.p, n, W->nscal.p);
-            HIPCHK(hipMemcpyAsync(hcol.data(), W->hacc.p, sizeof(cx<double>) * (j + 1), hipMemcpyDeviceToHost,
-                                  W->stream));
-            HIPCHK(hipMemcpyAsync(W->hpin, W->nscal.p, sizeof(double), hipMemcpyDeviceToHost, W->stream));
-            sync(W);
-            for (int i = 0; i <= j; ++i) Hij(i, j) = cd(hcol[i].x, hcol[i].y);
-            Hij(j + 1, j) = W->hpin[0];
+            look_post(W, j);
+        };
+        if (jmax > 0) step(0);
+        for (int j = 0; j < jmax; ++j) {
+            if (j + 1 < jmax) step(j + 1);  // lookahead (wasted only when column j converges)
+            applications += 1 + kin;
+            double hnext = 0.0;
+            const cx<double> *hc = look_wait(W, j, hnext);
+            for (int i = 0; i <= j; ++i) Hij(i, j) = cd(hc[i].x, hc[i].y);
+            Hij(j + 1, j) = hnext;
             for (int i = 0; i < j; ++i) {
                 cd h1 = Hij(i, j), h2 = Hij(i + 1, j);
                 Hij(i, j) = cs[i] * h1 + sn[i] * h2;
@@ -2175,6 +2204,9 @@ pimi_status pimi_workspace_create(pimi_plan *P, int32_t precision, int32_t krylo
         HIPCHK(hipEventCreate(&W->ev0));
         HIPCHK(hipEventCreate(&W->ev1));
         HIPCHK(hipHostMalloc((void **)&W->hpin, 64 * sizeof(double), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&W->hlook, 2 * ((size_t)W->m + 2) * sizeof(cx<double>), hipHostMallocDefault));
+        HIPCHK(hipEventCreateWithFlags(&W->look_ev[0], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&W->look_ev[1], hipEventDisableTiming));
         ensure_workspace(W.get());
         *out = W.release();
     });
@@ -2186,6 +2218,9 @@ void pimi_workspace_destroy(pimi_workspace *W) {
     if (W->stream) (void)hipStreamSynchronize(W->stream);
     if (W->inner_graph) (void)hipGraphExecDestroy(W->inner_graph);
     if (W->hpin) (void)hipHostFree(W->hpin);
+    if (W->hlook) (void)hipHostFree(W->hlook);
+    for (hipEvent_t e : W->look_ev)
+        if (e) (void)hipEventDestroy(e);
     if (W->ev0) (void)hipEventDestroy(W->ev0);
     if (W->ev1) (void)hipEventDestroy(W->ev1);
     if (W->stream) (void)hipStreamDestroy(W->stream);
