@@ -119,6 +119,8 @@ struct BCopy {  // one block per entry
     int64_t count;
 };
 
+constexpr int kDotChunks = 16;  // reduction chunks per workgroup in kb_dots2_partial
+
 template <typename T>
 __device__ __forceinline__ int seg_of(const T *__restrict__ seg, int nseg, int64_t b) {
     int lo = 0, hi = nseg - 1;
@@ -242,11 +244,14 @@ __global__ __launch_bounds__(256) void kb_trace(const BTrace *__restrict__ T) {
 // grouped sector GEMM tiles of all jobs, one wave per 16x16 tile
 template <typename R>
 __global__ __launch_bounds__(256) void kb_gemm(const BGemm *__restrict__ G, int ng, int64_t total, int transA,
-                                               int transB) {
+                                               int transB, int64_t per_xcd) {
     const int lane = threadIdx.x & 63;
     // the wave's tile and job are wave-uniform: scalar loads of the
-    // descriptors (constant address space), operands through raw buffers
-    const int64_t w = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // descriptors (constant address space), operands through raw buffers.
+    // Workgroup b runs on XCD b % 8, which takes one contiguous slab of the
+    // tile space, so a sector's tiles share their operand panels in one L2.
+    const int64_t g = (int64_t)(blockIdx.x % kNumXcd) * per_xcd + blockIdx.x / kNumXcd;
+    const int64_t w = g * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (w >= total) return;  // whole wave leaves together; no block barriers below
     const PIMI_CONST BGemm *Gc = cptr(G);
     int lo = 0, hi = ng - 1;
@@ -342,29 +347,39 @@ __global__ __launch_bounds__(kRedBlock) void kb_gemv(const BVec *__restrict__ S,
 template <typename R>
 __global__ __launch_bounds__(kRedBlock) void kb_dots2_partial(const BVec *__restrict__ S, int ns, int64_t total) {
     __shared__ double sh[4 * (kRedBlock / 64)];
-    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    // workgroup b runs on XCD b % 8 (the grid is a multiple of 8); each XCD
+    // takes a contiguous slab of every grid-stride pass, so the columns that
+    // share a chunk of p and w (consecutive vb) share one L2
+    const int64_t slab = gridDim.x / kNumXcd;
+    const int64_t b0 = (int64_t)(blockIdx.x % kNumXcd) * slab + blockIdx.x / kNumXcd;
+    for (int64_t vb = b0; vb < total; vb += gridDim.x) {
         const BVec J = S[seg_of(S, ns, vb)];
         const int64_t lb = vb - J.first;
-        const int64_t chunk = lb / J.ncol, c = lb % J.ncol;
+        // kDotChunks consecutive chunks of one column per workgroup: one job
+        // lookup for several chunks' partials (each summed as on its own)
+        const int64_t grp = lb / J.ncol, c = lb % J.ncol;
         const cx<R> *v = (const cx<R> *)J.a + c * J.n;
         const cx<R> *p = (const cx<R> *)J.a + (int64_t)(J.ncol - 1) * J.n;
         const cx<R> *w = (const cx<R> *)J.b;
-        double r0 = 0, i0 = 0, r1 = 0, i1 = 0;
-        for (int64_t k = chunk * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock) {
-            const cx<R> a = v[k], b = p[k], d = w[k];
-            r0 += (double)a.x * (double)b.x + (double)a.y * (double)b.y;
-            i0 += (double)a.x * (double)b.y - (double)a.y * (double)b.x;
-            r1 += (double)a.x * (double)d.x + (double)a.y * (double)d.y;
-            i1 += (double)a.x * (double)d.y - (double)a.y * (double)d.x;
+        const int64_t ch1 = min((int64_t)J.nblk, (grp + 1) * kDotChunks);
+        for (int64_t chunk = grp * kDotChunks; chunk < ch1; ++chunk) {
+            double r0 = 0, i0 = 0, r1 = 0, i1 = 0;
+            for (int64_t k = chunk * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock) {
+                const cx<R> a = v[k], b = p[k], d = w[k];
+                r0 += (double)a.x * (double)b.x + (double)a.y * (double)b.y;
+                i0 += (double)a.x * (double)b.y - (double)a.y * (double)b.x;
+                r1 += (double)a.x * (double)d.x + (double)a.y * (double)d.y;
+                i1 += (double)a.x * (double)d.y - (double)a.y * (double)d.x;
+            }
+            double sums[4] = {r0, i0, r1, i1};
+            block_sums(sums, sh);
+            if (threadIdx.x == 0) {
+                cx<double> *out = (cx<double> *)J.c + (c * J.nblk + chunk) * 2;
+                out[0] = cx<double>{sums[0], sums[1]};
+                out[1] = cx<double>{sums[2], sums[3]};
+            }
+            __syncthreads();
         }
-        double sums[4] = {r0, i0, r1, i1};
-        block_sums(sums, sh);
-        if (threadIdx.x == 0) {
-            cx<double> *out = (cx<double> *)J.c + (c * J.nblk + chunk) * 2;
-            out[0] = cx<double>{sums[0], sums[1]};
-            out[1] = cx<double>{sums[2], sums[3]};
-        }
-        __syncthreads();
     }
 }
 
@@ -799,7 +814,9 @@ struct Tick {
         }
         const size_t off = put(v);
         const int ns = (int)v.size();
-        const unsigned grid = (unsigned)std::min<int64_t>(tot, kMaxGrid);
+        // a multiple of the XCD count (kMaxGrid is one), for kernels that map
+        // workgroups to XCD slabs; workgroups past tot exit at once
+        const unsigned grid = (unsigned)std::min<int64_t>((tot + kNumXcd - 1) / kNumXcd * kNumXcd, kMaxGrid);
         hipStream_t st = stream;
         push([=](const char *d) {
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(bdim), 0, st, (const BVec *)(d + off), ns, tot);
@@ -901,8 +918,9 @@ struct Tick {
             const int ta = shapes[stage].ta, tb = shapes[stage].tb;
             hipStream_t st = stream;
             push([=](const char *d) {
-                hipLaunchKernelGGL(kb_gemm<R>, dim3((unsigned)((tot + 3) / 4)), dim3(256), 0, st,
-                                   (const BGemm *)(d + off), ng, tot, ta, tb);
+                const int64_t nwg = (tot + 3) / 4, per_xcd = (nwg + kNumXcd - 1) / kNumXcd;
+                hipLaunchKernelGGL(kb_gemm<R>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(256), 0, st,
+                                   (const BGemm *)(d + off), ng, tot, ta, tb, per_xcd);
             });
         }
         cls = saved;
@@ -944,7 +962,8 @@ struct Tick {
             u.push_back(BVec{0, J->V, J->h, J->w, J->scal(), J->n, J->nblk, nc, 0.0, 0});
         }
         cls = OC_DOTS;
-        vec(d, kb_dots2_partial<R>, [](const BVec &e) { return (int64_t)e.nblk * e.ncol; });
+        vec(d, kb_dots2_partial<R>,
+            [](const BVec &e) { return (int64_t)((e.nblk + kDotChunks - 1) / kDotChunks) * e.ncol; });
         cls = OC_OTHER;
         vec(r, kb_reduce_cols2, [](const BVec &e) { return (int64_t)e.ncol; }, 64);
         {
