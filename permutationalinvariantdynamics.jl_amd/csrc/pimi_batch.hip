@@ -636,8 +636,8 @@ struct Arena {
 enum Phase { PH_PROBE = 0, PH_INIT, PH_ARNOLDI, PH_END, PH_FINAL };
 enum JobState { JS_QUEUED = 0, JS_ACTIVE, JS_DONE, JS_FAILED };
 
-// mailbox slots (complex doubles) at the head of each job's mail region
-constexpr int kMailScalars = 8;   // 16 doubles: see the S_* indices
+// mailbox slots (complex doubles) at the head of each job's mail region:
+// slots 0-7 hold 16 doubles (see the S_* indices)
 constexpr int kMailTrace = 8;     // t . x
 constexpr int kMailHacc = 9;      // Hessenberg column h[0..j]
 enum {
@@ -780,8 +780,8 @@ namespace pimi {
 enum OpClass { OC_GEMM = 0, OC_APPLY, OC_DOTS, OC_UPDATE, OC_CYCLE_END, OC_FP64, OC_OTHER, OC_COUNT };
 
 struct Tick {
-    pimi_batch *E;
-    hipStream_t stream;
+    pimi_batch *E = nullptr;
+    hipStream_t stream = nullptr;
     std::vector<char> buf;
     std::vector<std::function<void(const char *)>> ops;
     std::vector<int> op_class;
@@ -1084,7 +1084,9 @@ static void launch_tick(pimi_batch *E, BLane &L) {
             case PH_END: end.push_back(J); break;
             default: fin.push_back(J); break;
         }
-    Tick T{E, L.stream, {}, {}};
+    Tick T;
+    T.E = E;
+    T.stream = L.stream;
     auto W64 = [](BJob *J) -> void * { return J->w64; };
     auto cX = [](BJob *J) -> const void * { return J->x; };
     auto cW64 = [](BJob *J) -> const void * { return J->w64; };

@@ -326,7 +326,6 @@ template <typename R, int TB, int SPLIT = 1>
 __device__ __forceinline__ void dia_tile(const cx<R> *xc_, cx<R> *yc_, R out_scale, ApplyTile tile,
                                          const int64_t *offs_, const int32_t *dims_, const int32_t *gbase_,
                                          const DiaGroup *groups_, const DiaEntry *ents_, const cx<R> *coef_) {
-    const PIMI_GLOBAL cx<R> *xc = gptr(xc_);
     PIMI_GLOBAL cx<R> *yc = gptr_w(yc_);
     const PIMI_CONST int64_t *offs = cptr(offs_);
     const PIMI_CONST int32_t *dims = cptr(dims_);

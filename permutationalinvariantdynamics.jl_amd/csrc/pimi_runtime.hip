@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kRedBlock) void k_gs_update_r(cx<R> *__restrict__ w
         if (lane == 0) hs[c] = cx<double>{re, im};
     }
     __syncthreads();
-    if (blockIdx.x == 0 && hacc && threadIdx.x < ncol)
+    if (blockIdx.x == 0 && hacc && (int)threadIdx.x < ncol)
         hacc[threadIdx.x] = cx<double>{hacc[threadIdx.x].x + hs[threadIdx.x].x, hacc[threadIdx.x].y + hs[threadIdx.x].y};
     double s = 0;
     for (int64_t k = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kRedBlock) {
