@@ -355,14 +355,16 @@ __global__ __launch_bounds__(kRedBlock) void kb_dots2_partial(const BVec *__rest
     for (int64_t vb = b0; vb < total; vb += gridDim.x) {
         const BVec J = S[seg_of(S, ns, vb)];
         const int64_t lb = vb - J.first;
-        // kDotChunks consecutive chunks of one column per workgroup: one job
-        // lookup for several chunks' partials (each summed as on its own)
+        // J.u0 consecutive chunks of one column per workgroup (one job lookup
+        // for several chunks' partials, each summed as on its own): many when
+        // many jobs fill the chip, one for a lone job
+        const int64_t cpw = J.u0 ? (int64_t)J.u0 : 1;
         const int64_t grp = lb / J.ncol, c = lb % J.ncol;
         const cx<R> *v = (const cx<R> *)J.a + c * J.n;
         const cx<R> *p = (const cx<R> *)J.a + (int64_t)(J.ncol - 1) * J.n;
         const cx<R> *w = (const cx<R> *)J.b;
-        const int64_t ch1 = min((int64_t)J.nblk, (grp + 1) * kDotChunks);
-        for (int64_t chunk = grp * kDotChunks; chunk < ch1; ++chunk) {
+        const int64_t ch1 = min((int64_t)J.nblk, (grp + 1) * cpw);
+        for (int64_t chunk = grp * cpw; chunk < ch1; ++chunk) {
             double r0 = 0, i0 = 0, r1 = 0, i1 = 0;
             for (int64_t k = chunk * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock) {
                 const cx<R> a = v[k], b = p[k], d = w[k];
@@ -461,32 +463,42 @@ __global__ __launch_bounds__(kRedBlock) void kb_dcgs_update(const BVec *__restri
         const double *slot = (const double *)J.d;
         const double inv = slot[kSlotInvRho], er = slot[kSlotE], ei = slot[kSlotE + 1];
         cx<R> *pcol = V + (int64_t)j * J.n, *ucol = V + (int64_t)(j + 1) * J.n;
-        for (int64_t k0 = lb * kRedBlock; k0 < J.n; k0 += (int64_t)J.nblk * kRedBlock) {
-            const int64_t k = k0 + threadIdx.x;
-            double ar = 0, ai = 0, br = 0, bi = 0;
-            for (int c0 = 0; c0 < j; c0 += kColChunk) {
-                const int cn = min(kColChunk, j - c0);
-                __syncthreads();
-                for (int c = threadIdx.x; c < cn; c += kRedBlock) {
-                    ca[c] = coef[c0 + c];
-                    cb[c] = coef[j + c0 + c];
-                }
-                __syncthreads();
-                if (k < J.n)
-                    for (int c = 0; c < cn; ++c) {
-                        const cx<R> v = V[k + (int64_t)(c0 + c) * J.n];
-                        ar += (double)v.x * ca[c].x - (double)v.y * ca[c].y;
-                        ai += (double)v.x * ca[c].y + (double)v.y * ca[c].x;
-                        br += (double)v.x * cb[c].x - (double)v.y * cb[c].y;
-                        bi += (double)v.x * cb[c].y + (double)v.y * cb[c].x;
-                    }
+        // one coordinate per thread (ceil(n / kRedBlock) workgroups per job);
+        // eight basis columns' loads in flight, accumulated in column order
+        const int64_t k = lb * kRedBlock + threadIdx.x;
+        double ar = 0, ai = 0, br = 0, bi = 0;
+        auto acc = [&](const cx<R> &v, int c) {
+            ar += (double)v.x * ca[c].x - (double)v.y * ca[c].y;
+            ai += (double)v.x * ca[c].y + (double)v.y * ca[c].x;
+            br += (double)v.x * cb[c].x - (double)v.y * cb[c].y;
+            bi += (double)v.x * cb[c].y + (double)v.y * cb[c].x;
+        };
+        for (int c0 = 0; c0 < j; c0 += kColChunk) {
+            const int cn = min(kColChunk, j - c0);
+            __syncthreads();
+            for (int c = threadIdx.x; c < cn; c += kRedBlock) {
+                ca[c] = coef[c0 + c];
+                cb[c] = coef[j + c0 + c];
             }
+            __syncthreads();
             if (k < J.n) {
-                const double px = pcol[k].x, py = pcol[k].y, wx = w[k].x, wy = w[k].y;
-                pcol[k] = cx<R>{(R)((px - ar) * inv), (R)((py - ai) * inv)};
-                ucol[k] = cx<R>{(R)((wx - (er * px - ei * py) - br) * inv),
-                                (R)((wy - (er * py + ei * px) - bi) * inv)};
+                const cx<R> *vk = V + k + (int64_t)c0 * J.n;
+                int c = 0;
+                for (; c + 8 <= cn; c += 8) {
+                    cx<R> v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) v[u] = vk[(int64_t)(c + u) * J.n];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) acc(v[u], c + u);
+                }
+                for (; c < cn; ++c) acc(vk[(int64_t)c * J.n], c);
             }
+        }
+        if (k < J.n) {
+            const double px = pcol[k].x, py = pcol[k].y, wx = w[k].x, wy = w[k].y;
+            pcol[k] = cx<R>{(R)((px - ar) * inv), (R)((py - ai) * inv)};
+            ucol[k] = cx<R>{(R)((wx - (er * px - ei * py) - br) * inv),
+                            (R)((wy - (er * py + ei * px) - bi) * inv)};
         }
         __syncthreads();
     }
@@ -731,6 +743,41 @@ struct BLane {
 };
 constexpr int kLanes = 2;
 
+// ---------------------------------------------------------------------------
+// The delayed-reorthogonalization step on one vector (the single-solve
+// flexible GMRES): the engine's kernels with one descriptor each.
+// ---------------------------------------------------------------------------
+int dcgs_single_nblk(int64_t n) { return blocks_for(n); }
+
+size_t dcgs_single_desc_bytes(int m) { return sizeof(BVec) * 4 * (size_t)m; }
+
+void dcgs_single_descs(void *host, int m, void *V, const void *w, void *partial, void *mail, void *h, double *slot,
+                       int64_t n) {
+    BVec *d = static_cast<BVec *>(host);
+    const int nblk = blocks_for(n);
+    for (int j = 0; j < m; ++j) {
+        const int nc = j + 1;
+        d[4 * j + 0] = BVec{0, V, w, partial, nullptr, n, nblk, nc, 0.0, 1};
+        d[4 * j + 1] = BVec{0, partial, nullptr, mail, nullptr, n, nblk, nc, 0.0, 0};
+        d[4 * j + 2] = BVec{0, mail, nullptr, h, slot, n, nblk, nc, 0.0, 0};
+        d[4 * j + 3] = BVec{0, V, h, const_cast<void *>(w), slot, n, nblk, nc, 0.0, 0};
+    }
+}
+
+void dcgs_single_step(hipStream_t st, const void *desc_dev, int j, int64_t n) {
+    const BVec *d = static_cast<const BVec *>(desc_dev) + 4 * (size_t)j;
+    const int64_t nblk = blocks_for(n), nc = j + 1;
+    auto grid8 = [](int64_t tot) {
+        return (unsigned)std::min<int64_t>((tot + kNumXcd - 1) / kNumXcd * kNumXcd, kMaxGrid);
+    };
+    const int64_t tdots = nblk * nc;  // one chunk per workgroup (the descriptor's u0 = 1)
+    hipLaunchKernelGGL(kb_dots2_partial<double>, dim3(grid8(tdots)), dim3(kRedBlock), 0, st, d + 0, 1, tdots);
+    hipLaunchKernelGGL(kb_reduce_cols2, dim3(grid8(nc)), dim3(64), 0, st, d + 1, 1, nc);
+    hipLaunchKernelGGL(kb_dcgs_coef, dim3(1), dim3(64), 0, st, d + 2);
+    const int64_t tupd = (n + kRedBlock - 1) / kRedBlock;
+    hipLaunchKernelGGL(kb_dcgs_update<double>, dim3(grid8(tupd)), dim3(kRedBlock), 0, st, d + 3, 1, tupd);
+}
+
 }  // namespace pimi
 
 struct pimi_batch {
@@ -962,6 +1009,7 @@ struct Tick {
             u.push_back(BVec{0, J->V, J->h, J->w, J->scal(), J->n, J->nblk, nc, 0.0, 0});
         }
         cls = OC_DOTS;
+        for (auto &e : d) e.u0 = kDotChunks;
         vec(d, kb_dots2_partial<R>,
             [](const BVec &e) { return (int64_t)((e.nblk + kDotChunks - 1) / kDotChunks) * e.ncol; });
         cls = OC_OTHER;
@@ -975,7 +1023,7 @@ struct Tick {
             });
         }
         cls = OC_UPDATE;
-        vec(u, kb_dcgs_update<R>, nblk_of);
+        vec(u, kb_dcgs_update<R>, [](const BVec &e) { return (e.n + kRedBlock - 1) / kRedBlock; });
         cls = saved;
     }
 

@@ -452,4 +452,19 @@ void ensure_plan_preconditioner(pimi_plan *P);
 // Plan-level sector-weighting tables (built once, on first use).
 void ensure_plan_weighting(pimi_plan *P);
 
+// Delayed-reorthogonalization Arnoldi step on one complex FP64 vector (the
+// batch engine's kernels, one job).  With Q = V[:, 0:j] orthonormal, p =
+// V[:, j] and w given, one pass over [Q, p] forms mail[0:j+1] = [Q p]^H p and
+// mail[j+1:2j+2] = [Q p]^H w, then slot[10] = rho = sqrt(p^H p - |Q^H p|^2),
+// slot[11] = 1/rho, slot[12:14] = e = (p^H w - s^H t)/rho^2, and a second
+// pass writes V[:, j] = (p - Q s)/rho and V[:, j+1] = (w - e p - Q(t - e s))/rho.
+// Descriptors for j = 0 .. m-1 are built once on the host (dcgs_single_descs,
+// dcgs_single_desc_bytes) and uploaded; partial holds 2 (m + 1) nblk, mail
+// 2 (m + 1), h 2 m complex doubles, slot 16 doubles.
+int dcgs_single_nblk(int64_t n);
+size_t dcgs_single_desc_bytes(int m);
+void dcgs_single_descs(void *host, int m, void *V, const void *w, void *partial, void *mail, void *h, double *slot,
+                       int64_t n);
+void dcgs_single_step(hipStream_t st, const void *desc_dev, int j, int64_t n);
+
 }  // namespace pimi

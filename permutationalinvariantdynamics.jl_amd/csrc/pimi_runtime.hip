@@ -425,6 +425,14 @@ struct pimi_workspace {
     pimi::DevBuf<pimi::cx<double>> zbuf, pt1, pt2;  // preconditioner work vectors
     // flexible GMRES: outer search directions Z and the inner solver's vectors
     pimi::DevBuf<pimi::cx<double>> Zo, Hin;
+    // flexible GMRES outer loop, delayed reorthogonalization: reduction
+    // partials, coefficient mailbox, update coefficients, scalars, per-step
+    // descriptors, and two pinned read-back slots
+    pimi::DevBuf<pimi::cx<double>> dc_partial, dc_mail, dc_h;
+    pimi::DevBuf<double> dc_slot;
+    pimi::DevBuf<char> dc_desc;
+    pimi::cx<double> *dc_look = nullptr;
+    bool dc_ready = false;
     pimi::DevBuf<char> inner_mem;  // (kin+1) basis + 4 work vectors in the inner precision
     pimi::DevBuf<double> inner_norms;
     pimi::DevBuf<pimi::cx<double>> inner_io, inner_y;  // fixed input/output vectors and LSQ solution
@@ -1212,8 +1220,8 @@ static void run_inner(pimi_workspace *W, int kin, double scale) {
     HIPCHK(hipGraphLaunch(W->inner_graph, W->stream));
 }
 
-static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol, double rtol, int64_t maxiter,
-                                 int64_t &applications, bool backward, int kin) {
+static GmresResult fgmres_device_cgs2(pimi_workspace *W, double invscale, double atol, double rtol,
+                                      int64_t maxiter, int64_t &applications, bool backward, int kin) {
     const int64_t n = W->plan->n;
     const int m = W->m;
     const double scale = 1.0 / invscale;
@@ -1310,6 +1318,207 @@ static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol
             if (est <= threshold()) break;
         }
         form_x(kdone, H, g, W->x.p, true);
+        restarts++;
+        steady_apply(W, W->x.p, W->w.p, invscale);
+        applications++;
+        hipLaunchKernelGGL(k_sub<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, W->r.p, W->b.p, W->w.p,
+                           n);
+        dev_norm<double>(W, W->r.p, n, W->nscal.p);
+        dev_norm<double>(W, W->x.p, n, W->nscal.p + 1);
+        d2h(W, W->hpin, W->nscal.p, 2 * sizeof(double));
+        beta = W->hpin[0];
+        xnorm = W->hpin[1];
+        if (beta <= threshold()) return {true, iterations, restarts, beta};
+    }
+    return {false, iterations, restarts, beta};
+}
+
+// Pinned read-back slot of the delayed outer loop: the step's mailbox
+// (2 (m + 1) entries) followed by its 16 scalars.
+static size_t dc_slot_entries(int m) { return 2 * ((size_t)m + 1) + 8; }
+
+static void ensure_dcgs(pimi_workspace *W) {
+    if (W->dc_ready) return;
+    const int m = W->m;
+    const int64_t n = W->plan->n;
+    const int nblk = dcgs_single_nblk(n);
+    W->dc_partial.alloc((size_t)2 * (m + 1) * nblk);
+    W->dc_mail.alloc((size_t)2 * (m + 1));
+    W->dc_h.alloc((size_t)2 * m + 2);
+    W->dc_slot.alloc(16);
+    std::vector<char> host(dcgs_single_desc_bytes(m));
+    dcgs_single_descs(host.data(), m, W->V.p, W->w.p, W->dc_partial.p, W->dc_mail.p, W->dc_h.p, W->dc_slot.p, n);
+    W->dc_desc.alloc(host.size());
+    h2d(W, W->dc_desc.p, host.data(), host.size());
+    HIPCHK(hipHostMalloc((void **)&W->dc_look, 2 * dc_slot_entries(m) * sizeof(cx<double>), hipHostMallocDefault));
+    W->dc_ready = true;
+}
+
+// Flexible GMRES, outer loop with delayed reorthogonalization (after
+// Swirydowicz, Langou, Ananthan, Yang and Thomas, "Low synchronization
+// Gram-Schmidt and GMRES", 2020; own derivation).  Step j takes the
+// once-orthogonalized candidate p_j = V[:, j], forms z'_j = inner(p_j) and
+// w = A z'_j, and one pass over [Q, p_j] (Q = V[:, 0:j]) gives s = Q^H p_j,
+// t = Q^H w, rho_j and e_j; a second pass writes q_j = (p_j - Q s)/rho_j and
+// the next candidate p_{j+1} = (w - e p_j - Q(t - e s))/rho_j.  With
+// z_j = z'_j / rho_j the flexible Arnoldi relation holds with Hessenberg
+// column j = (t/rho_j + s_{j+1}[0:j]; e_j + s_{j+1}[j]; rho_{j+1}), final
+// once step j+1 has run: two passes over the basis per step instead of
+// classical Gram-Schmidt's four.  Column j is finalized one step late, so a
+// converged cycle computes one (with the lookahead two) extra steps.
+static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol, double rtol, int64_t maxiter,
+                                 int64_t &applications, bool backward, int kin) {
+    {
+        const char *e = std::getenv("PIMI_OUTER_GS");
+        if (e && std::strcmp(e, "cgs2") == 0)
+            return fgmres_device_cgs2(W, invscale, atol, rtol, maxiter, applications, backward, kin);
+    }
+    const int64_t n = W->plan->n;
+    const int m = W->m;
+    const double scale = 1.0 / invscale;
+    ensure_inner(W, kin);
+    ensure_dcgs(W);
+    cx<double> *V = W->V.p, *Z = W->Zo.p;
+    const size_t se = dc_slot_entries(m);
+    auto inner = [&](const cx<double> *r, cx<double> *z) {
+        HIPCHK(hipMemcpyAsync(W->inner_io.p, r, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
+        run_inner(W, kin, scale);
+        HIPCHK(hipMemcpyAsync(z, W->inner_io.p + n, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
+    };
+    std::vector<cd> H((size_t)(m + 1) * m), g(m + 1), sn(m);
+    std::vector<double> cs(m), rho_col(m + 1);
+    auto Hij = [&](int i, int j) -> cd & { return H[(size_t)i + (size_t)j * (m + 1)]; };
+    dev_norm<double>(W, W->b.p, n, W->nscal.p);
+    dev_norm<double>(W, W->x.p, n, W->nscal.p + 1);
+    d2h(W, W->hpin, W->nscal.p, 2 * sizeof(double));
+    const double bnorm = W->hpin[0];
+    double xnorm = W->hpin[1];
+    auto threshold = [&]() { return backward ? 0.5 * (atol + rtol * xnorm) : atol + rtol * bnorm; };
+    // dst = (add_to_x ? dst : x) + Z' (y ./ rho), y the least-squares solution
+    auto form_x = [&](int k, cx<double> *dst, bool add_to_x) {
+        std::vector<cd> yy(k);
+        for (int i = k - 1; i >= 0; --i) {
+            cd acc = g[i];
+            for (int jj = i + 1; jj < k; ++jj) acc -= Hij(i, jj) * yy[jj];
+            yy[i] = acc / Hij(i, i);
+        }
+        std::vector<cx<double>> yc(k);
+        for (int i = 0; i < k; ++i) {
+            const cd v = yy[i] / rho_col[i];
+            yc[i] = cx<double>{v.real(), v.imag()};
+        }
+        h2d(W, W->coef.p, yc.data(), sizeof(cx<double>) * k);
+        if (!add_to_x) HIPCHK(hipMemcpyAsync(dst, W->x.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
+        hipLaunchKernelGGL(k_gemv<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, dst, Z, n, k, W->coef.p,
+                           n, 1.0, 0);
+    };
+    // column c of H from its rows 0..c+1, rotated into the triangular factor
+    auto finish_column = [&](int c, const std::vector<cd> &col) {
+        for (int i = 0; i <= c + 1; ++i) Hij(i, c) = col[i];
+        for (int i = 0; i < c; ++i) {
+            cd h1 = Hij(i, c), h2 = Hij(i + 1, c);
+            Hij(i, c) = cs[i] * h1 + sn[i] * h2;
+            Hij(i + 1, c) = -std::conj(sn[i]) * h1 + cs[i] * h2;
+        }
+        double cc;
+        cd ss;
+        complex_givens(Hij(c, c), Hij(c + 1, c), cc, ss);
+        cs[c] = cc;
+        sn[c] = ss;
+        cd h1 = Hij(c, c), h2 = Hij(c + 1, c);
+        Hij(c, c) = cc * h1 + ss * h2;
+        Hij(c + 1, c) = 0;
+        cd g1 = g[c], g2 = g[c + 1];
+        g[c] = cc * g1 + ss * g2;
+        g[c + 1] = -std::conj(ss) * g1 + cc * g2;
+    };
+    int64_t iterations = 0, restarts = 0;
+    steady_apply(W, W->x.p, W->w.p, invscale);
+    applications++;
+    hipLaunchKernelGGL(k_sub<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, W->r.p, W->b.p, W->w.p, n);
+    dev_norm<double>(W, W->r.p, n, W->nscal.p);
+    double beta = read_scalar(W, W->nscal.p);
+    if (beta <= threshold()) return {true, 0, 0, beta};
+    while (iterations < maxiter) {
+        std::fill(H.begin(), H.end(), cd(0, 0));
+        std::fill(g.begin(), g.end(), cd(0, 0));
+        W->hpin[0] = beta;
+        h2d(W, W->nscal.p + 1, W->hpin, sizeof(double));
+        hipLaunchKernelGGL(k_scale_div<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, V, W->r.p, n,
+                           W->nscal.p + 1);
+        g[0] = beta;
+        const int jmax = (int)std::min<int64_t>(m, maxiter - iterations);
+        auto step = [&](int j) {
+            inner(V + (int64_t)j * n, Z + (int64_t)j * n);
+            steady_apply(W, Z + (int64_t)j * n, W->w.p, invscale);
+            dcgs_single_step(W->stream, W->dc_desc.p, j, n);
+            cx<double> *slot = W->dc_look + (size_t)(j & 1) * se;
+            HIPCHK(hipMemcpyAsync(slot, W->dc_mail.p, sizeof(cx<double>) * 2 * (j + 1), hipMemcpyDeviceToHost,
+                                  W->stream));
+            HIPCHK(hipMemcpyAsync(slot + 2 * ((size_t)m + 1), W->dc_slot.p, 16 * sizeof(double),
+                                  hipMemcpyDeviceToHost, W->stream));
+            HIPCHK(hipEventRecord(W->look_ev[j & 1], W->stream));
+        };
+        std::vector<cd> tprev, col;
+        cd eprev(0, 0);
+        int kdone = 0;
+        bool stop = false;
+        // column c = j - 1 from the previous step's t, e, rho and this step's s, rho
+        auto close_column = [&](int j, const std::vector<cd> &sj, double rho_j) {
+            col.assign((size_t)j + 1, cd(0, 0));
+            for (int i = 0; i < j - 1; ++i) col[i] = tprev[i] / rho_col[j - 1] + sj[i];
+            col[j - 1] = eprev + sj[j - 1];
+            col[j] = rho_j;
+            finish_column(j - 1, col);
+            iterations++;
+            applications += 1 + kin;
+            kdone = j;
+            const double est = std::abs(g[j]);
+            if (backward && (kdone % 5 == 0 || est <= threshold()) && Hij(j - 1, j - 1) != cd(0, 0)) {
+                form_x(kdone, W->trial.p, false);
+                dev_norm<double>(W, W->trial.p, n, W->nscal.p + 5);
+                xnorm = read_scalar(W, W->nscal.p + 5);
+            }
+            if (W->trace_progress && (iterations % 10 == 0))
+                std::fprintf(stderr, "[pimi] fgmres iteration %lld  residual %.3e  |b| %.3e  |x| %.3e\n",
+                             (long long)iterations, est, bnorm, xnorm);
+            return est <= threshold();
+        };
+        if (jmax > 0) step(0);
+        for (int j = 0; j < jmax && !stop; ++j) {
+            if (j + 1 < jmax) step(j + 1);  // lookahead
+            HIPCHK(hipEventSynchronize(W->look_ev[j & 1]));
+            const cx<double> *sl = W->dc_look + (size_t)(j & 1) * se;
+            const double *sc = reinterpret_cast<const double *>(sl + 2 * ((size_t)m + 1));
+            std::vector<cd> sj(j), tj(j);
+            for (int i = 0; i < j; ++i) {
+                sj[i] = cd(sl[i].x, sl[i].y);
+                tj[i] = cd(sl[j + 1 + i].x, sl[j + 1 + i].y);
+            }
+            const double rho_j = sc[10];
+            const cd e_j(sc[12], sc[13]);
+            if (j == 0) g[0] = beta * rho_j;  // r = beta p_0 = beta rho_0 q_0
+            if (j >= 1) stop = close_column(j, sj, rho_j);
+            tprev = tj;
+            eprev = e_j;
+            rho_col[j] = rho_j;
+        }
+        if (!stop && jmax > 0) {
+            // the cycle ran its jmax steps: column jmax-1 needs s and rho of
+            // the last candidate p = V[:, jmax] against V[:, 0:jmax]
+            dev_dots<double>(W, V, n, jmax + 1, V + (int64_t)jmax * n, n, W->hvec.p, nullptr);
+            std::vector<cx<double>> hv((size_t)jmax + 1);
+            d2h(W, hv.data(), W->hvec.p, sizeof(cx<double>) * (jmax + 1));
+            std::vector<cd> sj(jmax);
+            double ss = 0;
+            for (int i = 0; i < jmax; ++i) {
+                sj[i] = cd(hv[i].x, hv[i].y);
+                ss += std::norm(sj[i]);
+            }
+            const double r2 = hv[jmax].x - ss;
+            close_column(jmax, sj, r2 > 0 ? std::sqrt(r2) : 0.0);
+        }
+        form_x(kdone, W->x.p, true);
         restarts++;
         steady_apply(W, W->x.p, W->w.p, invscale);
         applications++;
@@ -2219,6 +2428,7 @@ void pimi_workspace_destroy(pimi_workspace *W) {
     if (W->inner_graph) (void)hipGraphExecDestroy(W->inner_graph);
     if (W->hpin) (void)hipHostFree(W->hpin);
     if (W->hlook) (void)hipHostFree(W->hlook);
+    if (W->dc_look) (void)hipHostFree(W->dc_look);
     for (hipEvent_t e : W->look_ev)
         if (e) (void)hipEventDestroy(e);
     if (W->ev0) (void)hipEventDestroy(W->ev0);

@@ -267,6 +267,26 @@ def test_inner_outer_steady_state(precision):
     assert err <= 1e-8, err
 
 
+@pytest.mark.parametrize("outer", ["delayed", "cgs2"])
+def test_inner_outer_restarted_steady_state(monkeypatch, outer):
+    """Flexible GMRES restarted every 12 outer steps (each cycle closes its last
+    Hessenberg column with an extra reduction in the delayed form) reaches the
+    oracle's steady state, with the delayed and the classical twice-applied
+    Gram-Schmidt outer loops."""
+    if outer == "cgs2":
+        monkeypatch.setenv("PIMI_OUTER_GS", "cgs2")
+    comp = pimi.compile(_c2_model(24))
+    ref, _ = O.compile_plan(24, 2, _c2_terms())
+    alg = pimi.GMRESAlgorithm(12, 600, "sector_inner", inner_steps=4)
+    res = pimi.stationary_state(comp, algorithm=alg, atol=0.0, rtol=1e-12, return_info=True, criterion="backward")
+    x_ref, info_ref = ref.steady_state(krylovdim=400, maxiter=400, atol=0.0, rtol=1e-13, preconditioner="sector",
+                                       criterion="backward")
+    assert res.info["converged"] and info_ref["converged"]
+    assert res.info["restarts"] >= 2
+    err = np.linalg.norm(res.state.data - x_ref) / np.linalg.norm(x_ref)
+    assert err <= 1e-8, err
+
+
 def test_inner_graph_replays_match_eager(monkeypatch):
     """The captured inner-solve graph, replayed for every outer step (with the
     host's own work and synchronisation between replays), gives bitwise the
