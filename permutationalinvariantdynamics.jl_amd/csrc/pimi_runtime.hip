@@ -318,10 +318,14 @@ __global__ void k_zero_c64(cx<double> *__restrict__ p, int64_t count) {
 // nrm[k].  One lane; k <= 32.  A zero subdiagonal ends the space early.
 __global__ void k_inner_lsq(const cx<double> *__restrict__ Hin, const double *__restrict__ nrm, int k,
                             cx<double> *__restrict__ y) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    // One thread solves the small problem; its arrays live in LDS (dynamically
+    // indexed private arrays would go to scratch memory, ~100x the latency).
     constexpr int KMAX = 32;
-    double Rr[(KMAX + 1) * KMAX / 2 + KMAX], Ri[(KMAX + 1) * KMAX / 2 + KMAX];  // packed upper triangle
-    double gr[KMAX + 1], gi[KMAX + 1];
+    __shared__ double Rr[(KMAX + 1) * KMAX / 2 + KMAX], Ri[(KMAX + 1) * KMAX / 2 + KMAX];  // packed upper triangle
+    __shared__ double gr[KMAX + 1], gi[KMAX + 1];
+    __shared__ double csv[KMAX], snr[KMAX], sni[KMAX];
+    __shared__ cx<double> ys[KMAX];
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     auto idx = [](int i, int j) { return j * (j + 1) / 2 + i; };  // i <= j
     int kk = k;
     for (int j = 0; j < k; ++j)
@@ -329,7 +333,6 @@ __global__ void k_inner_lsq(const cx<double> *__restrict__ Hin, const double *__
     for (int i = 0; i <= kk; ++i) { gr[i] = 0; gi[i] = 0; }
     gr[0] = nrm[k];
     // rotate column by column
-    double csv[KMAX], snr[KMAX], sni[KMAX];
     for (int j = 0; j < kk; ++j) {
         // load column j (rows 0..j) and subdiagonal
         for (int i = 0; i <= j; ++i) { Rr[idx(i, j)] = Hin[i + (int64_t)j * (k + 1)].x; Ri[idx(i, j)] = Hin[i + (int64_t)j * (k + 1)].y; }
@@ -372,15 +375,15 @@ __global__ void k_inner_lsq(const cx<double> *__restrict__ Hin, const double *__
         double accr = gr[i], acci = gi[i];
         for (int jj = i + 1; jj < kk; ++jj) {
             double hr = Rr[idx(i, jj)], hi = Ri[idx(i, jj)];
-            accr -= hr * y[jj].x - hi * y[jj].y;
-            acci -= hr * y[jj].y + hi * y[jj].x;
+            accr -= hr * ys[jj].x - hi * ys[jj].y;
+            acci -= hr * ys[jj].y + hi * ys[jj].x;
         }
         double dr = Rr[idx(i, i)], di = Ri[idx(i, i)];
         double den = dr * dr + di * di;
-        y[i] = den == 0 ? cx<double>{0, 0}
-                        : cx<double>{(accr * dr + acci * di) / den, (acci * dr - accr * di) / den};
+        ys[i] = den == 0 ? cx<double>{0, 0}
+                         : cx<double>{(accr * dr + acci * di) / den, (acci * dr - accr * di) / den};
     }
-    for (int i = kk; i < k; ++i) y[i] = cx<double>{0, 0};
+    for (int i = 0; i < k; ++i) y[i] = i < kk ? ys[i] : cx<double>{0, 0};
 }
 
 // scatter a sparse vector into a zeroed dense one
