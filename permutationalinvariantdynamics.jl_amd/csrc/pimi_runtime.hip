@@ -1558,7 +1558,9 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, double *rh
     auto t0 = std::chrono::steady_clock::now();
     pimi_plan *P = W->plan;
     const int64_t n = P->n;
-    if (o->krylovdim != W->m) throw Error(PIMI_EINVAL, "opts.krylovdim must equal the workspace Krylov dimension");
+    // the workspace holds min(krylovdim, n) columns, as KrylovWorkspace (src/krylov.jl:74-82)
+    if (o->krylovdim < 1 || std::min<int64_t>(o->krylovdim, n) != W->m)
+        throw Error(PIMI_EINVAL, "opts.krylovdim must match the workspace Krylov dimension");
     if (!(o->atol >= 0) || !(o->rtol >= 0)) throw Error(PIMI_EINVAL, "atol and rtol must be nonnegative");
     // v = t / (t . t) (src/source_protocol.jl:107-113)
     double tt = 0;
@@ -1861,7 +1863,9 @@ static void expv_series(pimi_workspace *W, const double *rho0, const double *ts,
                         double *rho_final, pimi_solve_info *info) {
     auto t0 = std::chrono::steady_clock::now();
     const int64_t n = W->plan->n;
-    if (o->krylovdim != W->m) throw Error(PIMI_EINVAL, "opts.krylovdim must equal the workspace Krylov dimension");
+    // the workspace holds min(krylovdim, n) columns, as KrylovWorkspace (src/krylov.jl:74-82)
+    if (o->krylovdim < 1 || std::min<int64_t>(o->krylovdim, n) != W->m)
+        throw Error(PIMI_EINVAL, "opts.krylovdim must match the workspace Krylov dimension");
     if (!(o->safety > 0 && o->safety < 1)) throw Error(PIMI_EINVAL, "safety must lie strictly between zero and one");
     if (o->max_steps <= 0) throw Error(PIMI_EINVAL, "max_steps must be positive");
     for (int64_t k = 1; k < nts; ++k)

@@ -287,6 +287,21 @@ def test_inner_outer_restarted_steady_state(monkeypatch, outer):
     assert err <= 1e-8, err
 
 
+@pytest.mark.parametrize("N_", [1, 2, 3, 6])
+def test_inner_outer_small_spaces(N_):
+    """Flexible GMRES on PI spaces of dimension 4 to 84, where the outer Krylov
+    space can exhaust the whole space (krylovdim above n is clipped to n)."""
+    comp = pimi.compile(_c2_model(N_))
+    ref, _ = O.compile_plan(N_, 2, _c2_terms())
+    alg = pimi.GMRESAlgorithm(200, 400, "sector_inner", inner_steps=4)
+    res = pimi.stationary_state(comp, algorithm=alg, atol=0.0, rtol=1e-12, return_info=True, criterion="backward")
+    x_ref, info_ref = ref.steady_state(krylovdim=400, maxiter=400, atol=0.0, rtol=1e-13, preconditioner="sector",
+                                       criterion="backward")
+    assert res.info["converged"] and info_ref["converged"]
+    err = np.linalg.norm(res.state.data - x_ref) / np.linalg.norm(x_ref)
+    assert err <= 1e-9, err
+
+
 def test_inner_graph_replays_match_eager(monkeypatch):
     """The captured inner-solve graph, replayed for every outer step (with the
     host's own work and synchronisation between replays), gives bitwise the
