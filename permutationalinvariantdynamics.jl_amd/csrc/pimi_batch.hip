@@ -1589,7 +1589,8 @@ pimi_status pimi_batch_job_bytes(const pimi_plan *plan, int32_t precision, int32
         if (!plan || !bytes) throw Error(PIMI_EINVAL, "null argument");
         if (precision != PIMI_C64 && precision != PIMI_C32) throw Error(PIMI_EINVAL, "unknown precision");
         if (krylovdim < 1 || krylovdim > kMaxKrylov) throw Error(PIMI_EINVAL, "krylovdim must lie in 1..4000");
-        *bytes = (int64_t)job_bytes(plan->n, (int64_t)plan->tidx.size(), krylovdim, precision);
+        const int m = (int)std::min<int64_t>(krylovdim, plan->n);  // min(krylovdim, n) basis columns
+        *bytes = (int64_t)job_bytes(plan->n, (int64_t)plan->tidx.size(), m, precision);
     });
 }
 
@@ -1617,7 +1618,7 @@ pimi_status pimi_batch_submit(pimi_batch *E, pimi_plan *plan, const pimi_gmres_o
         J->o = *o;
         J->n = plan->n;
         J->nt = (int64_t)plan->tidx.size();
-        J->kdim = o->krylovdim;
+        J->kdim = (int)std::min<int64_t>(o->krylovdim, plan->n);  // as KrylovWorkspace (src/krylov.jl:74-82)
         J->nblk = blocks_for(J->n);
         J->bytes = job_bytes(J->n, J->nt, J->kdim, E->prec);
         {

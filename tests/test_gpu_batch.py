@@ -67,6 +67,24 @@ def test_batch_matches_oracle(precision, weighting):
 
 
 @pytest.mark.parametrize("precision", [N.PIMI_C64, N.PIMI_C32])
+def test_batch_small_spaces(precision):
+    """PI dimensions 4, 10 and 20 with krylovdim 150: each job's basis holds
+    min(krylovdim, n) columns, as the reference's KrylovWorkspace."""
+    print(f"batch small spaces precision={precision}", flush=True)
+    sizes = (1, 2, 3)
+    comps = [pimi.compile(_c2_model(n_)) for n_ in sizes]
+    alg = pimi.BatchAlgorithm(krylovdim=150, inner_rtol=1e-4)
+    res = pimi.batched_stationary_states(comps, alg, rtol=1e-11, precision=precision)
+    for n_, r in zip(sizes, res):
+        ref, _ = O.compile_plan(n_, 2, _c2_terms())
+        x_ref, info_ref = ref.steady_state(krylovdim=400, maxiter=400, atol=0.0, rtol=1e-12,
+                                           preconditioner="sector", criterion="backward")
+        assert info_ref["converged"] and r.info["converged"]
+        err = np.linalg.norm(r.state.data - x_ref) / np.linalg.norm(x_ref)
+        assert err <= 1e-8, (n_, err)
+
+
+@pytest.mark.parametrize("precision", [N.PIMI_C64, N.PIMI_C32])
 def test_batch_alone_equals_mixed(precision):
     """A job's result does not depend on the other jobs in its launches."""
     print(f"batch alone vs mixed precision={precision}", flush=True)
