@@ -184,6 +184,23 @@ def test_expv_series_parity_c1():
     assert np.linalg.norm(res.final_state.data - rho_ref) <= 1e-9 * np.linalg.norm(rho_ref)
 
 
+@pytest.mark.parametrize("N_", [1, 2])
+def test_expv_small_spaces(N_):
+    """expv with krylovdim 30 on PI dimensions 4 and 10 (the Krylov space
+    exhausts the whole space) against the dense exponential of the dense
+    Liouvillian."""
+    import scipy.linalg as sla
+    comp = pimi.compile(_c2_model(N_))
+    b = comp.basis
+    n = len(b)
+    Ld = np.column_stack([comp.apply(np.eye(n, dtype=np.complex128)[:, k]) for k in range(n)])
+    rho0 = pimi.computational_product_state(b, 2)
+    alg = pimi.ExpvAlgorithm(krylovdim=30, atol=0.0, rtol=1e-12)
+    res = pimi.solve_dynamics(comp, rho0, (0.0, 2.0), saveat=0.5, algorithm=alg, observables={"n": SP2.jp @ SP2.jm})
+    exact = sla.expm(2.0 * Ld) @ rho0.data
+    assert np.linalg.norm(res.final_state.data - exact) <= 1e-10 * np.linalg.norm(exact)
+
+
 def test_collective_operator_matches_oracle():
     comp = pimi.compile(_c2_model(12))
     b = L.Basis(12, 2)
