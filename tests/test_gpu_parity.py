@@ -304,13 +304,15 @@ def test_inner_outer_restarted_steady_state(monkeypatch, outer):
     assert err <= 1e-8, err
 
 
+@pytest.mark.parametrize("preconditioner", [None, "sector", "sector_inner"])
 @pytest.mark.parametrize("N_", [1, 2, 3, 6])
-def test_inner_outer_small_spaces(N_):
-    """Flexible GMRES on PI spaces of dimension 4 to 84, where the outer Krylov
-    space can exhaust the whole space (krylovdim above n is clipped to n)."""
+def test_inner_outer_small_spaces(N_, preconditioner):
+    """GMRES (plain, sector-preconditioned, flexible) on PI spaces of dimension
+    4 to 84, where the Krylov space can exhaust the whole space (krylovdim above
+    n is clipped to n)."""
     comp = pimi.compile(_c2_model(N_))
     ref, _ = O.compile_plan(N_, 2, _c2_terms())
-    alg = pimi.GMRESAlgorithm(200, 400, "sector_inner", inner_steps=4)
+    alg = pimi.GMRESAlgorithm(200, 400, preconditioner, inner_steps=4)
     res = pimi.stationary_state(comp, algorithm=alg, atol=0.0, rtol=1e-12, return_info=True, criterion="backward")
     x_ref, info_ref = ref.steady_state(krylovdim=400, maxiter=400, atol=0.0, rtol=1e-13, preconditioner="sector",
                                        criterion="backward")
