@@ -221,9 +221,9 @@ pimi_status pimi_plan_export(const pimi_plan *plan, int64_t *cp_base, int64_t *c
 pimi_status pimi_sector_preconditioner_factors(const pimi_plan *plan, double *U, double *inv_den);
 /* Build a device plan's stationary-solver tables ahead of use (the sector
  * preconditioner factors and the sector-weighting tables), so that a later
- * solve or batch submission does no host setup.  Not thread-safe for one
- * plan; different plans may be prepared concurrently.  No counterpart in the
- * reference. */
+ * solve or batch submission does no host setup.  Tables built on first use
+ * are built once per plan under a per-plan lock, so workspaces of one plan
+ * may run on several threads.  No counterpart in the reference. */
 pimi_status pimi_plan_prepare(pimi_plan *plan);
 void pimi_plan_destroy(pimi_plan *plan);
 
@@ -242,9 +242,11 @@ pimi_status pimi_apply(pimi_workspace *ws, const void *x, void *y, int64_t ncols
  * Same arguments as pimi_apply; the adjoint stencil is built on first use. */
 pimi_status pimi_apply_adjoint(pimi_workspace *ws, const void *x, void *y, int64_t ncols, int32_t on_device);
 
-/* Stationary state: solve (L/s + v t^H) rho = v from rho0 = v.  rho_out is a
- * host buffer of n complex doubles. */
-pimi_status pimi_steady_gmres(pimi_workspace *ws, const pimi_gmres_opts *opts, double *rho_out,
+/* Stationary state: solve (L/s + v t^H) rho = v from rho0 = x0, or from
+ * rho0 = v when x0 is NULL (krylov_steady_state's initial_state,
+ * src/krylov.jl:797-826).  x0 (may be NULL) and rho_out are host buffers of
+ * n complex doubles. */
+pimi_status pimi_steady_gmres(pimi_workspace *ws, const pimi_gmres_opts *opts, const double *x0, double *rho_out,
                               pimi_solve_info *info);
 
 /* Dynamics: rho(t_k) by expv over each interval, recording
@@ -395,6 +397,13 @@ pimi_status pimi_batch_next_finished(pimi_batch *batch, int64_t *job_id);
  * forget the job; PIMI_ENOTCONVERGED (state and info filled) for a job that
  * did not converge. */
 pimi_status pimi_batch_fetch(pimi_batch *batch, int64_t job_id, double *rho_out, pimi_solve_info *info);
+/* Multi-model apply: y_k = L_k x_k for every plan k in one launch (banded
+ * and ELL kernels over all plans' tiles), in the engine precision.  x, y:
+ * host buffers holding the plans' vectors one after another (complex
+ * double, sum of the plans' n entries); they must not overlap.  The batch
+ * apply! of src/liouvillian.jl:3232-3251 runs many columns of one plan; this
+ * runs one column of many plans. */
+pimi_status pimi_batch_apply(pimi_batch *batch, pimi_plan *const *plans, int32_t nplans, const double *x, double *y);
 /* Throughput probe: one launch applying L to a synthetic vector of every plan
  * (engine precision), `reps` times, event-timed; algorithmic bytes per launch
  * = sum over plans of 2 n s + stencil tables.  Measurement only. */

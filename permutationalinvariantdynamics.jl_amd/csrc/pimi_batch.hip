@@ -1817,39 +1817,34 @@ pimi_status pimi_batch_autotune_apply(pimi_batch *E, pimi_plan *const *plans, in
     });
 }
 
-pimi_status pimi_batch_apply_bench(pimi_batch *E, pimi_plan *const *plans, int32_t nplans, int32_t reps,
-                                   double *ms_avg, int64_t *algorithmic_bytes) {
-    return guarded([&] {
-        if (!E || !plans || nplans < 1 || reps < 1 || !ms_avg || !algorithmic_bytes)
-            throw Error(PIMI_EINVAL, "invalid argument");
-        BusyGuard guard(E->busy);
-        HIPCHK(hipSetDevice(E->ctx->device));
+// One launch applying L to one vector of every plan (engine precision):
+// descriptors for the banded and the ELL kernels, uploaded once; `launch`
+// enqueues the apply on the engine stream.  xs[k], ys[k]: device vectors of
+// plan k.  Shared by the multi-model apply and its throughput probe.
+struct MultiApply {
+    DevBuf<BApply> dv;
+    DevBuf<BDia> dvd;
+    int nv = 0, nvd = 0;
+    int64_t tot = 0, totd = 0, per_xcd = 0, bytes = 0;
+
+    void prepare(pimi_batch *E, pimi_plan *const *plans, int32_t nplans, void *const *xs, void *const *ys) {
         const size_t es = E->prec == PIMI_C32 ? sizeof(cx<float>) : sizeof(cx<double>);
-        std::vector<DevBuf<char>> xs((size_t)nplans), ys((size_t)nplans);
         std::vector<BApply> v;
         std::vector<BDia> vd;
-        int64_t tot = 0, totd = 0, bytes = 0;
         for (int32_t k = 0; k < nplans; ++k) {
             pimi_plan *P = plans[k];
-            if (!P || !P->ctx || P->ctx->device != E->ctx->device)
-                throw Error(PIMI_EINVAL, "every plan must live on the engine's device");
-            xs[k].alloc(es * (size_t)P->n);
-            ys[k].alloc(es * (size_t)P->n);
-            HIPCHK(hipMemsetAsync(xs[k].p, 0x3f, es * (size_t)P->n, E->stream));
             bytes += 2 * P->n * (int64_t)es + P->main.table_bytes((int)es);
             if (P->main.dia_ok) {
-                vd.push_back(E->prec == PIMI_C32 ? dia_desc<float>(totd, P, xs[k].p, ys[k].p, false, 1.0, E->dia_w)
-                                                 : dia_desc<double>(totd, P, xs[k].p, ys[k].p, false, 1.0, E->dia_w));
+                vd.push_back(E->prec == PIMI_C32 ? dia_desc<float>(totd, P, xs[k], ys[k], false, 1.0, E->dia_w)
+                                                 : dia_desc<double>(totd, P, xs[k], ys[k], false, 1.0, E->dia_w));
                 totd += P->ndtiles[E->dia_w];
                 continue;
             }
             const void *val = E->prec == PIMI_C32 ? (const void *)P->main.val32.p : (const void *)P->main.val64.p;
-            v.push_back(BApply{tot, xs[k].p, ys[k].p, P->offs.p, P->dims.p, P->tile_sector.p, P->atiles4.p,
+            v.push_back(BApply{tot, xs[k], ys[k], P->offs.p, P->dims.p, P->tile_sector.p, P->atiles4.p,
                                P->main.entry_base.p, P->main.entries.p, P->main.col.p, val, P->n, 1.0});
             tot += P->natiles4;
         }
-        DevBuf<BApply> dv;
-        DevBuf<BDia> dvd;
         if (!v.empty()) {
             dv.alloc(v.size());
             HIPCHK(hipMemcpyAsync(dv.p, v.data(), sizeof(BApply) * v.size(), hipMemcpyHostToDevice, E->stream));
@@ -1858,27 +1853,108 @@ pimi_status pimi_batch_apply_bench(pimi_batch *E, pimi_plan *const *plans, int32
             dvd.alloc(vd.size());
             HIPCHK(hipMemcpyAsync(dvd.p, vd.data(), sizeof(BDia) * vd.size(), hipMemcpyHostToDevice, E->stream));
         }
-        const int64_t per_xcd = (tot + kNumXcd - 1) / kNumXcd;
-        const int nv = (int)v.size(), nvd = (int)vd.size();
-        auto launch = [&] {
-            if (nvd) {
-                if (E->prec == PIMI_C32) launch_kb_dia<float>(E->stream, dvd.p, nvd, totd, E->dia_w);
-                else launch_kb_dia<double>(E->stream, dvd.p, nvd, totd, E->dia_w);
-            }
-            if (!nv) return;
+        HIPCHK(hipStreamSynchronize(E->stream));  // the host descriptor arrays go out of scope
+        per_xcd = (tot + kNumXcd - 1) / kNumXcd;
+        nv = (int)v.size();
+        nvd = (int)vd.size();
+    }
+    void launch(pimi_batch *E) const {
+        if (nvd) {
+            if (E->prec == PIMI_C32) launch_kb_dia<float>(E->stream, dvd.p, nvd, totd, E->dia_w);
+            else launch_kb_dia<double>(E->stream, dvd.p, nvd, totd, E->dia_w);
+        }
+        if (nv) {
             if (E->prec == PIMI_C32)
                 hipLaunchKernelGGL(kb_apply<float>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyWave), 0,
                                    E->stream, dv.p, nv, tot, per_xcd);
             else
                 hipLaunchKernelGGL(kb_apply<double>, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kApplyWave), 0,
                                    E->stream, dv.p, nv, tot, per_xcd);
-        };
-        launch();  // warm-up
+        }
+        HIPCHK(hipGetLastError());
+    }
+};
+
+static void check_multi_plans(pimi_batch *E, pimi_plan *const *plans, int32_t nplans) {
+    for (int32_t k = 0; k < nplans; ++k) {
+        pimi_plan *P = plans[k];
+        if (!P || !P->ctx || P->ctx->device != E->ctx->device)
+            throw Error(PIMI_EINVAL, "every plan must live on the engine's device");
+    }
+}
+
+pimi_status pimi_batch_apply(pimi_batch *E, pimi_plan *const *plans, int32_t nplans, const double *x, double *y) {
+    return guarded([&] {
+        if (!E || !plans || nplans < 1 || !x || !y) throw Error(PIMI_EINVAL, "invalid argument");
+        BusyGuard guard(E->busy);
+        HIPCHK(hipSetDevice(E->ctx->device));
+        check_multi_plans(E, plans, nplans);
+        const bool f32 = E->prec == PIMI_C32;
+        const size_t es = f32 ? sizeof(cx<float>) : sizeof(cx<double>);
+        int64_t total = 0;
+        std::vector<int64_t> off((size_t)nplans + 1, 0);
+        for (int32_t k = 0; k < nplans; ++k) off[(size_t)k + 1] = off[(size_t)k] + plans[k]->n;
+        total = off[(size_t)nplans];
+        const double *ylo = y, *yhi = y + 2 * total, *xlo = x, *xhi = x + 2 * total;
+        if (!(yhi <= xlo || xhi <= ylo)) throw Error(PIMI_EINVAL, "x and y must not overlap");
+        DevBuf<char> dx, dy;
+        dx.alloc(es * (size_t)total);
+        dy.alloc(es * (size_t)total);
+        std::vector<float> xf;
+        if (f32) {
+            xf.resize((size_t)(2 * total));
+            for (int64_t i = 0; i < 2 * total; ++i) xf[(size_t)i] = (float)x[i];
+            HIPCHK(hipMemcpyAsync(dx.p, xf.data(), es * (size_t)total, hipMemcpyHostToDevice, E->stream));
+        } else {
+            HIPCHK(hipMemcpyAsync(dx.p, x, es * (size_t)total, hipMemcpyHostToDevice, E->stream));
+        }
+        std::vector<void *> xs((size_t)nplans), ys((size_t)nplans);
+        for (int32_t k = 0; k < nplans; ++k) {
+            xs[(size_t)k] = dx.p + es * (size_t)off[(size_t)k];
+            ys[(size_t)k] = dy.p + es * (size_t)off[(size_t)k];
+        }
+        MultiApply M;
+        M.prepare(E, plans, nplans, xs.data(), ys.data());
+        M.launch(E);
+        if (f32) {
+            std::vector<float> yf((size_t)(2 * total));
+            HIPCHK(hipMemcpyAsync(yf.data(), dy.p, es * (size_t)total, hipMemcpyDeviceToHost, E->stream));
+            HIPCHK(hipStreamSynchronize(E->stream));
+            for (int64_t i = 0; i < 2 * total; ++i) y[i] = (double)yf[(size_t)i];
+        } else {
+            HIPCHK(hipMemcpyAsync(y, dy.p, es * (size_t)total, hipMemcpyDeviceToHost, E->stream));
+            HIPCHK(hipStreamSynchronize(E->stream));
+        }
+    });
+}
+
+pimi_status pimi_batch_apply_bench(pimi_batch *E, pimi_plan *const *plans, int32_t nplans, int32_t reps,
+                                   double *ms_avg, int64_t *algorithmic_bytes) {
+    return guarded([&] {
+        if (!E || !plans || nplans < 1 || reps < 1 || !ms_avg || !algorithmic_bytes)
+            throw Error(PIMI_EINVAL, "invalid argument");
+        BusyGuard guard(E->busy);
+        HIPCHK(hipSetDevice(E->ctx->device));
+        check_multi_plans(E, plans, nplans);
+        const size_t es = E->prec == PIMI_C32 ? sizeof(cx<float>) : sizeof(cx<double>);
+        std::vector<DevBuf<char>> xs((size_t)nplans), ys((size_t)nplans);
+        std::vector<void *> xp((size_t)nplans), yp((size_t)nplans);
+        for (int32_t k = 0; k < nplans; ++k) {
+            pimi_plan *P = plans[k];
+            xs[k].alloc(es * (size_t)P->n);
+            ys[k].alloc(es * (size_t)P->n);
+            HIPCHK(hipMemsetAsync(xs[k].p, 0x3f, es * (size_t)P->n, E->stream));
+            xp[(size_t)k] = xs[k].p;
+            yp[(size_t)k] = ys[k].p;
+        }
+        MultiApply M;
+        M.prepare(E, plans, nplans, xp.data(), yp.data());
+        M.launch(E);  // warm-up
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, E->stream));
-        for (int32_t r = 0; r < reps; ++r) launch();
+        for (int32_t r = 0; r < reps; ++r) M.launch(E);
         HIPCHK(hipEventRecord(e1, E->stream));
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventSynchronize(e1));
@@ -1887,7 +1963,7 @@ pimi_status pimi_batch_apply_bench(pimi_batch *E, pimi_plan *const *plans, int32
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         *ms_avg = ms / reps;
-        *algorithmic_bytes = bytes;
+        *algorithmic_bytes = M.bytes;
     });
 }
 

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -345,6 +346,11 @@ struct pimi_context {
 
 struct pimi_plan {
     pimi_context *ctx = nullptr;
+    // serializes the tables built on first use (preconditioner, weighting,
+    // sector-diagonal, adjoint and trajectory stencils) and the autotuner's
+    // width updates, so workspaces of one plan on several threads build each
+    // table once and never replace a buffer another stream reads
+    std::recursive_mutex lazy_mu;
     pimi::FusedPlan host;
     pimi::Stencil st;
     bool compiled = false;  // true: geometry available (collective operators)

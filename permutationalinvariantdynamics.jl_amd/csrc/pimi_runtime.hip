@@ -443,6 +443,7 @@ struct pimi_workspace {
     hipGraphExec_t inner_graph = nullptr;  // captured inner solve (one per (kin, precision))
     int inner_graph_k = 0, inner_graph_prec = -1;
     double inner_graph_scale = 0.0;
+    int inner_graph_width = -1;  // banded tile width index the graph was captured with
     bool use_graphs = true;  // PIMI_NO_GRAPH=1 runs the same kernels eagerly
     double *hpin = nullptr;  // pinned host staging (small results)
     bool trace_progress = false;  // PIMI_TRACE=1: solver progress lines on stderr
@@ -533,6 +534,7 @@ static void finalize_plan(pimi_plan *P) {
 // device operations on a workspace
 // ---------------------------------------------------------------------------
 static void ensure_diag_stencil(pimi_plan *P) {
+    std::lock_guard<std::recursive_mutex> lk(P->lazy_mu);
     if (P->diag_ready) return;
     Stencil st;
     build_stencil(P->host, st, true);
@@ -541,6 +543,7 @@ static void ensure_diag_stencil(pimi_plan *P) {
 }
 
 static void ensure_adjoint_stencil(pimi_plan *P) {
+    std::lock_guard<std::recursive_mutex> lk(P->lazy_mu);
     if (P->adj_ready) return;
     FusedPlan A;
     adjoint_plan(P->host, A);
@@ -852,6 +855,7 @@ static const cx<double> *look_wait(pimi_workspace *W, int j, double &hnext) {
 
 
 void ensure_plan_preconditioner(pimi_plan *P) {
+    std::lock_guard<std::recursive_mutex> lk(P->lazy_mu);
     if (!P->prec_ready) {
         std::vector<cd> U, inv;
         build_sector_preconditioner(P->host, U, inv);
@@ -925,6 +929,7 @@ void ensure_plan_preconditioner(pimi_plan *P) {
 }
 
 void ensure_plan_weighting(pimi_plan *P) {
+    std::lock_guard<std::recursive_mutex> lk(P->lazy_mu);
     if (P->weight_ready) return;
     const FusedPlan &F = P->host;
     P->sigma.assign((size_t)F.ns, 1.0);
@@ -1200,8 +1205,12 @@ static void run_inner(pimi_workspace *W, int kin, double scale) {
         else inner_solve_dev<double>(W, kin, scale);
     };
     if (!W->use_graphs) { body(); return; }
+    // the key covers everything the captured launches bake in: the inner
+    // Krylov dimension, the precision, the scale and the banded apply's tile
+    // width (which the autotuner may change after a capture)
+    const int width = W->plan->dia_w_single[W->prec == PIMI_C32 ? 1 : 0];
     if (!W->inner_graph || W->inner_graph_k != kin || W->inner_graph_prec != W->prec ||
-        W->inner_graph_scale != scale) {
+        W->inner_graph_scale != scale || W->inner_graph_width != width) {
         if (W->inner_graph) { HIPCHK(hipGraphExecDestroy(W->inner_graph)); W->inner_graph = nullptr; }
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(W->stream, hipStreamCaptureModeThreadLocal));
@@ -1219,6 +1228,7 @@ static void run_inner(pimi_workspace *W, int kin, double scale) {
         W->inner_graph_k = kin;
         W->inner_graph_prec = W->prec;
         W->inner_graph_scale = scale;
+        W->inner_graph_width = width;
     }
     HIPCHK(hipGraphLaunch(W->inner_graph, W->stream));
 }
@@ -1554,7 +1564,8 @@ static double operator_scale(pimi_workspace *W) {
     return scale;
 }
 
-static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, double *rho_out, pimi_solve_info *info) {
+static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, const double *x0, double *rho_out,
+                         pimi_solve_info *info) {
     auto t0 = std::chrono::steady_clock::now();
     pimi_plan *P = W->plan;
     const int64_t n = P->n;
@@ -1588,7 +1599,22 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, double *rh
     int64_t nt = (int64_t)vv.size();
     hipLaunchKernelGGL(k_scatter<double>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, W->stream, W->b.p,
                        P->d_tidx.p, W->d_vval.p, nt);
-    HIPCHK(hipMemcpyAsync(W->x.p, W->b.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
+    if (x0) {  // initial_state (src/krylov.jl:818-821); in weighted coordinates u0 = S x0
+        std::vector<cx<double>> xs((size_t)n);
+        std::memcpy(xs.data(), x0, sizeof(cx<double>) * (size_t)n);
+        for (const cx<double> &z : xs)
+            if (!std::isfinite(z.x) || !std::isfinite(z.y)) throw Error(PIMI_EINVAL, "initial_state must be finite");
+        if (weighted)
+            for (int q = 0; q < P->ns; ++q) {
+                const double sg = P->sigma[(size_t)q];
+                const int64_t m = P->host.dims[(size_t)q];
+                for (int64_t i = P->host.offs[(size_t)q]; i < P->host.offs[(size_t)q] + m * m; ++i)
+                    xs[(size_t)i] = cx<double>{xs[(size_t)i].x * sg, xs[(size_t)i].y * sg};
+            }
+        h2d(W, W->x.p, xs.data(), sizeof(cx<double>) * (size_t)n);
+    } else {
+        HIPCHK(hipMemcpyAsync(W->x.p, W->b.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
+    }
     int64_t apps = 3;
     const bool precond = o->preconditioner == PIMI_PRECOND_SECTOR;
     const bool inner_outer = o->preconditioner == PIMI_PRECOND_SECTOR_INNER;
@@ -1971,6 +1997,7 @@ __global__ void k_cols_div(cx<double> *__restrict__ X, const cx<double> *__restr
 }
 
 static void ensure_trajectory_tables(pimi_plan *P) {
+    std::lock_guard<std::recursive_mutex> lk(P->lazy_mu);
     if (P->traj_ready) return;
     const FusedPlan &F = P->host;
     if (!P->compiled || F.geom.sqrt_f.empty())
@@ -2546,8 +2573,11 @@ pimi_status pimi_workspace_autotune_apply(pimi_workspace *W, int64_t ncols, int3
                     best = w;
                 }
             }
-            if (pass == 0) P->dia_w_single[pi] = best;
-            else P->dia_w_multi[pi] = best;
+            {
+                std::lock_guard<std::recursive_mutex> lk(P->lazy_mu);
+                if (pass == 0) P->dia_w_single[pi] = best;
+                else P->dia_w_multi[pi] = best;
+            }
             widths_out[pass] = kDiaWidths[best];
         }
     });
@@ -2669,14 +2699,14 @@ pimi_status pimi_workspace_last_apply_ms(const pimi_workspace *W, double *ms) {
     });
 }
 
-pimi_status pimi_steady_gmres(pimi_workspace *W, const pimi_gmres_opts *opts, double *rho_out,
+pimi_status pimi_steady_gmres(pimi_workspace *W, const pimi_gmres_opts *opts, const double *x0, double *rho_out,
                               pimi_solve_info *info) {
     return guarded([&] {
         if (!W || !opts || !rho_out || !info) throw Error(PIMI_EINVAL, "null argument");
         if (!W->plan->compiled) throw Error(PIMI_EINVAL, "stationary solves need a compiled model");
         BusyGuard guard(W->busy);
         HIPCHK(hipSetDevice(W->plan->ctx->device));
-        steady_solve(W, opts, rho_out, info);
+        steady_solve(W, opts, x0, rho_out, info);
     });
 }
 

@@ -493,9 +493,11 @@ class SteadyStateResult:
 
 def stationary_state(compiled: CompiledPIModel, algorithm=None, atol=1e-10, rtol=1e-8, operator_scale=None,
                      return_info=False, precision=N.PIMI_C64, inner_rtol=1e-5, max_refinements=50,
-                     criterion="reference", weighting=None):
+                     criterion="reference", weighting=None, initial_state=None):
     """Trace-fixed stationary state by restarted GMRES on the device
     (src/krylov.jl:797-891).  Raises NotConverged like the reference.
+    initial_state (a PIState or coefficient vector) replaces the default
+    start v = t/(t.t), as the reference's keyword of that name.
 
     weighting="sector" iterates on u = S x with S = sqrt(f^nu) on sector nu,
     so that residual norms weigh sectors by multiplicity as observables do
@@ -517,8 +519,15 @@ def stationary_state(compiled: CompiledPIModel, algorithm=None, atol=1e-10, rtol
                        float(operator_scale or 0.0), float(inner_rtol), int(max_refinements), crit,
                        int(algorithm.inner_steps), wt, 0)
     rho = np.zeros(len(compiled.basis), dtype=np.complex128)
+    x0 = None
+    if initial_state is not None:
+        data = initial_state.data if isinstance(initial_state, PIState) else initial_state
+        x0 = np.ascontiguousarray(np.asarray(data, dtype=np.complex128))
+        if x0.shape != (len(compiled.basis),):
+            raise ValueError("initial_state has wrong length")
     info = N.SolveInfo()
-    N.check(N.lib().pimi_steady_gmres(ws, C.byref(opts), N.ptr(rho), C.byref(info)))
+    N.check(N.lib().pimi_steady_gmres(ws, C.byref(opts), None if x0 is None else N.ptr(x0), N.ptr(rho),
+                                      C.byref(info)))
     state = PIState(compiled.basis, rho)
     res = SteadyStateResult(state, info.as_dict(), algorithm)
     return res if return_info else state
@@ -571,43 +580,83 @@ def saved_times(tspan, saveat=None):
 
 @dataclass
 class DynamicsStreamResult:
+    """Saved times, observable series and the final state; `states` holds
+    the state at every saved time when save_states was requested (the
+    reference's DynamicsResult / DynamicsStreamResult states), else None."""
     t: np.ndarray
     values: dict
     final_state: PIState
     info: dict
+    states: list | None = None
 
 
 def solve_dynamics(compiled: CompiledPIModel, rho0: PIState, tspan, saveat=None, algorithm=None,
                    observables=None, save_states=False):
-    """Expv dynamics with streamed observables (src/highlevel.jl:614-733)."""
+    """Expv dynamics (src/highlevel.jl:559-733): observables streamed as
+    device dots at every saved time; with save_states=True the state at every
+    saved time is also returned (one device-to-host copy per saved time, the
+    reference's per-interval krylov_expv! with copy(current)).  As in the
+    reference, save_states=False needs at least one observable."""
     algorithm = algorithm or ExpvAlgorithm()
     if not isinstance(algorithm, ExpvAlgorithm):
         raise ValueError("the mi355x backend propagates with ExpvAlgorithm")
-    if save_states:
-        raise ValueError("the mi355x backend streams observables; use save_states=False")
-    if not observables:
+    if not observables and not save_states:
         raise ValueError("save_states=false requires at least one observable")
     if rho0.basis != compiled.basis:
         raise ValueError("Liouvillian source and initial state use incompatible PI bases")
     ts = saved_times(tspan, saveat)
     names, vecs = [], []
-    items = observables.items() if isinstance(observables, dict) else [("observable", observables)]
+    items = (observables.items() if isinstance(observables, dict) else [("observable", observables)]) \
+        if observables else []
     for name, op in items:
         names.append(name)
         if isinstance(op, PIState):
             vecs.append(np.asarray(op.data, dtype=np.complex128))
         else:
             vecs.append(compiled.collective_operator(op).data)
-    obs = np.ascontiguousarray(np.stack(vecs))
+    obs = np.ascontiguousarray(np.stack(vecs)) if vecs else None
     out = np.zeros((len(ts), len(names)), dtype=np.complex128)
     rho = np.ascontiguousarray(rho0.data, dtype=np.complex128)
     final = np.zeros_like(rho)
     ws = compiled.workspace(N.PIMI_C64, algorithm.krylovdim)
     opts = N.ExpvOpts(algorithm.krylovdim, 0, float(algorithm.atol), float(algorithm.rtol), float(algorithm.safety),
                       int(algorithm.max_steps))
-    info = N.SolveInfo()
     tsc = np.ascontiguousarray(ts, dtype=np.float64)
-    N.check(N.lib().pimi_expv_series(ws, N.ptr(rho), N.ptr(tsc), C.c_int64(len(ts)), C.byref(opts),
-                                     C.c_int32(len(names)), N.ptr(obs), N.ptr(out), N.ptr(final), C.byref(info)))
+
+    def run(t_seg, x_in, obs_seg):
+        info = N.SolveInfo()
+        N.check(N.lib().pimi_expv_series(ws, N.ptr(x_in), N.ptr(t_seg), C.c_int64(len(t_seg)), C.byref(opts),
+                                         C.c_int32(len(names)), None if obs is None else N.ptr(obs),
+                                         None if obs is None else N.ptr(obs_seg), N.ptr(final), C.byref(info)))
+        return info.as_dict()
+
+    if not save_states:
+        info = run(tsc, rho, out)
+        states = None
+    else:
+        # every interval restarts krylov_expv! in the reference too, so
+        # splitting the series at the saved times changes no arithmetic
+        states = [PIState(compiled.basis, rho.copy())]
+        info = None
+        seg_out = np.zeros((2, len(names)), dtype=np.complex128)
+        cur = rho.copy()
+        for k in range(1, len(ts)):
+            d = run(np.ascontiguousarray(tsc[k - 1:k + 1]), cur, seg_out)
+            if k == 1:
+                out[0] = seg_out[0]
+            out[k] = seg_out[1]
+            cur = final.copy()
+            states.append(PIState(compiled.basis, cur.copy()))
+            if info is None:
+                info = d
+            else:
+                for key in ("accepted", "rejected", "applications", "factorizations", "iterations"):
+                    info[key] += d[key]
+                info["estimated_error"] += d["estimated_error"]
+                info["seconds"] += d["seconds"]
+                info["converged"] = info["converged"] and d["converged"]
+        if info is None:  # a single saved time
+            info = run(tsc, rho, out)
+        final = cur
     values = {name: out[:, k].copy() for k, name in enumerate(names)}
-    return DynamicsStreamResult(ts, values, PIState(compiled.basis, final), info.as_dict())
+    return DynamicsStreamResult(ts, values, PIState(compiled.basis, final), info, states)

@@ -134,6 +134,27 @@ class BatchEngine:
                                                   C.byref(w)))
         return w.value
 
+    def apply(self, compiled_list, xs):
+        """Y_k = L_k X_k for every model k in one launch (the engine's
+        multi-model apply), computed in the engine precision; xs: one vector
+        per model.  The multi-model form of the batch apply!
+        (src/liouvillian.jl:3232-3251), across models instead of columns."""
+        if len(xs) != len(compiled_list):
+            raise ValueError("one input vector per model")
+        for c, x in zip(compiled_list, xs):
+            if np.shape(x) != (c.info["n"],):
+                raise ValueError("input vector length must equal the model's PI dimension")
+        x = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.complex128) for v in xs]))
+        y = np.empty_like(x)
+        arr = (C.c_void_p * len(compiled_list))(*[c._plan.value for c in compiled_list])
+        N.check(N.lib().pimi_batch_apply(self._h, arr, C.c_int32(len(compiled_list)), N.ptr(x), N.ptr(y)))
+        out, k = [], 0
+        for c in compiled_list:
+            n = c.info["n"]
+            out.append(y[k:k + n])
+            k += n
+        return out
+
     def apply_bench(self, compiled_list, reps=20):
         """One launch applying L to a vector of every model, event-timed:
         (ms per launch, algorithmic bytes per launch)."""

@@ -167,7 +167,11 @@ def load_checkpoint(path, format="auto") -> PIStateCheckpoint:
     if count != len(basis):
         raise ValueError("PI checkpoint coefficient length does not match its basis")
     pairs = np.frombuffer(take(2 * count * R.itemsize), dtype=R)
-    data = (pairs[0::2] + 1j * pairs[1::2]).astype(_COMPLEX[np.dtype(_CODES[code])])
+    # complex(re, im) exactly, as the reference builds it: re + 1j*im would
+    # turn (2, inf) into (nan, inf) and lose the sign of a -0.0 real part
+    data = np.empty(count, dtype=_COMPLEX[np.dtype(_CODES[code])])
+    data.real = pairs[0::2]
+    data.imag = pairs[1::2]
     (mcount,) = unpack("i")
     if mcount < 0:
         raise ValueError("invalid PI checkpoint metadata count")

@@ -99,3 +99,26 @@ def test_restricted_basis_cannot_compile():
         pimi.compile(model, host_only=True)
     with pytest.raises(ValueError):
         pimi.PIBasis(3, 2, sectors=[(2, 2)])
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_special_values_roundtrip_byte_for_byte(tmp_path, dtype):
+    """Coefficients are rebuilt as complex(re, im) exactly: -0.0 real parts,
+    infinite and NaN parts survive a save/load/save cycle bit for bit (a
+    re + 1j*im reconstruction turns (2, inf) into (nan, inf))."""
+    b = pimi.PIBasis(2, 2)  # n = 10
+    R = np.float64 if dtype == np.complex128 else np.float32
+    re = np.array([-0.0, 2.0, np.inf, -np.inf, np.nan, 0.0, 1.5, -0.0, 3.0, -2.0], dtype=R)
+    im = np.array([1.0, np.inf, -0.0, 2.0, 0.5, np.nan, -np.inf, -0.0, 0.0, 7.0], dtype=R)
+    data = np.empty(10, dtype=dtype)
+    data.real, data.imag = re, im
+    p1, p2 = str(tmp_path / "a.pid"), str(tmp_path / "b.pid")
+    pimi.save_checkpoint(p1, pimi.PIState(b, data))
+    got = pimi.load_checkpoint(p1)
+    assert got.state.data.dtype == dtype
+    np.testing.assert_array_equal(got.state.data.real.view(np.uint64 if R == np.float64 else np.uint32),
+                                  re.view(np.uint64 if R == np.float64 else np.uint32))
+    np.testing.assert_array_equal(got.state.data.imag.view(np.uint64 if R == np.float64 else np.uint32),
+                                  im.view(np.uint64 if R == np.float64 else np.uint32))
+    pimi.save_checkpoint(p2, got)
+    assert open(p1, "rb").read() == open(p2, "rb").read()

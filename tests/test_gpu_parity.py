@@ -1,8 +1,12 @@
 """GPU parity: libpimi on device 0 against the oracle on the same inputs.
 
 Tolerances: complex FP64 apply <= 1e-13 relative (max-norm); FP32 apply
-<= 5e-6 relative; steady states and dynamics <= 1e-9 relative after both
-solvers converge to rtol 1e-12 with atol 0.
+<= 5e-6 relative.  Plain FP64 solves (the reference's GMRES and expv) <= 1e-12
+relative against the oracle when both solve to rtol 1e-14 with atol 0 (C1,
+N=14, and the N=120 expmv over its first save intervals); preconditioned and
+flexible solves, whose iterates differ from the oracle's algorithm, to the
+bound their conditioning allows at the stated residual.  Measured errors are
+recorded with parity_log and printed at the end of the session.
 """
 import ctypes as C
 import os
@@ -10,6 +14,7 @@ import os
 import numpy as np
 import pytest
 
+import parity_log
 import pimi
 import planutil as PU
 from oracle import lowering as L
@@ -92,6 +97,90 @@ def test_apply_multicolumn_equals_columns():
         np.testing.assert_array_equal(Y[:, k], comp.apply(X[:, k]))
 
 
+@pytest.mark.parametrize("case", ["qubits_61", "qutrits_5"])
+def test_apply_multicolumn_against_oracle(case):
+    """The multi-column apply (batch apply!, src/liouvillian.jl:3232-3251):
+    every column against the oracle's serial apply, FP64 (<= 1e-13) and FP32
+    tables (<= 5e-6), banded (qubit) and ELL (qutrit) kernels."""
+    if case == "qubits_61":
+        comp = pimi.compile(_c2_model(61))
+        ref, _ = O.compile_plan(61, 2, _c2_terms())
+    else:
+        model, ref = _qutrit(5)
+        comp = pimi.compile(model)
+    X = _rand(comp.info["n"], 31, cols=5)
+    for prec, bar in ((pimi.PIMI_C64, 1e-13), (pimi.PIMI_C32, 5e-6)):
+        Y = comp.apply(X, precision=prec)
+        err = max(_relmax(Y[:, k], ref.apply(X[:, k])) for k in range(X.shape[1]))
+        parity_log.record(f"apply_multicolumn_{case}_{'c64' if prec == 0 else 'c32'}", err, bar)
+        assert err <= bar, err
+
+
+def test_multi_model_apply_against_oracle():
+    """The batch engine's multi-model apply (one launch over the tiles of
+    every plan, kb_apply_dia for qubit plans and kb_apply for qutrit plans)
+    against the oracle's apply of each model, FP64 and FP32 engines."""
+    sizes = (3, 20, 47, 120)
+    comps = [pimi.compile(_c2_model(n_)) for n_ in sizes]
+    refs = [O.compile_plan(n_, 2, _c2_terms())[0] for n_ in sizes]
+    qmodel, qref = _qutrit(4)
+    comps.append(pimi.compile(qmodel))
+    refs.append(qref)
+    xs = [_rand(r.n, 50 + k) for k, r in enumerate(refs)]
+    want = [r.apply(x, nthreads=4) for r, x in zip(refs, xs)]
+    for prec, bar in ((pimi.PIMI_C64, 1e-13), (pimi.PIMI_C32, 5e-6)):
+        eng = pimi.BatchEngine(precision=prec, memory_bytes=1 << 24)
+        try:
+            ys = eng.apply(comps, xs)
+        finally:
+            eng.close()
+        err = max(_relmax(y, w) for y, w in zip(ys, want))
+        parity_log.record(f"apply_multimodel_{'c64' if prec == 0 else 'c32'}", err, bar)
+        assert err <= bar, err
+
+
+def test_workspace_in_use_returns_busy():
+    """A workspace serves one caller at a time (the reference's CAS in
+    src/threaded_apply.jl:575-576): while a long dynamics call holds it on one
+    thread, applies on the same workspace from another thread get PIMI_EBUSY
+    and change nothing; the held call's result equals an undisturbed run."""
+    import threading
+    import time
+    import pimi._native as Nn
+    comp = pimi.compile(_c2_model(40))
+    b = comp.basis
+    rho0 = pimi.computational_product_state(b, 2)
+    alg = pimi.ExpvAlgorithm(krylovdim=30, atol=0.0, rtol=1e-12)
+    t0 = time.perf_counter()
+    ref = pimi.solve_dynamics(comp, rho0, (0.0, 20.0), saveat=0.02, algorithm=alg,
+                              observables={"n": SP2.jp @ SP2.jm})
+    t_ref = time.perf_counter() - t0
+    ws = comp.workspace(0, 30)
+    x = np.ascontiguousarray(_rand(comp.info["n"], 9))
+    y = np.zeros_like(x)
+    counts = {"busy": 0, "ok": 0, "other": 0}
+    out = {}
+
+    def long_call():
+        out["res"] = pimi.solve_dynamics(comp, rho0, (0.0, 20.0), saveat=0.02, algorithm=alg,
+                                         observables={"n": SP2.jp @ SP2.jm})
+
+    th = threading.Thread(target=long_call)
+    th.start()
+    time.sleep(min(0.25, 0.2 * t_ref))  # the long call is inside the library by now
+    while th.is_alive():
+        st = Nn.lib().pimi_apply(ws, Nn.ptr(x), Nn.ptr(y), C.c_int64(1), C.c_int32(0))
+        key = "busy" if st == Nn.PIMI_EBUSY else ("ok" if st == Nn.PIMI_OK else "other")
+        counts[key] += 1
+    th.join()
+    print(f"concurrent applies {counts} during a {t_ref:.2f} s call", flush=True)
+    assert "res" in out
+    assert counts["busy"] >= 1 and counts["other"] == 0
+    np.testing.assert_array_equal(out["res"].values["n"], ref.values["n"])
+    # the workspace is free again afterwards
+    Nn.check(Nn.lib().pimi_apply(ws, Nn.ptr(x), Nn.ptr(y), C.c_int64(1), C.c_int32(0)))
+
+
 def test_apply_fp32():
     comp = pimi.compile(_c2_model(40))
     ref, _ = O.compile_plan(40, 2, _c2_terms())
@@ -137,23 +226,28 @@ def test_apply_rejects_alias():
         N.check(N.lib().pimi_apply(ws, N.ptr(x), N.ptr(x), C.c_int64(1), C.c_int32(0)))
 
 
-@pytest.mark.parametrize("N", [8, 14])
-def test_steady_state_parity(N):
-    """Plain restarted GMRES(30), the reference's default (it stalls from
-    N ~ 20 on for this model family; see the preconditioned tests)."""
-    print(f"steady N={N}", flush=True)
+@pytest.mark.parametrize("N,kdim", [(8, 30), (8, 200), (14, 30), (14, 700)])
+def test_steady_state_parity(N, kdim):
+    """Plain restarted GMRES, the reference's algorithm (src/krylov.jl:732-891),
+    at the reference's default GMRES(30) and with the Krylov space as large as
+    the PI space (n = 165 at N=8, 680 at N=14), both sides to rtol 1e-14: the
+    device solution matches the oracle's to <= 1e-12 relative.  (GMRES(30)
+    stalls from N ~ 20 on for this model family; see the preconditioned
+    tests.)"""
+    print(f"steady N={N} krylovdim={kdim}", flush=True)
     model = _c1_model() if N == 8 else _c2_model(N)
     terms = _c1_terms() if N == 8 else _c2_terms()
     comp = pimi.compile(model)
     ref, _ = O.compile_plan(N, 2, terms)
-    res = pimi.stationary_state(comp, algorithm=pimi.GMRESAlgorithm(30, 5000), atol=0.0, rtol=1e-12,
+    res = pimi.stationary_state(comp, algorithm=pimi.GMRESAlgorithm(kdim, 20000), atol=0.0, rtol=1e-14,
                                 return_info=True)
-    x_ref, info_ref = ref.steady_state(krylovdim=30, maxiter=5000, atol=0.0, rtol=1e-12)
+    x_ref, info_ref = ref.steady_state(krylovdim=kdim, maxiter=20000, atol=0.0, rtol=1e-14, nthreads=4)
     assert res.info["converged"] and info_ref["converged"]
     assert abs(res.info["operator_scale"] - info_ref["operator_scale"]) <= 1e-12 * info_ref["operator_scale"]
     err = np.linalg.norm(res.state.data - x_ref) / np.linalg.norm(x_ref)
-    assert err <= 1e-9, err
-    assert res.info["normalized_residual"] <= 1e-12 * max(np.linalg.norm(res.state.data), 1.0)
+    parity_log.record(f"steady_gmres_N{N}_k{kdim}", err, 1e-12)
+    assert err <= 1e-12, err
+    assert res.info["normalized_residual"] <= 1e-14 * max(np.linalg.norm(res.state.data), 1.0)
 
 
 def test_steady_state_not_converged_raises():
@@ -163,6 +257,8 @@ def test_steady_state_not_converged_raises():
 
 
 def test_expv_series_parity_c1():
+    """C1 dynamics (expv over t in [0, 4], saveat 0.1, Krylov 30), both sides
+    to rtol 1e-14: observables and the final state <= 1e-12 relative."""
     comp = pimi.compile(_c1_model())
     ref, g = O.compile_plan(8, 2, _c1_terms())
     b = comp.basis
@@ -172,16 +268,61 @@ def test_expv_series_parity_c1():
     jp = L.collective_operator(g, sp["jp"])
     jm = L.collective_operator(g, sp["jm"])
     obs_coll = L.pi_product(ref.basis, jp, jm)
-    alg = pimi.ExpvAlgorithm(krylovdim=30, atol=0.0, rtol=1e-12)
+    alg = pimi.ExpvAlgorithm(krylovdim=30, atol=0.0, rtol=1e-14)
     res = pimi.solve_dynamics(comp, rho0, (0.0, 4.0), saveat=0.1, algorithm=alg,
                               observables={"n": SP2.jp @ SP2.jm, "JpJm": pimi.PIOperator(b, obs_coll)})
     ts = res.t
     assert len(ts) == 41
     out, rho_ref, info = ref.solve_dynamics(rho0.data, ts, [obs_local, obs_coll], krylovdim=30, atol=0.0,
-                                            rtol=1e-12)
-    assert _relmax(res.values["n"], out[:, 0]) <= 1e-9
-    assert _relmax(res.values["JpJm"], out[:, 1]) <= 1e-9
-    assert np.linalg.norm(res.final_state.data - rho_ref) <= 1e-9 * np.linalg.norm(rho_ref)
+                                            rtol=1e-14)
+    e_obs = max(_relmax(res.values["n"], out[:, 0]), _relmax(res.values["JpJm"], out[:, 1]))
+    e_state = np.linalg.norm(res.final_state.data - rho_ref) / np.linalg.norm(rho_ref)
+    parity_log.record("expv_c1_observables", e_obs, 1e-12)
+    parity_log.record("expv_c1_state", e_state, 1e-12)
+    assert e_obs <= 1e-12 and e_state <= 1e-12
+
+
+def test_expv_series_parity_n14():
+    """The N=14 qubit ensemble (C2 model, n = 680) from |e><e|^14 over
+    t in [0, 2], saveat 0.25, Krylov 30, both sides to rtol 1e-14."""
+    comp = pimi.compile(_c2_model(14))
+    ref, g = O.compile_plan(14, 2, _c2_terms())
+    rho0 = pimi.computational_product_state(comp.basis, 2)
+    sp = L.spin_matrices(2)
+    obs = [L.collective_operator(g, sp["jp"] @ sp["jm"]), L.collective_operator(g, sp["jx"])]
+    alg = pimi.ExpvAlgorithm(krylovdim=30, atol=0.0, rtol=1e-14)
+    res = pimi.solve_dynamics(comp, rho0, (0.0, 2.0), saveat=0.25, algorithm=alg,
+                              observables={"n": SP2.jp @ SP2.jm, "jx": SP2.jx})
+    out, rho_ref, _ = ref.solve_dynamics(rho0.data, res.t, obs, krylovdim=30, atol=0.0, rtol=1e-14, nthreads=4)
+    e_obs = max(_relmax(res.values["n"], out[:, 0]), _relmax(res.values["jx"], out[:, 1]))
+    e_state = np.linalg.norm(res.final_state.data - rho_ref) / np.linalg.norm(rho_ref)
+    parity_log.record("expv_n14_observables", e_obs, 1e-12)
+    parity_log.record("expv_n14_state", e_state, 1e-12)
+    assert e_obs <= 1e-12 and e_state <= 1e-12
+
+
+def test_expmv_c2_n120_first_intervals():
+    """C2 dynamics at full size (N=120, n = 302,621, Krylov 40, saveat 0.1):
+    the first three save intervals, device and oracle (all host threads) both
+    to rtol 1e-14; state and observables <= 1e-12 relative."""
+    print("expmv N=120 first intervals", flush=True)
+    comp = pimi.compile(_c2_model(120))
+    ref, g = O.compile_plan(120, 2, _c2_terms())
+    rho0 = pimi.computational_product_state(comp.basis, 2)
+    sp = L.spin_matrices(2)
+    obs = [L.collective_operator(g, sp["jp"] @ sp["jm"]), L.collective_operator(g, sp["jz"])]
+    alg = pimi.ExpvAlgorithm(krylovdim=40, atol=0.0, rtol=1e-14)
+    res = pimi.solve_dynamics(comp, rho0, (0.0, 0.3), saveat=0.1, algorithm=alg,
+                              observables={"n": SP2.jp @ SP2.jm, "jz": SP2.jz})
+    assert len(res.t) == 4
+    nth = max(1, min(16, os.cpu_count() or 1))
+    out, rho_ref, info = ref.solve_dynamics(rho0.data, res.t, obs, krylovdim=40, atol=0.0, rtol=1e-14,
+                                            nthreads=nth)
+    e_obs = max(_relmax(res.values["n"], out[:, 0]), _relmax(res.values["jz"], out[:, 1]))
+    e_state = np.linalg.norm(res.final_state.data - rho_ref) / np.linalg.norm(rho_ref)
+    parity_log.record("expmv_c2_n120_observables", e_obs, 1e-12)
+    parity_log.record("expmv_c2_n120_state", e_state, 1e-12)
+    assert e_obs <= 1e-12 and e_state <= 1e-12
 
 
 @pytest.mark.parametrize("N_", [1, 2])
@@ -506,3 +647,53 @@ def test_apply_autotune_picks_a_width_and_keeps_results():
     np.testing.assert_array_equal(comp.apply(X), Y0)
     model, _ = _qutrit(2)
     assert pimi.compile(model).autotune(reps=2) == (0, 0)
+
+
+def test_initial_state_restarts_from_given_state():
+    """initial_state (src/krylov.jl:818-821): started from its own converged
+    solution the solve stops at once with the same state; started from
+    |e><e|^N it reaches the default start's solution."""
+    comp = pimi.compile(_c2_model(12))
+    alg = pimi.GMRESAlgorithm(200, 2000, "sector")
+    base = pimi.stationary_state(comp, algorithm=alg, atol=0.0, rtol=1e-13, return_info=True,
+                                 criterion="backward")
+    again = pimi.stationary_state(comp, algorithm=alg, atol=0.0, rtol=1e-13, return_info=True,
+                                  criterion="backward", initial_state=base.state)
+    assert again.info["iterations"] <= 1
+    e0 = np.linalg.norm(again.state.data - base.state.data) / np.linalg.norm(base.state.data)
+    other = pimi.stationary_state(comp, algorithm=alg, atol=0.0, rtol=1e-13, return_info=True,
+                                  criterion="backward", weighting="sector",
+                                  initial_state=pimi.computational_product_state(comp.basis, 2))
+    e1 = np.linalg.norm(other.state.data - base.state.data) / np.linalg.norm(base.state.data)
+    parity_log.record("steady_initial_state_restart", e0, 1e-12)
+    parity_log.record("steady_initial_state_product", e1, 1e-9)
+    assert e0 <= 1e-12 and e1 <= 1e-9
+    with pytest.raises(ValueError):
+        pimi.stationary_state(comp, algorithm=alg, initial_state=np.zeros(3))
+
+
+def test_solve_dynamics_save_states():
+    """save_states=True returns the state at every saved time (the
+    reference's DynamicsResult); the series and the final state are bitwise
+    those of the streamed run, and each saved state matches the oracle."""
+    comp = pimi.compile(_c1_model())
+    ref, g = O.compile_plan(8, 2, _c1_terms())
+    rho0 = pimi.computational_product_state(comp.basis, 2)
+    alg = pimi.ExpvAlgorithm(krylovdim=30, atol=0.0, rtol=1e-14)
+    streamed = pimi.solve_dynamics(comp, rho0, (0.0, 1.0), saveat=0.25, algorithm=alg,
+                                   observables={"n": SP2.jp @ SP2.jm})
+    saved = pimi.solve_dynamics(comp, rho0, (0.0, 1.0), saveat=0.25, algorithm=alg,
+                                observables={"n": SP2.jp @ SP2.jm}, save_states=True)
+    only = pimi.solve_dynamics(comp, rho0, (0.0, 1.0), saveat=0.25, algorithm=alg, save_states=True)
+    assert len(saved.states) == len(saved.t) == 5
+    np.testing.assert_array_equal(saved.values["n"], streamed.values["n"])
+    np.testing.assert_array_equal(saved.final_state.data, streamed.final_state.data)
+    np.testing.assert_array_equal(only.states[-1].data, streamed.final_state.data)
+    err = 0.0
+    for k, t in enumerate(saved.t):
+        _, rho_t, _ = ref.solve_dynamics(rho0.data, np.array([0.0, t]) if t > 0 else np.array([0.0]),
+                                         [L.collective_operator(g, L.spin_matrices(2)["jz"])], krylovdim=30,
+                                         atol=0.0, rtol=1e-14)
+        err = max(err, np.linalg.norm(saved.states[k].data - rho_t) / np.linalg.norm(rho_t))
+    parity_log.record("expv_c1_saved_states", err, 1e-12)
+    assert err <= 1e-12

@@ -57,3 +57,38 @@ def test_serving_alone_equals_mix_and_closed_forms():
                 np.testing.assert_allclose(r.values["n"].real, ref, rtol=1e-6, atol=1e-9)
     finally:
         srv.close()
+
+
+def test_serving_large_n_fp32_against_closed_form():
+    """The top of the served size range, where FP32 range hazards live
+    (t.t = 2^N overflows FP32 from N = 128, f^nu reaches 8.8e45 at N = 160):
+    FP32-Arnoldi stationary jobs at N = 128 and 160 and an N = 160 dynamics
+    request to t = 10 (save every 0.2), against the product-state closed forms."""
+    from pimi.workloads import Instance
+    import parity_log
+    reqs = [Request(0.0, "steady", Instance(128, 0.7, 0.1, 0.02)),
+            Request(0.0, "steady", Instance(160, 1.3, 0.15, 0.05)),
+            Request(0.0, "dynamics", Instance(160, 0.9, 0.12, 0.03))]
+    srv = Server(precision=pimi.PIMI_C32, rtol=1e-11, memory_bytes=8 << 30, dynamics_workers=1,
+                 dynamics_memory_bytes=4 << 30, compile_workers=3, t_end=10.0, saveat=0.2)
+    try:
+        out = srv.serve(reqs, realtime=False)
+        for r, q in zip(out, reqs):
+            assert r is not None and r.converged, (r, q)
+            if q.kind == "steady":
+                assert r.info["relative_residual"] <= 1e-11
+                comp = pimi.compile(q.instance.model())
+                val = pimi.expect(comp.collective_operator(np.array([[0, 0], [0, 1]], dtype=complex)),
+                                  r.state).real
+                ref = _closed(q.instance)
+                err = abs(val - ref) / ref
+                parity_log.record(f"served_fp32_steady_N{q.instance.N}_n", err, 1e-6)
+                assert err <= 1e-6, (q.instance, val, ref)
+            else:
+                assert len(r.times) == 51
+                ref = _closed(q.instance, list(r.times))
+                err = np.abs(r.values["n"].real - ref).max() / np.abs(ref).max()
+                parity_log.record(f"served_dynamics_N{q.instance.N}_n", err, 1e-6)
+                assert err <= 1e-6
+    finally:
+        srv.close()
