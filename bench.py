@@ -31,7 +31,11 @@ Also reported:
                 FP64 device solve, observables against the FP64 reference values;
   serving       a Poisson stream (200 requests/s, 60 requests, 60 % stationary
                 states, 40 % dynamics to t=10) through the serving loop;
-  cpu_baseline  the oracle's OpenMP CPU restatement on a bounded sample.
+  reference_algorithm_c1  the reference's own GMRES(30) and expv on C1 (PI dim
+                165), the oracle on the host beside the device;
+  cpu_baseline  the oracle's OpenMP CPU restatement solving the headline in
+                full with the headline's algorithm (and plain sector GMRES
+                beside the device's plain path), no extrapolation.
 """
 from __future__ import annotations
 
@@ -101,38 +105,97 @@ def gemm_roofline(pimi, comp, reps=50):
     return per_launch_ms, per_launch_flops, per_launch_flops / (per_launch_ms * 1e-3) / 1e12
 
 
-def cpu_baseline(steady_iters, budget_s):
-    """Oracle (C/OpenMP restatement: output-sector-owned threaded apply, the
-    same sector preconditioner, MGS2 GMRES) on a bounded sample of the same
-    solve: runs of k1 and k2 Arnoldi steps fit t(k) = a k + b k^2 (apply and
-    preconditioner per step, Gram-Schmidt growing with k), evaluated at the
-    plain device solve's iteration count."""
+def _oracle_c2(N=N_QUBITS):
+    from oracle import lowering as L
+    from oracle import oracle as O
+    sp = L.spin_matrices(2)
+    ref, geom = O.compile_plan(N, 2, L.qubit_ensemble_terms(
+        hamiltonian=sp["jz"] + sp["jx"], emission=0.1, dephasing=0.2, pumping=0.02, collective_emission=0.1))
+    return ref, geom
+
+
+def cpu_baseline(device_ms, plain_device_ms, full_plain=True):
+    """The oracle (C/OpenMP restatement: output-sector-owned threaded apply,
+    the same sector preconditioner, multi-threaded BLAS-1) solving the
+    headline N=120 stationary state in full on every host core -- no fit, no
+    extrapolation: the headline's own algorithm (inner-outer flexible GMRES,
+    8 inner steps, outer Krylov 200) and, with `full_plain`, plain
+    sector-preconditioned GMRES (the device's plain FP64 path)."""
+    threads = min(os.cpu_count() or 1, 64)
+    ref, _ = _oracle_c2()
+    scale = ref.operator_scale(threads)
+    ref.sector_preconditioner()
+    t0 = time.perf_counter()
+    _, info = ref.steady_state(krylovdim=KRYLOV, maxiter=10 * KRYLOV, atol=0.0, rtol=RTOL, operator_scale=scale,
+                               nthreads=threads, preconditioner="sector_inner", inner_steps=INNER_STEPS,
+                               criterion="backward")
+    t_fg = time.perf_counter() - t0
+    out = {"value": 1.0 / t_fg, "unit": "solves/s", "cores": threads, "kind": "port",
+           "sample": (f"one full N=120 stationary solve by the oracle's inner-outer flexible GMRES (the headline "
+                      f"algorithm: {INNER_STEPS} inner steps, outer Krylov {KRYLOV}, rtol {RTOL:g}) on {threads} "
+                      f"OpenMP threads: {t_fg:.1f} s, {info['iterations']} outer steps"),
+           "speedup_headline_vs_cpu": t_fg / (device_ms * 1e-3)}
+    print(f"# cpu flexible GMRES: {t_fg:.1f} s, {info['iterations']} steps", file=sys.stderr, flush=True)
+    if full_plain:
+        t0 = time.perf_counter()
+        _, info2 = ref.steady_state(krylovdim=KRYLOV_PLAIN, maxiter=KRYLOV_PLAIN, atol=0.0, rtol=RTOL,
+                                    operator_scale=scale, nthreads=threads, preconditioner="sector",
+                                    criterion="backward")
+        t_pl = time.perf_counter() - t0
+        out["plain_sector_gmres"] = {
+            "cpu_s": t_pl, "iterations": int(info2["iterations"]), "device_ms": plain_device_ms,
+            "speedup_device_vs_cpu": t_pl / (plain_device_ms * 1e-3),
+            "sample": f"one full solve, the same algorithm on both sides, {threads} threads"}
+        print(f"# cpu plain sector GMRES: {t_pl:.1f} s, {info2['iterations']} steps", file=sys.stderr, flush=True)
+    return out
+
+
+def reference_algorithm_c1(pimi):
+    """C1 (driven_qubits(8), PI dim 165) with the reference's own algorithms on
+    both sides: GMRES(30) with two Gram-Schmidt passes to rtol 1e-12 (atol 0)
+    and the expv dynamics over t in [0, 4], saveat 0.1, Krylov 30, rtol 1e-12;
+    the oracle on one thread and on all host threads, the device through
+    libpimi; each timed warm over `reps` repetitions."""
     from oracle import lowering as L
     from oracle import oracle as O
     threads = min(os.cpu_count() or 1, 64)
-    sp = L.spin_matrices(2)
-    ref, _ = O.compile_plan(N_QUBITS, 2, L.qubit_ensemble_terms(
-        hamiltonian=sp["jz"] + sp["jx"], emission=0.1, dephasing=0.2, pumping=0.02, collective_emission=0.1))
-    scale = ref.operator_scale(threads)
-    ref.sector_preconditioner()
+    model = pimi.Models.driven_qubits(8, drive=1.0, detuning=1.0, emission=0.1, dephasing=0.2, pumping=0.0)
+    comp = pimi.compile(model)
+    ref, g = O.compile_plan(8, 2, L.driven_qubits_terms(drive=1.0, detuning=1.0, emission=0.1, dephasing=0.2,
+                                                        pumping=0.0))
+    sp = pimi.spin_matrices(2)
+    rho0 = pimi.computational_product_state(comp.basis, 2)
+    obs = L.collective_operator(g, L.spin_matrices(2)["jp"] @ L.spin_matrices(2)["jm"])
+    ts = pimi.saved_times((0.0, 4.0), 0.1)
+    alg_s = pimi.GMRESAlgorithm(30, 5000)
+    alg_d = pimi.ExpvAlgorithm(30, 0.0, 1e-12)
 
-    def run(k):
+    def timed(fn, reps):
+        fn()
         t0 = time.perf_counter()
-        ref.steady_state(krylovdim=k, maxiter=k, atol=0.0, rtol=RTOL, operator_scale=scale, nthreads=threads,
-                         preconditioner="sector", criterion="backward")
-        return time.perf_counter() - t0
+        for _ in range(reps):
+            r = fn()
+        return (time.perf_counter() - t0) / reps * 1e3, r
 
-    k1 = 4
-    t1 = run(k1)
-    k2 = int(max(8, min(200, budget_s / max(t1 / k1, 1e-3) / 2)))
-    t2 = run(k2)
-    b = max((t2 / k2 - t1 / k1) / (k2 - k1), 0.0)
-    a = max(t1 / k1 - b * k1, 0.0)
-    est = a * steady_iters + b * steady_iters ** 2
-    return {"value": 1.0 / est, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": (f"same preconditioned GMRES at N=120 on {threads} OpenMP threads: {k1} steps in {t1:.2f} s and "
-                       f"{k2} steps in {t2:.2f} s, fit t(k)={a * 1e3:.1f} ms*k + {b * 1e3:.3f} ms*k^2, evaluated at "
-                       f"the plain device solve's {steady_iters} steps = {est:.1f} s per solve")}
+    out = {"workload": "driven_qubits(8) (PI dim 165): GMRES(30), MGS2 (CGS2 on the device), rtol 1e-12; expv "
+                       "t in [0,4] saveat 0.1, Krylov 30, rtol 1e-12"}
+    ms, r = timed(lambda: pimi.stationary_state(comp, algorithm=alg_s, atol=0.0, rtol=1e-12, return_info=True), 5)
+    out["steady_gmres30"] = {"device_ms": ms, "iterations": int(r.info["iterations"])}
+    for nt in (1, threads):
+        ms_c, (_, inf) = timed(lambda: ref.steady_state(krylovdim=30, maxiter=5000, atol=0.0, rtol=1e-12,
+                                                          nthreads=nt), 3)
+        out["steady_gmres30"][f"cpu_ms_{nt}_threads"] = ms_c
+        out["steady_gmres30"]["cpu_iterations"] = int(inf["iterations"])
+    ms, r = timed(lambda: pimi.solve_dynamics(comp, rho0, (0.0, 4.0), saveat=0.1, algorithm=alg_d,
+                                              observables={"n": sp.jp @ sp.jm}), 5)
+    out["expv_dynamics"] = {"device_ms": ms, "applications": int(r.info["applications"])}
+    for nt in (1, threads):
+        ms_c, _ = timed(lambda: ref.solve_dynamics(rho0.data, ts, [obs], krylovdim=30, atol=0.0, rtol=1e-12,
+                                                   nthreads=nt), 3)
+        out["expv_dynamics"][f"cpu_ms_{nt}_threads"] = ms_c
+    out["cores"] = threads
+    print(f"# C1 reference algorithms: {out}", file=sys.stderr, flush=True)
+    return out
 
 
 def batched(pimi, args):
@@ -310,22 +373,23 @@ def dynamics_n120(pimi, comp, cpu=True):
            "accepted_steps": int(res.info["accepted"]), "rejected_steps": int(res.info["rejected"]),
            "n_final": float(res.values["n"][-1].real)}
     if cpu:
-        from oracle import lowering as L
-        from oracle import oracle as O
+        # the same first 10 save intervals (t in [0, 1]) timed on both sides; no scaling
         threads = min(os.cpu_count() or 1, 64)
-        lsp = L.spin_matrices(2)
-        ref, _ = O.compile_plan(N_QUBITS, 2, L.qubit_ensemble_terms(
-            hamiltonian=lsp["jz"] + lsp["jx"], emission=0.1, dephasing=0.2, pumping=0.02, collective_emission=0.1))
-        ts = np.array([0.0, 0.1, 0.2])
+        ref, _ = _oracle_c2()
         t0 = time.perf_counter()
-        ref.solve_dynamics(rho0.data, ts, [comp.collective_operator(sp.jp @ sp.jm).data], krylovdim=40, atol=0.0,
-                           rtol=1e-12, nthreads=threads)
-        t2 = time.perf_counter() - t0
-        est = t2 / 2 * 100
-        out["cpu_baseline"] = {"value_s": est, "cores": threads, "kind": "port",
-                               "sample": f"oracle OpenMP expv, first 2 of 100 save intervals in {t2:.2f} s on "
-                                         f"{threads} threads, scaled to 100 intervals"}
-        out["speedup_vs_cpu"] = est / secs
+        dev10 = pimi.solve_dynamics(comp, rho0, (0.0, 1.0), saveat=0.1, algorithm=alg, observables=obs)
+        dev_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cpu_out, _, _ = ref.solve_dynamics(rho0.data, np.array(dev10.t), [comp.collective_operator(sp.jp @ sp.jm).data],
+                                           krylovdim=40, atol=0.0, rtol=1e-12, nthreads=threads)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"cpu_s": cpu_s, "device_s": dev_s, "cores": threads, "kind": "port",
+                               "speedup_device_vs_cpu": cpu_s / dev_s,
+                               "max_rel_obs_diff": float(np.abs(cpu_out[:, 0] - dev10.values["n"]).max()
+                                                         / np.abs(cpu_out[:, 0]).max()),
+                               "sample": f"the first 10 of the 100 save intervals (t in [0, 1]) in full on both "
+                                         f"sides: oracle OpenMP expv on {threads} threads {cpu_s:.1f} s, device "
+                                         f"{dev_s * 1e3:.0f} ms"}
     print(f"# dynamics N=120: {out['ms']:.0f} ms, {out['applications']} applications", file=sys.stderr, flush=True)
     return out
 
@@ -355,7 +419,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-side", action="store_true", help="skip the plain and FP32-inner side solves")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-plain", action="store_true",
+                    help="skip the full CPU solve by plain sector-preconditioned GMRES (the longest CPU leg)")
     ap.add_argument("--profile", action="store_true",
                     help="apply, preconditioner and copy calibration only (counter runs)")
     args = ap.parse_args()
@@ -446,9 +511,10 @@ def main():
     serve = None if args.no_serve else serving(pimi)
     precision = None if args.no_batch else precision_form(pimi)
     dynamics = dynamics_n120(pimi, comp, cpu=not args.no_cpu)
-    cpu = None
+    cpu = c1 = None
     if not args.no_cpu:
-        cpu = cpu_baseline(side["plain_fp64"]["iterations"], args.cpu_seconds)
+        c1 = reference_algorithm_c1(pimi)
+        cpu = cpu_baseline(float(np.median(lat)) * 1e3, side["plain_fp64"]["ms"], full_plain=not args.no_cpu_plain)
     lat_ms = np.array(lat) * 1e3
     spmv = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
             "single_n120_fp64": {"achieved": achieved_apply, "frac": achieved_apply / HBM_PEAK_GBS,
@@ -494,6 +560,7 @@ def main():
         "precision_form": precision,
         "dynamics_n120": dynamics,
         "serving": serve,
+        "reference_algorithm_c1": c1,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -159,6 +159,13 @@ class OraclePlan:
     def n(self):
         return self.basis.n
 
+    def apply_diag(self, x, nthreads=1):
+        """y = D x, D the sector-diagonal part of L (the inner solver's operator)."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.empty_like(x)
+        lib().oracle_apply_diag(C.byref(self._P), _ptr(x), _ptr(y), C.c_int(nthreads))
+        return y
+
     def apply(self, x, nthreads=1):
         x = np.ascontiguousarray(x, dtype=np.complex128)
         y = np.empty_like(x)
@@ -204,8 +211,13 @@ class OraclePlan:
         return sig
 
     def steady_state(self, krylovdim=30, maxiter=500, atol=0.0, rtol=1e-10, operator_scale=0.0, nthreads=1,
-                     preconditioner=None, criterion="reference", weighting=None):
-        """weighting="sector": iterate on u = S x with S = sqrt(f^nu) per sector."""
+                     preconditioner=None, criterion="reference", weighting=None, inner_steps=0):
+        """weighting="sector": iterate on u = S x with S = sqrt(f^nu) per sector.
+        preconditioner="sector_inner": flexible GMRES whose directions are
+        `inner_steps` (default 8) sector-preconditioned GMRES steps on the
+        sector-diagonal part of L."""
+        if preconditioner == "sector_inner":
+            preconditioner, inner_steps = "sector", (inner_steps or 8)
         x = np.zeros(self.n, dtype=np.complex128)
         sig = None
         if weighting == "sector":
@@ -225,7 +237,7 @@ class OraclePlan:
                                   C.c_int64(len(self.t_idx)), C.c_int(krylovdim), C.c_int64(maxiter),
                                   C.c_double(atol), C.c_double(rtol), C.c_double(operator_scale), _ptr(x),
                                   C.byref(info), pu, pi, C.c_int(1 if criterion == "backward" else 0),
-                                  None if sig is None else _ptr(sig))
+                                  None if sig is None else _ptr(sig), C.c_int(int(inner_steps)))
         return x, {k: getattr(info, k) for k, _ in _SteadyInfo._fields_}
 
     def expv(self, b, t, krylovdim=30, atol=1e-10, rtol=1e-8, safety=0.9, max_steps=10000, nthreads=1):

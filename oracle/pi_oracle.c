@@ -13,6 +13,8 @@
  *   threaded apply         src/threaded_apply.jl:58-84, 506-549, 564-609
  *   GMRES(m) with MGS2     src/krylov.jl:710-786
  *   krylov_steady_state    src/krylov.jl:156-169, 797-891
+ *   (plus the product's sector preconditioner and inner-outer flexible GMRES,
+ *   which have no counterpart in the reference)
  *   krylov_expv!           src/krylov_extensions.jl:1346-1568
  *   solve_dynamics (expv)  src/highlevel.jl:489-525, 614-733
  *
@@ -256,12 +258,84 @@ static void do_apply(const oplan *P, const cplx *x, cplx *y, int nthreads) {
     else oracle_apply(P, x, y);
 }
 
-/* ---------------- BLAS-1 ---------------- */
+/* y = D x, D the sector-diagonal part of L: every term whose input and output
+ * sector coincide (Hamiltonian, loss, collective gains, and the one-body
+ * branches with br_in == br_out), in the order of oracle_apply. */
+void oracle_apply_diag(const oplan *P, const cplx *x, cplx *y, int nthreads) {
+    int workers = nthreads < P->ns ? nthreads : P->ns;
+    if (workers < 1) workers = 1;
+    int *owner = malloc(sizeof(int) * P->ns);
+    if (workers > 1) lpt_assign(P, workers, owner);
+    else for (int s = 0; s < P->ns; ++s) owner[s] = 0;
+    memset(y, 0, sizeof(cplx) * P->n);
+#pragma omp parallel num_threads(workers)
+    {
+        int wid = 0;
+#ifdef _OPENMP
+        wid = omp_get_thread_num();
+#endif
+        sector_scratch w;
+        alloc_scratch(&w, P->max_dim);
+        for (int s = 0; s < P->ns; ++s) {
+            if (owner[s] != wid) continue;
+            if (P->has_h) apply_h_sector(P, s, x, y, &w);
+            for (int g = 0; g < P->ncg; ++g) apply_cg_sector(P, g, s, x, y, &w);
+            if (P->has_q) apply_q_sector(P, s, x, y, &w);
+            for (int g = 0; g < P->nob; ++g)
+                for (int64_t br = P->ob_br_base[g]; br < P->ob_br_base[g + 1]; ++br)
+                    if (P->br_out[br] == s && P->br_in[br] == s)
+                        apply_branch(P, br, P->ob_scale[g] * P->br_r[br], x, y, &w);
+        }
+        free_scratch(&w);
+    }
+    free(owner);
+}
+
+/* ---------------- BLAS-1 ----------------
+ * The solvers' vector operations run on g_blas_threads OpenMP threads (set by
+ * each solver entry point from its nthreads), as the reference's threaded
+ * CPU path runs BLAS-1 over multi-threaded BLAS.  Dots are summed over a
+ * fixed chunking and the partial sums added in chunk order, so a result
+ * depends only on the thread count. */
+static int g_blas_threads = 1;
 
 static cplx zdot(const cplx *a, const cplx *b, int64_t n) {  /* sum conj(a) b */
+    int nt = g_blas_threads;
+    if (nt <= 1 || n < 65536) {
+        cplx s = 0;
+        for (int64_t i = 0; i < n; ++i) s += conj(a[i]) * b[i];
+        return s;
+    }
+    double re[256], im[256];
+    if (nt > 256) nt = 256;
+    int64_t chunk = (n + nt - 1) / nt;
+#pragma omp parallel for schedule(static) num_threads(nt)
+    for (int c = 0; c < nt; ++c) {
+        cplx s = 0;
+        int64_t i1 = (int64_t)(c + 1) * chunk < n ? (int64_t)(c + 1) * chunk : n;
+        for (int64_t i = (int64_t)c * chunk; i < i1; ++i) s += conj(a[i]) * b[i];
+        re[c] = creal(s);
+        im[c] = cimag(s);
+    }
     cplx s = 0;
-    for (int64_t i = 0; i < n; ++i) s += conj(a[i]) * b[i];
+    for (int c = 0; c < nt; ++c) s += re[c] + I * im[c];
     return s;
+}
+
+/* w -= h v */
+static void zaxpy_neg(cplx *w, cplx h, const cplx *v, int64_t n) {
+#pragma omp parallel for schedule(static) num_threads(g_blas_threads) if (n >= 65536)
+    for (int64_t k = 0; k < n; ++k) w[k] = w[k] - h * v[k];
+}
+
+/* out = V[:, 0:k] y (columns of length n) */
+static void zgemv_cols(cplx *out, const cplx *V, int64_t n, int k, const cplx *y) {
+#pragma omp parallel for schedule(static) num_threads(g_blas_threads) if (n >= 65536)
+    for (int64_t t = 0; t < n; ++t) {
+        cplx zz = 0;
+        for (int jj = 0; jj < k; ++jj) zz += V[(int64_t)jj * n + t] * y[jj];
+        out[t] = zz;
+    }
 }
 static double dznrm2(const cplx *a, int64_t n) {
     /* scaled 2-norm (LAPACK-style) so that tiny or huge vectors do not
@@ -426,11 +500,7 @@ static double iterate_norm(steady_op *op, const cplx *x, const cplx *V, const cp
         for (int jj = i + 1; jj < k; ++jj) acc -= H[i + jj * (m + 1)] * yv[jj];
         yv[i] = acc / H[i + i * (m + 1)];
     }
-    for (int64_t t = 0; t < n; ++t) {
-        cplx zz = 0;
-        for (int jj = 0; jj < k; ++jj) zz += V[(int64_t)jj * n + t] * yv[jj];
-        tmp[t] = zz;
-    }
+    zgemv_cols(tmp, V, n, k, yv);
     if (pc) pc_apply(op->P, pc, tmp, z, op->nthreads);
     else memcpy(z, tmp, sizeof(cplx) * n);
     for (int64_t t = 0; t < n; ++t) z[t] += x[t];
@@ -478,8 +548,7 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
                 for (int i = 0; i <= j; ++i) {
                     cplx hij = zdot(V + (int64_t)i * n, w, n);
                     Hij(i, j) += hij;
-                    const cplx *vi = V + (int64_t)i * n;
-                    for (int64_t k = 0; k < n; ++k) w[k] = w[k] - hij * vi[k];
+                    zaxpy_neg(w, hij, V + (int64_t)i * n, n);
                 }
             Hij(j + 1, j) = dznrm2(w, n);
             if (Hij(j + 1, j) != 0)
@@ -512,11 +581,7 @@ static int gmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, 
             for (int jj = i + 1; jj < kdone; ++jj) sacc -= Hij(i, jj) * yv[jj];
             yv[i] = sacc / Hij(i, i);
         }
-        for (int64_t k = 0; k < n; ++k) {
-            cplx zz = 0;
-            for (int jj = 0; jj < kdone; ++jj) zz += V[(int64_t)jj * n + k] * yv[jj];
-            w[k] = zz;
-        }
+        zgemv_cols(w, V, n, kdone, yv);
         if (pc) {
             pc_apply(op->P, pc, w, z, op->nthreads);
             for (int64_t k = 0; k < n; ++k) x[k] += z[k];
@@ -536,13 +601,179 @@ done:
     return conv;
 }
 
+/* ---------------- inner-outer flexible GMRES ----------------
+ * The product's PIMI_PRECOND_SECTOR_INNER solver (no counterpart in the
+ * reference, whose GMRES takes a fixed preconditioner): flexible GMRES on the
+ * bordered A = L/s + v t^H (modified Gram-Schmidt twice here; classical on the
+ * device) whose direction z_j = inner(v_j) is `kin` steps of
+ * sector-preconditioned GMRES on D/s from zero, D the sector-diagonal part of
+ * L (oracle_apply_diag), with one classical Gram-Schmidt pass per inner step
+ * and the inner least-squares problem solved by Givens rotations:
+ * inner(r) = M^{-1} V_i y, y = argmin |beta e1 - H_i y|, beta = |r|.  The
+ * stopping rules are those of gmres() above (the iterate norm refreshed every
+ * 5 outer steps under the backward rule, as the device does). */
+static void inner_solve(const steady_op *op, const sector_pc *pc, int kin, const cplx *r, cplx *z, cplx *Vi,
+                        cplx *t1, cplx *t2) {
+    const oplan *P = op->P;
+    int64_t n = P->n;
+    double beta = dznrm2(r, n);
+    if (beta == 0) { memset(z, 0, sizeof(cplx) * n); return; }
+    cplx *Hm = calloc((size_t)(kin + 1) * kin, sizeof(cplx));
+    cplx *h = malloc(sizeof(cplx) * (kin + 1));
+    for (int64_t t = 0; t < n; ++t) Vi[t] = r[t] / beta;
+    int kk = 0;
+    for (int i = 0; i < kin; ++i) {
+        pc_apply(P, pc, Vi + (int64_t)i * n, t1, op->nthreads);
+        oracle_apply_diag(P, t1, t2, op->nthreads);
+        for (int64_t t = 0; t < n; ++t) t2[t] *= op->invscale;
+        for (int k = 0; k <= i; ++k) h[k] = zdot(Vi + (int64_t)k * n, t2, n);   /* one classical pass */
+        for (int k = 0; k <= i; ++k) {
+            Hm[k + i * (kin + 1)] = h[k];
+            zaxpy_neg(t2, h[k], Vi + (int64_t)k * n, n);
+        }
+        double hn = dznrm2(t2, n);
+        Hm[i + 1 + i * (kin + 1)] = hn;
+        kk = i + 1;
+        if (hn == 0) break;
+        for (int64_t t = 0; t < n; ++t) Vi[(int64_t)(i + 1) * n + t] = t2[t] / hn;
+    }
+    /* least squares min |beta e1 - H y| by Givens rotations */
+    cplx *g = calloc(kk + 1, sizeof(cplx)), *y = malloc(sizeof(cplx) * (kk > 0 ? kk : 1));
+    g[0] = beta;
+    {
+        double *cs = malloc(sizeof(double) * (kk > 0 ? kk : 1));
+        cplx *sn = malloc(sizeof(cplx) * (kk > 0 ? kk : 1));
+        for (int j = 0; j < kk; ++j) {
+#define HM(i, jj) Hm[(i) + (jj) * (kin + 1)]
+            for (int i = 0; i < j; ++i) {
+                cplx h1 = HM(i, j), h2 = HM(i + 1, j);
+                HM(i, j) = cs[i] * h1 + sn[i] * h2;
+                HM(i + 1, j) = -conj(sn[i]) * h1 + cs[i] * h2;
+            }
+            double c; cplx sg;
+            complex_givens(HM(j, j), HM(j + 1, j), &c, &sg);
+            cs[j] = c; sn[j] = sg;
+            cplx h1 = HM(j, j), h2 = HM(j + 1, j);
+            HM(j, j) = c * h1 + sg * h2;
+            HM(j + 1, j) = 0;
+            cplx g1 = g[j], g2 = g[j + 1];
+            g[j] = c * g1 + sg * g2;
+            g[j + 1] = -conj(sg) * g1 + c * g2;
+        }
+        for (int i = kk - 1; i >= 0; --i) {
+            cplx acc = g[i];
+            for (int jj = i + 1; jj < kk; ++jj) acc -= HM(i, jj) * y[jj];
+            y[i] = acc / HM(i, i);
+        }
+#undef HM
+        free(cs); free(sn);
+    }
+    zgemv_cols(t2, Vi, n, kk, y);
+    pc_apply(P, pc, t2, z, op->nthreads);
+    free(Hm); free(h); free(g); free(y);
+}
+
+static int fgmres(steady_op *op, cplx *x, const cplx *b, int m, int64_t maxiter, double atol, double rtol,
+                  int64_t *iters_out, int64_t *restarts_out, double *res_out, const sector_pc *pc, int backward,
+                  int kin) {
+    int64_t n = op->P->n;
+    cplx *V = malloc(sizeof(cplx) * n * (m + 1)), *Z = malloc(sizeof(cplx) * n * m);
+    cplx *Vi = malloc(sizeof(cplx) * n * (kin + 1));
+    cplx *H = malloc(sizeof(cplx) * (m + 1) * m);
+    cplx *g = malloc(sizeof(cplx) * (m + 1));
+    cplx *sn = malloc(sizeof(cplx) * m), *yv = malloc(sizeof(cplx) * m);
+    double *cs = malloc(sizeof(double) * m);
+    cplx *w = malloc(sizeof(cplx) * n), *r = malloc(sizeof(cplx) * n);
+    cplx *t1 = malloc(sizeof(cplx) * n), *t2 = malloc(sizeof(cplx) * n);
+#define Hij(i, j) H[(i) + (j) * (m + 1)]
+    double bnorm = dznrm2(b, n);
+    double xnorm = dznrm2(x, n);
+    double tol = backward ? 0.5 * (atol + rtol * xnorm) : atol + rtol * bnorm;
+    int64_t iterations = 0, restarts = 0;
+    steady_apply(op, x, w);
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i] - w[i];
+    double beta = dznrm2(r, n);
+    int conv = 0;
+    if (beta <= tol) { conv = 1; goto done; }
+    while (iterations < maxiter) {
+        memset(H, 0, sizeof(cplx) * (m + 1) * m);
+        memset(g, 0, sizeof(cplx) * (m + 1));
+        for (int64_t i = 0; i < n; ++i) V[i] = r[i] / beta;
+        g[0] = beta;
+        int kdone = 0;
+        int jmax = (int)((maxiter - iterations) < m ? (maxiter - iterations) : m);
+        for (int j = 0; j < jmax; ++j) {
+            inner_solve(op, pc, kin, V + (int64_t)j * n, Z + (int64_t)j * n, Vi, t1, t2);
+            op->applications += kin;
+            steady_apply(op, Z + (int64_t)j * n, w);
+            for (int pass = 0; pass < 2; ++pass)
+                for (int i = 0; i <= j; ++i) {
+                    cplx hij = zdot(V + (int64_t)i * n, w, n);
+                    Hij(i, j) += hij;
+                    zaxpy_neg(w, hij, V + (int64_t)i * n, n);
+                }
+            Hij(j + 1, j) = dznrm2(w, n);
+            if (Hij(j + 1, j) != 0)
+                for (int64_t k = 0; k < n; ++k) V[(int64_t)(j + 1) * n + k] = w[k] / Hij(j + 1, j);
+            for (int i = 0; i < j; ++i) {
+                cplx h1 = Hij(i, j), h2 = Hij(i + 1, j);
+                double c = cs[i]; cplx s = sn[i];
+                Hij(i, j) = c * h1 + s * h2;
+                Hij(i + 1, j) = -conj(s) * h1 + c * h2;
+            }
+            double c; cplx s;
+            complex_givens(Hij(j, j), Hij(j + 1, j), &c, &s);
+            cs[j] = c; sn[j] = s;
+            cplx h1 = Hij(j, j), h2 = Hij(j + 1, j);
+            Hij(j, j) = c * h1 + s * h2;
+            Hij(j + 1, j) = 0;
+            cplx g1 = g[j], g2 = g[j + 1];
+            g[j] = c * g1 + s * g2;
+            g[j + 1] = -conj(s) * g1 + c * g2;
+            iterations++;
+            kdone = j + 1;
+            if (backward && (kdone % 5 == 0 || cabs(g[j + 1]) <= tol) && Hij(j, j) != 0) {
+                for (int i = kdone - 1; i >= 0; --i) {
+                    cplx sacc = g[i];
+                    for (int jj = i + 1; jj < kdone; ++jj) sacc -= Hij(i, jj) * yv[jj];
+                    yv[i] = sacc / Hij(i, i);
+                }
+                zgemv_cols(t1, Z, n, kdone, yv);
+                for (int64_t k = 0; k < n; ++k) t1[k] += x[k];
+                xnorm = dznrm2(t1, n);
+                tol = 0.5 * (atol + rtol * xnorm);
+            }
+            if (cabs(g[j + 1]) <= tol) break;
+        }
+        for (int i = kdone - 1; i >= 0; --i) {
+            cplx sacc = g[i];
+            for (int jj = i + 1; jj < kdone; ++jj) sacc -= Hij(i, jj) * yv[jj];
+            yv[i] = sacc / Hij(i, i);
+        }
+        zgemv_cols(w, Z, n, kdone, yv);
+        for (int64_t k = 0; k < n; ++k) x[k] += w[k];
+        restarts++;
+        steady_apply(op, x, w);
+        for (int64_t i = 0; i < n; ++i) r[i] = b[i] - w[i];
+        beta = dznrm2(r, n);
+        if (backward) { xnorm = dznrm2(x, n); tol = 0.5 * (atol + rtol * xnorm); }
+        if (beta <= tol) { conv = 1; break; }
+    }
+done:
+#undef Hij
+    *iters_out = iterations; *restarts_out = restarts; *res_out = beta;
+    free(V); free(Z); free(Vi); free(H); free(g); free(sn); free(yv); free(cs); free(w); free(r); free(t1); free(t2);
+    return conv;
+}
+
 /* krylov_steady_state (src/krylov.jl:797-891) with x0 = v = t/(t.t).
  * operator_scale <= 0 selects the deterministic probe estimate. */
 int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const double *tvals, int64_t nt,
                         int krylovdim, int64_t maxiter, double atol, double rtol, double operator_scale,
                         cplx *x_out, steady_info *info, const cplx *pc_U, const cplx *pc_inv, int backward,
-                        const double *sigma) {
+                        const double *sigma, int inner_steps) {
     int64_t n = P->n;
+    g_blas_threads = nthreads > 0 ? nthreads : 1;
     cplx *tv = malloc(sizeof(cplx) * nt), *vv = malloc(sizeof(cplx) * nt);
     cplx *tw = malloc(sizeof(cplx) * nt), *vw = malloc(sizeof(cplx) * nt);
     cplx tt = 0;
@@ -569,8 +800,12 @@ int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const
     cplx *lw = sigma ? malloc(sizeof(cplx) * n) : NULL;
     for (int pass = 0;; ++pass) {
         int64_t it1, rs1;
-        conv = gmres(&op, x_out, b, krylovdim, maxiter - it, atol * tight, rtol * tight, &it1, &rs1, &res,
-                     pc_U ? &pc : NULL, backward);
+        if (inner_steps > 0 && pc_U)
+            conv = fgmres(&op, x_out, b, krylovdim, maxiter - it, atol * tight, rtol * tight, &it1, &rs1, &res,
+                          &pc, backward, inner_steps);
+        else
+            conv = gmres(&op, x_out, b, krylovdim, maxiter - it, atol * tight, rtol * tight, &it1, &rs1, &res,
+                         pc_U ? &pc : NULL, backward);
         it += it1; rs += rs1;
         if (!sigma || !conv || pass >= 4 || it >= maxiter) break;
         for (int64_t i = 0; i < n; ++i) scr[i] = x_out[i] / sigma[i];
@@ -748,8 +983,7 @@ static void arnoldi_factorization(expv_ws *W, const oplan *P, int nthreads) {
             for (int i = 0; i <= j; ++i) {
                 cplx a = zdot(V + (int64_t)i * n, W->w, n);
                 HH(i, j) += a;
-                const cplx *vi = V + (int64_t)i * n;
-                for (int64_t t = 0; t < n; ++t) W->w[t] = W->w[t] - a * vi[t];
+                zaxpy_neg(W->w, a, V + (int64_t)i * n, n);
             }
         HH(j + 1, j) = dznrm2(W->w, n);
         k = j + 1;
@@ -772,8 +1006,7 @@ static void arnoldi_factorization(expv_ws *W, const oplan *P, int nthreads) {
             for (int i = 0; i < q; ++i) {
                 cplx a = zdot(V + (int64_t)i * n, W->w, n);
                 SS(i, q - 1) += a;
-                const cplx *vi = V + (int64_t)i * n;
-                for (int64_t t = 0; t < n; ++t) W->w[t] = W->w[t] - a * vi[t];
+                zaxpy_neg(W->w, a, V + (int64_t)i * n, n);
             }
         next_remainder = dznrm2(W->w, n);
     }
@@ -794,6 +1027,7 @@ static double project_factorization(expv_ws *W, double h) {
         for (int i = 0; i < q; ++i) S[i + j * q] = h * W->small[i + j * (m + 1)];
     oracle_expm(q, S, E);
     int cc = W->coeff_count;
+#pragma omp parallel for schedule(static) num_threads(g_blas_threads) if (n >= 65536)
     for (int64_t t = 0; t < n; ++t) {
         cplx z = 0;
         for (int j = 0; j < cc; ++j) z += W->V[(int64_t)j * n + t] * E[j];
@@ -809,6 +1043,7 @@ static double project_factorization(expv_ws *W, double h) {
 
 int oracle_expv(const oplan *P, int nthreads, cplx *y, const cplx *b, double t, int krylovdim,
                 double atol, double rtol, double safety, int64_t max_steps, expv_info *info) {
+    g_blas_threads = nthreads > 0 ? nthreads : 1;
     int64_t n = P->n;
     int m = krylovdim < n ? krylovdim : (int)n;
     expv_ws W;

@@ -697,3 +697,23 @@ def test_solve_dynamics_save_states():
         err = max(err, np.linalg.norm(saved.states[k].data - rho_t) / np.linalg.norm(rho_t))
     parity_log.record("expv_c1_saved_states", err, 1e-12)
     assert err <= 1e-12
+
+
+@pytest.mark.parametrize("N", [16, 40])
+def test_flexible_gmres_against_oracle_flexible_gmres(N):
+    """The headline solver (flexible GMRES, FP64 sector-preconditioned inner
+    solves on the sector-diagonal part of L) against the oracle's restatement
+    of the same algorithm: the same number of outer steps (within one) and the
+    same state."""
+    print(f"flexible GMRES vs oracle N={N}", flush=True)
+    comp = pimi.compile(_c2_model(N))
+    ref, _ = O.compile_plan(N, 2, _c2_terms())
+    alg = pimi.GMRESAlgorithm(200, 400, "sector_inner", inner_steps=8)
+    res = pimi.stationary_state(comp, algorithm=alg, atol=0.0, rtol=1e-13, return_info=True, criterion="backward")
+    x_ref, info_ref = ref.steady_state(krylovdim=200, maxiter=400, atol=0.0, rtol=1e-13,
+                                       preconditioner="sector_inner", inner_steps=8, criterion="backward", nthreads=4)
+    assert res.info["converged"] and info_ref["converged"]
+    assert abs(res.info["iterations"] - info_ref["iterations"]) <= 1, (res.info["iterations"], info_ref["iterations"])
+    err = np.linalg.norm(res.state.data - x_ref) / np.linalg.norm(x_ref)
+    parity_log.record(f"fgmres_vs_oracle_fgmres_N{N}", err, 1e-10)
+    assert err <= 1e-10, err

@@ -450,3 +450,44 @@ def test_sector_preconditioner_approximates_drive_and_is_nonsingular():
         X = rng.standard_normal(m * m) + 1j * rng.standard_normal(m * m)
         errs.append(np.linalg.norm(Ppc @ (A @ X) - X) / np.linalg.norm(X))
     assert max(errs) < 0.3
+
+
+def test_sector_diagonal_apply_is_the_block_diagonal_of_L():
+    """oracle_apply_diag (the inner solver's D) equals the sector-block
+    diagonal of the dense L built from the full apply (itself pinned by brute
+    force): each sector's m^2 x m^2 diagonal block, and nothing else."""
+    P, _ = _c2(7)
+    M = O.dense_liouvillian(P)
+    b = P.basis
+    D = np.zeros_like(M)
+    for s in range(len(b.dims)):
+        o0, o1 = b.offsets[s], b.offsets[s + 1]
+        D[o0:o1, o0:o1] = M[o0:o1, o0:o1]
+    rng = np.random.default_rng(11)
+    for _ in range(3):
+        x = rng.standard_normal(P.n) + 1j * rng.standard_normal(P.n)
+        y = P.apply_diag(x, nthreads=3)
+        assert np.linalg.norm(y - D @ x) <= 1e-13 * np.linalg.norm(D @ x)
+    assert np.linalg.norm(M - D) > 0.02 * np.linalg.norm(M)   # the off-diagonal gains (4 % here) are left out
+
+
+@pytest.mark.parametrize("N,backward", [(12, True), (24, True), (8, False)])
+def test_oracle_flexible_gmres_reaches_the_preconditioned_solution(N, backward):
+    """The oracle's inner-outer flexible GMRES (the product's sector_inner
+    solver) reaches the state of plain sector-preconditioned GMRES, in fewer
+    outer steps, and the dense null space at small N."""
+    P, _ = _c2(N)
+    crit = "backward" if backward else "reference"
+    x1, i1 = P.steady_state(krylovdim=400, maxiter=400, atol=0.0, rtol=1e-13, preconditioner="sector",
+                            criterion=crit, nthreads=4)
+    x2, i2 = P.steady_state(krylovdim=200, maxiter=400, atol=0.0, rtol=1e-13, preconditioner="sector_inner",
+                            inner_steps=6, criterion=crit, nthreads=4)
+    assert i1["converged"] and i2["converged"]
+    assert i2["iterations"] < i1["iterations"]
+    assert np.linalg.norm(x2 - x1) <= 1e-10 * np.linalg.norm(x1)
+    if N <= 8:
+        M = O.dense_liouvillian(P)
+        w, V = np.linalg.eig(M)
+        v = V[:, np.argmin(np.abs(w))]
+        v = v / np.dot(P.t_val, v[P.t_idx])
+        assert np.linalg.norm(x2 - v) <= 1e-10 * np.linalg.norm(v)
