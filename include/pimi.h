@@ -303,6 +303,22 @@ pimi_status pimi_trajectory_channels(const pimi_plan *plan, int32_t *count);
  * E = sum_k rate_k L_k^dagger L_k, intensities[c] = lambda of column c. */
 pimi_status pimi_conditional_action(pimi_workspace *ws, const void *X, void *Y, int64_t ncols, double *intensities,
                                     int32_t on_device);
+/* Allocation-free variant of pimi_conditional_action, for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops
+ * that forbid hipMalloc).  No counterpart in the reference.  X, Y: device
+ * pointers (n x ncols complex double, column-major); intensities: device
+ * pointer to ncols doubles, receiving max(0, lambda_c) as the host array of
+ * pimi_conditional_action does.  The call allocates nothing: the intensity
+ * reduction uses a fixed device array of the library (one per GPU, held
+ * until the process ends, about 1 MB), so ncols is limited to 1..256
+ * (PIMI_EINVAL beyond).  The plan's trajectory tables must already exist
+ * (they are built, with allocation, by the first pimi_conditional_action,
+ * pimi_conditional_rk4, pimi_channel_intensities or pimi_apply_jumps call
+ * on the plan); otherwise PIMI_EINVAL.  The same sign check as
+ * pimi_conditional_action gives PIMI_EINVAL.  Calls that overlap on one GPU
+ * share the array and race; the call returns after its work completes. */
+pimi_status pimi_conditional_action_noalloc(pimi_workspace *ws, const void *X, void *Y, int64_t ncols,
+                                            double *intensities);
 /* batched_conditional_rk4!: one fixed RK4 step of size h for every column,
  * then each column trace-normalized (X in place). */
 pimi_status pimi_conditional_rk4(pimi_workspace *ws, void *X, int64_t ncols, double h, int32_t on_device);

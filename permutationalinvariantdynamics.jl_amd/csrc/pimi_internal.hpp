@@ -342,6 +342,10 @@ struct DevStencil {
 
 struct pimi_context {
     int device = 0;
+    // device address of the library's fixed scratch array on this device
+    // (hipGetSymbolAddress at creation); used by the allocation-free entry
+    // points only
+    void *noalloc_scratch = nullptr;
 };
 
 struct pimi_plan {

@@ -2089,24 +2089,62 @@ static void launch_stencil(pimi_workspace *W, const DevStencil &S, const cx<doub
 }
 
 // h[c] = sum_i conj(X[i, c]) ebar[i] = conj(sum_i X e), whose real part is the intensity
+// (partial: kDotBlocks x ncols reduction partials)
 static void cols_intensity(pimi_workspace *W, const cx<double> *X, int64_t ncols, const cx<double> *ebar,
-                           cx<double> *h) {
+                           cx<double> *h, cx<double> *partial) {
     const int64_t n = W->plan->n;
     hipLaunchKernelGGL(k_dots_partial<double>, dim3(kDotBlocks, (unsigned)ncols), dim3(kRedBlock), 0, W->stream, X,
-                       n, ebar, n, W->tPartial.p);
-    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)ncols), dim3(kRedBlock), 0, W->stream, W->tPartial.p,
-                       kDotBlocks, h, (cx<double> *)nullptr);
+                       n, ebar, n, partial);
+    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)ncols), dim3(kRedBlock), 0, W->stream, partial, kDotBlocks, h,
+                       (cx<double> *)nullptr);
+}
+static void cols_intensity(pimi_workspace *W, const cx<double> *X, int64_t ncols, const cx<double> *ebar,
+                           cx<double> *h) {
+    cols_intensity(W, X, ncols, ebar, h, W->tPartial.p);
 }
 
-// Y = conditional drift of X; lam (ncols) receives the unclamped total intensities
+// Y = conditional drift of X; lam (ncols) receives the unclamped total
+// intensities; h (ncols) and partial (kDotBlocks x ncols) are scratch
 static void conditional_action(pimi_workspace *W, const cx<double> *X, cx<double> *Y, int64_t ncols,
-                               cx<double> *lam) {
+                               cx<double> *lam, cx<double> *h, cx<double> *partial) {
     pimi_plan *P = W->plan;
     const int64_t n = P->n;
     launch_stencil(W, P->cond, X, Y, ncols);
-    cols_intensity(W, X, ncols, P->ebar.p, W->tH.p);
-    hipLaunchKernelGGL(k_cols_axpy_lam, dim3(grid_for(n * ncols)), dim3(kRedBlock), 0, W->stream, Y, X, W->tH.p, lam,
-                       n, ncols);
+    cols_intensity(W, X, ncols, P->ebar.p, h, partial);
+    hipLaunchKernelGGL(k_cols_axpy_lam, dim3(grid_for(n * ncols)), dim3(kRedBlock), 0, W->stream, Y, X, h, lam, n,
+                       ncols);
+}
+static void conditional_action(pimi_workspace *W, const cx<double> *X, cx<double> *Y, int64_t ncols,
+                               cx<double> *lam) {
+    conditional_action(W, X, Y, ncols, lam, W->tH.p, W->tPartial.p);
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free entry point's scratch: one fixed array per device
+// (each GPU the library runs on holds its own copy until the process ends).
+// Host code never passes the array's name to a kernel: pimi_context_create
+// resolves its device address with hipGetSymbolAddress, with the context's
+// device current, and the kernels take that address.  Layout: the intensity
+// reduction's partials (kDotBlocks x ncols), then h (ncols), lam (ncols) and
+// one status slot.  Calls that overlap on one GPU share it and race.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocMaxCols = 256;
+constexpr int64_t kNoAllocScratch = kDotBlocks * kNoAllocMaxCols + 2 * kNoAllocMaxCols + 1;  // complex doubles
+static __device__ __align__(256) cx<double> scratch[kNoAllocScratch];
+
+// intensities[c] = max(0, Re lam[c]); *status = 1 when any Re lam[c] is
+// nonfinite or below -1e-11 (the reference's sign check), else 0.  One block;
+// the status is written unconditionally, so it needs no clearing.
+__global__ __launch_bounds__(256) void k_lam_clamp_check(const cx<double> *__restrict__ lam, int64_t ncols,
+                                                         double *__restrict__ intensities, cx<double> *status) {
+    int bad = 0;
+    for (int64_t c = threadIdx.x; c < ncols; c += 256) {
+        const double v = lam[c].x;
+        bad |= !(v == v) || v - v != 0.0 || v < -1e-11;  // NaN, +-inf, or a negative trace
+        intensities[c] = v > 0.0 ? v : 0.0;
+    }
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) *status = cx<double>{bad ? 1.0 : 0.0, 0.0};
 }
 
 // Intensities are finite and not below -max(1e-11, 100 eps) (src/trajectories.jl:755)
@@ -2153,8 +2191,15 @@ pimi_status pimi_context_create(int32_t device_ordinal, pimi_context **out) {
         HIPCHK(hipGetDeviceCount(&count));
         if (device_ordinal < 0 || device_ordinal >= count) throw Error(PIMI_EINVAL, "device ordinal out of range");
         HIPCHK(hipSetDevice(device_ordinal));
+        // the allocation-free entry points' scratch array: its address on
+        // this device (each device has its own copy)
+        void *sp = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !sp)
+            throw Error(PIMI_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
         auto *c = new pimi_context();
         c->device = device_ordinal;
+        c->noalloc_scratch = sp;
         *out = c;
     });
 }
@@ -2747,6 +2792,36 @@ pimi_status pimi_conditional_action(pimi_workspace *W, const void *X, void *Y, i
         sync(W);
         check_intensities(h, "total jump intensity");
         for (int64_t c = 0; c < ncols; ++c) intensities[c] = std::max(0.0, h[(size_t)c].x);
+    });
+}
+
+pimi_status pimi_conditional_action_noalloc(pimi_workspace *W, const void *X, void *Y, int64_t ncols,
+                                            double *intensities) {
+    return guarded([&] {
+        if (!W || !X || !Y || !intensities) throw Error(PIMI_EINVAL, "null argument");
+        if (X == Y) throw Error(PIMI_EINVAL, "batched trajectory source and destination must not alias");
+        if (ncols < 1 || ncols > kNoAllocMaxCols)
+            throw Error(PIMI_EINVAL, "pimi_conditional_action_noalloc takes 1..256 columns (its fixed scratch size)");
+        if (W->prec != PIMI_C64) throw Error(PIMI_EINVAL, "trajectory operations need a complex FP64 workspace");
+        BusyGuard guard(W->busy);
+        pimi_plan *P = W->plan;
+        HIPCHK(hipSetDevice(P->ctx->device));
+        {
+            std::lock_guard<std::recursive_mutex> lk(P->lazy_mu);
+            if (!P->traj_ready)
+                throw Error(PIMI_EINVAL, "the plan's trajectory tables are not built yet (building them allocates): "
+                                         "call pimi_conditional_action or pimi_channel_intensities on it first");
+        }
+        cx<double> *base = (cx<double> *)P->ctx->noalloc_scratch;
+        if (!base) throw Error(PIMI_EDEVICE, "no scratch address for this device");
+        cx<double> *partial = base, *h = base + kDotBlocks * kNoAllocMaxCols, *lam = h + kNoAllocMaxCols;
+        cx<double> *status = lam + kNoAllocMaxCols;
+        conditional_action(W, (const cx<double> *)X, (cx<double> *)Y, ncols, lam, h, partial);
+        hipLaunchKernelGGL(k_lam_clamp_check, dim3(1), dim3(256), 0, W->stream, lam, ncols, intensities, status);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(W->hpin, status, sizeof(double), hipMemcpyDeviceToHost, W->stream));  // pinned
+        sync(W);
+        if (W->hpin[0] != 0.0) throw Error(PIMI_EINVAL, "total jump intensity is nonfinite or has a negative trace");
     });
 }
 
