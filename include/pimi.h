@@ -208,6 +208,13 @@ pimi_status pimi_plan_dims(const pimi_plan *plan, int64_t *dims);
 pimi_status pimi_trace_functional(const pimi_plan *plan, int64_t *idx, double *vals);
 /* PI coefficient vector of sum_i X_i for a d x d complex X (host in, host out) */
 pimi_status pimi_collective_operator(const pimi_plan *plan, const double *X, double *out);
+/* iid_state (src/operators.jl:146-250): PI coefficient vector (n complex,
+ * host) of rho^{otimes N} for a d x d one-particle matrix rho (complex,
+ * column-major, host), by the one-box Clebsch-Gordan recurrence
+ * D_lambda = sum_{i,o} rho[i,o] U_i D_mu U_o^T over particle numbers 1..N.
+ * rho is used as given (the Python and Julia callers validate Hermiticity,
+ * positivity and trace first, as the reference does). */
+pimi_status pimi_iid_state(const pimi_plan *plan, const double *rho, double *out);
 /* fused plan export of a compiled model (host arrays); lengths via pimi_plan_export_sizes */
 pimi_status pimi_plan_export_sizes(const pimi_plan *plan, int64_t *sizes /* 12 entries */);
 pimi_status pimi_plan_export(const pimi_plan *plan, int64_t *cp_base, int64_t *colptr, int32_t *rows,

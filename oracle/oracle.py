@@ -211,7 +211,8 @@ class OraclePlan:
         return sig
 
     def steady_state(self, krylovdim=30, maxiter=500, atol=0.0, rtol=1e-10, operator_scale=0.0, nthreads=1,
-                     preconditioner=None, criterion="reference", weighting=None, inner_steps=0):
+                     preconditioner=None, criterion="reference", weighting=None, inner_steps=0,
+                     initial_state=None):
         """weighting="sector": iterate on u = S x with S = sqrt(f^nu) per sector.
         preconditioner="sector_inner": flexible GMRES whose directions are
         `inner_steps` (default 8) sector-preconditioned GMRES steps on the
@@ -226,6 +227,7 @@ class OraclePlan:
             sig = np.ascontiguousarray(self.sector_sigma())
         elif weighting is not None:
             raise ValueError("unknown weighting")
+        x0 = None if initial_state is None else np.ascontiguousarray(initial_state, dtype=np.complex128)
         info = _SteadyInfo()
         pu = pi = None
         if preconditioner == "sector":
@@ -237,7 +239,8 @@ class OraclePlan:
                                   C.c_int64(len(self.t_idx)), C.c_int(krylovdim), C.c_int64(maxiter),
                                   C.c_double(atol), C.c_double(rtol), C.c_double(operator_scale), _ptr(x),
                                   C.byref(info), pu, pi, C.c_int(1 if criterion == "backward" else 0),
-                                  None if sig is None else _ptr(sig), C.c_int(int(inner_steps)))
+                                  None if sig is None else _ptr(sig), C.c_int(int(inner_steps)),
+                                  None if x0 is None else _ptr(x0))
         return x, {k: getattr(info, k) for k, _ in _SteadyInfo._fields_}
 
     def expv(self, b, t, krylovdim=30, atol=1e-10, rtol=1e-8, safety=0.9, max_steps=10000, nthreads=1):

@@ -766,12 +766,13 @@ done:
     return conv;
 }
 
-/* krylov_steady_state (src/krylov.jl:797-891) with x0 = v = t/(t.t).
+/* krylov_steady_state (src/krylov.jl:797-891) with x0 = v = t/(t.t), or the
+ * given initial state.
  * operator_scale <= 0 selects the deterministic probe estimate. */
 int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const double *tvals, int64_t nt,
                         int krylovdim, int64_t maxiter, double atol, double rtol, double operator_scale,
                         cplx *x_out, steady_info *info, const cplx *pc_U, const cplx *pc_inv, int backward,
-                        const double *sigma, int inner_steps) {
+                        const double *sigma, int inner_steps, const cplx *x0) {
     int64_t n = P->n;
     g_blas_threads = nthreads > 0 ? nthreads : 1;
     cplx *tv = malloc(sizeof(cplx) * nt), *vv = malloc(sizeof(cplx) * nt);
@@ -789,7 +790,10 @@ int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const
     steady_op op = {P, nthreads, 1.0 / scale, tidx, tw, vw, nt, 0, sigma, scr};
     cplx *b = calloc(n, sizeof(cplx));
     for (int64_t k = 0; k < nt; ++k) b[tidx[k]] += vw[k];
-    memcpy(x_out, b, sizeof(cplx) * n);
+    if (x0)  /* initial_state (src/krylov.jl:818-821), in the iteration's coordinates */
+        for (int64_t i = 0; i < n; ++i) x_out[i] = sigma ? x0[i] * sigma[i] : x0[i];
+    else
+        memcpy(x_out, b, sizeof(cplx) * n);
     int64_t it = 0, rs = 0; double res = 0;
     sector_pc pc = {pc_U, pc_inv, scale};
     /* with weighting the iteration controls the weighted residual; if

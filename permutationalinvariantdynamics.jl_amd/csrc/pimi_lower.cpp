@@ -574,6 +574,69 @@ void collective_operator(const FusedPlan &P, const cd *X, cd *out) {
     }
 }
 
+// iid_state (src/operators.jl:146-250): the physical Schur blocks of
+// rho^{otimes N} by the one-box Clebsch-Gordan recurrence
+//   D_lambda = sum_{i,o} A[i,o] U_i D_mu U_o^T,   D_(0,...,0) = [1],
+// mu = lambda without its first removable corner, U_l[a, c] the one-box
+// isometry of (mu, lambda) at local label l (the transition tables of the
+// geometry of lambda's particle number), one particle number at a time; the
+// coefficient block of sector lambda is sqrt(f^lambda) D_lambda.  A is the
+// (validated, Hermitian-averaged) d x d one-particle matrix, column-major.
+void iid_state(const FusedPlan &P, const cd *A, cd *out) {
+    const int N = P.geom.N, d = P.geom.d;
+    std::map<Partition, std::vector<cd>> prev;
+    prev[Partition((size_t)d, 0)] = {cd(1.0, 0.0)};
+    for (int nn = 1; nn <= N; ++nn) {
+        Geometry Gl;
+        const Geometry *G = &P.geom;
+        if (nn < N) {
+            Gl.build(nn, d);
+            G = &Gl;
+        }
+        std::map<Partition, std::vector<cd>> cur;
+        for (size_t s = 0; s < G->sectors.size(); ++s) {
+            const Partition &mu = G->removed[s].front();
+            const std::vector<cd> &D = prev.at(mu);
+            const Transitions &T = G->trans.at({(int)s, mu});
+            const int64_t m = G->dim((int)s), mm = (int64_t)T.size();
+            // W_o[p, b] = sum_c D[p, c] U_o[b, c]
+            std::vector<cd> W((size_t)d * (size_t)(mm * m), cd(0, 0));
+            for (int64_t c = 0; c < mm; ++c)
+                for (const auto &t : T[(size_t)c]) {
+                    const int64_t b = std::get<0>(t);
+                    const int o = std::get<1>(t);
+                    const double v = std::get<2>(t);
+                    if (v == 0.0) continue;
+                    cd *w = W.data() + (size_t)o * (size_t)(mm * m) + (size_t)(b * mm);
+                    for (int64_t p = 0; p < mm; ++p) w[p] += D[(size_t)(p + c * mm)] * v;
+                }
+            // D_lambda[a, b] = sum_{i,o} A[i,o] sum_p U_i[a, p] W_o[p, b]
+            std::vector<cd> Dl((size_t)(m * m), cd(0, 0));
+            for (int64_t p = 0; p < mm; ++p)
+                for (const auto &t : T[(size_t)p]) {
+                    const int64_t a = std::get<0>(t);
+                    const int i = std::get<1>(t);
+                    const double v = std::get<2>(t);
+                    if (v == 0.0) continue;
+                    for (int o = 0; o < d; ++o) {
+                        const cd coef = A[i + o * d] * v;
+                        if (coef == cd(0, 0)) continue;
+                        const cd *w = W.data() + (size_t)o * (size_t)(mm * m);
+                        for (int64_t b = 0; b < m; ++b) Dl[(size_t)(a + b * m)] += coef * w[(size_t)(p + b * mm)];
+                    }
+                }
+            cur[G->sectors[s]] = std::move(Dl);
+        }
+        prev = std::move(cur);
+    }
+    for (int s = 0; s < P.ns; ++s) {
+        const std::vector<cd> &D = prev.at(P.geom.sectors[(size_t)s]);
+        const double sc = P.geom.sqrt_f[(size_t)s];
+        const int64_t m = P.dims[(size_t)s];
+        for (int64_t k = 0; k < m * m; ++k) out[P.offs[(size_t)s] + k] = sc * D[(size_t)k];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // device stencil tables
 // ---------------------------------------------------------------------------

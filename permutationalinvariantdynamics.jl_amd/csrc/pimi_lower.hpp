@@ -102,6 +102,10 @@ void lower_model(int N, int d, const std::vector<TermDesc> &terms, FusedPlan &ou
 // Collective operator sum_i X_i as a PI coefficient vector.
 void collective_operator(const FusedPlan &plan, const cd *X, cd *out);
 
+// PI coefficients of A^{otimes N} (A: d x d column-major) by the one-box
+// Clebsch-Gordan recurrence of iid_state.
+void iid_state(const FusedPlan &plan, const cd *A, cd *out);
+
 // Device stencil: per output sector, terms Y_l += s * A X_n B^H with A and B
 // given as padded ELL rows (row-major over slots: slot k of row a at
 // base + k*m_l + a).  A or B may be the identity.

@@ -2371,6 +2371,21 @@ pimi_status pimi_collective_operator(const pimi_plan *P, const double *X, double
     });
 }
 
+pimi_status pimi_iid_state(const pimi_plan *P, const double *rho, double *out) {
+    return guarded([&] {
+        if (!P || !rho || !out) throw Error(PIMI_EINVAL, "null argument");
+        if (!P->compiled) throw Error(PIMI_EINVAL, "iid_state needs a plan compiled from a model");
+        const int d = P->host.geom.d;
+        std::vector<cd> A((size_t)d * d);
+        for (int q = 0; q < d * d; ++q) {
+            A[(size_t)q] = cd(rho[2 * q], rho[2 * q + 1]);
+            if (!std::isfinite(rho[2 * q]) || !std::isfinite(rho[2 * q + 1]))
+                throw Error(PIMI_EINVAL, "rho must be finite");
+        }
+        iid_state(P->host, A.data(), reinterpret_cast<cd *>(out));
+    });
+}
+
 pimi_status pimi_plan_export_sizes(const pimi_plan *P, int64_t *sz) {
     return guarded([&] {
         if (!P || !sz) throw Error(PIMI_EINVAL, "null argument");

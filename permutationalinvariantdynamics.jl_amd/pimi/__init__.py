@@ -8,7 +8,7 @@ from .api import (  # noqa: F401
     CollectiveHamiltonian, CollectiveJump, CollectiveOperator, CollectiveProduct, CompiledPIModel,
     DirectPIHamiltonian, DynamicsStreamResult, ExpvAlgorithm, GMRESAlgorithm, LocalHamiltonian, LocalJump, Models,
     PIBasis, PIModel, PIOperator, PIState, SteadyStateResult, collective_operator, compile,
-    computational_product_state, expect, qubit_ensemble_model, saved_times, solve_dynamics, spin_matrices,
+    computational_product_state, expect, iid_state, qubit_ensemble_model, saved_times, solve_dynamics, spin_matrices,
     stationary_state,
 )
 from .batch import BatchAlgorithm, BatchEngine, batched_stationary_states  # noqa: F401

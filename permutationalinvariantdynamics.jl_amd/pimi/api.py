@@ -272,6 +272,72 @@ def computational_product_state(basis: PIBasis, level: int, T=np.float64) -> PIS
     return PIState(basis, v)
 
 
+_IID_PLANS = {}
+
+
+def iid_state(basis: PIBasis, rho, atol=0.0, rtol=None) -> PIState:
+    """PI representation of rho^{otimes N} for a one-particle density matrix
+    (iid_state, src/operators.jl:197-250): validated as the reference does
+    (Hermitian within atol + rtol |rho|, positive within atol + rtol |A| with
+    A = (rho + rho^H)/2, trace one within atol + rtol, the N-th power of the
+    trace too), then every Schur block is built on the host by the one-box
+    Clebsch-Gordan recurrence D_lambda = sum_{i,o} A[i,o] U_i D_mu U_o^T
+    (pimi_iid_state); the coefficient block is sqrt(f^lambda) D_lambda.  A
+    restricted basis must retain every sector that carries weight."""
+    rho = np.asarray(rho, dtype=np.complex128)
+    d, Np = basis.d, basis.N
+    if rho.shape != (d, d):
+        raise ValueError(f"rho must have size ({d}, {d})")
+    rtol = math.sqrt(np.finfo(np.float64).eps) if rtol is None else float(rtol)
+    atol = float(atol)
+    if atol < 0 or rtol < 0:
+        raise ValueError("atol and rtol must be nonnegative")
+    herr = float(np.abs(rho - rho.conj().T).max())
+    if not herr <= atol + rtol * float(np.abs(rho).max()):
+        raise ValueError(f"rho must be Hermitian: error={herr}")
+    A = (rho + rho.conj().T) / 2
+    tol = atol + rtol * float(np.abs(A).max())
+    lo = float(np.linalg.eigvalsh(A).min())
+    if lo < -(tol + 32 * d * np.finfo(np.float64).eps * max(float(np.abs(A).max()), 1.0)):
+        raise ValueError(f"rho must be positive within tolerance {tol}")
+    tr = float(np.trace(A).real)
+    if not abs(tr - 1.0) <= atol + rtol:
+        raise ValueError(f"rho must have trace one: trace={tr}")
+    tt = tr ** Np
+    if not (math.isfinite(tt) and abs(tt - 1.0) <= atol + rtol):
+        raise ValueError(f"the accepted one-particle trace error is amplified at N={Np}: the tensor-power trace "
+                         f"would be {tt}")
+    full = PIBasis(Np, d)
+    if Np == 0:
+        out = np.ones(1, dtype=np.complex128)
+    else:
+        key = (Np, d)
+        if key not in _IID_PLANS:
+            _IID_PLANS[key] = CompiledPIModel(PIModel(full, []), host_only=True)
+        plan = _IID_PLANS[key]
+        Ac = np.ascontiguousarray(A.reshape(-1, order="F"))
+        out = np.zeros(len(full), dtype=np.complex128)
+        N.check(N.lib().pimi_iid_state(plan._plan, N.ptr(Ac), N.ptr(out)))
+    if basis.complete:
+        return PIState(basis, out)
+    data = np.zeros(len(basis), dtype=np.complex128)
+    for s, p in enumerate(basis.sectors):
+        fs = full.sectors.index(p)
+        data[basis.offsets[s]:basis.offsets[s + 1]] = out[full.offsets[fs]:full.offsets[fs + 1]]
+    if Np == 0:
+        tr_b = float(data.sum().real)
+    else:
+        idx, tv = _IID_PLANS[(Np, d)].trace_functional()
+        keep = np.zeros(len(full), dtype=bool)
+        for p in basis.sectors:
+            fs = full.sectors.index(p)
+            keep[full.offsets[fs]:full.offsets[fs + 1]] = True
+        tr_b = float(sum(tv[k] * out[i_].real for k, i_ in enumerate(idx) if keep[i_]))
+    if not abs(tr_b - 1.0) <= atol + rtol:
+        raise ValueError(f"basis omits Schur-sector support of rho^N: represented trace={tr_b}")
+    return PIState(basis, data)
+
+
 def expect(op, state):
     return complex(np.vdot(op.data, state.data))
 
