@@ -14,7 +14,7 @@ from .api import (  # noqa: F401
 from .batch import BatchAlgorithm, BatchEngine, batched_stationary_states  # noqa: F401
 from .trajectories import (  # noqa: F401
     batched_apply_jumps, batched_channel_intensities, batched_conditional_action, batched_conditional_rk4,
-    trajectory_channels,
+    batched_sample_jumps, batched_trajectory_rngs, trajectory_channels,
 )
 from .checkpoints import (  # noqa: F401
     PI_CHECKPOINT_VERSION, PIStateCheckpoint, checkpoint_state, load_checkpoint, make_checkpoint, save_checkpoint,
