@@ -16,9 +16,16 @@ stationary solver's host setup run on a thread pool off the loop thread.
 A request's result does not depend on the other requests in flight: the
 engine's arithmetic per job depends only on that job, and every dynamics
 request runs on its own workspace with deterministic reductions.
+
+With `snapshot_dir`, every finished request's state (the stationary state,
+or a dynamics request's state at t_end) is written as a PIDCHKPT v1
+checkpoint (src/checkpoints.jl:114-138) named by its request index, with the
+request's parameters and solver figures as metadata, so served results can
+be reloaded here or by the reference package.
 """
 from __future__ import annotations
 
+import os
 import queue
 import threading
 import time
@@ -30,6 +37,7 @@ import numpy as np
 from . import _native as N
 from .api import ExpvAlgorithm, PIState, compile as pi_compile, computational_product_state, solve_dynamics
 from .batch import BatchAlgorithm, BatchEngine
+from .checkpoints import save_checkpoint
 
 
 @dataclass
@@ -41,10 +49,11 @@ class ServedResult:
     finished: float
     latency: float
     converged: bool
-    state: PIState | None = None          # stationary state (trace one)
+    state: PIState | None = None          # stationary state (trace one), or the dynamics state at t_end
     values: dict | None = None            # dynamics: observable series
     times: np.ndarray | None = None
     info: dict = field(default_factory=dict)
+    snapshot: str | None = None           # PIDCHKPT file of the state (with snapshot_dir)
 
 
 def dynamics_bytes(n, krylovdim):
@@ -55,7 +64,8 @@ def dynamics_bytes(n, krylovdim):
 class Server:
     def __init__(self, device=0, precision=N.PIMI_C32, steady_algorithm: BatchAlgorithm | None = None,
                  dynamics_algorithm: ExpvAlgorithm | None = None, rtol=1e-8, memory_bytes=0,
-                 dynamics_workers=2, dynamics_memory_bytes=32 << 30, compile_workers=8, t_end=10.0, saveat=0.2):
+                 dynamics_workers=2, dynamics_memory_bytes=32 << 30, compile_workers=8, t_end=10.0, saveat=0.2,
+                 snapshot_dir=None):
         self.device = device
         self.rtol = rtol
         self.steady_algorithm = steady_algorithm or BatchAlgorithm(krylovdim=200, inner_rtol=1e-4)
@@ -66,6 +76,9 @@ class Server:
         self.compile_workers = compile_workers
         self.t_end = t_end
         self.saveat = saveat
+        self.snapshot_dir = snapshot_dir
+        if snapshot_dir is not None:
+            os.makedirs(snapshot_dir, exist_ok=True)
 
     def close(self):
         self.engine.close()
@@ -83,6 +96,20 @@ class Server:
         res = solve_dynamics(comp, rho0, (0.0, self.t_end), saveat=self.saveat, algorithm=self.dynamics_algorithm,
                              observables={"n": np.array([[0, 0], [0, 1]], dtype=complex)})
         return res
+
+    def _snapshot(self, result, req):
+        """Write a finished request's state as a PIDCHKPT v1 file."""
+        if self.snapshot_dir is None or result.state is None:
+            return
+        inst = req.instance
+        md = {"request": result.index, "kind": result.kind, "N": inst.N, "omega": repr(inst.omega),
+              "gamma": repr(inst.gamma), "gamma_phi": repr(inst.gamma_phi), "converged": int(result.converged)}
+        for key in ("relative_residual", "iterations", "accepted", "applications"):
+            if key in result.info:
+                md[key] = repr(result.info[key])
+        path = os.path.join(self.snapshot_dir, f"request_{result.index:06d}_{result.kind}.pid")
+        result.snapshot = save_checkpoint(path, result.state, time=self.t_end if result.kind == "dynamics" else None,
+                                          metadata=md)
 
     def serve(self, requests, realtime=True):
         """Run the stream; `requests` carry `arrival` (seconds), `kind` and
@@ -127,10 +154,12 @@ class Server:
                 try:
                     res = self._run_dynamics(comp, reqs[i])
                     t = now()
+                    r = ServedResult(i, "dynamics", comp.basis.N, reqs[i].arrival, t, t - reqs[i].arrival,
+                                     bool(res.info.get("converged", 1)), state=res.final_state, values=res.values,
+                                     times=res.t, info=res.info)
+                    self._snapshot(r, reqs[i])
                     with lock:
-                        results[i] = ServedResult(i, "dynamics", comp.basis.N, reqs[i].arrival, t,
-                                                  t - reqs[i].arrival, bool(res.info.get("converged", 1)),
-                                                  values=res.values, times=res.t, info=res.info)
+                        results[i] = r
                 except Exception as e:
                     errors.append(e)
                 finally:
@@ -181,6 +210,7 @@ class Server:
                     t = now()
                     results[i] = ServedResult(i, "steady", comp.basis.N, reqs[i].arrival, t, t - reqs[i].arrival,
                                               bool(r.info["converged"]), state=r.state, info=r.info)
+                    self._snapshot(results[i], reqs[i])
                     done_steady += 1
             dispatcher.join()
             pool.shutdown(wait=True)

@@ -92,3 +92,24 @@ def test_serving_large_n_fp32_against_closed_form():
                 assert err <= 1e-6
     finally:
         srv.close()
+
+
+def test_serving_snapshots_reload_bit_for_bit(tmp_path):
+    """With snapshot_dir every finished request leaves a PIDCHKPT v1 file whose
+    state reloads bit for bit, with the request's parameters as metadata."""
+    from pimi.workloads import Instance
+    reqs = [Request(0.0, "steady", Instance(12, 0.6, 0.1, 0.02)), Request(0.0, "dynamics", Instance(9, 0.8, 0.1, 0.0))]
+    srv = Server(precision=pimi.PIMI_C64, rtol=1e-10, memory_bytes=1 << 30, dynamics_workers=1,
+                 dynamics_memory_bytes=1 << 30, compile_workers=2, t_end=1.0, saveat=0.5,
+                 snapshot_dir=str(tmp_path))
+    try:
+        out = srv.serve(reqs, realtime=False)
+    finally:
+        srv.close()
+    for r, q in zip(out, reqs):
+        assert r.snapshot and os.path.exists(r.snapshot)
+        cp = pimi.load_checkpoint(r.snapshot)
+        np.testing.assert_array_equal(cp.state.data, r.state.data)
+        assert cp.metadata["kind"] == q.kind and int(cp.metadata["N"]) == q.instance.N
+        assert float(cp.metadata["omega"]) == q.instance.omega
+        assert (cp.time is None) == (q.kind == "steady")
