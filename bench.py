@@ -29,8 +29,9 @@ Also reported:
                 oracle's CPU expv on a bounded sample of it;
   precision_form  spin-1 N=18: FP32 Arnoldi in FP64 refinement against the plain
                 FP64 device solve, observables against the FP64 reference values;
-  serving       a Poisson stream (200 requests/s, 60 requests, 60 % stationary
-                states, 40 % dynamics to t=10) through the serving loop;
+  serving       the Poisson stream at 50, 200, 1000 and 2000 requests/s (24
+                requests each, 60 % stationary states, 40 % dynamics to
+                t=10) through the serving loop;
   reference_algorithm_c1  the reference's own GMRES(30) and expv on C1 (PI dim
                 165), the oracle on the host beside the device;
   cpu_baseline  the oracle's OpenMP CPU restatement solving the headline in
@@ -65,6 +66,49 @@ BATCH_PLAIN_SAMPLE = 32
 # the banded apply kernels (tile width picked by the autotuner at run time)
 APPLY_KERNEL = "k_apply_dia_split<double, {}>"  # single-vector banded apply, FP64
 BATCH_APPLY_KERNEL = "kb_apply_dia<float, {}>"  # batch engine's banded apply, FP32
+
+
+def c2_reference():
+    """The oracle's weighted N=120 stationary state, summarised
+    (tests/golden/c2_steady_reference.json, written by
+    tests/golden/make_c2_steady_reference.py)."""
+    with open(os.path.join(ROOT, "tests", "golden", "c2_steady_reference.json")) as fh:
+        return json.load(fh)
+
+
+_OBS_CACHE = {}
+
+
+def observable_errors(pimi, comp, state, gold):
+    """Relative errors of <sum n_i>, <Jz>, <Jx>, <J+J-> and the largest
+    absolute sector-population error against the oracle's reference state."""
+    if "ops" not in _OBS_CACHE:
+        sp = pimi.spin_matrices(2)
+        jp = comp.collective_operator(sp.jp).data
+        jm = comp.collective_operator(sp.jm).data
+        idx, tv = comp.trace_functional()
+        jj = np.zeros_like(jp)
+        k = 0
+        for sec, m in enumerate(comp.basis.dims):  # (A B) coefficient blocks = A_nu B_nu / sqrt(f)
+            o0, o1 = comp.basis.offsets[sec], comp.basis.offsets[sec + 1]
+            jj[o0:o1] = ((jp[o0:o1].reshape(m, m, order="F") @ jm[o0:o1].reshape(m, m, order="F")) /
+                         tv[k]).reshape(-1, order="F")
+            k += m
+        _OBS_CACHE["ops"] = {"n": comp.collective_operator(sp.jp @ sp.jm).data,
+                             "jz": comp.collective_operator(sp.jz).data,
+                             "jx": comp.collective_operator(sp.jx).data, "JpJm": jj}
+        _OBS_CACHE["tv"] = tv
+    x = state.data
+    errs = {k: abs(np.vdot(op, x).real - gold["observables"][k]) / abs(gold["observables"][k])
+            for k, op in _OBS_CACHE["ops"].items()}
+    tv = _OBS_CACHE["tv"]
+    pops, k = [], 0
+    for sec, m in enumerate(comp.basis.dims):
+        blk = x[comp.basis.offsets[sec]:comp.basis.offsets[sec + 1]].reshape(m, m, order="F")
+        pops.append((tv[k] * np.trace(blk)).real)
+        k += m
+    return {"max_observable_rel_err": max(errs.values()), "observable_rel_err": errs,
+            "max_population_abs_err": float(max(abs(a - b) for a, b in zip(pops, gold["populations"])))}
 
 
 def build_model(pimi):
@@ -128,11 +172,12 @@ def cpu_baseline(device_ms, plain_device_ms, full_plain=True):
     t0 = time.perf_counter()
     _, info = ref.steady_state(krylovdim=KRYLOV, maxiter=10 * KRYLOV, atol=0.0, rtol=RTOL, operator_scale=scale,
                                nthreads=threads, preconditioner="sector_inner", inner_steps=INNER_STEPS,
-                               criterion="backward")
+                               criterion="weighted", weighting="sector")
     t_fg = time.perf_counter() - t0
     out = {"value": 1.0 / t_fg, "unit": "solves/s", "cores": threads, "kind": "port",
-           "sample": (f"one full N=120 stationary solve by the oracle's inner-outer flexible GMRES (the headline "
-                      f"algorithm: {INNER_STEPS} inner steps, outer Krylov {KRYLOV}, rtol {RTOL:g}) on {threads} "
+           "sample": (f"one full N=120 stationary solve by the oracle's inner-outer flexible GMRES in sector-"
+                      f"weighted coordinates (the headline algorithm: {INNER_STEPS} inner steps, outer Krylov "
+                      f"{KRYLOV}, weighted rtol {RTOL:g}) on {threads} "
                       f"OpenMP threads: {t_fg:.1f} s, {info['iterations']} outer steps"),
            "speedup_headline_vs_cpu": t_fg / (device_ms * 1e-3)}
     print(f"# cpu flexible GMRES: {t_fg:.1f} s, {info['iterations']} steps", file=sys.stderr, flush=True)
@@ -140,7 +185,7 @@ def cpu_baseline(device_ms, plain_device_ms, full_plain=True):
         t0 = time.perf_counter()
         _, info2 = ref.steady_state(krylovdim=KRYLOV_PLAIN, maxiter=KRYLOV_PLAIN, atol=0.0, rtol=RTOL,
                                     operator_scale=scale, nthreads=threads, preconditioner="sector",
-                                    criterion="backward")
+                                    criterion="weighted", weighting="sector")
         t_pl = time.perf_counter() - t0
         out["plain_sector_gmres"] = {
             "cpu_s": t_pl, "iterations": int(info2["iterations"]), "device_ms": plain_device_ms,
@@ -394,20 +439,39 @@ def dynamics_n120(pimi, comp, cpu=True):
     return out
 
 
-def serving(pimi):
+SERVE_RATES = (50.0, 200.0, 1000.0, 2000.0)
+SERVE_COUNT = 24
+
+
+def serving(pimi, rates=SERVE_RATES, count=SERVE_COUNT):
+    """The C5 stream (60 % stationary, 40 % dynamics to t=10 with 50 save
+    points, N ~ U{16..160}) offered at each arrival rate: `count` requests
+    per rate through one server (FP32 stationary jobs continuously batched
+    in the engine, dynamics on worker streams); p50/p99 latency and served
+    requests/s per rate.  Latency includes queueing once the offered rate
+    exceeds the service rate."""
     from pimi.serving import Server, summarize
     from pimi.workloads import served_requests
-    reqs = served_requests(200.0, 60)
     srv = Server(precision=pimi.PIMI_C32)
-    t0 = time.perf_counter()
-    res = srv.serve(reqs)
-    wall = time.perf_counter() - t0
-    srv.close()
-    s = summarize(res, wall)
-    s["rate_per_s"] = 200.0
-    s["wall_s"] = wall
-    s["max_steady_relative_residual"] = max(r.info["relative_residual"] for r in res if r.kind == "steady")
-    return s
+    out = {"workload": f"{count} requests per rate, seeded Poisson arrivals, 60% stationary (FP32 Arnoldi in FP64 "
+                       f"refinement, rtol 1e-8) / 40% expv dynamics to t=10 (51 samples), N ~ U{{16..160}}",
+           "rates": {}}
+    try:
+        srv.serve(served_requests(1000.0, 4, seed=7), realtime=False)  # warm-up (compilation, workspaces)
+        for rate in rates:
+            reqs = served_requests(rate, count)
+            t0 = time.perf_counter()
+            res = srv.serve(reqs)
+            wall = time.perf_counter() - t0
+            s = summarize(res, wall)
+            s["wall_s"] = wall
+            s["max_steady_relative_residual"] = max(r.info["relative_residual"] for r in res if r.kind == "steady")
+            out["rates"][f"{rate:g}"] = s
+            print(f"# serving {rate:g}/s: {s['all']['per_s']:.2f} req/s, p50 {s['all']['p50_ms']:.0f} ms, "
+                  f"p99 {s['all']['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
+    finally:
+        srv.close()
+    return out
 
 
 def main():
@@ -419,8 +483,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-side", action="store_true", help="skip the plain and FP32-inner side solves")
-    ap.add_argument("--no-cpu-plain", action="store_true",
-                    help="skip the full CPU solve by plain sector-preconditioned GMRES (the longest CPU leg)")
+    ap.add_argument("--cpu-plain", action="store_true",
+                    help="also solve in full on the CPU with plain sector-preconditioned GMRES (several minutes)")
     ap.add_argument("--profile", action="store_true",
                     help="apply, preconditioner and copy calibration only (counter runs)")
     args = ap.parse_args()
@@ -471,8 +535,14 @@ def main():
     widths = {"c64": comp.autotune(pimi.PIMI_C64), "c32": comp.autotune(pimi.PIMI_C32)}
     alg = pimi.GMRESAlgorithm(krylovdim=KRYLOV, maxiter=10 * KRYLOV, preconditioner="sector_inner",
                               inner_steps=INNER_STEPS)
+    gold = c2_reference()
 
-    def solve(algorithm=alg, precision=pimi.PIMI_C64):
+    def solve(algorithm=alg, precision=pimi.PIMI_C64, weighted=True):
+        # sector-weighted coordinates, accepted on the weighted relative residual:
+        # the coefficient-space residual does not resolve the populated sectors at N=120
+        if weighted:
+            return pimi.stationary_state(comp, algorithm=algorithm, atol=0.0, rtol=RTOL, return_info=True,
+                                         criterion="weighted", weighting="sector", precision=precision)
         return pimi.stationary_state(comp, algorithm=algorithm, atol=0.0, rtol=RTOL, return_info=True,
                                      criterion="backward", precision=precision)
 
@@ -486,24 +556,35 @@ def main():
         r = solve()
         lat.append(time.perf_counter() - s)
         infos.append(r.info)
-        print(f"# solve {len(lat)}: {lat[-1]:.3f} s, iterations {r.info['iterations']}, "
-              f"relative residual {r.info['relative_residual']:.3e}", file=sys.stderr, flush=True)
+        print(f"# solve {len(lat)}: {lat[-1]:.3f} s, iterations {r.info['iterations']}, weighted relative "
+              f"residual {r.info['weighted_relative_residual']:.3e}", file=sys.stderr, flush=True)
     total = time.perf_counter() - t0
+    head_err = observable_errors(pimi, comp, r.state, gold)
     iters = int(np.median([i["iterations"] for i in infos]))
     side = {}
     if args.no_side:
         print(json.dumps({"solves_per_s": args.steps / total, "ms_per_step": total / args.steps * 1e3,
                           "iterations": iters}))
         return
-    for name, algorithm, prec in (
+    for name, algorithm, prec, weighted in (
             ("plain_fp64", pimi.GMRESAlgorithm(krylovdim=KRYLOV_PLAIN, maxiter=KRYLOV_PLAIN,
-                                               preconditioner="sector"), pimi.PIMI_C64),
-            ("fp32_inner", alg, pimi.PIMI_C32)):
+                                               preconditioner="sector"), pimi.PIMI_C64, True),
+            ("fp32_inner", alg, pimi.PIMI_C32, True),
+            ("unweighted_flexible", alg, pimi.PIMI_C64, False)):
         s = time.perf_counter()
-        r = solve(algorithm, prec)
+        try:
+            r = solve(algorithm, prec, weighted)
+        except pimi.NotConverged as e:
+            side[name] = {"error": str(e)}
+            continue
         side[name] = {"ms": (time.perf_counter() - s) * 1e3, "iterations": int(r.info["iterations"]),
-                      "relative_residual": r.info["relative_residual"]}
+                      "relative_residual": r.info["relative_residual"],
+                      "weighted_relative_residual": r.info["weighted_relative_residual"]}
+        side[name].update(observable_errors(pimi, comp, r.state, gold))
         print(f"# {name}: {side[name]}", file=sys.stderr, flush=True)
+    side["unweighted_flexible"]["note"] = (
+        "the same solver in PI coefficients, accepted on |L x|/(s|x|) <= 1e-8: faster, but that residual does not "
+        "resolve the populated sectors at N=120, and its observables miss the oracle's by the errors shown")
     ms_apply, bytes_apply, achieved_apply = apply_roofline(pimi, comp)
     ms_gemm, flops_gemm, tfs_gemm = gemm_roofline(pimi, comp)
     batch, spmv_batch = (None, None) if args.no_batch else batched(pimi, args)
@@ -514,7 +595,7 @@ def main():
     cpu = c1 = None
     if not args.no_cpu:
         c1 = reference_algorithm_c1(pimi)
-        cpu = cpu_baseline(float(np.median(lat)) * 1e3, side["plain_fp64"]["ms"], full_plain=not args.no_cpu_plain)
+        cpu = cpu_baseline(float(np.median(lat)) * 1e3, side["plain_fp64"]["ms"], full_plain=args.cpu_plain)
     lat_ms = np.array(lat) * 1e3
     spmv = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
             "single_n120_fp64": {"achieved": achieved_apply, "frac": achieved_apply / HBM_PEAK_GBS,
@@ -541,15 +622,21 @@ def main():
         "data": "synthetic",
         "config": {"workload": "qubit ensemble N=120 (PI dim 302621): H=sum(jz+jx), local emission 0.1, "
                                "dephasing 0.2, pumping 0.02, collective emission 0.1; FP64 sector-"
-                               "preconditioned flexible GMRES (8 inner steps) stationary state to |L rho|/(s|rho|) <= 1e-8",
+                               "preconditioned flexible GMRES (8 inner steps) in sector-weighted coordinates, "
+                               "stationary state to weighted relative residual |S L rho|/(s|S rho|) <= 1e-8, "
+                               "observables checked against the FP64 oracle",
                    "n": comp.info["n"], "krylovdim": KRYLOV, "inner_steps": INNER_STEPS, "rtol": RTOL},
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "gmres_iterations": iters,
+        "weighted_relative_residual": max(i["weighted_relative_residual"] for i in infos),
         "relative_residual": max(i["relative_residual"] for i in infos),
         "trace_error": max(i["trace_error"] for i in infos),
+        "error_vs_oracle": dict(head_err, reference="oracle weighted flexible GMRES, weighted relative residual "
+                                                    f"{gold['weighted_relative_residual']:.1e}"),
         "plain_fp64_sector_gmres": side["plain_fp64"],
         "fp32_inner": side["fp32_inner"],
+        "unweighted_flexible": side["unweighted_flexible"],
         "roofline": {"bound": "mfma", "achieved": tfs_gemm, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
                      "frac": tfs_gemm / FP64_MFMA_PEAK_TFS,
                      "traffic": measured_traffic("k_sector_gemm_mfma<double>"),

@@ -148,7 +148,13 @@ typedef struct {
        PIMI_CRITERION_BACKWARD:  |r| <= (atol + rtol |x|) / 2 and |L x|/s <= atol + rtol |x|, i.e. for
          atol = 0 the scale-invariant relative residual |L x|/(s |x|) <= rtol (t^H L = 0 bounds
          |L x|/s by 2 |r|); at large N |b| = 2^(-N/2) is far below eps |x|, where the
-         reference rule cannot be met in floating point. */
+         reference rule cannot be met in floating point.
+       PIMI_CRITERION_WEIGHTED (with PIMI_WEIGHT_SECTOR only): the backward rule on the weighted
+         iterate u = S x, accepted when the weighted relative residual |S L x|/(s |S x|) <= atol + rtol.
+         The coefficient-space residual |L x|/(s |x|) weighs a sector by its coefficients, which
+         scale as 1/sqrt(f^nu) of its population: at large N (f^nu up to 1e34 at N = 120) it leaves
+         the populated sectors, and every observable, unresolved, while the weighted residual
+         weighs sectors as traces and observables do.  No counterpart in the reference. */
     int32_t criterion;
     int32_t inner_steps;    /* PIMI_PRECOND_SECTOR_INNER: inner GMRES steps per outer step */
     /* PIMI_WEIGHT_NONE: iterate on the PI coefficients x, as the reference does.
@@ -163,7 +169,7 @@ typedef struct {
     int32_t reserved;
 } pimi_gmres_opts;
 
-enum { PIMI_CRITERION_REFERENCE = 0, PIMI_CRITERION_BACKWARD = 1 };
+enum { PIMI_CRITERION_REFERENCE = 0, PIMI_CRITERION_BACKWARD = 1, PIMI_CRITERION_WEIGHTED = 2 };
 enum { PIMI_WEIGHT_NONE = 0, PIMI_WEIGHT_SECTOR = 1 };
 
 typedef struct {
@@ -186,6 +192,8 @@ typedef struct {
     double estimated_error;
     double seconds;
     double relative_residual;   /* |L x|_2 / (operator_scale |x|_2) */
+    double weighted_relative_residual; /* |S L x|_2 / (operator_scale |S x|_2), S = sqrt(f^nu) per sector
+                                          (stationary single solves; NaN where not formed) */
 } pimi_solve_info;
 
 const char *pimi_last_error(void);
@@ -215,6 +223,16 @@ pimi_status pimi_collective_operator(const pimi_plan *plan, const double *X, dou
  * rho is used as given (the Python and Julia callers validate Hermiticity,
  * positivity and trace first, as the reference does). */
 pimi_status pimi_iid_state(const pimi_plan *plan, const double *rho, double *out);
+/* The banded apply tables of a qubit plan (host copies, for checking the
+ * table construction against the operator on the CPU): counts[4] = group
+ * base entries (ns + 1), groups, entries, coefficient slots (complex); each
+ * array argument may be NULL.  groups: 7 int64 per group (input block
+ * offset, input block size, row shift, first strip column, strip span, entry
+ * range [e0, e1)); ents: 6 int32 per entry (kind, strip column, qa, qb, qa2,
+ * qb2); coef: complex doubles.  counts are zero for plans on ELL tables.  No
+ * counterpart in the reference (a device table layout). */
+pimi_status pimi_plan_banded_export(const pimi_plan *plan, int64_t *counts, int32_t *gbase, int64_t *groups,
+                                    int32_t *ents, double *coef);
 /* fused plan export of a compiled model (host arrays); lengths via pimi_plan_export_sizes */
 pimi_status pimi_plan_export_sizes(const pimi_plan *plan, int64_t *sizes /* 12 entries */);
 pimi_status pimi_plan_export(const pimi_plan *plan, int64_t *cp_base, int64_t *colptr, int32_t *rows,

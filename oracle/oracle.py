@@ -55,7 +55,8 @@ class _Plan(C.Structure):
 class _SteadyInfo(C.Structure):
     _fields_ = [("iterations", C.c_int64), ("restarts", C.c_int64), ("applications", C.c_int64),
                 ("converged", C.c_int32), ("residual", C.c_double), ("normalized_residual", C.c_double),
-                ("trace_error", C.c_double), ("operator_scale", C.c_double), ("linear_residual", C.c_double)]
+                ("trace_error", C.c_double), ("operator_scale", C.c_double), ("linear_residual", C.c_double),
+                ("weighted_relative_residual", C.c_double)]
 
 
 class _ExpvInfo(C.Structure):
@@ -213,7 +214,8 @@ class OraclePlan:
     def steady_state(self, krylovdim=30, maxiter=500, atol=0.0, rtol=1e-10, operator_scale=0.0, nthreads=1,
                      preconditioner=None, criterion="reference", weighting=None, inner_steps=0,
                      initial_state=None):
-        """weighting="sector": iterate on u = S x with S = sqrt(f^nu) per sector.
+        """weighting="sector": iterate on u = S x with S = sqrt(f^nu) per sector;
+        criterion="weighted" (with it) accepts on |S L x|/(s |S x|).
         preconditioner="sector_inner": flexible GMRES whose directions are
         `inner_steps` (default 8) sector-preconditioned GMRES steps on the
         sector-diagonal part of L."""
@@ -222,8 +224,8 @@ class OraclePlan:
         x = np.zeros(self.n, dtype=np.complex128)
         sig = None
         if weighting == "sector":
-            if criterion != "backward":
-                raise ValueError("sector weighting needs the backward criterion")
+            if criterion not in ("backward", "weighted"):
+                raise ValueError("sector weighting needs the backward or the weighted criterion")
             sig = np.ascontiguousarray(self.sector_sigma())
         elif weighting is not None:
             raise ValueError("unknown weighting")
@@ -238,7 +240,7 @@ class OraclePlan:
         lib().oracle_steady_state(C.byref(self._P), C.c_int(nthreads), _ptr(self.t_idx), _ptr(self.t_val),
                                   C.c_int64(len(self.t_idx)), C.c_int(krylovdim), C.c_int64(maxiter),
                                   C.c_double(atol), C.c_double(rtol), C.c_double(operator_scale), _ptr(x),
-                                  C.byref(info), pu, pi, C.c_int(1 if criterion == "backward" else 0),
+                                  C.byref(info), pu, pi, C.c_int({"backward": 1, "weighted": 2}.get(criterion, 0)),
                                   None if sig is None else _ptr(sig), C.c_int(int(inner_steps)),
                                   None if x0 is None else _ptr(x0))
         return x, {k: getattr(info, k) for k, _ in _SteadyInfo._fields_}

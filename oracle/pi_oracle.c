@@ -441,6 +441,7 @@ typedef struct {
     int64_t iterations, restarts, applications;
     int32_t converged;
     double residual, normalized_residual, trace_error, operator_scale, linear_residual;
+    double weighted_relative_residual;
 } steady_info;
 
 typedef struct {
@@ -811,7 +812,8 @@ int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const
             conv = gmres(&op, x_out, b, krylovdim, maxiter - it, atol * tight, rtol * tight, &it1, &rs1, &res,
                          pc_U ? &pc : NULL, backward);
         it += it1; rs += rs1;
-        if (!sigma || !conv || pass >= 4 || it >= maxiter) break;
+        /* backward == 2 (weighted criterion): accept on the weighted residual, no tightening */
+        if (!sigma || backward == 2 || !conv || pass >= 4 || it >= maxiter) break;
         for (int64_t i = 0; i < n; ++i) scr[i] = x_out[i] / sigma[i];
         do_apply(P, scr, lw, nthreads);
         op.applications++;
@@ -836,7 +838,19 @@ int oracle_steady_state(const oplan *P, int nthreads, const int64_t *tidx, const
     double terr = cabs(tr - 1.0);
     double xn = dznrm2(x_out, n);
     double tol = backward ? atol + rtol * xn : atol + rtol * (xn > 1.0 ? xn : 1.0);
-    info->converged = conv && nres <= tol && terr <= atol + rtol;
+    /* weighted relative residual |S L x| / (s |S x|) (when the weights are given) */
+    double wres = NAN;
+    if (sigma) {
+        cplx *sx = malloc(sizeof(cplx) * n);
+        for (int64_t i = 0; i < n; ++i) sx[i] = sigma[i] * w[i];
+        double a = dznrm2(sx, n);
+        for (int64_t i = 0; i < n; ++i) sx[i] = sigma[i] * x_out[i];
+        double b2 = dznrm2(sx, n);
+        wres = b2 > 0 ? a / (scale * b2) : INFINITY;
+        free(sx);
+    }
+    info->weighted_relative_residual = wres;
+    info->converged = conv && terr <= atol + rtol && (backward == 2 ? wres <= atol + rtol : nres <= tol);
     info->iterations = it; info->restarts = rs; info->applications = op.applications;
     info->residual = resid; info->normalized_residual = nres; info->trace_error = terr;
     info->operator_scale = scale; info->linear_residual = res;

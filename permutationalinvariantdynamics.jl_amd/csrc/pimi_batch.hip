@@ -1510,6 +1510,7 @@ static void complete_tick(pimi_batch *E, BLane &L) {
                 I.operator_scale = J->scale;
                 I.linear_residual = J->beta;
                 I.relative_residual = xn > 0 ? nres / xn : INFINITY;
+                I.weighted_relative_residual = NAN;  // not formed by the engine
                 I.seconds =
                     std::chrono::duration<double>(std::chrono::steady_clock::now() - J->t_admit).count();
                 if (!ok && J->error.empty()) {

@@ -162,7 +162,7 @@ struct DevStencil {
         };
         struct PendingEntry {
             int kind, dB;
-            std::vector<cd> ca, wb;
+            std::vector<cd> ca, wb, ca2, wb2;
         };
         for (int q = 0; q < ns; ++q) {
             const int64_t m = dims[(size_t)q];
@@ -235,10 +235,60 @@ struct DevStencil {
                         p[i / 2] = i % 2 ? cd(p[i / 2].real(), v[i].real()) : cd(v[i].real(), p[i / 2].imag());
                     return p;
                 };
-                for (auto &E : pend[g]) {
-                    DiaEntry D{E.kind, E.dB - lo, 0, 0};
-                    if (E.kind != 1) D.qa = put(E.kind == 3 ? packed(E.ca) : E.ca);
-                    if (E.kind != 0) D.qb = put(E.kind == 3 ? packed(E.wb) : E.wb);
+                // specialised kinds (arithmetic only; every output still sums
+                // the group's entries in one fixed order for every tile width):
+                // purely imaginary left/right coefficients, a complex left and
+                // right entry on one strip column merged, and real sandwiches
+                // on one strip column paired
+                std::vector<PendingEntry> &P = pend[g];
+                auto imag_only = [](const std::vector<cd> &v) {
+                    for (const cd &z : v)
+                        if (z.real() != 0.0) return false;
+                    return true;
+                };
+                for (auto &E : P) {
+                    if (E.kind == 0 && imag_only(E.ca)) {
+                        E.kind = 4;
+                        for (cd &z : E.ca) z = cd(z.imag(), 0.0);
+                    } else if (E.kind == 1 && imag_only(E.wb)) {
+                        E.kind = 5;
+                        for (cd &z : E.wb) z = cd(z.imag(), 0.0);
+                    }
+                }
+                std::vector<PendingEntry> merged;
+                std::vector<bool> used(P.size(), false);
+                for (size_t i = 0; i < P.size(); ++i) {
+                    if (used[i]) continue;
+                    PendingEntry E = P[i];
+                    used[i] = true;
+                    const int want = E.kind == 0 ? 1 : E.kind == 1 ? 0 : E.kind == 3 ? 3 : -1;
+                    if (want >= 0)
+                        for (size_t k = i + 1; k < P.size(); ++k) {
+                            if (used[k] || P[k].kind != want || P[k].dB != E.dB) continue;
+                            used[k] = true;
+                            if (E.kind == 3) {  // pair: the second sandwich in ca2/wb2
+                                E.kind = 7;
+                                E.ca2 = P[k].ca;
+                                E.wb2 = P[k].wb;
+                            } else {            // left + right: the left's row vector, the right's column vector
+                                if (E.kind == 1) E.ca = P[k].ca;
+                                else E.wb = P[k].wb;
+                                E.kind = 6;
+                            }
+                            break;
+                        }
+                    merged.push_back(std::move(E));
+                }
+                for (auto &E : merged) {
+                    DiaEntry D{E.kind, E.dB - lo, 0, 0, 0, 0};
+                    const bool real_rows = E.kind == 3 || E.kind == 4 || E.kind == 7;
+                    const bool real_cols = E.kind == 3 || E.kind == 5 || E.kind == 7;
+                    if (E.kind != 1 && E.kind != 5) D.qa = put(real_rows ? packed(E.ca) : E.ca);
+                    if (E.kind != 0 && E.kind != 4) D.qb = put(real_cols ? packed(E.wb) : E.wb);
+                    if (E.kind == 7) {
+                        D.qa2 = put(packed(E.ca2));
+                        D.qb2 = put(packed(E.wb2));
+                    }
                     if (h_coef.size() >= (size_t)INT32_MAX) return;
                     h_dents.push_back(D);
                     h_dent_sector.push_back(q);
@@ -277,10 +327,19 @@ struct DevStencil {
             const double r = sigma[(size_t)h_dent_sector[e]] / sigma[(size_t)h_dent_in[e]];
             if (r == 1.0) continue;
             const DiaEntry &D = h_dents[e];
-            const int32_t q = D.kind == 1 ? D.qb : D.qa;
             const size_t m = (size_t)dims_of_entry(e);
-            const size_t len = D.kind == 3 ? (m + 1) / 2 : m;  // packed reals: two per slot
-            for (size_t k = 0; k < len; ++k) w[(size_t)q + k] *= r;
+            auto scale = [&](int32_t q, bool packed_reals) {  // packed reals: two per slot
+                const size_t len = packed_reals ? (m + 1) / 2 : m;
+                for (size_t k = 0; k < len; ++k) w[(size_t)q + k] *= r;
+            };
+            switch (D.kind) {  // the side that carries the scale
+                case 1: scale(D.qb, false); break;
+                case 5: scale(D.qb, true); break;
+                case 3: case 4: scale(D.qa, true); break;
+                case 6: scale(D.qa, false); scale(D.qb, false); break;  // (ca + w): both sides
+                case 7: scale(D.qa, true); scale(D.qa2, true); break;
+                default: scale(D.qa, false); break;
+            }
         }
         std::vector<cx<double>> c64(w.size());
         std::vector<cx<float>> c32(w.size());
@@ -404,7 +463,7 @@ struct pimi_plan {
     // the transformed trace functional (ones on the block diagonals) and 1/sigma
     bool weight_ready = false;
     std::vector<double> sigma;  // per sector
-    pimi::DevBuf<double> d_ones, d_inv_sigma;
+    pimi::DevBuf<double> d_ones, d_inv_sigma, d_sigma;
 };
 
 namespace pimi {

@@ -185,7 +185,16 @@ __global__ __launch_bounds__(kApplyWave) void k_apply(
 // Kinds:  left (B = I): Y += cA'[a] X;  right (A = I): Y += w[b] X;
 //         sandwich: Y += (cA'[a] w[b]) X;  real sandwich: the same with real
 //         cA', w (two real FMAs per product instead of four complex ones);
+//         imaginary left / right: a left / right entry whose coefficients
+//         are purely imaginary (i c X: two real FMAs), stored as the real c;
+//         left + right: a left and a right entry on the same strip column,
+//         (cA'[a] + w[b]) X;  real sandwich pair: two real sandwiches on the
+//         same strip column, (cA1[a] w1[b] + cA2[a] w2[b]) X.
 // cA' carries the term scale, w[b] = conj(cB[b]) (times the scale for right).
+// For qubit plans (tridiagonal H, diagonal loss, one-diagonal contractions)
+// the builder turns the off-diagonal left/right entries into imaginary
+// ones, merges the two diagonal ones, and pairs the two children's
+// sandwiches of each branch, which takes about a third of the arithmetic.
 // Every output sums its groups in list order and each group's entries in
 // list order with explicit FMAs, so the result of an output does not depend
 // on the tile width TB.
@@ -198,9 +207,11 @@ struct DiaGroup {
     int32_t e0, e1;     // entry range
 };
 struct DiaEntry {
-    int32_t kind;       // 0 left, 1 right, 2 sandwich, 3 real sandwich
+    int32_t kind;       // 0 left, 1 right, 2 sandwich, 3 real sandwich, 4 imaginary left, 5 imaginary right,
+                        // 6 left + right on one strip column, 7 two real sandwiches on one strip column
     int32_t dB;         // strip column offset, 0..2
     int32_t qa, qb;     // coefficient vectors: per output row (A side), per output column (B side)
+    int32_t qa2, qb2;   // kind 7: the second sandwich's vectors
 };
 constexpr int kDiaWaves = 4;  // tiles (waves) per workgroup
 constexpr int kNumDiaWidths = 3;
@@ -257,7 +268,36 @@ template <typename R, int TB, int KIND, int DB>
 __device__ __forceinline__ void dia_entry(cx<R> (&acc)[TB], const cx<R> (&st)[TB + 2], const DiaEntry &E,
                                           __amdgpu_buffer_rsrc_t pool, const PIMI_CONST cx<R> *coefs, int va,
                                           int vr, int b0) {
-    if constexpr (KIND == 0) {
+    if constexpr (KIND == 4) {  // i c[a] X
+        const R c = dia_row_real<R>(pool, vr, E.qa);
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+            acc[j].x = rfma(-c, st[j + DB].y, acc[j].x);
+            acc[j].y = rfma(c, st[j + DB].x, acc[j].y);
+        }
+    } else if constexpr (KIND == 5) {  // i w[b] X
+        const DiaColsR<R, TB> w = *(const PIMI_CONST DiaColsR<R, TB> *)((const PIMI_CONST R *)(coefs + E.qb) + b0);
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+            acc[j].x = rfma(-w.v[j], st[j + DB].y, acc[j].x);
+            acc[j].y = rfma(w.v[j], st[j + DB].x, acc[j].y);
+        }
+    } else if constexpr (KIND == 6) {  // (cA[a] + w[b]) X
+        const cx<R> ca = dia_row<R>(pool, va, E.qa);
+        const DiaCols<R, TB> w = *(const PIMI_CONST DiaCols<R, TB> *)(coefs + E.qb + b0);
+#pragma unroll
+        for (int j = 0; j < TB; ++j) acc[j] = cfma(cx<R>{ca.x + w.v[j].x, ca.y + w.v[j].y}, st[j + DB], acc[j]);
+    } else if constexpr (KIND == 7) {  // (cA1[a] w1[b] + cA2[a] w2[b]) X, all real
+        const R c1 = dia_row_real<R>(pool, vr, E.qa), c2 = dia_row_real<R>(pool, vr, E.qa2);
+        const DiaColsR<R, TB> w1 = *(const PIMI_CONST DiaColsR<R, TB> *)((const PIMI_CONST R *)(coefs + E.qb) + b0);
+        const DiaColsR<R, TB> w2 = *(const PIMI_CONST DiaColsR<R, TB> *)((const PIMI_CONST R *)(coefs + E.qb2) + b0);
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+            const R c = rfma(c1, w1.v[j], c2 * w2.v[j]);
+            acc[j].x = rfma(c, st[j + DB].x, acc[j].x);
+            acc[j].y = rfma(c, st[j + DB].y, acc[j].y);
+        }
+    } else if constexpr (KIND == 0) {
         const cx<R> ca = dia_row<R>(pool, va, E.qa);
 #pragma unroll
         for (int j = 0; j < TB; ++j) acc[j] = cfma(ca, st[j + DB], acc[j]);
@@ -373,6 +413,18 @@ __device__ __forceinline__ void dia_tile(const cx<R> *xc_, cx<R> *yc_, R out_sca
                 case 12: dia_entry<R, TB, 3, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
                 case 13: dia_entry<R, TB, 3, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
                 case 14: dia_entry<R, TB, 3, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 16: dia_entry<R, TB, 4, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 17: dia_entry<R, TB, 4, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 18: dia_entry<R, TB, 4, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 20: dia_entry<R, TB, 5, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 21: dia_entry<R, TB, 5, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 22: dia_entry<R, TB, 5, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 24: dia_entry<R, TB, 6, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 25: dia_entry<R, TB, 6, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 26: dia_entry<R, TB, 6, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 28: dia_entry<R, TB, 7, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 29: dia_entry<R, TB, 7, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
+                case 30: dia_entry<R, TB, 7, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
                 default: break;
             }
         }

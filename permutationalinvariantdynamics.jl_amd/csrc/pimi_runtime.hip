@@ -940,6 +940,7 @@ void ensure_plan_weighting(pimi_plan *P) {
     P->main.push_weighted(P->sigma);
     P->d_ones.upload(ones.data(), ones.size());
     P->d_inv_sigma.upload(inv.data(), inv.size());
+    P->d_sigma.upload(P->sigma.data(), P->sigma.size());
     P->weight_ready = true;
 }
 
@@ -1581,8 +1582,11 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, const doub
     if (o->weighting != PIMI_WEIGHT_NONE && o->weighting != PIMI_WEIGHT_SECTOR)
         throw Error(PIMI_EINVAL, "unknown weighting");
     const bool weighted = o->weighting == PIMI_WEIGHT_SECTOR;
-    if (weighted && o->criterion != PIMI_CRITERION_BACKWARD)
-        throw Error(PIMI_EINVAL, "sector weighting needs the backward criterion");
+    if (weighted && o->criterion == PIMI_CRITERION_REFERENCE)
+        throw Error(PIMI_EINVAL, "sector weighting needs the backward or the weighted criterion");
+    if (!weighted && o->criterion == PIMI_CRITERION_WEIGHTED)
+        throw Error(PIMI_EINVAL, "the weighted criterion needs sector weighting");
+    const bool weighted_accept = o->criterion == PIMI_CRITERION_WEIGHTED;
     W->weighted = false;
     std::vector<double> vw(vv);
     if (weighted) {  // v -> S v on the trace support (ordered by sector, m entries each)
@@ -1626,12 +1630,15 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, const doub
         throw Error(PIMI_EINVAL, "complex FP32 workspaces solve with the inner-outer preconditioner");
     if (precond || inner_outer) ensure_preconditioner(W);
     if (inner_outer) ensure_diag_stencil(P);
-    const bool backward = o->criterion == PIMI_CRITERION_BACKWARD;
+    const bool backward = o->criterion == PIMI_CRITERION_BACKWARD || weighted_accept;
     if (o->criterion != PIMI_CRITERION_REFERENCE && !backward) throw Error(PIMI_EINVAL, "unknown criterion");
     W->weighted = weighted;
     // With sector weighting the iteration controls |S A S^{-1} u - S v| against
-    // |u|; the acceptance below measures |L x| / (s |x|) on x = S^{-1} u.  If
-    // that has not followed, the iteration continues with a tighter threshold.
+    // |u|.  Under the backward criterion the acceptance below measures
+    // |L x| / (s |x|) on x = S^{-1} u; if that has not followed, the iteration
+    // continues with a tighter threshold.  Under the weighted criterion the
+    // acceptance is the weighted relative residual |S L x| / (s |S x|), the
+    // quantity the iteration controls.
     GmresResult gr{false, 0, 0, 0.0};
     double tight = 1.0;
     for (int pass = 0;; ++pass) {
@@ -1644,7 +1651,7 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, const doub
         gr.iterations += g.iterations;
         gr.restarts += g.restarts;
         gr.residual = g.residual;
-        if (!weighted || !g.converged || pass >= 4 || gr.iterations >= o->maxiter) break;
+        if (!weighted || weighted_accept || !g.converged || pass >= 4 || gr.iterations >= o->maxiter) break;
         hipLaunchKernelGGL(k_sector_scale<double>, dim3((unsigned)P->ntiles), dim3(kApplyBlock), 0, W->stream,
                            W->x.p, W->tmp.p, n, P->offs.p, P->tile_sector.p, P->d_inv_sigma.p);
         launch_apply<double>(W, W->tmp.p, W->w.p, 1.0);
@@ -1688,6 +1695,19 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, const doub
     HIPCHK(hipMemcpyAsync(W->hpin, W->nscal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, W->stream));
     sync(W);
     double resid = W->hpin[0], xn = W->hpin[1];
+    // the weighted relative residual |S L x| / (s |S x|), S = sqrt(f^nu) per sector
+    double wres = NAN;
+    if (P->compiled && !P->host.geom.sqrt_f.empty()) {
+        ensure_plan_weighting(P);
+        hipLaunchKernelGGL(k_sector_scale<double>, dim3((unsigned)P->ntiles), dim3(kApplyBlock), 0, W->stream,
+                           W->w.p, W->tmp.p, n, P->offs.p, P->tile_sector.p, P->d_sigma.p);
+        dev_norm<double>(W, W->tmp.p, n, W->nscal.p + 2);
+        hipLaunchKernelGGL(k_sector_scale<double>, dim3((unsigned)P->ntiles), dim3(kApplyBlock), 0, W->stream,
+                           W->x.p, W->tmp.p, n, P->offs.p, P->tile_sector.p, P->d_sigma.p);
+        dev_norm<double>(W, W->tmp.p, n, W->nscal.p + 3);
+        d2h(W, W->hpin + 2, W->nscal.p + 2, 2 * sizeof(double));
+        wres = W->hpin[3] > 0 ? W->hpin[2] / (scale * W->hpin[3]) : INFINITY;
+    }
     cd tr(0, 0);
     for (size_t k = 0; k < P->tidx.size(); ++k) tr += P->tval[k] * cd(xh[P->tidx[k]].x, xh[P->tidx[k]].y);
     double terr = std::abs(tr - 1.0);
@@ -1695,7 +1715,8 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, const doub
     double tol = backward ? o->atol + o->rtol * xn : o->atol + o->rtol * std::max(xn, 1.0);
     std::memcpy(rho_out, xh.data(), sizeof(cx<double>) * n);
     std::memset(info, 0, sizeof(*info));
-    info->converged = gr.converged && nres <= tol && terr <= o->atol + o->rtol;
+    info->converged = gr.converged && terr <= o->atol + o->rtol &&
+                      (weighted_accept ? wres <= o->atol + o->rtol : nres <= tol);
     info->iterations = gr.iterations;
     info->restarts = gr.restarts;
     info->applications = apps;
@@ -1705,6 +1726,7 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, const doub
     info->operator_scale = scale;
     info->linear_residual = gr.residual;
     info->relative_residual = xn > 0 ? nres / xn : INFINITY;
+    info->weighted_relative_residual = wres;
     info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!info->converged) {
         char buf[256];
@@ -2383,6 +2405,40 @@ pimi_status pimi_iid_state(const pimi_plan *P, const double *rho, double *out) {
                 throw Error(PIMI_EINVAL, "rho must be finite");
         }
         iid_state(P->host, A.data(), reinterpret_cast<cd *>(out));
+    });
+}
+
+pimi_status pimi_plan_banded_export(const pimi_plan *P, int64_t *counts, int32_t *gbase, int64_t *groups,
+                                    int32_t *ents, double *coef) {
+    return guarded([&] {
+        if (!P || !counts) throw Error(PIMI_EINVAL, "null argument");
+        const DevStencil &S = P->main;
+        if (!S.dia_ok) {
+            counts[0] = counts[1] = counts[2] = counts[3] = 0;
+            return;
+        }
+        counts[0] = (int64_t)S.h_gbase.size();
+        counts[1] = (int64_t)S.h_groups.size();
+        counts[2] = (int64_t)S.h_dents.size();
+        counts[3] = (int64_t)S.h_coef.size();
+        if (gbase) std::copy(S.h_gbase.begin(), S.h_gbase.end(), gbase);
+        if (groups)
+            for (size_t g = 0; g < S.h_groups.size(); ++g) {
+                const DiaGroup &G = S.h_groups[g];
+                const int64_t v[7] = {G.in_off, G.mn, G.dA, G.c0, G.span, G.e0, G.e1};
+                std::copy(v, v + 7, groups + 7 * g);
+            }
+        if (ents)
+            for (size_t e = 0; e < S.h_dents.size(); ++e) {
+                const DiaEntry &D = S.h_dents[e];
+                const int32_t v[6] = {D.kind, D.dB, D.qa, D.qb, D.qa2, D.qb2};
+                std::copy(v, v + 6, ents + 6 * e);
+            }
+        if (coef)
+            for (size_t k = 0; k < S.h_coef.size(); ++k) {
+                coef[2 * k] = S.h_coef[k].real();
+                coef[2 * k + 1] = S.h_coef[k].imag();
+            }
     });
 }
 

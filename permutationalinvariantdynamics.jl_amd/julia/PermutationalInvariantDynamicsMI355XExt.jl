@@ -161,6 +161,7 @@ mutable struct PimiSolveInfo
     estimated_error::Float64
     seconds::Float64
     relative_residual::Float64
+    weighted_relative_residual::Float64
     PimiSolveInfo() = new()
 end
 
@@ -474,7 +475,8 @@ function PID.steady_state(A::MI355XOperator; method=:krylov, basis=nothing, trac
     ws = _workspace(A, prec, min(krylovdim, n))
     opts = PimiGmresOpts(Int32(krylovdim), _preconditioner_code(preconditioner), Int64(maxiter), Float64(atol),
                          Float64(rtol), operator_scale === nothing ? 0.0 : Float64(operator_scale), 1e-5,
-                         Int64(50), criterion === :backward ? Int32(1) : Int32(0), Int32(inner_steps),
+                         Int64(50), criterion === :weighted ? Int32(2) : criterion === :backward ? Int32(1) : Int32(0),
+                         Int32(inner_steps),
                          weighting === :sector ? Int32(1) : Int32(0), Int32(0))
     x0 = initial_state === nothing ? nothing :
          Vector{ComplexF64}(initial_state isa PID.PIState ? initial_state.data : initial_state)
@@ -495,7 +497,8 @@ function PID.steady_state(A::MI355XOperator; method=:krylov, basis=nothing, trac
      krylov_variant=:restarted, recycle_dimension=0, recycled_initially=false, recycle_reset=false,
      operator_applications=Int(info.applications), preconditioned=preconditioner !== nothing,
      preconditioner=preconditioner === nothing ? nothing : typeof(preconditioner), preconditioner_cost=nothing,
-     relative_residual=info.relative_residual, device_seconds=info.seconds)
+     relative_residual=info.relative_residual, weighted_relative_residual=info.weighted_relative_residual,
+     device_seconds=info.seconds)
 end
 
 PID.krylov_steady_state(A::MI355XOperator; kwargs...) = PID.steady_state(A; method=:krylov, kwargs...)

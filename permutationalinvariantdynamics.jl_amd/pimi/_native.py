@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libpimi.so")
 PIMI_OK, PIMI_EINVAL, PIMI_EDIM, PIMI_ENOMEM, PIMI_EDEVICE, PIMI_EBUSY, PIMI_ENOTCONVERGED, PIMI_ERANGE = range(8)
 PIMI_C64, PIMI_C32 = 0, 1
 PRECOND_NONE, PRECOND_SECTOR, PRECOND_SECTOR_INNER = 0, 1, 2
-CRITERION_REFERENCE, CRITERION_BACKWARD = 0, 1
+CRITERION_REFERENCE, CRITERION_BACKWARD, CRITERION_WEIGHTED = 0, 1, 2
 WEIGHT_NONE, WEIGHT_SECTOR = 0, 1
 TERM_LOCAL_H, TERM_COLLECTIVE_H, TERM_LOCAL_JUMP, TERM_COLLECTIVE_JUMP, TERM_DIRECT_PRODUCT_H = range(5)
 
@@ -22,7 +22,7 @@ TERM_LOCAL_H, TERM_COLLECTIVE_H, TERM_LOCAL_JUMP, TERM_COLLECTIVE_JUMP, TERM_DIR
 EXPORTS = [
     "pimi_last_error", "pimi_version", "pimi_context_create", "pimi_context_destroy", "pimi_compile",
     "pimi_plan_create_fused", "pimi_plan_info_get", "pimi_plan_dims", "pimi_trace_functional",
-    "pimi_collective_operator", "pimi_iid_state", "pimi_plan_export_sizes", "pimi_plan_export", "pimi_plan_destroy",
+    "pimi_collective_operator", "pimi_iid_state", "pimi_plan_banded_export", "pimi_plan_export_sizes", "pimi_plan_export", "pimi_plan_destroy",
     "pimi_workspace_create", "pimi_workspace_destroy", "pimi_apply", "pimi_apply_adjoint", "pimi_steady_gmres", "pimi_expv_series",
     "pimi_workspace_last_apply_ms", "pimi_apply_bench", "pimi_copy_bench",
     "pimi_sector_preconditioner_factors", "pimi_precond_apply",
@@ -103,7 +103,8 @@ class SolveInfo(C.Structure):
                 ("rejected", C.c_int64), ("factorizations", C.c_int64), ("refinements", C.c_int64),
                 ("residual", C.c_double), ("normalized_residual", C.c_double), ("trace_error", C.c_double),
                 ("operator_scale", C.c_double), ("linear_residual", C.c_double), ("estimated_error", C.c_double),
-                ("seconds", C.c_double), ("relative_residual", C.c_double)]
+                ("seconds", C.c_double), ("relative_residual", C.c_double),
+                ("weighted_relative_residual", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -567,7 +567,9 @@ def stationary_state(compiled: CompiledPIModel, algorithm=None, atol=1e-10, rtol
 
     weighting="sector" iterates on u = S x with S = sqrt(f^nu) on sector nu,
     so that residual norms weigh sectors by multiplicity as observables do
-    (needs criterion="backward"); None iterates on the PI coefficients as the
+    (needs criterion="backward" or "weighted"; "weighted" stops and accepts on
+    the weighted relative residual |S L x|/(s |S x|), which weighs sectors as
+    observables do); None iterates on the PI coefficients as the
     reference does."""
     algorithm = algorithm or GMRESAlgorithm()
     if not isinstance(algorithm, GMRESAlgorithm):
@@ -575,9 +577,10 @@ def stationary_state(compiled: CompiledPIModel, algorithm=None, atol=1e-10, rtol
     ws = compiled.workspace(precision, algorithm.krylovdim)
     pc = {None: N.PRECOND_NONE, "sector": N.PRECOND_SECTOR,
           "sector_inner": N.PRECOND_SECTOR_INNER}[algorithm.preconditioner]
-    if criterion not in ("reference", "backward"):
-        raise ValueError("criterion must be \"reference\" or \"backward\"")
-    crit = N.CRITERION_BACKWARD if criterion == "backward" else N.CRITERION_REFERENCE
+    if criterion not in ("reference", "backward", "weighted"):
+        raise ValueError("criterion must be \"reference\", \"backward\" or \"weighted\"")
+    crit = {"reference": N.CRITERION_REFERENCE, "backward": N.CRITERION_BACKWARD,
+            "weighted": N.CRITERION_WEIGHTED}[criterion]
     if weighting not in (None, "sector"):
         raise ValueError("weighting must be None or \"sector\"")
     wt = N.WEIGHT_SECTOR if weighting == "sector" else N.WEIGHT_NONE
