@@ -57,9 +57,10 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
 N_QUBITS = 120
-RTOL = 1e-8
+RTOL = 1e-8           # the metric's residual bar (batched and served forms, the side solves)
+RTOL_HEADLINE = 1e-11  # weighted residual of the headline solve: observables within 1e-8 of the oracle's
 KRYLOV = 200          # outer (flexible) Krylov dimension of the headline solver
-INNER_STEPS = 8
+INNER_STEPS = 6
 KRYLOV_PLAIN = 1000   # plain sector-preconditioned GMRES, no restart needed
 BATCH_COUNT = 512
 BATCH_PLAIN_SAMPLE = 32
@@ -170,20 +171,20 @@ def cpu_baseline(device_ms, plain_device_ms, full_plain=True):
     scale = ref.operator_scale(threads)
     ref.sector_preconditioner()
     t0 = time.perf_counter()
-    _, info = ref.steady_state(krylovdim=KRYLOV, maxiter=10 * KRYLOV, atol=0.0, rtol=RTOL, operator_scale=scale,
-                               nthreads=threads, preconditioner="sector_inner", inner_steps=INNER_STEPS,
-                               criterion="weighted", weighting="sector")
+    _, info = ref.steady_state(krylovdim=KRYLOV, maxiter=10 * KRYLOV, atol=0.0, rtol=RTOL_HEADLINE,
+                               operator_scale=scale, nthreads=threads, preconditioner="sector_inner",
+                               inner_steps=INNER_STEPS, criterion="weighted", weighting="sector")
     t_fg = time.perf_counter() - t0
     out = {"value": 1.0 / t_fg, "unit": "solves/s", "cores": threads, "kind": "port",
            "sample": (f"one full N=120 stationary solve by the oracle's inner-outer flexible GMRES in sector-"
                       f"weighted coordinates (the headline algorithm: {INNER_STEPS} inner steps, outer Krylov "
-                      f"{KRYLOV}, weighted rtol {RTOL:g}) on {threads} "
+                      f"{KRYLOV}, weighted rtol {RTOL_HEADLINE:g}) on {threads} "
                       f"OpenMP threads: {t_fg:.1f} s, {info['iterations']} outer steps"),
            "speedup_headline_vs_cpu": t_fg / (device_ms * 1e-3)}
     print(f"# cpu flexible GMRES: {t_fg:.1f} s, {info['iterations']} steps", file=sys.stderr, flush=True)
     if full_plain:
         t0 = time.perf_counter()
-        _, info2 = ref.steady_state(krylovdim=KRYLOV_PLAIN, maxiter=KRYLOV_PLAIN, atol=0.0, rtol=RTOL,
+        _, info2 = ref.steady_state(krylovdim=KRYLOV_PLAIN, maxiter=KRYLOV_PLAIN, atol=0.0, rtol=RTOL_HEADLINE,
                                     operator_scale=scale, nthreads=threads, preconditioner="sector",
                                     criterion="weighted", weighting="sector")
         t_pl = time.perf_counter() - t0
@@ -537,13 +538,13 @@ def main():
                               inner_steps=INNER_STEPS)
     gold = c2_reference()
 
-    def solve(algorithm=alg, precision=pimi.PIMI_C64, weighted=True):
+    def solve(algorithm=alg, precision=pimi.PIMI_C64, weighted=True, rtol=RTOL_HEADLINE):
         # sector-weighted coordinates, accepted on the weighted relative residual:
         # the coefficient-space residual does not resolve the populated sectors at N=120
         if weighted:
-            return pimi.stationary_state(comp, algorithm=algorithm, atol=0.0, rtol=RTOL, return_info=True,
+            return pimi.stationary_state(comp, algorithm=algorithm, atol=0.0, rtol=rtol, return_info=True,
                                          criterion="weighted", weighting="sector", precision=precision)
-        return pimi.stationary_state(comp, algorithm=algorithm, atol=0.0, rtol=RTOL, return_info=True,
+        return pimi.stationary_state(comp, algorithm=algorithm, atol=0.0, rtol=rtol, return_info=True,
                                      criterion="backward", precision=precision)
 
     for _ in range(args.warmup):
@@ -566,18 +567,19 @@ def main():
         print(json.dumps({"solves_per_s": args.steps / total, "ms_per_step": total / args.steps * 1e3,
                           "iterations": iters}))
         return
-    for name, algorithm, prec, weighted in (
+    for name, algorithm, prec, weighted, rt in (
+            ("at_weighted_residual_1e-8", alg, pimi.PIMI_C64, True, RTOL),
             ("plain_fp64", pimi.GMRESAlgorithm(krylovdim=KRYLOV_PLAIN, maxiter=KRYLOV_PLAIN,
-                                               preconditioner="sector"), pimi.PIMI_C64, True),
-            ("fp32_inner", alg, pimi.PIMI_C32, True),
-            ("unweighted_flexible", alg, pimi.PIMI_C64, False)):
+                                               preconditioner="sector"), pimi.PIMI_C64, True, RTOL_HEADLINE),
+            ("fp32_inner", alg, pimi.PIMI_C32, True, RTOL_HEADLINE),
+            ("unweighted_flexible", alg, pimi.PIMI_C64, False, RTOL)):
         s = time.perf_counter()
         try:
-            r = solve(algorithm, prec, weighted)
+            r = solve(algorithm, prec, weighted, rt)
         except pimi.NotConverged as e:
             side[name] = {"error": str(e)}
             continue
-        side[name] = {"ms": (time.perf_counter() - s) * 1e3, "iterations": int(r.info["iterations"]),
+        side[name] = {"ms": (time.perf_counter() - s) * 1e3, "iterations": int(r.info["iterations"]), "rtol": rt,
                       "relative_residual": r.info["relative_residual"],
                       "weighted_relative_residual": r.info["weighted_relative_residual"]}
         side[name].update(observable_errors(pimi, comp, r.state, gold))
@@ -622,18 +624,19 @@ def main():
         "data": "synthetic",
         "config": {"workload": "qubit ensemble N=120 (PI dim 302621): H=sum(jz+jx), local emission 0.1, "
                                "dephasing 0.2, pumping 0.02, collective emission 0.1; FP64 sector-"
-                               "preconditioned flexible GMRES (8 inner steps) in sector-weighted coordinates, "
-                               "stationary state to weighted relative residual |S L rho|/(s|S rho|) <= 1e-8, "
-                               "observables checked against the FP64 oracle",
-                   "n": comp.info["n"], "krylovdim": KRYLOV, "inner_steps": INNER_STEPS, "rtol": RTOL},
+                               "preconditioned flexible GMRES (6 inner steps) in sector-weighted coordinates, "
+                               "stationary state to weighted relative residual |S L rho|/(s|S rho|) <= 1e-11 "
+                               "(<= the metric's 1e-8; observables within 1e-8 of the FP64 oracle's)",
+                   "n": comp.info["n"], "krylovdim": KRYLOV, "inner_steps": INNER_STEPS, "rtol": RTOL_HEADLINE},
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "gmres_iterations": iters,
         "weighted_relative_residual": max(i["weighted_relative_residual"] for i in infos),
-        "relative_residual": max(i["relative_residual"] for i in infos),
+        "coefficient_relative_residual": max(i["relative_residual"] for i in infos),
         "trace_error": max(i["trace_error"] for i in infos),
         "error_vs_oracle": dict(head_err, reference="oracle weighted flexible GMRES, weighted relative residual "
                                                     f"{gold['weighted_relative_residual']:.1e}"),
+        "at_weighted_residual_1e-8": side["at_weighted_residual_1e-8"],
         "plain_fp64_sector_gmres": side["plain_fp64"],
         "fp32_inner": side["fp32_inner"],
         "unweighted_flexible": side["unweighted_flexible"],

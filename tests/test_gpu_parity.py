@@ -725,32 +725,36 @@ def _c2_fixture():
         return json.load(fh)
 
 
-@pytest.mark.parametrize("algorithm", ["flexible", "plain"])
-def test_c2_steady_n120_against_oracle_fixture(algorithm):
+@pytest.mark.parametrize("algorithm,rtol,obs_bar,pop_bar", [("flexible", 1e-10, 1e-7, 1e-8),
+                                                             ("plain", 1e-10, 1e-7, 1e-8),
+                                                             ("flexible", 1e-11, 1e-8, 1e-9)])
+def test_c2_steady_n120_against_oracle_fixture(algorithm, rtol, obs_bar, pop_bar):
     """C2 stationary state at full size (N=120) against the oracle's weighted
-    solve (tests/golden/c2_steady_reference.json): the device solve in
-    sector-weighted coordinates accepted on the weighted relative residual
-    at rtol 1e-10 (flexible GMRES, the headline's solver, or plain
+    solve (tests/golden/c2_steady_reference.json, weighted residual 4.5e-16):
+    the device solve in sector-weighted coordinates accepted on the weighted
+    relative residual (flexible GMRES, the headline's solver, or plain
     sector-preconditioned GMRES) reproduces the oracle's collective
-    observables and every sector population."""
+    observables and every sector population.  The observable error is about
+    400 times the residual on this model (measured: 4e-8 at rtol 1e-10, 4e-9
+    at 1e-11, the headline's tolerance)."""
     import sys
     sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
     import make_c2_steady_reference as M
     gold = _c2_fixture()
-    print(f"C2 N=120 steady vs oracle fixture ({algorithm})", flush=True)
+    print(f"C2 N=120 steady vs oracle fixture ({algorithm}, rtol {rtol:g})", flush=True)
     comp = pimi.compile(_c2_model(120))
-    alg = pimi.GMRESAlgorithm(300, 600, "sector_inner", inner_steps=8) if algorithm == "flexible" else \
+    alg = pimi.GMRESAlgorithm(300, 600, "sector_inner", inner_steps=6) if algorithm == "flexible" else \
         pimi.GMRESAlgorithm(1500, 1500, "sector")
-    res = pimi.stationary_state(comp, algorithm=alg, atol=0.0, rtol=1e-10, return_info=True, criterion="weighted",
+    res = pimi.stationary_state(comp, algorithm=alg, atol=0.0, rtol=rtol, return_info=True, criterion="weighted",
                                 weighting="sector")
-    assert res.info["converged"] and res.info["weighted_relative_residual"] <= 1e-10
+    assert res.info["converged"] and res.info["weighted_relative_residual"] <= rtol
     b = L.Basis(120, 2)
     g = L.OneBodyGeometry(b)
     _, tv = comp.trace_functional()
     got = M.summarize(b, g, tv, res.state.data)
     e_obs = max(abs(got["observables"][k] - v) / abs(v) for k, v in gold["observables"].items())
     e_pop = max(abs(a - c) for a, c in zip(got["populations"], gold["populations"]))
-    parity_log.record(f"c2_n120_steady_{algorithm}_observables", e_obs, 1e-8)
-    parity_log.record(f"c2_n120_steady_{algorithm}_populations", e_pop, 1e-9)
-    assert e_obs <= 1e-8, (got["observables"], gold["observables"])
-    assert e_pop <= 1e-9
+    parity_log.record(f"c2_n120_steady_{algorithm}_rtol{rtol:g}_observables", e_obs, obs_bar)
+    parity_log.record(f"c2_n120_steady_{algorithm}_rtol{rtol:g}_populations", e_pop, pop_bar)
+    assert e_obs <= obs_bar, (got["observables"], gold["observables"])
+    assert e_pop <= pop_bar
