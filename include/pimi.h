@@ -120,7 +120,7 @@ typedef struct {
     int64_t table_bytes_c64;/* device stencil table bytes (complex FP64 tables) */
     int64_t table_bytes_c32;
     int32_t apply_form;     /* 1: banded (diagonal) apply tables, 0: ELL tables (see PIMI_APPLY_FORM) */
-    int32_t banded_groups;  /* banded form: (input sector, row shift) groups over all sectors */
+    int32_t banded_groups;  /* banded form: chains (one strip load per tile each) over all sectors */
 } pimi_plan_info;
 
 typedef enum {
@@ -224,15 +224,30 @@ pimi_status pimi_collective_operator(const pimi_plan *plan, const double *X, dou
  * positivity and trace first, as the reference does). */
 pimi_status pimi_iid_state(const pimi_plan *plan, const double *rho, double *out);
 /* The banded apply tables of a qubit plan (host copies, for checking the
- * table construction against the operator on the CPU): counts[4] = group
- * base entries (ns + 1), groups, entries, coefficient slots (complex); each
- * array argument may be NULL.  groups: 7 int64 per group (input block
- * offset, input block size, row shift, first strip column, strip span, entry
- * range [e0, e1)); ents: 6 int32 per entry (kind, strip column, qa, qb, qa2,
- * qb2); coef: complex doubles.  counts are zero for plans on ELL tables.  No
+ * table construction against the operator on the CPU): counts[4] = chain
+ * base entries (ns + 1), chains, entries, coefficient slots (complex); each
+ * array argument may be NULL.  chains: 7 int64 per chain (input block
+ * offset, input block size, row shift of the loaded strip, first strip
+ * column, strip span, entry range [e0, e1)); ents: 7 int32 per entry (kind,
+ * strip column, qa, qb, qa2, qb2, row shift beyond the chain's); coef:
+ * complex doubles.  counts are zero for plans on ELL tables.  No
  * counterpart in the reference (a device table layout). */
-pimi_status pimi_plan_banded_export(const pimi_plan *plan, int64_t *counts, int32_t *gbase, int64_t *groups,
+pimi_status pimi_plan_banded_export(const pimi_plan *plan, int64_t *counts, int32_t *cbase, int64_t *chains,
                                     int32_t *ents, double *coef);
+/* The compiled banded programs of a qubit plan (straight-line device code
+ * per sector structure, built with hiprtc when a device plan is created):
+ * the module source (copied into source when non-NULL, at most cap - 1
+ * characters and a terminating NUL; *len = its full length), the number of
+ * distinct programs and each output sector's program id (pid: ns int32,
+ * -1 = the sector's tables are interpreted; may be NULL).  compile = 1 also
+ * compiles the source for gfx950 (no device needed; the FP32 kernels of tile
+ * width 8) and reports a failure as PIMI_EDEVICE.  A device plan compiles
+ * each kernel on its first launch outside graph capture (once per process
+ * for every plan of the same structure); PIMI_DIA_PROGRAMS=0 keeps every
+ * plan on the interpreting kernels.  No counterpart in the reference (a
+ * device code generator). */
+pimi_status pimi_plan_banded_programs(const pimi_plan *plan, int32_t compile, char *source, int64_t cap,
+                                      int64_t *len, int32_t *nprograms, int32_t *pid);
 /* fused plan export of a compiled model (host arrays); lengths via pimi_plan_export_sizes */
 pimi_status pimi_plan_export_sizes(const pimi_plan *plan, int64_t *sizes /* 12 entries */);
 pimi_status pimi_plan_export(const pimi_plan *plan, int64_t *cp_base, int64_t *colptr, int32_t *rows,

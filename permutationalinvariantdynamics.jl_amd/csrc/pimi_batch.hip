@@ -60,21 +60,6 @@ struct BApply {
     double out_scale;
 };
 
-struct BDia {  // banded apply of one job (k_apply_dia's tables)
-    int64_t first;
-    const void *x;
-    void *y;
-    const int64_t *offs;
-    const int32_t *dims;
-    const ApplyTile *tiles;
-    const int32_t *gbase;
-    const DiaGroup *groups;
-    const DiaEntry *ents;
-    const void *coef;
-    int64_t n;
-    double out_scale;
-};
-
 struct BTrace {  // one block per job
     const void *z;    // input
     void *y;          // bordered: y += (t . z) v
@@ -155,37 +140,46 @@ __global__ __launch_bounds__(kApplyWave) void kb_apply(const BApply *__restrict_
                               A.dims, A.entry_base, A.entries, A.col, (const cx<R> *)A.val);
 }
 
-// The same through the banded form: kDiaWaves tiles (waves) per workgroup,
-// workgroups split into one contiguous slab per XCD; each wave finds its job.
+// The same through the banded form, interpreting the tables
 template <typename R, int TB>
 __global__ __launch_bounds__(kApplyWave *kDiaWaves) void kb_apply_dia(const BDia *__restrict__ J, int nj,
                                                                       int64_t total, int64_t per_xcd) {
-    const int64_t bid = blockIdx.x;
-    const int64_t blk = (bid % kNumXcd) * per_xcd + bid / kNumXcd;
-    const int64_t g = blk * kDiaWaves + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    if (g >= total) return;
-    const PIMI_CONST BDia *Jc = cptr(J);
-    int lo = 0, hi = nj - 1;  // the wave's job: binary search on first (scalar)
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (Jc[mid].first <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    const BDia A = Jc[lo];
-    dia_tile<R, TB>((const cx<R> *)A.x, (cx<R> *)A.y, (R)A.out_scale, cptr(A.tiles)[g - A.first], A.offs, A.dims,
-                    A.gbase, A.groups, A.ents, (const cx<R> *)A.coef);
+    dia_batch_body<DiaInterpreted, R, TB>(J, nj, total, per_xcd);
 }
 
+// S: a stencil whose compiled programs every job's main stencil shares
+// (null: interpret the tables)
 template <typename R>
-static void launch_kb_dia(hipStream_t st, const BDia *d, int nj, int64_t tot, int w) {
+static void launch_kb_dia(hipStream_t st, const BDia *d, int nj, int64_t tot, int w, const DevStencil *S) {
     const int64_t nblk = (tot + kDiaWaves - 1) / kDiaWaves;
     const int64_t per_xcd = (nblk + kNumXcd - 1) / kNumXcd;
     const dim3 grid((unsigned)(per_xcd * kNumXcd)), block(kApplyWave * kDiaWaves);
+    if (hipFunction_t f = S ? dia_program(*S, 1, sizeof(R) == 8 ? 0 : 1, w, st) : nullptr) {
+        void *args[] = {(void *)&d, (void *)&nj, (void *)&tot, (void *)&per_xcd};
+        HIPCHK(hipModuleLaunchKernel(f, grid.x, 1, 1, block.x, 1, 1, 0, st, args, nullptr));
+        return;
+    }
     switch (kDiaWidths[w]) {
         case 4: hipLaunchKernelGGL((kb_apply_dia<R, 4>), grid, block, 0, st, d, nj, tot, per_xcd); break;
         case 8: hipLaunchKernelGGL((kb_apply_dia<R, 8>), grid, block, 0, st, d, nj, tot, per_xcd); break;
         default: hipLaunchKernelGGL((kb_apply_dia<R, 16>), grid, block, 0, st, d, nj, tot, per_xcd); break;
     }
+}
+
+// a main stencil whose compiled programs those of every banded plan in
+// `plans` equal (same structure, same device), or null
+template <typename It>
+static const DevStencil *common_programs(It first, It last) {
+    const DevStencil *M = nullptr;
+    for (It it = first; it != last; ++it) {
+        const pimi_plan *P = *it;
+        if (!P->main.dia_ok) continue;
+        const DevStencil &S = P->main;
+        if (S.prog_sig.empty()) return nullptr;
+        if (M && (S.prog_sig != M->prog_sig || S.prog_device != M->prog_device)) return nullptr;
+        if (!M) M = &S;
+    }
+    return M;
 }
 
 // banded-apply descriptor of one plan (coefficients in precision R, weighted or not)
@@ -196,7 +190,7 @@ static BDia dia_desc(int64_t first, const pimi_plan *P, const void *x, void *y, 
     const void *coef;
     if constexpr (sizeof(R) == 8) coef = weighted ? (const void *)S.coefw64.p : (const void *)S.coef64.p;
     else coef = weighted ? (const void *)S.coefw32.p : (const void *)S.coef32.p;
-    return BDia{first, x, y, P->offs.p, P->dims.p, P->dtiles[w].p, S.gbase.p, S.groups.p, S.dents.p, coef, P->n,
+    return BDia{first, x, y, P->offs.p, P->dims.p, P->dtiles[w].p, S.tables(), coef, P->n,
                 out_scale};
 }
 
@@ -902,7 +896,10 @@ struct Tick {
         if (!vd.empty()) {
             const size_t offd = put(vd);
             const int njd = (int)vd.size();
-            push([=](const char *d) { launch_kb_dia<R>(st, (const BDia *)(d + offd), njd, totd, w); });
+            std::vector<const pimi_plan *> ps;
+            for (BJob *J : js) ps.push_back(J->plan);
+            const DevStencil *M = common_programs(ps.begin(), ps.end());
+            push([=](const char *d) { launch_kb_dia<R>(st, (const BDia *)(d + offd), njd, totd, w, M); });
         }
         if (v.empty()) return;
         const size_t off = put(v);
@@ -1794,9 +1791,10 @@ pimi_status pimi_batch_autotune_apply(pimi_batch *E, pimi_plan *const *plans, in
             }
             HIPCHK(hipMemcpyAsync(dvd.p, vd.data(), sizeof(BDia) * vd.size(), hipMemcpyHostToDevice, E->stream));
             const int nvd = (int)vd.size();
+            const DevStencil *M = common_programs(banded.begin(), banded.end());
             auto launch = [&] {
-                if (E->prec == PIMI_C32) launch_kb_dia<float>(E->stream, dvd.p, nvd, totd, w);
-                else launch_kb_dia<double>(E->stream, dvd.p, nvd, totd, w);
+                if (E->prec == PIMI_C32) launch_kb_dia<float>(E->stream, dvd.p, nvd, totd, w, M);
+                else launch_kb_dia<double>(E->stream, dvd.p, nvd, totd, w, M);
             };
             launch();  // warm
             HIPCHK(hipEventRecord(e0, E->stream));
@@ -1825,6 +1823,7 @@ pimi_status pimi_batch_autotune_apply(pimi_batch *E, pimi_plan *const *plans, in
 struct MultiApply {
     DevBuf<BApply> dv;
     DevBuf<BDia> dvd;
+    const DevStencil *programs = nullptr;
     int nv = 0, nvd = 0;
     int64_t tot = 0, totd = 0, per_xcd = 0, bytes = 0;
 
@@ -1832,6 +1831,7 @@ struct MultiApply {
         const size_t es = E->prec == PIMI_C32 ? sizeof(cx<float>) : sizeof(cx<double>);
         std::vector<BApply> v;
         std::vector<BDia> vd;
+        programs = common_programs(plans, plans + nplans);
         for (int32_t k = 0; k < nplans; ++k) {
             pimi_plan *P = plans[k];
             bytes += 2 * P->n * (int64_t)es + P->main.table_bytes((int)es);
@@ -1861,8 +1861,8 @@ struct MultiApply {
     }
     void launch(pimi_batch *E) const {
         if (nvd) {
-            if (E->prec == PIMI_C32) launch_kb_dia<float>(E->stream, dvd.p, nvd, totd, E->dia_w);
-            else launch_kb_dia<double>(E->stream, dvd.p, nvd, totd, E->dia_w);
+            if (E->prec == PIMI_C32) launch_kb_dia<float>(E->stream, dvd.p, nvd, totd, E->dia_w, programs);
+            else launch_kb_dia<double>(E->stream, dvd.p, nvd, totd, E->dia_w, programs);
         }
         if (nv) {
             if (E->prec == PIMI_C32)

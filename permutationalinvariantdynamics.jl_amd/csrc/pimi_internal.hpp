@@ -115,15 +115,26 @@ struct DevStencil {
     // Banded form (k_apply_dia): built when every factor is a sum of
     // diagonals and each (input sector, row shift) group's column shifts
     // span at most three diagonals; dia_ok false otherwise (the ELL entries
-    // above are then the only form).
+    // above are then the only form).  Groups of one input sector whose row
+    // shifts lie within kDiaShift and whose columns fit one strip of TB + 2
+    // share a chain (one strip load per tile).
     bool dia_ok = false;
-    std::vector<int32_t> h_gbase;  // ns + 1
-    std::vector<DiaGroup> h_groups;
+    std::vector<int32_t> h_cbase;  // ns + 1: chains of each output sector
+    std::vector<int32_t> h_pid;    // ns: compiled program of each output sector (-1: interpreted)
+    std::vector<DiaChain> h_chains;
     std::vector<DiaEntry> h_dents;
     std::vector<cd> h_coef;
     std::vector<int32_t> h_dent_sector, h_dent_in;  // per entry: output and input sector (weighting)
-    DevBuf<int32_t> gbase;
-    DevBuf<DiaGroup> groups;
+    DevBuf<int32_t> cbase;
+    DevBuf<DiaChain> chains;
+    DevBuf<int32_t> pid;  // per output sector: compiled program (-1: interpreted)
+    // compiled banded programs: the structure signature (empty: none, the
+    // tables are interpreted), the programs' source, and the kernels
+    // compiled so far, [0: single-vector, 1: batched][0: FP64, 1: FP32][tile width]
+    std::string prog_sig, prog_src;
+    int prog_device = -1;
+    mutable std::atomic<hipFunction_t> prog_fn[2][2][kNumDiaWidths] = {};
+    DiaTables tables() const { return DiaTables{cbase.p, chains.p, dents.p, pid.p}; }
     DevBuf<DiaEntry> dents;
     DevBuf<cx<double>> coef64, coefw64;
     DevBuf<cx<float>> coef32, coefw32;
@@ -135,8 +146,8 @@ struct DevStencil {
         // independent kernel for the same operator)
         if (const char *f = std::getenv("PIMI_APPLY_FORM"))
             if (std::string(f) == "ell") return;
-        h_gbase.assign(1, 0);
-        h_groups.clear();
+        h_cbase.assign(1, 0);
+        h_chains.clear();
         h_dents.clear();
         h_coef.clear();
         h_dent_sector.clear();
@@ -207,6 +218,12 @@ struct DevStencil {
                         pend[g].push_back(std::move(E));
                     }
             }
+            struct GroupOut {  // one (input sector, row shift) group, entries ready
+                int64_t in_off;
+                int32_t mn, dA, c0, span, nu;
+                std::vector<DiaEntry> ents;
+            };
+            std::vector<GroupOut> outs;
             for (size_t g = 0; g < keys.size(); ++g) {
                 int lo = INT32_MAX, hi = INT32_MIN;
                 for (auto &E : pend[g]) {
@@ -215,8 +232,7 @@ struct DevStencil {
                 }
                 if (hi - lo > 2) return;  // not banded enough: ELL form only
                 const int nu = keys[g].first;
-                DiaGroup G{offs[(size_t)nu], (int32_t)dims[(size_t)nu], keys[g].second, lo, hi - lo,
-                           (int32_t)h_dents.size(), 0};
+                GroupOut O{offs[(size_t)nu], (int32_t)dims[(size_t)nu], keys[g].second, lo, hi - lo, nu, {}};
                 // every coefficient vector starts on a kDiaCoefPad boundary and
                 // is zero-padded to a multiple of it, so a tile reads its TB
                 // column coefficients as one unclamped contiguous (scalar) load
@@ -280,7 +296,7 @@ struct DevStencil {
                     merged.push_back(std::move(E));
                 }
                 for (auto &E : merged) {
-                    DiaEntry D{E.kind, E.dB - lo, 0, 0, 0, 0};
+                    DiaEntry D{E.kind, E.dB - lo, 0, 0, 0, 0, 0, 0};
                     const bool real_rows = E.kind == 3 || E.kind == 4 || E.kind == 7;
                     const bool real_cols = E.kind == 3 || E.kind == 5 || E.kind == 7;
                     if (E.kind != 1 && E.kind != 5) D.qa = put(real_rows ? packed(E.ca) : E.ca);
@@ -290,14 +306,47 @@ struct DevStencil {
                         D.qb2 = put(packed(E.wb2));
                     }
                     if (h_coef.size() >= (size_t)INT32_MAX) return;
-                    h_dents.push_back(D);
-                    h_dent_sector.push_back(q);
-                    h_dent_in.push_back(nu);
+                    O.ents.push_back(D);
                 }
-                G.e1 = (int32_t)h_dents.size();
-                h_groups.push_back(G);
+                outs.push_back(std::move(O));
             }
-            h_gbase.push_back((int32_t)h_groups.size());
+            // chains: the sector's groups ordered by (input block, row shift),
+            // first-appearance order among equals; a chain takes the next
+            // group while it reads the same block, its row shift stays within
+            // kDiaShift of the chain's first and the column union stays
+            // within TB + 2
+            std::vector<size_t> order(outs.size());
+            for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+            std::stable_sort(order.begin(), order.end(), [&](size_t u, size_t v) {
+                return outs[u].in_off != outs[v].in_off ? outs[u].in_off < outs[v].in_off : outs[u].dA < outs[v].dA;
+            });
+            for (size_t i = 0; i < order.size();) {
+                const GroupOut &F = outs[order[i]];
+                int lo = F.c0, hi = F.c0 + F.span;
+                size_t j = i + 1;
+                for (; j < order.size(); ++j) {
+                    const GroupOut &H = outs[order[j]];
+                    const int nlo = std::min(lo, H.c0), nhi = std::max(hi, H.c0 + H.span);
+                    if (H.in_off != F.in_off || H.dA - F.dA > kDiaShift || nhi - nlo > 2) break;
+                    lo = nlo;
+                    hi = nhi;
+                }
+                DiaChain C{F.in_off, F.mn, F.dA, lo, hi - lo, (int32_t)h_dents.size(), 0};
+                for (size_t k = i; k < j; ++k) {
+                    const GroupOut &H = outs[order[k]];
+                    for (DiaEntry D : H.ents) {
+                        D.dB += H.c0 - lo;  // strip column relative to the chain's first
+                        D.sh = H.dA - F.dA;
+                        h_dents.push_back(D);
+                        h_dent_sector.push_back(q);
+                        h_dent_in.push_back(H.nu);
+                    }
+                }
+                C.e1 = (int32_t)h_dents.size();
+                h_chains.push_back(C);
+                i = j;
+            }
+            h_cbase.push_back((int32_t)h_chains.size());
         }
         if (h_coef.empty()) h_coef.push_back(cd(0.0, 0.0));
         dia_ok = true;
@@ -310,9 +359,11 @@ struct DevStencil {
             c64[k] = cx<double>{h_coef[k].real(), h_coef[k].imag()};
             c32[k] = cx<float>{(float)h_coef[k].real(), (float)h_coef[k].imag()};
         }
-        gbase.upload(h_gbase.data(), h_gbase.size());
-        if (h_groups.empty()) groups.alloc(1);  // no entries: the kernel reads no group
-        else groups.upload(h_groups.data(), h_groups.size());
+        cbase.upload(h_cbase.data(), h_cbase.size());
+        if (h_pid.size() + 1 != h_cbase.size()) h_pid.assign(h_cbase.size() - 1, -1);
+        pid.upload(h_pid.data(), h_pid.size());
+        if (h_chains.empty()) chains.alloc(1);  // no entries: the kernel reads no chain
+        else chains.upload(h_chains.data(), h_chains.size());
         if (h_dents.empty()) dents.alloc(1);
         else dents.upload(h_dents.data(), h_dents.size());
         coef64.upload(c64.data(), c64.size());
@@ -354,8 +405,8 @@ struct DevStencil {
     std::vector<int64_t> h_dims;  // sector sizes (banded coefficient vector lengths)
     int64_t dims_of_entry(size_t e) const { return h_dims[(size_t)h_dent_sector[e]]; }
     int64_t dia_table_bytes(int elem) const {
-        return (int64_t)(h_groups.size() * sizeof(DiaGroup) + h_dents.size() * sizeof(DiaEntry) +
-                         h_gbase.size() * 4) +
+        return (int64_t)(h_chains.size() * sizeof(DiaChain) + h_dents.size() * sizeof(DiaEntry) +
+                         h_cbase.size() * 4) +
                (int64_t)h_coef.size() * elem;
     }
 
@@ -535,5 +586,31 @@ size_t dcgs_single_desc_bytes(int m);
 void dcgs_single_descs(void *host, int m, void *V, const void *w, void *partial, void *mail, void *h, double *slot,
                        int64_t n);
 void dcgs_single_step(hipStream_t st, const void *desc_dev, int j, int64_t n);
+
+// Set up the banded programs of stencil S (of a plan on device `device`)
+// when wanted: per-sector program ids and the programs' source; kernels are
+// compiled on first use (dia_program).  No-op for stencils on the ELL form.
+void dia_setup_programs(DevStencil &S, int device);
+// The compiled kernel of S's programs for (form 0 single-vector / 1 batched,
+// precision 0 FP64 / 1 FP32, tile width index w), compiled now if need be;
+// null when S has no programs or `stream` is capturing a graph and the
+// kernel is not compiled yet (the caller interprets the tables).
+hipFunction_t dia_program(const DevStencil &S, int form, int prec, int w, hipStream_t stream);
+
+// Launch the banded apply of y = out_scale * S x on ncols vectors through
+// S's compiled programs (true) or report that it has none for this
+// precision and width (false).
+template <typename R>
+bool launch_dia_program(hipStream_t stream, const DevStencil &S, int w, const cx<R> *x, cx<R> *y, int64_t n,
+                        const ApplyTile *tiles, int64_t nt, int64_t per_xcd, const int64_t *offs, const int32_t *dims,
+                        const cx<R> *coef, R out_scale, int64_t ncols, dim3 grid, dim3 block) {
+    hipFunction_t f = dia_program(S, 0, sizeof(R) == 8 ? 0 : 1, w, stream);
+    if (!f) return false;
+    DiaTables T = S.tables();
+    void *args[] = {(void *)&x,    (void *)&y,    (void *)&n, (void *)&tiles, (void *)&nt,        (void *)&per_xcd,
+                    (void *)&offs, (void *)&dims, (void *)&T, (void *)&coef,  (void *)&out_scale, (void *)&ncols};
+    HIPCHK(hipModuleLaunchKernel(f, grid.x, grid.y, grid.z, block.x, block.y, block.z, 0, stream, args, nullptr));
+    return true;
+}
 
 }  // namespace pimi

@@ -1,55 +1,15 @@
 // Device kernels for the PI Liouvillian apply and the Krylov BLAS-1/2 steps.
 //
-// Complex vectors are arrays of cx<R> (R = double or float), 8/16-byte pairs.
 // All reductions are fixed-shape trees (wave shuffles + LDS), so results are
 // deterministic from run to run.
 #pragma once
 
-#include <hip/hip_runtime.h>
-#include <stdint.h>
-
 #include <type_traits>
 
+#include "pimi_base.hpp"
+#include "pimi_dia.hpp"
+
 namespace pimi {
-
-template <typename R>
-struct alignas(2 * sizeof(R)) cx {
-    R x, y;
-};
-
-template <typename R>
-__device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) { return {a.x + b.x, a.y + b.y}; }
-template <typename R>
-__device__ __forceinline__ cx<R> csub(cx<R> a, cx<R> b) { return {a.x - b.x, a.y - b.y}; }
-template <typename R>
-__device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
-// conj(a) * b
-template <typename R>
-__device__ __forceinline__ cx<R> cjmul(cx<R> a, cx<R> b) { return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }
-
-// Address spaces: tables that no kernel writes while it runs are read through
-// the constant address space (wave-uniform reads become scalar loads, also
-// when the table pointer itself was loaded from a descriptor); vectors
-// through the global one (global_load, not flat_load).
-// (The attributes apply to the device pass only; the host pass sees plain
-// pointers.)
-#if defined(__HIP_DEVICE_COMPILE__)
-#define PIMI_CONST __attribute__((address_space(4)))
-#define PIMI_GLOBAL __attribute__((address_space(1)))
-#else
-#define PIMI_CONST
-#define PIMI_GLOBAL
-#endif
-template <typename T>
-__device__ __forceinline__ const PIMI_CONST T *cptr(const T *p) { return (const PIMI_CONST T *)p; }
-template <typename T>
-__device__ __forceinline__ const PIMI_GLOBAL T *gptr(const T *p) { return (const PIMI_GLOBAL T *)p; }
-template <typename T>
-__device__ __forceinline__ PIMI_GLOBAL T *gptr_w(T *p) { return (PIMI_GLOBAL T *)p; }
-
-constexpr int kApplyBlock = 256;
-constexpr int kRedBlock = 256;
-constexpr int kNumXcd = 8;
 
 // One stencil entry of an output sector: its contribution to Y[a, b] is
 //   s * A[a] * conj(B[b]) * X_in[colA(a), colB(b)],
@@ -78,10 +38,6 @@ constexpr int kEntryChunk = 8;
 // loads); the A-side (row) slots are loaded once per entry for all nb
 // columns of a lane.  Every output sums its entries in list order, as
 // (s * conj(B)) * A * X.
-struct ApplyTile {
-    int32_t sector, b0, a0, nb;
-};
-constexpr int kApplyWave = 64;
 constexpr int kApplyCols = 4;  // columns per lane of the batched apply (one for a single vector)
 
 template <typename R, int CB>
@@ -168,307 +124,15 @@ __global__ __launch_bounds__(kApplyWave) void k_apply(
         apply_tile<R, 1>(x + c * n, y + c * n, out_scale, tile, offs, dims, entry_base, entries, col, val);
 }
 
-// ---------------------------------------------------------------------------
-// Banded (diagonal) form of the stencil apply.
-//
-// When every factor of every term is a sum of a few diagonals,
-//   A = sum_dA diag-shift(cA_dA),  A[a, a + dA] = cA_dA[a],
-// a term s A X_in B^H adds to Y[a, b] the products
-//   s cA_dA[a] conj(cB_dB[b]) X_in[a + dA, b + dB].
-// The qubit sectors are of this form (H, Q and J- tridiagonal or narrower,
-// every one-body contraction one diagonal).  Entries are grouped by (input
-// sector, dA); a group's lanes read one strip X_in[a + dA, b0 + c0 + k],
-// k = 0 .. TB + span - 1, with plain coalesced loads (no column tables), and
-// every entry of the group combines the strip at its own column offset
-// dB - c0 in 0..2.  Coefficient vectors are zero wherever a + dA or b + dB
-// leaves the input block, so clamped load indices never change a result.
-// Kinds:  left (B = I): Y += cA'[a] X;  right (A = I): Y += w[b] X;
-//         sandwich: Y += (cA'[a] w[b]) X;  real sandwich: the same with real
-//         cA', w (two real FMAs per product instead of four complex ones);
-//         imaginary left / right: a left / right entry whose coefficients
-//         are purely imaginary (i c X: two real FMAs), stored as the real c;
-//         left + right: a left and a right entry on the same strip column,
-//         (cA'[a] + w[b]) X;  real sandwich pair: two real sandwiches on the
-//         same strip column, (cA1[a] w1[b] + cA2[a] w2[b]) X.
-// cA' carries the term scale, w[b] = conj(cB[b]) (times the scale for right).
-// For qubit plans (tridiagonal H, diagonal loss, one-diagonal contractions)
-// the builder turns the off-diagonal left/right entries into imaginary
-// ones, merges the two diagonal ones, and pairs the two children's
-// sandwiches of each branch, which takes about a third of the arithmetic.
-// Every output sums its groups in list order and each group's entries in
-// list order with explicit FMAs, so the result of an output does not depend
-// on the tile width TB.
-struct DiaGroup {
-    int64_t in_off;     // offset of the input block
-    int32_t mn;         // input block size
-    int32_t dA;         // row shift of the strip
-    int32_t c0;         // first strip column relative to the tile's first column
-    int32_t span;       // strip columns beyond TB (0..2)
-    int32_t e0, e1;     // entry range
-};
-struct DiaEntry {
-    int32_t kind;       // 0 left, 1 right, 2 sandwich, 3 real sandwich, 4 imaginary left, 5 imaginary right,
-                        // 6 left + right on one strip column, 7 two real sandwiches on one strip column
-    int32_t dB;         // strip column offset, 0..2
-    int32_t qa, qb;     // coefficient vectors: per output row (A side), per output column (B side)
-    int32_t qa2, qb2;   // kind 7: the second sandwich's vectors
-};
-constexpr int kDiaWaves = 4;  // tiles (waves) per workgroup
-constexpr int kNumDiaWidths = 3;
-constexpr int kDiaWidths[kNumDiaWidths] = {4, 8, 16};  // tile columns TB
-constexpr int kDiaCoefPad = 16;  // coefficient vectors: aligned and zero-padded to this many elements
-
-__device__ __forceinline__ float rfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ double rfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
-
-template <typename R>
-__device__ __forceinline__ cx<R> cfma(cx<R> a, cx<R> b, cx<R> c) {
-    c.x = rfma(a.x, b.x, c.x);
-    c.x = rfma(-a.y, b.y, c.x);
-    c.y = rfma(a.x, b.y, c.y);
-    c.y = rfma(a.y, b.x, c.y);
-    return c;
-}
-template <typename R>
-__device__ __forceinline__ cx<R> cmulf(cx<R> a, cx<R> b) {
-    return cx<R>{rfma(a.x, b.x, -a.y * b.y), rfma(a.x, b.y, a.y * b.x)};
-}
-
-
-// TB consecutive column coefficients, read as one block (complex, or the
-// packed reals of a real sandwich)
-template <typename R, int TB>
-struct DiaCols {
-    cx<R> v[TB];
-};
-template <typename R, int TB>
-struct DiaColsR {
-    R v[TB];
-};
-
-// Per-lane (row) coefficients come through a raw buffer over the pool: the
-// lane's offset (a times the element size) is fixed for the tile, the
-// vector's start is a scalar offset, so a load needs no address arithmetic.
-template <typename R>
-__device__ __forceinline__ cx<R> dia_row(__amdgpu_buffer_rsrc_t pool, int voff, int q) {
-    if constexpr (sizeof(R) == 8)
-        return __builtin_bit_cast(cx<R>, __builtin_amdgcn_raw_buffer_load_b128(pool, voff, q * 16, 0));
-    else
-        return __builtin_bit_cast(cx<R>, __builtin_amdgcn_raw_buffer_load_b64(pool, voff, q * 8, 0));
-}
-template <typename R>
-__device__ __forceinline__ R dia_row_real(__amdgpu_buffer_rsrc_t pool, int voff_real, int q) {
-    if constexpr (sizeof(R) == 8)
-        return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(pool, voff_real, q * 16, 0));
-    else
-        return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(pool, voff_real, q * 8, 0));
-}
-
-template <typename R, int TB, int KIND, int DB>
-__device__ __forceinline__ void dia_entry(cx<R> (&acc)[TB], const cx<R> (&st)[TB + 2], const DiaEntry &E,
-                                          __amdgpu_buffer_rsrc_t pool, const PIMI_CONST cx<R> *coefs, int va,
-                                          int vr, int b0) {
-    if constexpr (KIND == 4) {  // i c[a] X
-        const R c = dia_row_real<R>(pool, vr, E.qa);
-#pragma unroll
-        for (int j = 0; j < TB; ++j) {
-            acc[j].x = rfma(-c, st[j + DB].y, acc[j].x);
-            acc[j].y = rfma(c, st[j + DB].x, acc[j].y);
-        }
-    } else if constexpr (KIND == 5) {  // i w[b] X
-        const DiaColsR<R, TB> w = *(const PIMI_CONST DiaColsR<R, TB> *)((const PIMI_CONST R *)(coefs + E.qb) + b0);
-#pragma unroll
-        for (int j = 0; j < TB; ++j) {
-            acc[j].x = rfma(-w.v[j], st[j + DB].y, acc[j].x);
-            acc[j].y = rfma(w.v[j], st[j + DB].x, acc[j].y);
-        }
-    } else if constexpr (KIND == 6) {  // (cA[a] + w[b]) X
-        const cx<R> ca = dia_row<R>(pool, va, E.qa);
-        const DiaCols<R, TB> w = *(const PIMI_CONST DiaCols<R, TB> *)(coefs + E.qb + b0);
-#pragma unroll
-        for (int j = 0; j < TB; ++j) acc[j] = cfma(cx<R>{ca.x + w.v[j].x, ca.y + w.v[j].y}, st[j + DB], acc[j]);
-    } else if constexpr (KIND == 7) {  // (cA1[a] w1[b] + cA2[a] w2[b]) X, all real
-        const R c1 = dia_row_real<R>(pool, vr, E.qa), c2 = dia_row_real<R>(pool, vr, E.qa2);
-        const DiaColsR<R, TB> w1 = *(const PIMI_CONST DiaColsR<R, TB> *)((const PIMI_CONST R *)(coefs + E.qb) + b0);
-        const DiaColsR<R, TB> w2 = *(const PIMI_CONST DiaColsR<R, TB> *)((const PIMI_CONST R *)(coefs + E.qb2) + b0);
-#pragma unroll
-        for (int j = 0; j < TB; ++j) {
-            const R c = rfma(c1, w1.v[j], c2 * w2.v[j]);
-            acc[j].x = rfma(c, st[j + DB].x, acc[j].x);
-            acc[j].y = rfma(c, st[j + DB].y, acc[j].y);
-        }
-    } else if constexpr (KIND == 0) {
-        const cx<R> ca = dia_row<R>(pool, va, E.qa);
-#pragma unroll
-        for (int j = 0; j < TB; ++j) acc[j] = cfma(ca, st[j + DB], acc[j]);
-    } else if constexpr (KIND == 3) {
-        const DiaColsR<R, TB> w = *(const PIMI_CONST DiaColsR<R, TB> *)((const PIMI_CONST R *)(coefs + E.qb) + b0);
-        const R ca = dia_row_real<R>(pool, vr, E.qa);
-#pragma unroll
-        for (int j = 0; j < TB; ++j) {
-            const R c = ca * w.v[j];
-            acc[j].x = rfma(c, st[j + DB].x, acc[j].x);
-            acc[j].y = rfma(c, st[j + DB].y, acc[j].y);
-        }
-    } else {
-        const DiaCols<R, TB> w = *(const PIMI_CONST DiaCols<R, TB> *)(coefs + E.qb + b0);
-        if constexpr (KIND == 1) {
-#pragma unroll
-            for (int j = 0; j < TB; ++j) acc[j] = cfma(w.v[j], st[j + DB], acc[j]);
-        } else {
-            const cx<R> ca = dia_row<R>(pool, va, E.qa);
-#pragma unroll
-            for (int j = 0; j < TB; ++j) acc[j] = cfma(cmulf(ca, w.v[j]), st[j + DB], acc[j]);
-        }
-    }
-}
-
-// the strip X_in[r, cfirst + k], k < NK, of one group through a raw buffer
-// over the input block, the whole offset ((clamped) row + column times the
-// block size) in voffset, which the hardware range check covers: a column
-// outside the block gives an offset outside the resource (a negative one
-// reads as a huge unsigned offset), which reads zero -- exactly the zero the
-// coefficient vectors hold there.  (The scalar offset operand is not used:
-// the range check need not cover it.)  0x00020000: the gfx9-family word 3 of
-// a raw buffer resource.
-template <typename R, int TB, int NK>
-__device__ __forceinline__ void dia_strip(cx<R> (&st)[TB + 2], const cx<R> *Xn, int r, int mn, int cfirst) {
-    constexpr int es = (int)sizeof(cx<R>);
-    const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc((void *)Xn, (short)0, mn * mn * es, 0x00020000);
-    const int vr = r * es;
-    const int step = mn * es;
-    const int s0 = cfirst * step;
-#pragma unroll
-    for (int k = 0; k < TB + 2; ++k) {
-        const int vo = vr + (s0 + k * step);
-        if constexpr (sizeof(R) == 8) {
-            st[k] = k < NK ? __builtin_bit_cast(cx<R>, __builtin_amdgcn_raw_buffer_load_b128(rs, vo, 0, 0))
-                           : cx<R>{0, 0};
-        } else {
-            st[k] = k < NK ? __builtin_bit_cast(cx<R>, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, 0, 0))
-                           : cx<R>{0, 0};
-        }
-    }
-}
-
-// One banded tile: rows a0..a0+63 (one per lane) of columns b0..b0+nb-1
-// (nb <= TB) of sector s's output block.  Everything but the lane's row is
-// wave-uniform: descriptors and column coefficients are scalar loads, the
-// entry switch a scalar branch.
-//
-// SPLIT > 1: the tile's groups are dealt to the SPLIT waves of the workgroup
-// (wave w takes groups w, w + SPLIT, ...), which shortens each wave's chain of
-// dependent strip loads; the partial sums meet in LDS and wave 0 adds them in
-// the order w = 0 .. SPLIT-1 (deterministic).  Used for the single-solve
-// paths, where one vector has too few tiles to hide the chain's latency.
-template <typename R, int TB, int SPLIT = 1>
-__device__ __forceinline__ void dia_tile(const cx<R> *xc_, cx<R> *yc_, R out_scale, ApplyTile tile,
-                                         const int64_t *offs_, const int32_t *dims_, const int32_t *gbase_,
-                                         const DiaGroup *groups_, const DiaEntry *ents_, const cx<R> *coef_) {
-    PIMI_GLOBAL cx<R> *yc = gptr_w(yc_);
-    const PIMI_CONST int64_t *offs = cptr(offs_);
-    const PIMI_CONST int32_t *dims = cptr(dims_);
-    const PIMI_CONST int32_t *gbase = cptr(gbase_);
-    const PIMI_CONST DiaGroup *groups = cptr(groups_);
-    const PIMI_CONST DiaEntry *ents = cptr(ents_);
-    const PIMI_CONST cx<R> *coefs = cptr(coef_);
-    const int s = __builtin_amdgcn_readfirstlane(tile.sector);
-    const int b0 = __builtin_amdgcn_readfirstlane(tile.b0);
-    const int a0 = __builtin_amdgcn_readfirstlane(tile.a0);
-    const int nb = __builtin_amdgcn_readfirstlane(tile.nb);
-    const int m = dims[s];
-    const int arow = a0 + (int)(threadIdx.x & (kApplyWave - 1));
-    const bool active = arow < m;
-    const int a = active ? arow : m - 1;  // inactive lanes evaluate the last row and store nothing
-    // the coefficient pool as one raw buffer (every read is in range; the
-    // pool is far below 2 GiB), and the lane's byte offsets into a vector:
-    // complex element a, and real element a of a packed real vector
-    const __amdgpu_buffer_rsrc_t pool =
-        __builtin_amdgcn_make_buffer_rsrc((void *)coef_, (short)0, 0x7fffffff, 0x00020000);
-    const int va = a * (int)sizeof(cx<R>), vr = a * (int)sizeof(R);
-    cx<R> acc[TB];
-#pragma unroll
-    for (int j = 0; j < TB; ++j) acc[j] = cx<R>{0, 0};
-    const int wave = SPLIT > 1 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
-    const int g0 = gbase[s], g1 = gbase[s + 1];
-    for (int g = g0 + wave; g < g1; g += SPLIT) {
-        const DiaGroup G = groups[g];
-        cx<R> st[TB + 2];
-        const int r = min(max(a + G.dA, 0), G.mn - 1);
-        if (G.span == 0) dia_strip<R, TB, TB>(st, xc_ + G.in_off, r, G.mn, b0 + G.c0);
-        else dia_strip<R, TB, TB + 2>(st, xc_ + G.in_off, r, G.mn, b0 + G.c0);
-        for (int e = G.e0; e < G.e1; ++e) {
-            const DiaEntry E = ents[e];
-            switch (E.kind * 4 + E.dB) {
-                case 0: dia_entry<R, TB, 0, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 1: dia_entry<R, TB, 0, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 2: dia_entry<R, TB, 0, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 4: dia_entry<R, TB, 1, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 5: dia_entry<R, TB, 1, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 6: dia_entry<R, TB, 1, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 8: dia_entry<R, TB, 2, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 9: dia_entry<R, TB, 2, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 10: dia_entry<R, TB, 2, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 12: dia_entry<R, TB, 3, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 13: dia_entry<R, TB, 3, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 14: dia_entry<R, TB, 3, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 16: dia_entry<R, TB, 4, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 17: dia_entry<R, TB, 4, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 18: dia_entry<R, TB, 4, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 20: dia_entry<R, TB, 5, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 21: dia_entry<R, TB, 5, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 22: dia_entry<R, TB, 5, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 24: dia_entry<R, TB, 6, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 25: dia_entry<R, TB, 6, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 26: dia_entry<R, TB, 6, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 28: dia_entry<R, TB, 7, 0>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 29: dia_entry<R, TB, 7, 1>(acc, st, E, pool, coefs, va, vr, b0); break;
-                case 30: dia_entry<R, TB, 7, 2>(acc, st, E, pool, coefs, va, vr, b0); break;
-                default: break;
-            }
-        }
-    }
-    if constexpr (SPLIT > 1) {
-        __shared__ cx<R> part[SPLIT - 1][TB][kApplyWave];
-        const int lane = threadIdx.x & (kApplyWave - 1);
-        if (wave > 0)
-#pragma unroll
-            for (int j = 0; j < TB; ++j) part[wave - 1][j][lane] = acc[j];
-        __syncthreads();
-        if (wave == 0)
-#pragma unroll
-            for (int w = 0; w < SPLIT - 1; ++w)
-#pragma unroll
-                for (int j = 0; j < TB; ++j) acc[j] = cadd(acc[j], part[w][j][lane]);
-        __syncthreads();  // the LDS is free for the workgroup's next column vector
-        if (wave > 0) return;
-    }
-    if (active) {
-        const int64_t o = offs[s] + a;
-#pragma unroll
-        for (int j = 0; j < TB; ++j)
-            if (j < nb) yc[o + (int64_t)(b0 + j) * m] = cx<R>{out_scale * acc[j].x, out_scale * acc[j].y};
-    }
-}
-
-// y = out_scale * L x for ncols column vectors through the banded form, one
-// tile per workgroup of kDiaSplit waves that share the tile's groups; tiles
-// spread so that XCD k (workgroup id mod 8) handles one contiguous slab of
-// the sector-major tile list.
-constexpr int kDiaSplit = 4;
+// the banded apply through the stencil's tables (programs compiled for a
+// plan run the same body, see the plan's compiled banded module)
 template <typename R, int TB>
 __global__ __launch_bounds__(kApplyWave *kDiaSplit) void k_apply_dia_split(
     const cx<R> *__restrict__ x, cx<R> *__restrict__ y, int64_t n, const ApplyTile *__restrict__ tiles,
     int64_t ntiles, int64_t blocks_per_xcd, const int64_t *__restrict__ offs, const int32_t *__restrict__ dims,
-    const int32_t *__restrict__ gbase, const DiaGroup *__restrict__ groups, const DiaEntry *__restrict__ ents,
-    const cx<R> *__restrict__ coef, R out_scale, int64_t ncols) {
-    const int64_t bid = blockIdx.x;
-    const int64_t t = (bid % kNumXcd) * blocks_per_xcd + bid / kNumXcd;
-    if (t >= ntiles) return;  // the whole workgroup leaves together
-    const ApplyTile tile = cptr(tiles)[t];
-    for (int64_t c = blockIdx.y; c < ncols; c += gridDim.y)
-        dia_tile<R, TB, kDiaSplit>(x + c * n, y + c * n, out_scale, tile, offs, dims, gbase, groups, ents, coef);
+    DiaTables T, const cx<R> *__restrict__ coef, R out_scale, int64_t ncols) {
+    dia_split_body<DiaInterpreted, R, TB>(x, y, n, tiles, ntiles, blocks_per_xcd, offs, dims, T, coef, out_scale,
+                                          ncols);
 }
 
 // y[i] = x[i] * w[sector of i], one coordinate per lane over the plan's tiles

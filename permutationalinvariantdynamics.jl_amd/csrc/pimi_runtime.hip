@@ -485,17 +485,17 @@ static void finalize_plan(pimi_plan *P) {
         tsec[(size_t)t] = s;
     }
     // apply tiles: sector-major, then column groups, then 64-row chunks
-    auto make_tiles = [&](int cols) {
+    auto make_tiles = [&](int cols, int rows) {
         std::vector<ApplyTile> t;
         for (int q = 0; q < F.ns; ++q) {
             const int32_t m = (int32_t)F.dims[q];
             for (int32_t b0 = 0; b0 < m; b0 += cols)
-                for (int32_t a0 = 0; a0 < m; a0 += kApplyWave)
+                for (int32_t a0 = 0; a0 < m; a0 += rows)
                     t.push_back(ApplyTile{q, b0, a0, std::min<int32_t>(cols, m - b0)});
         }
         return t;
     };
-    std::vector<ApplyTile> at = make_tiles(1), at4 = make_tiles(kApplyCols);
+    std::vector<ApplyTile> at = make_tiles(1, kApplyWave), at4 = make_tiles(kApplyCols, kApplyWave);
     P->natiles = (int64_t)at.size();
     P->natiles4 = (int64_t)at4.size();
     P->atiles_per_xcd = (P->natiles + kNumXcd - 1) / kNumXcd;
@@ -519,13 +519,14 @@ static void finalize_plan(pimi_plan *P) {
     P->atiles.upload(at.data(), at.size());
     P->atiles4.upload(at4.data(), at4.size());
     for (int w = 0; w < kNumDiaWidths; ++w) {
-        std::vector<ApplyTile> dt = make_tiles(kDiaWidths[w]);
+        std::vector<ApplyTile> dt = make_tiles(kDiaWidths[w], kDiaRows);
         P->ndtiles[w] = (int64_t)dt.size();
         P->dtiles[w].upload(dt.data(), dt.size());
     }
     P->dia_w_single[0] = P->dia_w_single[1] = dia_width_default("PIMI_DIA_TB_SINGLE", 0);
     P->dia_w_multi[0] = P->dia_w_multi[1] = dia_width_default("PIMI_DIA_TB_MULTI", 1);
     P->main.push();
+    dia_setup_programs(P->main, P->ctx->device);
     P->d_tidx.upload(P->tidx.data(), P->tidx.size());
     P->d_tval.upload(P->tval.data(), P->tval.size());
 }
@@ -539,6 +540,7 @@ static void ensure_diag_stencil(pimi_plan *P) {
     Stencil st;
     build_stencil(P->host, st, true);
     P->diag.upload(st, P->host.dims, P->host.offs);
+    dia_setup_programs(P->diag, P->ctx->device);
     P->diag_ready = true;
 }
 
@@ -550,6 +552,7 @@ static void ensure_adjoint_stencil(pimi_plan *P) {
     Stencil st;
     build_stencil(A, st);
     P->adj.upload(st, P->host.dims, P->host.offs);
+    dia_setup_programs(P->adj, P->ctx->device);
     P->adj_ready = true;
 }
 
@@ -570,9 +573,12 @@ static void launch_dia(hipStream_t stream, const pimi_plan *P, const DevStencil 
     const int64_t per_xcd = (nt + kNumXcd - 1) / kNumXcd;
     dim3 grid((unsigned)(per_xcd * kNumXcd), (unsigned)std::min<int64_t>(ncols, 65535));
     dim3 block(kApplyWave * kDiaSplit);
+    if (launch_dia_program<R>(stream, S, w, x, y, P->n, P->dtiles[w].p, nt, per_xcd, P->offs.p, P->dims.p, coef,
+                              (R)out_scale, ncols, grid, block))
+        return;
 #define PIMI_DIA_LAUNCH(TB)                                                                                   \
     hipLaunchKernelGGL((k_apply_dia_split<R, TB>), grid, block, 0, stream, x, y, P->n, P->dtiles[w].p, nt,     \
-                       per_xcd, P->offs.p, P->dims.p, S.gbase.p, S.groups.p, S.dents.p, coef, (R)out_scale, ncols)
+                       per_xcd, P->offs.p, P->dims.p, S.tables(), coef, (R)out_scale, ncols)
     switch (kDiaWidths[w]) {
         case 4: PIMI_DIA_LAUNCH(4); break;
         case 8: PIMI_DIA_LAUNCH(8); break;
@@ -2362,7 +2368,7 @@ pimi_status pimi_plan_info_get(const pimi_plan *P, pimi_plan_info *info) {
         info->table_bytes_c64 = P->main.table_bytes(16);
         info->table_bytes_c32 = P->main.table_bytes(8);
         info->apply_form = P->main.dia_ok ? 1 : 0;
-        info->banded_groups = P->main.dia_ok ? (int32_t)P->main.h_groups.size() : 0;
+        info->banded_groups = P->main.dia_ok ? (int32_t)P->main.h_chains.size() : 0;
     });
 }
 
@@ -2408,7 +2414,7 @@ pimi_status pimi_iid_state(const pimi_plan *P, const double *rho, double *out) {
     });
 }
 
-pimi_status pimi_plan_banded_export(const pimi_plan *P, int64_t *counts, int32_t *gbase, int64_t *groups,
+pimi_status pimi_plan_banded_export(const pimi_plan *P, int64_t *counts, int32_t *cbase, int64_t *chains,
                                     int32_t *ents, double *coef) {
     return guarded([&] {
         if (!P || !counts) throw Error(PIMI_EINVAL, "null argument");
@@ -2417,22 +2423,22 @@ pimi_status pimi_plan_banded_export(const pimi_plan *P, int64_t *counts, int32_t
             counts[0] = counts[1] = counts[2] = counts[3] = 0;
             return;
         }
-        counts[0] = (int64_t)S.h_gbase.size();
-        counts[1] = (int64_t)S.h_groups.size();
+        counts[0] = (int64_t)S.h_cbase.size();
+        counts[1] = (int64_t)S.h_chains.size();
         counts[2] = (int64_t)S.h_dents.size();
         counts[3] = (int64_t)S.h_coef.size();
-        if (gbase) std::copy(S.h_gbase.begin(), S.h_gbase.end(), gbase);
-        if (groups)
-            for (size_t g = 0; g < S.h_groups.size(); ++g) {
-                const DiaGroup &G = S.h_groups[g];
+        if (cbase) std::copy(S.h_cbase.begin(), S.h_cbase.end(), cbase);
+        if (chains)
+            for (size_t g = 0; g < S.h_chains.size(); ++g) {
+                const DiaChain &G = S.h_chains[g];
                 const int64_t v[7] = {G.in_off, G.mn, G.dA, G.c0, G.span, G.e0, G.e1};
-                std::copy(v, v + 7, groups + 7 * g);
+                std::copy(v, v + 7, chains + 7 * g);
             }
         if (ents)
             for (size_t e = 0; e < S.h_dents.size(); ++e) {
                 const DiaEntry &D = S.h_dents[e];
-                const int32_t v[6] = {D.kind, D.dB, D.qa, D.qb, D.qa2, D.qb2};
-                std::copy(v, v + 6, ents + 6 * e);
+                const int32_t v[7] = {D.kind, D.dB, D.qa, D.qb, D.qa2, D.qb2, D.sh};
+                std::copy(v, v + 7, ents + 7 * e);
             }
         if (coef)
             for (size_t k = 0; k < S.h_coef.size(); ++k) {

@@ -22,7 +22,8 @@ TERM_LOCAL_H, TERM_COLLECTIVE_H, TERM_LOCAL_JUMP, TERM_COLLECTIVE_JUMP, TERM_DIR
 EXPORTS = [
     "pimi_last_error", "pimi_version", "pimi_context_create", "pimi_context_destroy", "pimi_compile",
     "pimi_plan_create_fused", "pimi_plan_info_get", "pimi_plan_dims", "pimi_trace_functional",
-    "pimi_collective_operator", "pimi_iid_state", "pimi_plan_banded_export", "pimi_plan_export_sizes", "pimi_plan_export", "pimi_plan_destroy",
+    "pimi_collective_operator", "pimi_iid_state", "pimi_plan_banded_export", "pimi_plan_banded_programs",
+    "pimi_plan_export_sizes", "pimi_plan_export", "pimi_plan_destroy",
     "pimi_workspace_create", "pimi_workspace_destroy", "pimi_apply", "pimi_apply_adjoint", "pimi_steady_gmres", "pimi_expv_series",
     "pimi_workspace_last_apply_ms", "pimi_apply_bench", "pimi_copy_bench",
     "pimi_sector_preconditioner_factors", "pimi_precond_apply",
