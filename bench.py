@@ -65,8 +65,8 @@ KRYLOV_PLAIN = 1000   # plain sector-preconditioned GMRES, no restart needed
 BATCH_COUNT = 512
 BATCH_PLAIN_SAMPLE = 32
 # the banded apply kernels (tile width picked by the autotuner at run time)
-APPLY_KERNEL = "k_apply_dia_split<double, {}>"  # single-vector banded apply, FP64
-BATCH_APPLY_KERNEL = "kb_apply_dia<float, {}>"  # batch engine's banded apply, FP32
+APPLY_KERNEL = "pimi_dia_split_f64_{}"  # single-vector banded apply (compiled programs), FP64
+BATCH_APPLY_KERNEL = "pimi_dia_batch_f32_{}"  # batch engine's banded apply (compiled programs), FP32
 
 
 def c2_reference():
