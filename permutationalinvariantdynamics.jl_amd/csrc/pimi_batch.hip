@@ -336,43 +336,97 @@ __global__ __launch_bounds__(kRedBlock) void kb_gemv(const BVec *__restrict__ S,
 //   q_j = (p - Q s) / rho,  u_{j+1} = (w - e p - Q (t - e s)) / rho  (one pass)
 // u_{j+1} is B q_j projected once against [Q, q_j]: B Q s lies in span[Q, q_j].
 
+// The sum of v over the 64 lanes of a wave, in lane 63, by DPP moves (no
+// LDS traffic): pairs, quads, half rows and rows by quad permutes and
+// mirrors, then rows 0+1 and 2+3 by row broadcasts (the fixed tree of the
+// gfx9 DPP reduction).  Lanes a step does not write add zero.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_f64(double v) {
+    int2 w = __builtin_bit_cast(int2, v);
+    w.x = __builtin_amdgcn_update_dpp(0, w.x, CTRL, ROW_MASK, 0xf, false);
+    w.y = __builtin_amdgcn_update_dpp(0, w.y, CTRL, ROW_MASK, 0xf, false);
+    return __builtin_bit_cast(double, w);
+}
+__device__ __forceinline__ double wave_total_lane63(double v) {
+    v += dpp_f64<0xB1>(v);        // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);        // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);       // row_half_mirror
+    v += dpp_f64<0x140>(v);       // row_mirror: every lane holds its row's sum
+    v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
+    v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3: lane 63 holds the total
+    return v;
+}
+
+// block sums of N values with one barrier: each wave's total (lane 63) to
+// LDS, then the waves' totals added in wave order; valid in every thread.
+// `sh` holds N * blockDim.x / 64 doubles.
+template <int N>
+__device__ __forceinline__ void block_sums_dpp(double (&v)[N], double *sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = wave_total_lane63(v[q]);
+    if (lane == 63)
+#pragma unroll
+        for (int q = 0; q < N; ++q) sh[q * nw + wid] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        double r = 0;
+        for (int k = 0; k < nw; ++k) r += sh[q * nw + k];
+        v[q] = r;
+    }
+}
+
 // partial[(col * nblk + chunk) * 2 + {0, 1}] = chunk's part of conj(V_col) . p
-// and conj(V_col) . w for col = 0..ncol-1, p = V[:, ncol-1]; a = V, b = w
+// and conj(V_col) . w for col = 0..ncol-1, p = V[:, ncol-1]; a = V, b = w.
+// A workgroup takes J.u0 consecutive chunks and kDotCols consecutive columns:
+// p and w are read once per element for all its columns.
+constexpr int kDotCols = 4;
 template <typename R>
 __global__ __launch_bounds__(kRedBlock) void kb_dots2_partial(const BVec *__restrict__ S, int ns, int64_t total) {
-    __shared__ double sh[4 * (kRedBlock / 64)];
+    __shared__ double sh[4 * kDotCols * (kRedBlock / 64)];
     // workgroup b runs on XCD b % 8 (the grid is a multiple of 8); each XCD
-    // takes a contiguous slab of every grid-stride pass, so the columns that
-    // share a chunk of p and w (consecutive vb) share one L2
+    // takes a contiguous slab of every grid-stride pass, so the column groups
+    // that share a chunk of p and w (consecutive vb) share one L2
     const int64_t slab = gridDim.x / kNumXcd;
     const int64_t b0 = (int64_t)(blockIdx.x % kNumXcd) * slab + blockIdx.x / kNumXcd;
     for (int64_t vb = b0; vb < total; vb += gridDim.x) {
         const BVec J = S[seg_of(S, ns, vb)];
         const int64_t lb = vb - J.first;
-        // J.u0 consecutive chunks of one column per workgroup (one job lookup
-        // for several chunks' partials, each summed as on its own): many when
-        // many jobs fill the chip, one for a lone job
         const int64_t cpw = J.u0 ? (int64_t)J.u0 : 1;
-        const int64_t grp = lb / J.ncol, c = lb % J.ncol;
-        const cx<R> *v = (const cx<R> *)J.a + c * J.n;
+        const int ngrp = (J.ncol + kDotCols - 1) / kDotCols;
+        const int64_t grp = lb / ngrp;
+        const int c0 = (int)(lb % ngrp) * kDotCols;
+        const int nc = min(kDotCols, J.ncol - c0);
+        const cx<R> *v = (const cx<R> *)J.a + (int64_t)c0 * J.n;
         const cx<R> *p = (const cx<R> *)J.a + (int64_t)(J.ncol - 1) * J.n;
         const cx<R> *w = (const cx<R> *)J.b;
         const int64_t ch1 = min((int64_t)J.nblk, (grp + 1) * cpw);
         for (int64_t chunk = grp * cpw; chunk < ch1; ++chunk) {
-            double r0 = 0, i0 = 0, r1 = 0, i1 = 0;
+            double sums[4 * kDotCols];
+#pragma unroll
+            for (int q = 0; q < 4 * kDotCols; ++q) sums[q] = 0;
             for (int64_t k = chunk * kRedBlock + threadIdx.x; k < J.n; k += (int64_t)J.nblk * kRedBlock) {
-                const cx<R> a = v[k], b = p[k], d = w[k];
-                r0 += (double)a.x * (double)b.x + (double)a.y * (double)b.y;
-                i0 += (double)a.x * (double)b.y - (double)a.y * (double)b.x;
-                r1 += (double)a.x * (double)d.x + (double)a.y * (double)d.y;
-                i1 += (double)a.x * (double)d.y - (double)a.y * (double)d.x;
+                const cx<R> b = p[k], d = w[k];
+                // columns past the job's last (a partial group) repeat the last
+                // column's loads; their sums are not written
+                cx<R> a[kDotCols];
+#pragma unroll
+                for (int u = 0; u < kDotCols; ++u) a[u] = v[k + (int64_t)min(u, nc - 1) * J.n];
+#pragma unroll
+                for (int u = 0; u < kDotCols; ++u) {
+                    sums[4 * u + 0] += (double)a[u].x * (double)b.x + (double)a[u].y * (double)b.y;
+                    sums[4 * u + 1] += (double)a[u].x * (double)b.y - (double)a[u].y * (double)b.x;
+                    sums[4 * u + 2] += (double)a[u].x * (double)d.x + (double)a[u].y * (double)d.y;
+                    sums[4 * u + 3] += (double)a[u].x * (double)d.y - (double)a[u].y * (double)d.x;
+                }
             }
-            double sums[4] = {r0, i0, r1, i1};
-            block_sums(sums, sh);
-            if (threadIdx.x == 0) {
-                cx<double> *out = (cx<double> *)J.c + (c * J.nblk + chunk) * 2;
-                out[0] = cx<double>{sums[0], sums[1]};
-                out[1] = cx<double>{sums[2], sums[3]};
+            block_sums_dpp(sums, sh);
+            if (threadIdx.x < nc) {
+                const int u = threadIdx.x;
+                cx<double> *out = (cx<double> *)J.c + ((int64_t)(c0 + u) * J.nblk + chunk) * 2;
+                out[0] = cx<double>{sums[4 * u + 0], sums[4 * u + 1]};
+                out[1] = cx<double>{sums[4 * u + 2], sums[4 * u + 3]};
             }
             __syncthreads();
         }
@@ -764,7 +818,7 @@ void dcgs_single_step(hipStream_t st, const void *desc_dev, int j, int64_t n) {
     auto grid8 = [](int64_t tot) {
         return (unsigned)std::min<int64_t>((tot + kNumXcd - 1) / kNumXcd * kNumXcd, kMaxGrid);
     };
-    const int64_t tdots = nblk * nc;  // one chunk per workgroup (the descriptor's u0 = 1)
+    const int64_t tdots = nblk * ((nc + kDotCols - 1) / kDotCols);  // one chunk per workgroup (u0 = 1)
     hipLaunchKernelGGL(kb_dots2_partial<double>, dim3(grid8(tdots)), dim3(kRedBlock), 0, st, d + 0, 1, tdots);
     hipLaunchKernelGGL(kb_reduce_cols2, dim3(grid8(nc)), dim3(64), 0, st, d + 1, 1, nc);
     hipLaunchKernelGGL(kb_dcgs_coef, dim3(1), dim3(64), 0, st, d + 2);
@@ -1008,7 +1062,9 @@ struct Tick {
         cls = OC_DOTS;
         for (auto &e : d) e.u0 = kDotChunks;
         vec(d, kb_dots2_partial<R>,
-            [](const BVec &e) { return (int64_t)((e.nblk + kDotChunks - 1) / kDotChunks) * e.ncol; });
+            [](const BVec &e) {
+                return (int64_t)((e.nblk + kDotChunks - 1) / kDotChunks) * ((e.ncol + kDotCols - 1) / kDotCols);
+            });
         cls = OC_OTHER;
         vec(r, kb_reduce_cols2, [](const BVec &e) { return (int64_t)e.ncol; }, 64);
         {
