@@ -453,6 +453,26 @@ pimi_status pimi_batch_next_finished(pimi_batch *batch, int64_t *job_id);
  * forget the job; PIMI_ENOTCONVERGED (state and info filled) for a job that
  * did not converge. */
 pimi_status pimi_batch_fetch(pimi_batch *batch, int64_t job_id, double *rho_out, pimi_solve_info *info);
+/* A dynamics job in the batch engine: the adaptive Krylov exponential of
+ * pimi_expv_series (solve_dynamics with ExpvAlgorithm, src/highlevel.jl:559-733;
+ * krylov_expv!, src/krylov_extensions.jl:1452-1568) from rho0 (n complex)
+ * over the output times ts[0..nts-1] (nondecreasing), observables
+ * <obs_q, rho(t)> of nobs vectors (nobs x n complex, may be 0).  Its Arnoldi
+ * steps, projections and observations run in the engine's ticks beside the
+ * other jobs (one launch per step kind over all jobs); the step control and
+ * the small exponential are the single-vector path's.  FP64 engines only.
+ * Results agree with pimi_expv_series to the exponential's tolerance (the
+ * Arnoldi basis is orthogonalized by delayed reorthogonalization, not CGS2). */
+pimi_status pimi_batch_submit_expv(pimi_batch *batch, pimi_plan *plan, const double *rho0, const double *ts,
+                                   int64_t nts, const pimi_expv_opts *opts, int32_t nobs, const double *obs,
+                                   int64_t *job_id);
+/* A finished dynamics job's observables (nts x nobs complex, row per output
+ * time; may be NULL when nobs = 0), its final state (n complex; may be NULL)
+ * and info (accepted, rejected, applications, factorizations, iterations =
+ * attempts); forgets the job.  PIMI_ENOTCONVERGED (outputs filled up to the
+ * failure) when the exponential did not reach the last output time. */
+pimi_status pimi_batch_fetch_expv(pimi_batch *batch, int64_t job_id, double *obs_out, double *rho_final,
+                                  pimi_solve_info *info);
 /* Multi-model apply: y_k = L_k x_k for every plan k in one launch (banded
  * and ELL kernels over all plans' tiles), in the engine precision.  x, y:
  * host buffers holding the plans' vectors one after another (complex

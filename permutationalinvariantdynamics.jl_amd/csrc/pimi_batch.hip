@@ -37,6 +37,7 @@
 #include <memory>
 
 #include "pimi_internal.hpp"
+#include "pimi_expv.hpp"
 
 namespace pimi {
 
@@ -694,6 +695,11 @@ struct Arena {
 };
 
 enum Phase { PH_PROBE = 0, PH_INIT, PH_ARNOLDI, PH_END, PH_FINAL };
+// phases of a dynamics job (adaptive Krylov exponential, src/krylov_extensions.jl:1452-1568):
+// observe (observables of x at a save time, |x|), Arnoldi ticks of one
+// factorization, and projections (trial = beta V exp(h S) e1, |trial|)
+enum DynPhase { PX_OBSERVE = 10, PX_ARNOLDI, PX_PROJECT };
+enum JobKind { JK_STEADY = 0, JK_DYNAMICS = 1 };
 enum JobState { JS_QUEUED = 0, JS_ACTIVE, JS_DONE, JS_FAILED };
 
 // mailbox slots (complex doubles) at the head of each job's mail region:
@@ -737,6 +743,23 @@ struct BJob {
     std::vector<cx<double>> result;
     pimi_solve_info info{};
     std::string error;
+    // dynamics jobs (kind JK_DYNAMICS): save times, observables (nobs vectors
+    // plus one zero column on the device), the state's samples, and the
+    // exponential's factorization and step control
+    int kind = JK_STEADY;
+    pimi_expv_opts eo{};
+    std::vector<double> ts;
+    int64_t sample = 0;  // next save time to record
+    int32_t nobs = 0;
+    std::vector<cx<double>> rho0, obs_host, obs_out;
+    cx<double> *obs = nullptr;
+    Factorization F;
+    ExpvControl X;
+    bool have = false, near = false, enlarging = false;
+    double err = 0;
+    std::vector<cx<double>> coeffs;
+    int64_t accepted = 0, rejected = 0, factorizations = 0, attempts = 0;
+    double error_sum = 0;
 };
 
 // Workgroups of a job's BLAS-1 passes and reduction partials: one per 2048
@@ -1100,6 +1123,23 @@ struct Tick {
     }
 };
 
+// Device bytes of one dynamics job (FP64 engine): x and the trial vector, the
+// basis with room for one enlargement step, w, reduction partials (for the
+// basis or the observables, whichever has more columns), Hessenberg
+// coefficients, mailbox, norm partials and the observables.
+static size_t dyn_job_bytes(int64_t n, int kdim, int nobs) {
+    const int nblk = blocks_for(n);
+    const int cols = std::max(kdim + 2, nobs + 1);
+    size_t b = 0;
+    b += 2 * align_up(sizeof(cx<double>) * n, 256);
+    b += align_up(sizeof(cx<double>) * n * (size_t)(kdim + 3), 256) + align_up(sizeof(cx<double>) * n, 256);
+    b += align_up(sizeof(cx<double>) * nblk * 2 * (size_t)cols, 256);
+    b += 2 * align_up(sizeof(cx<double>) * (size_t)(2 * (cols + 1) + kMailHacc), 256);
+    b += align_up(sizeof(double) * 4 * nblk, 256);
+    b += align_up(sizeof(cx<double>) * n * (size_t)(nobs + 1), 256);
+    return b;
+}
+
 template <typename R>
 static char *vcol(BJob *J, int c) {
     return J->V + sizeof(cx<R>) * (size_t)J->n * (size_t)c;
@@ -1114,6 +1154,21 @@ static void carve(pimi_batch *E, BJob *J) {
         return q;
     };
     const int64_t n = J->n;
+    if (J->kind == JK_DYNAMICS) {
+        const int cols = std::max(J->kdim + 2, J->nobs + 1);
+        J->x = (cx<double> *)take(sizeof(cx<double>) * n);
+        J->r = (cx<double> *)take(sizeof(cx<double>) * n);
+        J->V = take(sizeof(cx<double>) * n * (size_t)(J->kdim + 3));
+        J->w = take(sizeof(cx<double>) * n);
+        J->partial = (cx<double> *)take(sizeof(cx<double>) * J->nblk * 2 * (size_t)cols);
+        J->h = (cx<double> *)take(sizeof(cx<double>) * (size_t)(2 * (cols + 1) + kMailHacc));
+        J->mail = (cx<double> *)take(sizeof(cx<double>) * (size_t)(2 * (cols + 1) + kMailHacc));
+        J->npart = (double *)take(sizeof(double) * 4 * J->nblk);
+        J->obs = (cx<double> *)take(sizeof(cx<double>) * n * (size_t)(J->nobs + 1));
+        if ((size_t)(p - (E->arena.base + J->off)) > J->bytes)
+            throw Error(PIMI_EINVAL, "job layout exceeds its footprint");
+        return;
+    }
     J->x = (cx<double> *)take(sizeof(cx<double>) * n);
     J->b = (cx<double> *)take(sizeof(cx<double>) * n);
     J->r = (cx<double> *)take(sizeof(cx<double>) * n);
@@ -1145,6 +1200,20 @@ static void admit(pimi_batch *E, BLane &L) {
         E->queue.pop_front();
         J->off = off;
         carve(E, J);
+        if (J->kind == JK_DYNAMICS) {
+            // the initial state and the observables (host copies kept until
+            // the job ends: the copies are asynchronous on the lane's stream)
+            HIPCHK(hipMemcpyAsync(J->x, J->rho0.data(), sizeof(cx<double>) * J->n, hipMemcpyHostToDevice, L.stream));
+            if (J->nobs > 0)
+                HIPCHK(hipMemcpyAsync(J->obs, J->obs_host.data(), sizeof(cx<double>) * J->n * J->nobs,
+                                      hipMemcpyHostToDevice, L.stream));
+            HIPCHK(hipMemsetAsync(J->obs + (size_t)J->n * J->nobs, 0, sizeof(cx<double>) * J->n, L.stream));
+            J->state = JS_ACTIVE;
+            J->phase = PX_OBSERVE;
+            J->t_admit = std::chrono::steady_clock::now();
+            L.active.push_back(J);
+            continue;
+        }
         HIPCHK(hipMemcpyAsync(J->vval, J->vv.data(), sizeof(double) * J->nt, hipMemcpyHostToDevice, L.stream));
         J->state = JS_ACTIVE;
         J->phase = J->o.operator_scale > 0 ? PH_INIT : PH_PROBE;
@@ -1156,6 +1225,25 @@ static void admit(pimi_batch *E, BLane &L) {
 
 static void finish(pimi_batch *E, BJob *J, bool ok) {
     J->state = ok ? JS_DONE : JS_FAILED;
+    if (J->kind == JK_DYNAMICS) {
+        if ((int64_t)J->result.size() != J->n) J->result.assign((size_t)J->n, cx<double>{0, 0});
+        pimi_solve_info &I = J->info;
+        std::memset(&I, 0, sizeof I);
+        I.converged = ok ? 1 : 0;
+        I.accepted = J->accepted;
+        I.rejected = J->rejected;
+        I.applications = J->applications;
+        I.factorizations = J->factorizations;
+        I.iterations = J->attempts;
+        I.estimated_error = J->error_sum;
+        I.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - J->t_submit).count();
+        E->arena.release(J->off, J->bytes);
+        J->x = J->r = nullptr;
+        J->V = J->w = nullptr;
+        J->obs = nullptr;
+        E->finished.push_back(J->id);
+        return;
+    }
     if ((int64_t)J->result.size() != J->n) {  // failed before its final phase
         J->result.assign((size_t)J->n, cx<double>{0, 0});
         std::memset(&J->info, 0, sizeof J->info);
@@ -1175,6 +1263,7 @@ static void launch_tick(pimi_batch *E, BLane &L) {
     admit(E, L);
     if (L.active.empty()) return;
     std::vector<BJob *> probe, init, arn, begin, end, fin;
+    std::vector<BJob *> xobs, xbegin, xarn, xproj;  // dynamics jobs
     for (BJob *J : L.active) switch (J->phase) {
             case PH_PROBE: probe.push_back(J); break;
             case PH_INIT: init.push_back(J); break;
@@ -1183,6 +1272,12 @@ static void launch_tick(pimi_batch *E, BLane &L) {
                 if (J->begin_cycle) begin.push_back(J);
                 break;
             case PH_END: end.push_back(J); break;
+            case PX_OBSERVE: xobs.push_back(J); break;
+            case PX_ARNOLDI:
+                xarn.push_back(J);
+                if (J->j == 0) xbegin.push_back(J);
+                break;
+            case PX_PROJECT: xproj.push_back(J); break;
             default: fin.push_back(J); break;
         }
     Tick T;
@@ -1272,6 +1367,60 @@ static void launch_tick(pimi_batch *E, BLane &L) {
         T.norm<double>(resid, cR, S_RNORM);
         T.norm<double>(resid, cX, S_XNORM, 1);
     }
+    // -- dynamics jobs: observables <obs_q, x> at a save time and |x|; an
+    // Arnoldi tick of a factorization (V0 = x / |x| on its first); a
+    // projection trial = V c (c = beta exp(h S) e1, staged) and |trial|
+    if (!xobs.empty()) {
+        T.cls = OC_OTHER;
+        std::vector<BJob *> withobs;
+        for (BJob *J : xobs)
+            if (J->nobs > 0) withobs.push_back(J);
+        if (!withobs.empty()) {
+            std::vector<BVec> d, r;
+            for (BJob *J : withobs) {
+                const int nc = J->nobs + 1;  // the observables and a zero column in the "p" role
+                d.push_back(BVec{0, J->obs, J->x, J->partial, nullptr, J->n, J->nblk, nc, 0.0, (uint64_t)kDotChunks});
+                r.push_back(BVec{0, J->partial, nullptr, J->mail + kMailHacc, nullptr, J->n, J->nblk, nc, 0.0, 0});
+            }
+            T.vec(d, kb_dots2_partial<double>, [](const BVec &e) {
+                return (int64_t)((e.nblk + kDotChunks - 1) / kDotChunks) * ((e.ncol + kDotCols - 1) / kDotCols);
+            });
+            T.vec(r, kb_reduce_cols2, [](const BVec &e) { return (int64_t)e.ncol; }, 64);
+        }
+        T.norm<double>(xobs, cX, S_RNORM);
+    }
+    T.cls = OC_OTHER;
+    T.vec(vecs(xbegin, [](BJob *J) {
+              return BVec{0, J->x, nullptr, J->V, J->scal() + S_RNORM, J->n, J->nblk, 1, 0.0, 0};
+          }),
+          kb_scale_div<R, double>, Tick::nblk_of);
+    if (!xarn.empty()) {
+        T.cls = OC_APPLY;
+        T.apply<R>(xarn, [](BJob *J) -> const void * { return vcol<R>(J, J->j); },
+                   [](BJob *J) -> void * { return J->w; }, true);
+        T.dcgs<R>(xarn);
+    }
+    if (!xproj.empty()) {
+        T.cls = OC_CYCLE_END;
+        std::vector<BVec> u, z;
+        std::vector<size_t> coffs;
+        std::vector<BJob *> proj;
+        for (BJob *J : xproj) {
+            if (J->F.zero_state) {
+                z.push_back(BVec{0, nullptr, nullptr, J->r, nullptr, J->n, J->nblk, 1, 0.0, 0});
+                continue;
+            }
+            T.bytes[OC_CYCLE_END] += (double)(J->F.coeff_count + 1) * J->n * sizeof(cx<R>);
+            coffs.push_back(T.put(J->coeffs));  // coefficients travel with the descriptors
+            u.push_back(BVec{0, J->V, nullptr, J->r, nullptr, J->n, J->nblk, J->F.coeff_count, 0.0, 0});
+        }
+        const size_t uoff = T.gemv<R>(u, 1);
+        for (size_t k = 0; k < coffs.size(); ++k)
+            T.relocs.push_back({uoff + k * sizeof(BVec) + offsetof(BVec, b), coffs[k]});
+        T.vec(z, kb_zero, Tick::nblk_of);
+        T.cls = OC_OTHER;
+        T.norm<double>(xproj, cR, S_RNORM);
+    }
     // -- final: back to coefficients (x = S^{-1} u for weighted jobs), |L x|, |x| and t . x
     auto cXo = [](BJob *J) -> const void * { return J->weighted ? (const void *)J->xo : (const void *)J->x; };
     if (!fin.empty()) {
@@ -1306,7 +1455,9 @@ static void launch_tick(pimi_batch *E, BLane &L) {
     moff.assign(L.active.size(), 0);
     for (size_t k = 0; k < L.active.size(); ++k) {
         BJob *J = L.active[k];
-        const int64_t cnt = kMailHacc + (J->phase == PH_ARNOLDI ? 2 * (J->j + 1) : 0);
+        const int64_t cnt = kMailHacc + ((J->phase == PH_ARNOLDI || J->phase == PX_ARNOLDI) ? 2 * (J->j + 1)
+                                         : J->phase == PX_OBSERVE                          ? 2 * (J->nobs + 1)
+                                                                                           : 0);
         moff[k] = mtot;
         g.push_back(BCopy{J->mail, nullptr, cnt});
         mtot += (size_t)cnt;
@@ -1359,12 +1510,139 @@ static void launch_tick(pimi_batch *E, BLane &L) {
         HIPCHK(hipMemcpyAsync(J->result.data(), J->weighted ? J->xo : J->x, sizeof(cx<double>) * J->n,
                               hipMemcpyDeviceToHost, L.stream));
     }
+    for (BJob *J : xobs)  // a dynamics job's last save time: its final state
+        if (J->sample + 1 == (int64_t)J->ts.size()) {
+            J->result.resize((size_t)J->n);
+            HIPCHK(hipMemcpyAsync(J->result.data(), J->x, sizeof(cx<double>) * J->n, hipMemcpyDeviceToHost,
+                                  L.stream));
+        }
     L.op_class = T.op_class;
     for (int c = 0; c < OC_COUNT; ++c) {
         L.bytes[c] = T.bytes[c];
         L.flops[c] = T.flops[c];
     }
     L.inflight = true;
+}
+
+// Dynamics jobs: the next attempt of the current interval (a new
+// factorization when the last one was used by an accepted step, else a
+// projection with the new step), as the loop of krylov_expv!
+// (src/krylov_extensions.jl:1452-1568).  Returns false when the job has used
+// its max_steps attempts.
+static bool dyn_attempt(BJob *J) {
+    if (J->attempts >= J->eo.max_steps) return false;
+    J->attempts++;
+    if (!J->have) {
+        J->F = Factorization();
+        J->factorizations++;
+        J->have = true;
+        if (J->xnorm == 0) {  // zero state: exp(h A) 0 = 0
+            J->F.zero_state = true;
+            J->F.completed = true;
+        } else {
+            J->F.beta = J->xnorm;
+            J->j = 0;
+            J->near = J->enlarging = false;
+            std::fill(J->Hraw.begin(), J->Hraw.end(), cd(0, 0));
+            J->phase = PX_ARNOLDI;
+            return true;
+        }
+    }
+    J->err = J->F.zero_state ? 0.0 : expv_coefficients(J->F, J->kdim + 1, J->X.step(), J->coeffs);
+    if (J->F.zero_state) J->X.step();
+    for (auto &c : J->coeffs) c = cx<double>{J->F.beta * c.x, J->F.beta * c.y};
+    J->phase = PX_PROJECT;
+    return true;
+}
+
+// After an observation: the next interval, or the end of the job.  Returns
+// false when the job cannot continue (max_steps).
+static bool dyn_next_interval(BJob *J, bool &done) {
+    if (J->sample == (int64_t)J->ts.size()) {
+        done = true;
+        return true;
+    }
+    const double dt = J->ts[(size_t)J->sample] - J->ts[(size_t)J->sample - 1];
+    if (!J->X.begin(dt)) {  // a zero-length interval: the same state again
+        J->phase = PX_OBSERVE;
+        return true;
+    }
+    return dyn_attempt(J);
+}
+
+// An Arnoldi tick of a dynamics job's factorization: Hessenberg column j-1
+// final (as for the stationary jobs), breakdown tests as arnoldi() of the
+// single-vector path (exact: a zero subdiagonal or the whole space; near:
+// subdiagonal <= sqrt(eps) |A v_j|, with |A v_j| the norm of the final column),
+// and one more step to enlarge the factorization after a near breakdown.
+static void dyn_arnoldi(BJob *J, const cx<double> *mb, const double *sc, bool &done, bool &ok) {
+    const int m = J->kdim, jj = J->j, ld = m + 2;
+    auto Hw = [&](int i, int c) -> cd & { return J->Hraw[(size_t)i + (size_t)c * ld]; };
+    const cx<double> *sv = mb + kMailHacc, *tv = sv + (jj + 1);
+    auto S_ = [&](int i) { return cd(sv[i].x, sv[i].y); };
+    auto T_ = [&](int i) { return cd(tv[i].x, tv[i].y); };
+    const double rho = sc[kSlotRho];
+    const cd e(sc[kSlotE], sc[kSlotE + 1]);
+    bool complete = false, exact = false;
+    if (jj >= 1) {
+        for (int i = 0; i < jj; ++i) Hw(i, jj - 1) = J->cprov[(size_t)i] + S_(i);
+        Hw(jj, jj - 1) = rho;
+        if (J->enlarging) {
+            complete = true;
+        } else {
+            double img2 = 0;
+            for (int i = 0; i <= jj; ++i) img2 += std::norm(Hw(i, jj - 1));
+            const double breakfactor = std::sqrt(2.220446049250313e-16);
+            if (rho == 0 || (int64_t)jj == J->n) {
+                exact = complete = true;
+            } else {
+                J->near |= rho <= breakfactor * std::max(std::sqrt(img2), 2.2250738585072014e-308);
+                if (jj == m) {
+                    if (J->near) J->enlarging = true;
+                    else complete = true;
+                }
+            }
+        }
+    } else if (!(rho > 0) || !std::isfinite(rho)) {
+        J->error = "zero or non-finite start vector of a Krylov factorization";
+        done = true;
+        ok = false;
+        return;
+    }
+    if (!complete) {
+        for (int i = 0; i < jj; ++i) {
+            cd acc = T_(i);
+            for (int c = 0; c < jj; ++c) acc -= Hw(i, c) * S_(c);
+            J->cprov[(size_t)i] = acc / rho;
+        }
+        J->cprov[(size_t)jj] = e - (jj >= 1 ? S_(jj - 1) : cd(0, 0));
+        J->j = jj + 1;
+        return;
+    }
+    // the factorization, laid out as arnoldi() of the single-vector path
+    Factorization &F = J->F;
+    const bool enlarged = J->enlarging;
+    const int k = enlarged ? jj - 1 : jj, fl = m + 1;
+    F.S.assign((size_t)fl * fl, cd(0, 0));
+    for (int c = 0; c < k; ++c)
+        for (int i = 0; i < k; ++i) F.S[(size_t)i + (size_t)c * fl] = Hw(i, c);
+    const int q = k + (exact ? 0 : 1);
+    if (!exact) F.S[(size_t)k + (size_t)(k - 1) * fl] = Hw(k, k - 1);
+    F.used = k;
+    if (enlarged) {
+        F.enlarged = true;
+        F.used += 1;
+        for (int i = 0; i < q; ++i) F.S[(size_t)i + (size_t)(q - 1) * fl] += Hw(i, k);
+        F.next_remainder = Hw(k + 1, k).real();
+    }
+    F.k = k;
+    F.q = q;
+    F.coeff_count = enlarged ? q : k;
+    F.completed = exact || (enlarged && ((int64_t)q == J->n || F.next_remainder == 0));
+    J->applications += F.used;
+    J->err = expv_coefficients(F, fl, J->X.step(), J->coeffs);
+    for (auto &c : J->coeffs) c = cx<double>{F.beta * c.x, F.beta * c.y};
+    J->phase = PX_PROJECT;
 }
 
 // Wait for lane L's tick and advance every one of its jobs' state machines.
@@ -1517,6 +1795,53 @@ static void complete_tick(pimi_batch *E, BLane &L) {
                 if (E->trace && J->iterations % 50 == 0 && jj >= 1)
                     std::fprintf(stderr, "[pimi batch] job %lld iteration %lld estimate %.3e |x| %.3e\n",
                                  (long long)J->id, (long long)J->iterations, est, J->xnorm);
+                break;
+            }
+            case PX_OBSERVE: {
+                const cx<double> *ov = mb + kMailHacc + (J->nobs + 1);
+                for (int32_t q = 0; q < J->nobs; ++q) J->obs_out[(size_t)(J->sample * J->nobs + q)] = ov[q];
+                J->xnorm = sc[S_RNORM];
+                J->sample++;
+                if (!dyn_next_interval(J, done)) {
+                    J->error = "adaptive Krylov exponential action did not reach the requested time";
+                    done = true;
+                } else if (done) {
+                    ok = true;
+                }
+                break;
+            }
+            case PX_ARNOLDI:
+                dyn_arnoldi(J, mb, sc, done, ok);
+                break;
+            case PX_PROJECT: {
+                int d = -1;
+                try {
+                    d = J->X.decide(J->err, sc[S_RNORM], [&]() { return J->xnorm; }, J->F, J->eo);
+                } catch (const Error &er) {
+                    J->error = er.what();
+                    done = true;
+                    break;
+                }
+                if (d > 0) {
+                    std::swap(J->x, J->r);
+                    J->xnorm = sc[S_RNORM];
+                    J->accepted++;
+                    J->error_sum += J->err;
+                    J->have = false;
+                } else {
+                    J->rejected++;
+                }
+                bool go;
+                if (d > 0 && J->X.done()) {
+                    J->phase = PX_OBSERVE;
+                    go = true;
+                } else {
+                    go = d >= 0 && dyn_attempt(J);
+                }
+                if (!go) {
+                    J->error = "adaptive Krylov exponential action did not reach the requested time";
+                    done = true;
+                }
                 break;
             }
             default: {  // PH_FINAL
@@ -1711,6 +2036,57 @@ pimi_status pimi_batch_submit(pimi_batch *E, pimi_plan *plan, const pimi_gmres_o
     });
 }
 
+pimi_status pimi_batch_submit_expv(pimi_batch *E, pimi_plan *plan, const double *rho0, const double *ts,
+                                   int64_t nts, const pimi_expv_opts *o, int32_t nobs, const double *obs,
+                                   int64_t *job_id) {
+    return guarded([&] {
+        if (!E || !plan || !rho0 || !ts || !o || !job_id || (nobs > 0 && !obs)) throw Error(PIMI_EINVAL, "null argument");
+        BusyGuard guard(E->busy);
+        if (!plan->ctx || plan->ctx->device != E->ctx->device)
+            throw Error(PIMI_EINVAL, "the plan must live on the engine's device");
+        if (E->prec != PIMI_C64) throw Error(PIMI_EINVAL, "dynamics jobs need an FP64 engine");
+        if (o->krylovdim < 1 || o->krylovdim > kMaxKrylov) throw Error(PIMI_EINVAL, "krylovdim must lie in 1..4000");
+        if (!(o->safety > 0 && o->safety < 1)) throw Error(PIMI_EINVAL, "safety must lie strictly between zero and one");
+        if (o->max_steps <= 0) throw Error(PIMI_EINVAL, "max_steps must be positive");
+        if (!(o->atol >= 0) || !(o->rtol >= 0)) throw Error(PIMI_EINVAL, "atol and rtol must be nonnegative");
+        if (nts < 1) throw Error(PIMI_EINVAL, "at least one output time is required");
+        if (nobs < 0) throw Error(PIMI_EINVAL, "nobs must be nonnegative");
+        for (int64_t k = 0; k < nts; ++k)
+            if (!std::isfinite(ts[k])) throw Error(PIMI_EINVAL, "output times must be finite");
+        for (int64_t k = 1; k < nts; ++k)
+            if (!(ts[k] >= ts[k - 1])) throw Error(PIMI_EINVAL, "Krylov exponential output times must be nondecreasing");
+        HIPCHK(hipSetDevice(E->ctx->device));
+        std::unique_ptr<BJob> J(new BJob());
+        J->kind = JK_DYNAMICS;
+        J->plan = plan;
+        J->eo = *o;
+        J->n = plan->n;
+        J->nt = 0;
+        J->kdim = (int)std::min<int64_t>(o->krylovdim, plan->n);  // as KrylovWorkspace (src/krylov.jl:74-82)
+        J->nblk = blocks_for(J->n);
+        J->nobs = nobs;
+        J->bytes = dyn_job_bytes(J->n, J->kdim, nobs);
+        J->table_bytes64 = (double)plan->main.table_bytes(16);
+        J->table_bytes32 = (double)plan->main.table_bytes(8);
+        if (J->bytes > E->arena.size)
+            throw Error(PIMI_ENOMEM, "the job's device state exceeds the engine's memory reservation");
+        J->ts.assign(ts, ts + nts);
+        J->rho0.resize((size_t)J->n);
+        std::memcpy(J->rho0.data(), rho0, sizeof(cx<double>) * (size_t)J->n);
+        J->obs_host.resize((size_t)J->n * (size_t)nobs);
+        if (nobs > 0) std::memcpy(J->obs_host.data(), obs, sizeof(cx<double>) * J->obs_host.size());
+        J->obs_out.assign((size_t)nts * (size_t)nobs, cx<double>{0, 0});
+        const int m = J->kdim;
+        J->Hraw.assign((size_t)(m + 2) * (m + 1), cd(0, 0));
+        J->cprov.assign((size_t)m + 3, cd(0, 0));
+        J->t_submit = std::chrono::steady_clock::now();
+        J->id = E->next_id++;
+        *job_id = J->id;
+        E->queue.push_back(J.get());
+        E->jobs[J->id] = std::move(J);
+    });
+}
+
 pimi_status pimi_batch_run(pimi_batch *E, int32_t max_ticks, int32_t stop_when_finished, int64_t *finished) {
     return guarded([&] {
         if (!E) throw Error(PIMI_EINVAL, "null argument");
@@ -1779,6 +2155,38 @@ pimi_status pimi_batch_next_finished(pimi_batch *E, int64_t *job_id) {
         *job_id = E->finished.front();
         E->finished.pop_front();
     });
+}
+
+pimi_status pimi_batch_fetch_expv(pimi_batch *E, int64_t job_id, double *obs_out, double *rho_final,
+                                  pimi_solve_info *info) {
+    pimi_status st = PIMI_OK;
+    std::string msg;
+    pimi_status g = guarded([&] {
+        if (!E || !info) throw Error(PIMI_EINVAL, "null argument");
+        BusyGuard guard(E->busy);
+        auto it = E->jobs.find(job_id);
+        if (it == E->jobs.end()) throw Error(PIMI_EINVAL, "unknown batch job");
+        BJob *J = it->second.get();
+        if (J->kind != JK_DYNAMICS) throw Error(PIMI_EINVAL, "not a dynamics job");
+        if (J->state != JS_DONE && J->state != JS_FAILED) throw Error(PIMI_EBUSY, "the batch job has not finished");
+        if (J->nobs > 0 && !obs_out) throw Error(PIMI_EINVAL, "null argument");
+        if (obs_out) std::memcpy(obs_out, J->obs_out.data(), sizeof(cx<double>) * J->obs_out.size());
+        if (rho_final) std::memcpy(rho_final, J->result.data(), sizeof(cx<double>) * (size_t)J->n);
+        *info = J->info;
+        if (J->state == JS_FAILED) {
+            st = PIMI_ENOTCONVERGED;
+            msg = J->error.empty() ? "adaptive Krylov exponential action did not reach the requested time" : J->error;
+        }
+        for (auto q = E->finished.begin(); q != E->finished.end(); ++q)
+            if (*q == job_id) {
+                E->finished.erase(q);
+                break;
+            }
+        E->jobs.erase(it);
+    });
+    if (g != PIMI_OK) return g;
+    if (st != PIMI_OK) set_error(msg);
+    return st;
 }
 
 pimi_status pimi_batch_fetch(pimi_batch *E, int64_t job_id, double *rho_out, pimi_solve_info *info) {

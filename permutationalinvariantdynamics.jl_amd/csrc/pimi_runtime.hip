@@ -31,6 +31,7 @@
 #include <cfloat>
 
 #include "pimi_internal.hpp"
+#include "pimi_expv.hpp"
 
 namespace pimi {
 
@@ -689,107 +690,6 @@ static void probe_vector(uint64_t &st, std::vector<cx<double>> &x) {
         double r = std::sqrt(-2.0 * std::log(u)), a = 6.283185307179586 * v;
         z = cx<double>{r * std::cos(a) * 0.7071067811865476, r * std::sin(a) * 0.7071067811865476};
     }
-}
-
-// ---------------------------------------------------------------------------
-// small dense exponential (host): scaling and squaring with diagonal Pade
-// approximants of degree 3..13, Higham (2005) thresholds.
-// ---------------------------------------------------------------------------
-using cdv = std::vector<cd>;
-static void mm(int q, const cdv &A, const cdv &B, cdv &C) {
-    C.assign((size_t)q * q, cd(0, 0));
-    for (int j = 0; j < q; ++j)
-        for (int k = 0; k < q; ++k) {
-            cd b = B[(size_t)k + (size_t)j * q];
-            if (b == cd(0, 0)) continue;
-            for (int i = 0; i < q; ++i) C[(size_t)i + (size_t)j * q] += A[(size_t)i + (size_t)k * q] * b;
-        }
-}
-static void solve_inplace(int q, cdv A, cdv &B) {
-    for (int k = 0; k < q; ++k) {
-        int p = k;
-        for (int i = k + 1; i < q; ++i)
-            if (std::abs(A[i + (size_t)k * q]) > std::abs(A[p + (size_t)k * q])) p = i;
-        if (p != k)
-            for (int j = 0; j < q; ++j) {
-                std::swap(A[k + (size_t)j * q], A[p + (size_t)j * q]);
-                std::swap(B[k + (size_t)j * q], B[p + (size_t)j * q]);
-            }
-        cd piv = A[k + (size_t)k * q];
-        if (piv == cd(0, 0)) throw Error(PIMI_EINVAL, "singular Pade denominator");
-        for (int i = k + 1; i < q; ++i) {
-            cd f = A[i + (size_t)k * q] / piv;
-            if (f == cd(0, 0)) continue;
-            for (int j = k; j < q; ++j) A[i + (size_t)j * q] -= f * A[k + (size_t)j * q];
-            for (int j = 0; j < q; ++j) B[i + (size_t)j * q] -= f * B[k + (size_t)j * q];
-        }
-    }
-    for (int j = 0; j < q; ++j)
-        for (int i = q - 1; i >= 0; --i) {
-            cd s = B[i + (size_t)j * q];
-            for (int k = i + 1; k < q; ++k) s -= A[i + (size_t)k * q] * B[k + (size_t)j * q];
-            B[i + (size_t)j * q] = s / A[i + (size_t)i * q];
-        }
-}
-static cdv expm(int q, const cdv &Ain) {
-    static const double c13[] = {64764752532480000., 32382376266240000., 7771770303897600., 1187353796428800.,
-                                 129060195264000., 10559470521600., 670442572800., 33522128640., 1323241920.,
-                                 40840800., 960960., 16380., 182., 1.};
-    static const double c9[] = {17643225600., 8821612800., 2075673600., 302702400., 30270240.,
-                                2162160., 110880., 3960., 90., 1.};
-    static const double c7[] = {17297280., 8648640., 1995840., 277200., 25200., 1512., 56., 1.};
-    static const double c5[] = {30240., 15120., 3360., 420., 30., 1.};
-    static const double c3[] = {120., 60., 12., 1.};
-    static const double th[] = {1.495585217958292e-2, 2.539398330063230e-1, 9.504178996162932e-1,
-                                2.097847961257068e0, 5.371920351148152e0};
-    cdv A = Ain;
-    double nrm = 0;
-    for (int j = 0; j < q; ++j) {
-        double s = 0;
-        for (int i = 0; i < q; ++i) s += std::abs(A[i + (size_t)j * q]);
-        nrm = std::max(nrm, s);
-    }
-    int deg = 13, sq = 0;
-    const double *c = c13;
-    if (nrm <= th[0]) { deg = 3; c = c3; }
-    else if (nrm <= th[1]) { deg = 5; c = c5; }
-    else if (nrm <= th[2]) { deg = 7; c = c7; }
-    else if (nrm <= th[3]) { deg = 9; c = c9; }
-    else if (nrm > th[4]) {
-        sq = std::max(0, (int)std::ceil(std::log2(nrm / th[4])));
-        double f = std::ldexp(1.0, -sq);
-        for (auto &z : A) z *= f;
-    }
-    const size_t qq = (size_t)q * q;
-    cdv I(qq, cd(0, 0));
-    for (int i = 0; i < q; ++i) I[i + (size_t)i * q] = 1.0;
-    cdv A2, U(qq), Vv(qq), T(qq);
-    mm(q, A, A, A2);
-    if (deg == 13) {
-        cdv A4, A6, W1, W2;
-        mm(q, A2, A2, A4);
-        mm(q, A4, A2, A6);
-        for (size_t k = 0; k < qq; ++k) T[k] = c[13] * A6[k] + c[11] * A4[k] + c[9] * A2[k];
-        mm(q, A6, T, W1);
-        for (size_t k = 0; k < qq; ++k) W1[k] += c[7] * A6[k] + c[5] * A4[k] + c[3] * A2[k] + c[1] * I[k];
-        mm(q, A, W1, U);
-        for (size_t k = 0; k < qq; ++k) T[k] = c[12] * A6[k] + c[10] * A4[k] + c[8] * A2[k];
-        mm(q, A6, T, W2);
-        for (size_t k = 0; k < qq; ++k) Vv[k] = W2[k] + c[6] * A6[k] + c[4] * A4[k] + c[2] * A2[k] + c[0] * I[k];
-    } else {
-        cdv Pw = I, odd(qq, cd(0, 0)), nxt;
-        std::fill(Vv.begin(), Vv.end(), cd(0, 0));
-        for (int k = 0; 2 * k <= deg; ++k) {
-            for (size_t e = 0; e < qq; ++e) { odd[e] += c[2 * k + 1] * Pw[e]; Vv[e] += c[2 * k] * Pw[e]; }
-            if (2 * (k + 1) <= deg) { mm(q, Pw, A2, nxt); Pw.swap(nxt); }
-        }
-        mm(q, A, odd, U);
-    }
-    cdv Num(qq), Den(qq);
-    for (size_t k = 0; k < qq; ++k) { Num[k] = Vv[k] + U[k]; Den[k] = Vv[k] - U[k]; }
-    solve_inplace(q, Den, Num);
-    for (int k = 0; k < sq; ++k) { mm(q, Num, Num, T); Num.swap(T); }
-    return Num;
 }
 
 // ---------------------------------------------------------------------------
@@ -1747,13 +1647,6 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, const doub
 // ---------------------------------------------------------------------------
 // adaptive Krylov exponential (src/krylov_extensions.jl:1346-1568)
 // ---------------------------------------------------------------------------
-struct Factorization {
-    double beta = 0, next_remainder = 0;
-    bool zero_state = false, completed = false, enlarged = false;
-    int k = 0, used = 0, q = 0, coeff_count = 0;
-    std::vector<cd> S;  // (m+1) x (m+1)
-};
-
 static void arnoldi(pimi_workspace *W, Factorization &F) {
     const int64_t n = W->plan->n;
     const int m = W->m;
@@ -1831,24 +1724,16 @@ static void arnoldi(pimi_workspace *W, Factorization &F) {
 // trial = beta V exp(h S) e1 ; returns error estimate
 static double project(pimi_workspace *W, const Factorization &F, double h) {
     const int64_t n = W->plan->n;
-    const int m = W->m;
     if (F.zero_state) {
         HIPCHK(hipMemsetAsync(W->trial.p, 0, sizeof(cx<double>) * n, W->stream));
         return 0.0;
     }
-    int q = F.q;
-    cdv S((size_t)q * q);
-    for (int j = 0; j < q; ++j)
-        for (int i = 0; i < q; ++i) S[(size_t)i + (size_t)j * q] = h * F.S[(size_t)i + (size_t)j * (m + 1)];
-    cdv E = expm(q, S);
-    std::vector<cx<double>> c(F.coeff_count);
-    for (int j = 0; j < F.coeff_count; ++j) c[j] = cx<double>{E[j].real(), E[j].imag()};
+    std::vector<cx<double>> c;
+    const double err = expv_coefficients(F, W->m + 1, h, c);
     h2d(W, W->coef.p, c.data(), sizeof(cx<double>) * F.coeff_count);
     hipLaunchKernelGGL(k_gemv<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, W->trial.p, W->V.p, n,
                        F.coeff_count, W->coef.p, n, F.beta, 1);
-    if (F.completed) return 0.0;
-    if (F.enlarged) return F.beta * std::fabs(h) * F.next_remainder * std::abs(E[(size_t)(q - 1)]);
-    return F.beta * std::abs(E[(size_t)F.k]);
+    return err;
 }
 
 struct ExpvStats {
@@ -1859,57 +1744,38 @@ struct ExpvStats {
 // W->cur holds b on entry and exp(t A) b on exit.
 static bool expv_device(pimi_workspace *W, double t, const pimi_expv_opts *o, ExpvStats &S) {
     const int64_t n = W->plan->n;
-    double total = std::fabs(t);
-    if (total == 0) return true;
-    double hmax = total, hmin = 0.0, h = hmax;
-    double reached = 0.0, direction = t > 0 ? 1.0 : -1.0;
+    ExpvControl X;
+    if (!X.begin(t)) return true;
     Factorization F;
     bool have = false;
-    while (reached < total && S.attempts < o->max_steps) {
+    while (!X.done() && S.attempts < o->max_steps) {
         S.attempts++;
-        h = std::min(h, total - reached);
+        const double hs = X.step();
         if (!have) {
             arnoldi(W, F);
             S.applications += F.used;
             S.factorizations++;
             have = true;
         }
-        double err = project(W, F, direction * h);
-        bool breakdown = F.completed;
+        const double err = project(W, F, hs);
         dev_norm<double>(W, W->trial.p, n, W->nscal.p + 4);
-        double trial_norm = read_scalar(W, W->nscal.p + 4);
-        bool finite = std::isfinite(err) && std::isfinite(trial_norm);
-        if (!finite) {
+        const double trial_norm = read_scalar(W, W->nscal.p + 4);
+        auto cur_norm = [&]() {
             dev_norm<double>(W, W->cur.p, n, W->nscal.p + 4);
-            trial_norm = read_scalar(W, W->nscal.p + 4);
-        }
-        double localtol = (h / total) * (o->atol + o->rtol * trial_norm);
-        if (!std::isfinite(localtol)) throw Error(PIMI_EINVAL, "exponential-action local tolerance overflowed");
-        bool accept = finite && (breakdown || err <= localtol);
-        if (accept) {
+            return read_scalar(W, W->nscal.p + 4);
+        };
+        const int d = X.decide(err, trial_norm, cur_norm, F, *o);
+        if (d > 0) {
             std::swap(W->cur.p, W->trial.p);
-            double remaining = total - reached;
-            reached = (h == remaining) ? total : reached + h;
             S.accepted++;
             S.error_sum += err;
             have = false;
         } else {
             S.rejected++;
+            if (d < 0) break;
         }
-        double factor;
-        if (!finite) factor = 0.2;
-        else if (breakdown) factor = 5.0;
-        else if (err == 0) factor = 2.0;
-        else {
-            factor = o->safety * std::pow(localtol / err, 1.0 / (double)std::max(F.k, 1));
-            factor = std::min(5.0, std::max(0.2, factor));
-        }
-        if (!accept) factor = std::min(factor, 0.5);
-        double hnew = std::min(hmax, h * factor);
-        if (!accept && (hnew < hmin || hnew == 0 || (reached > 0 && reached + hnew == reached))) break;
-        h = std::max(hnew, hmin);
     }
-    return reached == total;
+    return X.done();
 }
 
 static void expv_series(pimi_workspace *W, const double *rho0, const double *ts, int64_t nts,

@@ -659,18 +659,12 @@ class DynamicsStreamResult:
     states: list | None = None
 
 
-def solve_dynamics(compiled: CompiledPIModel, rho0: PIState, tspan, saveat=None, algorithm=None,
-                   observables=None, save_states=False):
-    """Expv dynamics (src/highlevel.jl:559-733): observables streamed as
-    device dots at every saved time; with save_states=True the state at every
-    saved time is also returned (one device-to-host copy per saved time, the
-    reference's per-interval krylov_expv! with copy(current)).  As in the
-    reference, save_states=False needs at least one observable."""
-    algorithm = algorithm or ExpvAlgorithm()
+def _dynamics_inputs(compiled, rho0, tspan, saveat, algorithm, observables):
+    """Saved times, observable names, the observables as rows of PI vectors
+    (or None) and the native options, validated as solve_dynamics does
+    (src/highlevel.jl:559-608)."""
     if not isinstance(algorithm, ExpvAlgorithm):
         raise ValueError("the mi355x backend propagates with ExpvAlgorithm")
-    if not observables and not save_states:
-        raise ValueError("save_states=false requires at least one observable")
     if rho0.basis != compiled.basis:
         raise ValueError("Liouvillian source and initial state use incompatible PI bases")
     ts = saved_times(tspan, saveat)
@@ -684,12 +678,26 @@ def solve_dynamics(compiled: CompiledPIModel, rho0: PIState, tspan, saveat=None,
         else:
             vecs.append(compiled.collective_operator(op).data)
     obs = np.ascontiguousarray(np.stack(vecs)) if vecs else None
+    opts = N.ExpvOpts(algorithm.krylovdim, 0, float(algorithm.atol), float(algorithm.rtol), float(algorithm.safety),
+                      int(algorithm.max_steps))
+    return ts, names, obs, opts
+
+
+def solve_dynamics(compiled: CompiledPIModel, rho0: PIState, tspan, saveat=None, algorithm=None,
+                   observables=None, save_states=False):
+    """Expv dynamics (src/highlevel.jl:559-733): observables streamed as
+    device dots at every saved time; with save_states=True the state at every
+    saved time is also returned (one device-to-host copy per saved time, the
+    reference's per-interval krylov_expv! with copy(current)).  As in the
+    reference, save_states=False needs at least one observable."""
+    algorithm = algorithm or ExpvAlgorithm()
+    if not observables and not save_states:
+        raise ValueError("save_states=false requires at least one observable")
+    ts, names, obs, opts = _dynamics_inputs(compiled, rho0, tspan, saveat, algorithm, observables)
     out = np.zeros((len(ts), len(names)), dtype=np.complex128)
     rho = np.ascontiguousarray(rho0.data, dtype=np.complex128)
     final = np.zeros_like(rho)
     ws = compiled.workspace(N.PIMI_C64, algorithm.krylovdim)
-    opts = N.ExpvOpts(algorithm.krylovdim, 0, float(algorithm.atol), float(algorithm.rtol), float(algorithm.safety),
-                      int(algorithm.max_steps))
     tsc = np.ascontiguousarray(ts, dtype=np.float64)
 
     def run(t_seg, x_in, obs_seg):

@@ -1,14 +1,17 @@
-"""Batched stationary solves on one device (libpimi's batch engine).
+"""Batched stationary solves and dynamics on one device (libpimi's batch engine).
 
 Many independent models (different N, rates, fields) are solved together:
 the engine admits queued jobs while its device-memory reservation holds
 their Krylov state, advances all admitted jobs one step per tick with one
 kernel launch per operation, and frees a job's memory as soon as it
-finishes.  Each job is the stationary solve of src/krylov.jl:797-891 with
-the sector preconditioner and the scale-invariant stopping rule, run as GMRES
+finishes.  A stationary job is the solve of src/krylov.jl:797-891 with the
+sector preconditioner and the scale-invariant stopping rule, run as GMRES
 cycles inside FP64 iterative refinement (complex FP32 Arnoldi with
-``precision=PIMI_C32``).  The reference solves one model at a time; this
-module has no counterpart there.
+``precision=PIMI_C32``).  A dynamics job (FP64 engines) is solve_dynamics
+with ExpvAlgorithm (src/highlevel.jl:559-733): its Arnoldi steps,
+projections and observations share the ticks' launches with the other
+jobs.  The reference solves one model at a time; this module has no
+counterpart there.
 """
 from __future__ import annotations
 
@@ -18,7 +21,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native as N
-from .api import CompiledPIModel, PIState, SteadyStateResult, _context
+from .api import (CompiledPIModel, DynamicsStreamResult, ExpvAlgorithm, PIState, SteadyStateResult, _context,
+                  _dynamics_inputs)
 
 
 @dataclass
@@ -84,6 +88,45 @@ class BatchEngine:
         N.check(N.lib().pimi_batch_submit(self._h, compiled._plan, C.byref(opts), C.byref(jid)))
         self._jobs[jid.value] = (compiled, a)
         return jid.value
+
+    def submit_dynamics(self, compiled: CompiledPIModel, rho0: PIState, tspan, saveat=None, algorithm=None,
+                        observables=None) -> int:
+        """Queue solve_dynamics(compiled, rho0, tspan; saveat, ExpvAlgorithm,
+        observables) as a job of this (FP64) engine; fetch_dynamics returns it."""
+        if compiled.device != self.device:
+            raise ValueError("the compiled model lives on another device")
+        algorithm = algorithm or ExpvAlgorithm()
+        if not observables:
+            raise ValueError("a batched dynamics job requires at least one observable")
+        ts, names, obs, opts = _dynamics_inputs(compiled, rho0, tspan, saveat, algorithm, observables)
+        rho = np.ascontiguousarray(rho0.data, dtype=np.complex128)
+        tsc = np.ascontiguousarray(ts, dtype=np.float64)
+        jid = C.c_int64()
+        N.check(N.lib().pimi_batch_submit_expv(self._h, compiled._plan, N.ptr(rho), N.ptr(tsc), C.c_int64(len(tsc)),
+                                               C.byref(opts), C.c_int32(len(names)), N.ptr(obs), C.byref(jid)))
+        self._jobs[jid.value] = (compiled, ("dynamics", ts, names))
+        return jid.value
+
+    def fetch_dynamics(self, job: int, check=True) -> DynamicsStreamResult:
+        """Result of a finished dynamics job; raises NotConverged (like
+        solve_dynamics) unless check is False."""
+        compiled, (_, ts, names) = self._jobs[job]
+        out = np.zeros((len(ts), len(names)), dtype=np.complex128)
+        final = np.zeros(len(compiled.basis), dtype=np.complex128)
+        info = N.SolveInfo()
+        st = N.lib().pimi_batch_fetch_expv(self._h, C.c_int64(job), N.ptr(out), N.ptr(final), C.byref(info))
+        if st not in (N.PIMI_OK, N.PIMI_ENOTCONVERGED):
+            N.check(st)
+        del self._jobs[job]
+        res = DynamicsStreamResult(ts, {name: out[:, k].copy() for k, name in enumerate(names)},
+                                   PIState(compiled.basis, final), info.as_dict())
+        if st == N.PIMI_ENOTCONVERGED and check:
+            N.check(st)
+        return res
+
+    def is_dynamics(self, job: int) -> bool:
+        meta = self._jobs[job][1]
+        return isinstance(meta, tuple) and meta[0] == "dynamics"
 
     def run(self, max_ticks=1 << 30, stop_when_finished=False) -> int:
         fin = C.c_int64()

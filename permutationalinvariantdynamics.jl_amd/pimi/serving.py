@@ -8,14 +8,17 @@ Stationary requests are continuously batched in one BatchEngine: a request
 that arrives while others are being solved is admitted between two ticks as
 soon as the engine's memory reservation holds its Krylov state (HBM-footprint
 admission), and finished jobs free their memory at once.  Dynamics requests
-run on a small pool of worker threads, each with its own device workspace and
-HIP stream, admitted while their workspaces fit a second memory budget; their
-kernels overlap the engine's on the device.  Model compilation and the
-stationary solver's host setup run on a thread pool off the loop thread.
+are continuously batched the same way in a second, FP64 engine (its own
+memory reservation and streams): their Arnoldi steps, projections and
+observations are one launch per step kind over every dynamics request in
+flight, and the loop ticks both engines in turn.  (`dynamics_mode="workers"`
+runs each dynamics request on its own worker thread, workspace and HIP
+stream instead.)  Model compilation and the stationary solver's host setup
+run on a thread pool off the loop thread.
 
 A request's result does not depend on the other requests in flight: the
-engine's arithmetic per job depends only on that job, and every dynamics
-request runs on its own workspace with deterministic reductions.
+engines' arithmetic per job depends only on that job, and a worker's
+dynamics request runs on its own workspace with deterministic reductions.
 
 With `snapshot_dir`, every finished request's state (the stationary state,
 or a dynamics request's state at t_end) is written as a PIDCHKPT v1
@@ -65,11 +68,17 @@ class Server:
     def __init__(self, device=0, precision=N.PIMI_C32, steady_algorithm: BatchAlgorithm | None = None,
                  dynamics_algorithm: ExpvAlgorithm | None = None, rtol=1e-8, memory_bytes=0,
                  dynamics_workers=2, dynamics_memory_bytes=32 << 30, compile_workers=8, t_end=10.0, saveat=0.2,
-                 snapshot_dir=None):
+                 snapshot_dir=None, dynamics_mode="engine"):
+        if dynamics_mode not in ("engine", "workers"):
+            raise ValueError("dynamics_mode must be 'engine' or 'workers'")
         self.device = device
         self.rtol = rtol
         self.steady_algorithm = steady_algorithm or BatchAlgorithm(krylovdim=200, inner_rtol=1e-4)
         self.dynamics_algorithm = dynamics_algorithm or ExpvAlgorithm(krylovdim=30, atol=0.0, rtol=rtol)
+        self.dynamics_mode = dynamics_mode
+        # the dynamics engine first: the stationary one reserves a share of what is left
+        self.dyn_engine = BatchEngine(precision=N.PIMI_C64, memory_bytes=dynamics_memory_bytes, device=device) \
+            if dynamics_mode == "engine" else None
         self.engine = BatchEngine(precision=precision, memory_bytes=memory_bytes, device=device)
         self.dynamics_workers = dynamics_workers
         self.dynamics_memory_bytes = dynamics_memory_bytes
@@ -82,6 +91,12 @@ class Server:
 
     def close(self):
         self.engine.close()
+        if self.dyn_engine is not None:
+            self.dyn_engine.close()
+
+    def _dynamics_inputs(self, comp):
+        rho0 = computational_product_state(comp.basis, 2)   # every particle in the upper level
+        return rho0, {"n": np.array([[0, 0], [0, 1]], dtype=complex)}
 
     # -- request preparation (host, thread pool) --------------------------------
     def _prepare(self, req):
@@ -91,11 +106,9 @@ class Server:
         return comp
 
     def _run_dynamics(self, comp, req):
-        b = comp.basis
-        rho0 = computational_product_state(b, 2)   # every particle in the upper level
-        res = solve_dynamics(comp, rho0, (0.0, self.t_end), saveat=self.saveat, algorithm=self.dynamics_algorithm,
-                             observables={"n": np.array([[0, 0], [0, 1]], dtype=complex)})
-        return res
+        rho0, obs = self._dynamics_inputs(comp)
+        return solve_dynamics(comp, rho0, (0.0, self.t_end), saveat=self.saveat, algorithm=self.dynamics_algorithm,
+                              observables=obs)
 
     def _snapshot(self, result, req):
         """Write a finished request's state as a PIDCHKPT v1 file."""
@@ -123,7 +136,8 @@ class Server:
         now = lambda: time.perf_counter() - t0  # noqa: E731
         lock = threading.Lock()
         prepared_steady = queue.Queue()
-        dyn_queue = queue.Queue()
+        prepared_dyn = queue.Queue()   # dynamics requests for the dynamics engine
+        dyn_queue = queue.Queue()      # dynamics requests for the workers
         dyn_budget = [self.dynamics_memory_bytes]
         budget_cv = threading.Condition()
         errors = []
@@ -137,6 +151,8 @@ class Server:
                 return
             if req.kind == "steady":
                 prepared_steady.put((i, comp))
+            elif self.dyn_engine is not None:
+                prepared_dyn.put((i, comp))
             else:
                 dyn_queue.put((i, comp))
 
@@ -169,7 +185,8 @@ class Server:
                         budget_cv.notify_all()
 
         pool = ThreadPoolExecutor(max_workers=self.compile_workers)
-        workers = [threading.Thread(target=dyn_worker, daemon=True) for _ in range(self.dynamics_workers)]
+        nworkers = self.dynamics_workers if self.dyn_engine is None else 0
+        workers = [threading.Thread(target=dyn_worker, daemon=True) for _ in range(nworkers)]
         for w in workers:
             w.start()
 
@@ -184,11 +201,13 @@ class Server:
         dispatcher = threading.Thread(target=dispatch, daemon=True)
         dispatcher.start()
         n_steady = sum(1 for r in reqs if r.kind == "steady")
-        jobs = {}      # engine job id -> (request index, compiled model)
-        done_steady = 0
-        eng = self.engine
+        n_dyn = len(reqs) - n_steady if self.dyn_engine is not None else 0
+        jobs = {}      # steady engine job id -> (request index, compiled model)
+        djobs = {}     # dynamics engine job id -> (request index, compiled model)
+        done_steady = done_dyn = 0
+        eng, deng = self.engine, self.dyn_engine
         try:
-            while done_steady < n_steady:
+            while done_steady < n_steady or done_dyn < n_dyn:
                 if errors:
                     raise errors[0]
                 # admit every prepared request that has arrived (between ticks)
@@ -199,10 +218,24 @@ class Server:
                         break
                     jid = eng.submit(comp, self.steady_algorithm, rtol=self.rtol)
                     jobs[jid] = (i, comp)
-                c = eng.counts()
-                if c["active"] + c["queued"] > 0:
-                    eng.run(max_ticks=1)
-                else:
+                while deng is not None:
+                    try:
+                        i, comp = prepared_dyn.get_nowait()
+                    except queue.Empty:
+                        break
+                    rho0, obs = self._dynamics_inputs(comp)
+                    jid = deng.submit_dynamics(comp, rho0, (0.0, self.t_end), saveat=self.saveat,
+                                               algorithm=self.dynamics_algorithm, observables=obs)
+                    djobs[jid] = (i, comp)
+                busy = False
+                for e in (eng, deng):
+                    if e is None:
+                        continue
+                    c = e.counts()
+                    if c["active"] + c["queued"] > 0:
+                        e.run(max_ticks=1)
+                        busy = True
+                if not busy:
                     time.sleep(2e-4)
                 while (jid := eng.next_finished()) is not None:
                     i, comp = jobs.pop(jid)
@@ -212,6 +245,15 @@ class Server:
                                               bool(r.info["converged"]), state=r.state, info=r.info)
                     self._snapshot(results[i], reqs[i])
                     done_steady += 1
+                while deng is not None and (jid := deng.next_finished()) is not None:
+                    i, comp = djobs.pop(jid)
+                    res = deng.fetch_dynamics(jid, check=False)
+                    t = now()
+                    results[i] = ServedResult(i, "dynamics", comp.basis.N, reqs[i].arrival, t, t - reqs[i].arrival,
+                                              bool(res.info.get("converged", 1)), state=res.final_state,
+                                              values=res.values, times=res.t, info=res.info)
+                    self._snapshot(results[i], reqs[i])
+                    done_dyn += 1
             dispatcher.join()
             pool.shutdown(wait=True)
             for _ in workers:

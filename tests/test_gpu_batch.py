@@ -13,6 +13,7 @@ import sys
 import numpy as np
 import pytest
 
+import parity_log
 import pimi
 import pimi._native as N
 from oracle import lowering as L
@@ -204,3 +205,118 @@ def test_batch_autotune_keeps_results():
     eng.close()
     for a, b in zip(ref, out):
         np.testing.assert_array_equal(a.state.data, b.state.data)
+
+
+def _excited(comp):
+    return pimi.computational_product_state(comp.basis, 2)
+
+
+def _dyn_observables(comp):
+    return {"n": SP2.jp @ SP2.jm, "jx": SP2.jx}
+
+
+def _relerr(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(np.asarray(b))), 1e-300))
+
+
+def test_batched_dynamics_match_the_single_vector_path():
+    """Dynamics jobs in the batch engine (the adaptive Krylov exponential of
+    solve_dynamics, src/highlevel.jl:559-733, with its Arnoldi steps,
+    projections and observations in the engine's ticks beside a stationary
+    job) against pimi.solve_dynamics on the same models: observables at every
+    saved time and final states agree to 1e-10 relative (both at rtol 1e-12;
+    the engine's Arnoldi basis is orthogonalized by delayed
+    reorthogonalization, the single path's by CGS2), and the job statistics
+    are those of an adaptive run (accepted steps, factorizations)."""
+    print("batched dynamics vs single path", flush=True)
+    alg = pimi.ExpvAlgorithm(krylovdim=30, rtol=1e-12)
+    comps = [pimi.compile(pimi.Models.driven_qubits(8, drive=1.0, detuning=1.0, emission=0.1, dephasing=0.2,
+                                                    pumping=0.0)),
+             pimi.compile(_c2_model(20))]
+    eng = pimi.BatchEngine(precision=N.PIMI_C64, memory_bytes=1 << 30)
+    try:
+        steady = eng.submit(pimi.compile(_c2_model(13)), pimi.BatchAlgorithm(krylovdim=150), rtol=1e-10)
+        jobs = [eng.submit_dynamics(c, _excited(c), (0.0, 4.0), saveat=0.1, algorithm=alg,
+                                    observables=_dyn_observables(c)) for c in comps]
+        eng.run()
+        res = [eng.fetch_dynamics(j) for j in jobs]
+        assert eng.fetch(steady).info["converged"]
+    finally:
+        eng.close()
+    for c, r in zip(comps, res):
+        ref = pimi.solve_dynamics(c, _excited(c), (0.0, 4.0), saveat=0.1, algorithm=alg,
+                                  observables=_dyn_observables(c))
+        assert np.array_equal(r.t, ref.t)
+        err = max(_relerr(r.values[k], ref.values[k]) for k in ref.values)
+        serr = _relerr(r.final_state.data, ref.final_state.data)
+        parity_log.record(f"batched_dynamics_n{c.basis.N}_observables", err, 1e-10)
+        parity_log.record(f"batched_dynamics_n{c.basis.N}_state", serr, 1e-10)
+        assert err <= 1e-10 and serr <= 1e-10, (err, serr)
+        assert r.info["converged"] and r.info["accepted"] >= 40 and r.info["factorizations"] >= r.info["accepted"]
+
+
+def test_batched_dynamics_c1_closed_form_and_alone_equals_mixed():
+    """The C1 dynamics (driven_qubits(8) from the excited state, t in [0, 4])
+    as a batch-engine job against the closed form of the single-qubit
+    Lindbladian (<sum n_i>(t), 41 times), and bit for bit the same alone and
+    in a batch with other dynamics and stationary jobs."""
+    print("batched dynamics C1 closed form and determinism", flush=True)
+    model = pimi.Models.driven_qubits(8, drive=1.0, detuning=1.0, emission=0.1, dephasing=0.2, pumping=0.0)
+    comp = pimi.compile(model)
+    alg = pimi.ExpvAlgorithm(krylovdim=30, rtol=1e-12)
+    obs = {"n": SP2.jp @ SP2.jm}
+
+    def run(extra):
+        eng = pimi.BatchEngine(precision=N.PIMI_C64, memory_bytes=1 << 30)
+        try:
+            others = []
+            if extra:
+                c2 = pimi.compile(_c2_model(17))
+                others.append(eng.submit_dynamics(c2, _excited(c2), (0.0, 2.0), saveat=0.5, algorithm=alg,
+                                                  observables=_dyn_observables(c2)))
+                eng.submit(pimi.compile(_c2_model(9)), pimi.BatchAlgorithm(krylovdim=100), rtol=1e-10)
+            j = eng.submit_dynamics(comp, _excited(comp), (0.0, 4.0), saveat=0.1, algorithm=alg, observables=obs)
+            eng.run()
+            return eng.fetch_dynamics(j)
+        finally:
+            eng.close()
+
+    alone, mixed = run(False), run(True)
+    assert np.array_equal(alone.values["n"], mixed.values["n"])
+    assert np.array_equal(alone.final_state.data, mixed.final_state.data)
+    import json
+    with open(os.path.join(os.path.dirname(__file__), "golden", "closed_form.json")) as fh:
+        g = json.load(fh)["c1"]
+    np.testing.assert_allclose(alone.t, g["t"], rtol=0, atol=0)
+    gold = np.asarray(g["n"])
+    err = _relerr(alone.values["n"].real, gold)
+    parity_log.record("batched_dynamics_c1_closed_form", err, 1e-8)
+    assert err <= 1e-8, err
+
+
+def test_batched_dynamics_rejects_bad_requests():
+    comp = pimi.compile(_c2_model(6))
+    eng32 = pimi.BatchEngine(precision=N.PIMI_C32, memory_bytes=1 << 26)
+    try:
+        with pytest.raises(pimi.PIMIError):
+            eng32.submit_dynamics(comp, _excited(comp), (0.0, 1.0), saveat=0.5, observables=_dyn_observables(comp))
+    finally:
+        eng32.close()
+    eng = pimi.BatchEngine(precision=N.PIMI_C64, memory_bytes=1 << 26)
+    try:
+        with pytest.raises(ValueError):
+            eng.submit_dynamics(comp, _excited(comp), (0.0, 1.0), saveat=0.5)
+        with pytest.raises(ValueError):
+            eng.submit_dynamics(comp, _excited(comp), (0.0, 1.0), saveat=0.5, observables=_dyn_observables(comp),
+                                algorithm=pimi.ExpvAlgorithm(safety=1.5))
+        # the native checks behind the Python ones: non-monotone output times
+        opts = N.ExpvOpts(30, 0, 0.0, 1e-10, 0.9, 1000)
+        rho = np.ascontiguousarray(_excited(comp).data)
+        ts = np.array([0.0, 1.0, 0.5])
+        obs = np.ascontiguousarray(comp.collective_operator(SP2.jz).data[None, :])
+        jid = C.c_int64()
+        st = N.lib().pimi_batch_submit_expv(eng._h, comp._plan, N.ptr(rho), N.ptr(ts), C.c_int64(3), C.byref(opts),
+                                            C.c_int32(1), N.ptr(obs), C.byref(jid))
+        assert st == N.PIMI_EINVAL
+    finally:
+        eng.close()

@@ -113,3 +113,31 @@ def test_serving_snapshots_reload_bit_for_bit(tmp_path):
         assert cp.metadata["kind"] == q.kind and int(cp.metadata["N"]) == q.instance.N
         assert float(cp.metadata["omega"]) == q.instance.omega
         assert (cp.time is None) == (q.kind == "steady")
+
+
+def test_serving_dynamics_engine_matches_workers():
+    """Dynamics requests batched in the FP64 dynamics engine (the default)
+    against the same requests on worker streams (pimi_expv_series): the
+    observable series agree to 1e-10 relative, and every request's series
+    follows the closed form."""
+    from pimi.workloads import Instance
+    import parity_log
+    reqs = [Request(0.0, "dynamics", Instance(n_, om, 0.1, gp))
+            for n_, om, gp in ((12, 0.7, 0.02), (30, 1.1, 0.0), (57, 0.4, 0.05))]
+    out = {}
+    for mode in ("engine", "workers"):
+        srv = Server(precision=pimi.PIMI_C64, rtol=1e-11, memory_bytes=1 << 30, dynamics_workers=2,
+                     dynamics_memory_bytes=2 << 30, compile_workers=3, t_end=3.0, saveat=0.25, dynamics_mode=mode)
+        try:
+            out[mode] = srv.serve(reqs, realtime=False)
+        finally:
+            srv.close()
+    worst = 0.0
+    for a, b, q in zip(out["engine"], out["workers"], reqs):
+        assert a.converged and b.converged
+        np.testing.assert_array_equal(a.times, b.times)
+        worst = max(worst, float(np.abs(a.values["n"] - b.values["n"]).max() / np.abs(b.values["n"]).max()))
+        ref = _closed(q.instance, list(a.times))
+        np.testing.assert_allclose(a.values["n"].real, ref, rtol=1e-8, atol=1e-10)
+    parity_log.record("served_dynamics_engine_vs_workers", worst, 1e-10)
+    assert worst <= 1e-10, worst
