@@ -444,13 +444,53 @@ SERVE_RATES = (50.0, 200.0, 1000.0, 2000.0)
 SERVE_COUNT = 24
 
 
+def batched_dynamics(pimi, count=64, single_sample=8):
+    """`count` seeded driven-qubit models (N in 20..120) propagated from the
+    excited product state over t in [0, 10], saved every 0.2 (expv, Krylov 30,
+    rtol 1e-8; observable sum_i n_i), as dynamics jobs of one FP64 batch
+    engine, against solve_dynamics one model at a time on a sample."""
+    from pimi.workloads import batched_instances
+    inst = batched_instances(count, seed=20261206)
+    comps = [pimi.compile(i.model()) for i in inst]
+    alg = pimi.ExpvAlgorithm(krylovdim=30, atol=0.0, rtol=1e-8)
+    obs = {"n": np.array([[0, 0], [0, 1]], dtype=complex)}
+    eng = pimi.BatchEngine(precision=pimi.PIMI_C64)
+    eng.autotune(comps)
+    t0 = time.perf_counter()
+    ids = [eng.submit_dynamics(c, pimi.computational_product_state(c.basis, 2), (0.0, 10.0), saveat=0.2,
+                               algorithm=alg, observables=obs) for c in comps]
+    out = {}
+    while len(out) < len(ids):
+        eng.run(stop_when_finished=True)
+        while (j := eng.next_finished()) is not None:
+            out[j] = eng.fetch_dynamics(j)
+    wall = time.perf_counter() - t0
+    eng.close()
+    res = [out[j] for j in ids]
+    t0 = time.perf_counter()
+    single = [pimi.solve_dynamics(c, pimi.computational_product_state(c.basis, 2), (0.0, 10.0), saveat=0.2,
+                                  algorithm=alg, observables=obs) for c in comps[:single_sample]]
+    single_rate = single_sample / (time.perf_counter() - t0)
+    diff = max(float(np.abs(r.values["n"] - s_.values["n"]).max() / np.abs(s_.values["n"]).max())
+               for r, s_ in zip(res, single))
+    print(f"# batched dynamics: {len(res)} jobs in {wall:.2f} s", file=sys.stderr, flush=True)
+    return {"workload": f"{count} seeded driven-qubit models (N in 20..120), excited product state, t in [0, 10] "
+                        f"saved every 0.2, expv Krylov 30, rtol 1e-8, FP64 batch engine",
+            "jobs_per_s": len(res) / wall, "wall_s": wall,
+            "converged": int(sum(bool(r.info["converged"]) for r in res)),
+            "single_path_jobs_per_s": single_rate, "single_sample": f"first {single_sample} of the same models, "
+                                                                    f"one at a time",
+            "speedup_vs_single_path": (len(res) / wall) / single_rate,
+            "max_rel_obs_diff_vs_single_path": diff}
+
+
 def serving(pimi, rates=SERVE_RATES, count=SERVE_COUNT):
     """The C5 stream (60 % stationary, 40 % dynamics to t=10 with 50 save
     points, N ~ U{16..160}) offered at each arrival rate: `count` requests
     per rate through one server (FP32 stationary jobs continuously batched
-    in the engine, dynamics on worker streams); p50/p99 latency and served
-    requests/s per rate.  Latency includes queueing once the offered rate
-    exceeds the service rate."""
+    in one engine, dynamics jobs continuously batched in an FP64 engine);
+    p50/p99 latency and served requests/s per rate.  Latency includes
+    queueing once the offered rate exceeds the service rate."""
     from pimi.serving import Server, summarize
     from pimi.workloads import served_requests
     srv = Server(precision=pimi.PIMI_C32)
@@ -591,6 +631,7 @@ def main():
     ms_gemm, flops_gemm, tfs_gemm = gemm_roofline(pimi, comp)
     batch, spmv_batch = (None, None) if args.no_batch else batched(pimi, args)
     batch120 = None if args.no_batch else batched_n120(pimi)
+    batch_dyn = None if args.no_batch else batched_dynamics(pimi)
     serve = None if args.no_serve else serving(pimi)
     precision = None if args.no_batch else precision_form(pimi)
     dynamics = dynamics_n120(pimi, comp, cpu=not args.no_cpu)
@@ -647,6 +688,7 @@ def main():
         "spmv": spmv,
         "batched": batch,
         "batched_n120": batch120,
+        "batched_dynamics": batch_dyn,
         "precision_form": precision,
         "dynamics_n120": dynamics,
         "serving": serve,

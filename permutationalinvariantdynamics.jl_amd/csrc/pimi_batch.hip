@@ -760,7 +760,15 @@ struct BJob {
     std::vector<cx<double>> coeffs;
     int64_t accepted = 0, rejected = 0, factorizations = 0, attempts = 0;
     double error_sum = 0;
+    // a factorization's Arnoldi steps run in one tick, step i reporting in
+    // mail region i (dyn_mail_stride complex each)
+    int arn_steps = 0;
+    int64_t mail_stride = 0;
 };
+
+static int64_t dyn_mail_stride(int kdim, int nobs) {
+    return kMailHacc + 2 * (int64_t)std::max(kdim + 3, nobs + 2);
+}
 
 // Workgroups of a job's BLAS-1 passes and reduction partials: one per 2048
 // coordinates (so a lone large job still fills the chip), at most 1024; fixed
@@ -1068,26 +1076,40 @@ struct Tick {
     // One delayed-reorthogonalization Arnoldi step (see kb_dcgs_update):
     // V[:, j] becomes q_j and V[:, j+1] the next candidate; the reductions
     // V^H p, V^H w land in the mailbox for the host's Hessenberg update.
+    // Step j of job J is J->j (or step_of(J)); its reductions and scalars go to
+    // the mail region mail_of(J) (default: the head of J's mailbox).
     template <typename R>
-    void dcgs(const std::vector<BJob *> &js) {
+    void dcgs(const std::vector<BJob *> &js, std::function<int(BJob *)> step_of = nullptr,
+              std::function<cx<double> *(BJob *)> mail_of = nullptr) {
         if (js.empty()) return;
         const int saved = cls;
         std::vector<BVec> d, r, c, u;
         for (BJob *J : js) {
-            const int nc = J->j + 1;
+            const int jj = step_of ? step_of(J) : J->j;
+            cx<double> *mb = mail_of ? mail_of(J) : J->mail;
+            const int nc = jj + 1;
             bytes[OC_DOTS] += (double)(nc + 1) * J->n * sizeof(cx<R>);
-            bytes[OC_UPDATE] += (double)(J->j + 4) * J->n * sizeof(cx<R>);
+            bytes[OC_UPDATE] += (double)(jj + 4) * J->n * sizeof(cx<R>);
             d.push_back(BVec{0, J->V, J->w, J->partial, nullptr, J->n, J->nblk, nc, 0.0, 0});
-            r.push_back(BVec{0, J->partial, nullptr, J->mail + kMailHacc, nullptr, J->n, J->nblk, nc, 0.0, 0});
-            c.push_back(BVec{0, J->mail + kMailHacc, nullptr, J->h, J->scal(), J->n, J->nblk, nc, 0.0, 0});
-            u.push_back(BVec{0, J->V, J->h, J->w, J->scal(), J->n, J->nblk, nc, 0.0, 0});
+            r.push_back(BVec{0, J->partial, nullptr, mb + kMailHacc, nullptr, J->n, J->nblk, nc, 0.0, 0});
+            c.push_back(BVec{0, mb + kMailHacc, nullptr, J->h, (double *)mb, J->n, J->nblk, nc, 0.0, 0});
+            u.push_back(BVec{0, J->V, J->h, J->w, (double *)mb, J->n, J->nblk, nc, 0.0, 0});
         }
         cls = OC_DOTS;
-        for (auto &e : d) e.u0 = kDotChunks;
-        vec(d, kb_dots2_partial<R>,
-            [](const BVec &e) {
-                return (int64_t)((e.nblk + kDotChunks - 1) / kDotChunks) * ((e.ncol + kDotCols - 1) / kDotCols);
-            });
+        // chunks per workgroup: kDotChunks when the launch has workgroups
+        // enough to fill the chip, fewer when few jobs share it (each chunk's
+        // partial is summed on its own, so results do not depend on this)
+        int64_t cpw = kDotChunks;
+        auto groups = [&](int64_t c) {
+            int64_t g = 0;
+            for (const BVec &e : d) g += ((e.nblk + c - 1) / c) * ((e.ncol + kDotCols - 1) / kDotCols);
+            return g;
+        };
+        while (cpw > 1 && groups(cpw) < 2048) cpw /= 2;  // eight workgroups per CU
+        for (auto &e : d) e.u0 = (uint64_t)cpw;
+        vec(d, kb_dots2_partial<R>, [](const BVec &e) {
+            return (int64_t)((e.nblk + (int64_t)e.u0 - 1) / (int64_t)e.u0) * ((e.ncol + kDotCols - 1) / kDotCols);
+        });
         cls = OC_OTHER;
         vec(r, kb_reduce_cols2, [](const BVec &e) { return (int64_t)e.ncol; }, 64);
         {
@@ -1127,6 +1149,7 @@ struct Tick {
 // basis with room for one enlargement step, w, reduction partials (for the
 // basis or the observables, whichever has more columns), Hessenberg
 // coefficients, mailbox, norm partials and the observables.
+static int64_t dyn_mail_stride(int kdim, int nobs);
 static size_t dyn_job_bytes(int64_t n, int kdim, int nobs) {
     const int nblk = blocks_for(n);
     const int cols = std::max(kdim + 2, nobs + 1);
@@ -1134,7 +1157,8 @@ static size_t dyn_job_bytes(int64_t n, int kdim, int nobs) {
     b += 2 * align_up(sizeof(cx<double>) * n, 256);
     b += align_up(sizeof(cx<double>) * n * (size_t)(kdim + 3), 256) + align_up(sizeof(cx<double>) * n, 256);
     b += align_up(sizeof(cx<double>) * nblk * 2 * (size_t)cols, 256);
-    b += 2 * align_up(sizeof(cx<double>) * (size_t)(2 * (cols + 1) + kMailHacc), 256);
+    b += align_up(sizeof(cx<double>) * (size_t)(2 * (cols + 1) + kMailHacc), 256);
+    b += align_up(sizeof(cx<double>) * (size_t)(dyn_mail_stride(kdim, nobs) * (kdim + 2)), 256);
     b += align_up(sizeof(double) * 4 * nblk, 256);
     b += align_up(sizeof(cx<double>) * n * (size_t)(nobs + 1), 256);
     return b;
@@ -1162,7 +1186,8 @@ static void carve(pimi_batch *E, BJob *J) {
         J->w = take(sizeof(cx<double>) * n);
         J->partial = (cx<double> *)take(sizeof(cx<double>) * J->nblk * 2 * (size_t)cols);
         J->h = (cx<double> *)take(sizeof(cx<double>) * (size_t)(2 * (cols + 1) + kMailHacc));
-        J->mail = (cx<double> *)take(sizeof(cx<double>) * (size_t)(2 * (cols + 1) + kMailHacc));
+        J->mail_stride = dyn_mail_stride(J->kdim, J->nobs);
+        J->mail = (cx<double> *)take(sizeof(cx<double>) * (size_t)(J->mail_stride * (J->kdim + 2)));
         J->npart = (double *)take(sizeof(double) * 4 * J->nblk);
         J->obs = (cx<double> *)take(sizeof(cx<double>) * n * (size_t)(J->nobs + 1));
         if ((size_t)(p - (E->arena.base + J->off)) > J->bytes)
@@ -1395,10 +1420,23 @@ static void launch_tick(pimi_batch *E, BLane &L) {
           }),
           kb_scale_div<R, double>, Tick::nblk_of);
     if (!xarn.empty()) {
-        T.cls = OC_APPLY;
-        T.apply<R>(xarn, [](BJob *J) -> const void * { return vcol<R>(J, J->j); },
-                   [](BJob *J) -> void * { return J->w; }, true);
-        T.dcgs<R>(xarn);
+        // a new factorization stages its steps 0..m (column m-1 is final after
+        // step m); the enlargement after a near breakdown is one more step
+        int most = 0;
+        for (BJob *J : xarn) {
+            J->arn_steps = J->enlarging ? 1 : J->kdim + 1;
+            most = std::max(most, J->arn_steps);
+        }
+        for (int i = 0; i < most; ++i) {
+            std::vector<BJob *> at;
+            for (BJob *J : xarn)
+                if (i < J->arn_steps) at.push_back(J);
+            T.cls = OC_APPLY;
+            T.apply<R>(at, [i](BJob *J) -> const void * { return vcol<R>(J, J->j + i); },
+                       [](BJob *J) -> void * { return J->w; }, true);
+            T.dcgs<R>(at, [i](BJob *J) { return J->j + i; },
+                      [i](BJob *J) { return J->mail + (size_t)i * (size_t)J->mail_stride; });
+        }
     }
     if (!xproj.empty()) {
         T.cls = OC_CYCLE_END;
@@ -1455,9 +1493,10 @@ static void launch_tick(pimi_batch *E, BLane &L) {
     moff.assign(L.active.size(), 0);
     for (size_t k = 0; k < L.active.size(); ++k) {
         BJob *J = L.active[k];
-        const int64_t cnt = kMailHacc + ((J->phase == PH_ARNOLDI || J->phase == PX_ARNOLDI) ? 2 * (J->j + 1)
-                                         : J->phase == PX_OBSERVE                          ? 2 * (J->nobs + 1)
-                                                                                           : 0);
+        const int64_t cnt = J->phase == PX_ARNOLDI  ? J->arn_steps * J->mail_stride
+                            : J->phase == PH_ARNOLDI ? kMailHacc + 2 * (J->j + 1)
+                            : J->phase == PX_OBSERVE ? kMailHacc + 2 * (J->nobs + 1)
+                                                     : kMailHacc;
         moff[k] = mtot;
         g.push_back(BCopy{J->mail, nullptr, cnt});
         mtot += (size_t)cnt;
@@ -1811,7 +1850,13 @@ static void complete_tick(pimi_batch *E, BLane &L) {
                 break;
             }
             case PX_ARNOLDI:
-                dyn_arnoldi(J, mb, sc, done, ok);
+                // the staged steps in order, until the factorization is
+                // complete (a breakdown ends it early: later steps' values
+                // are never read) or the job fails
+                for (int i = 0; i < J->arn_steps && J->phase == PX_ARNOLDI && !done; ++i) {
+                    const cx<double> *mi = mb + (size_t)i * (size_t)J->mail_stride;
+                    dyn_arnoldi(J, mi, (const double *)mi, done, ok);
+                }
                 break;
             case PX_PROJECT: {
                 int d = -1;

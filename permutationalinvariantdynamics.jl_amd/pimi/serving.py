@@ -203,11 +203,46 @@ class Server:
         n_steady = sum(1 for r in reqs if r.kind == "steady")
         n_dyn = len(reqs) - n_steady if self.dyn_engine is not None else 0
         jobs = {}      # steady engine job id -> (request index, compiled model)
-        djobs = {}     # dynamics engine job id -> (request index, compiled model)
-        done_steady = done_dyn = 0
         eng, deng = self.engine, self.dyn_engine
+
+        def dyn_loop():   # the dynamics engine's own loop (its ticks block only this thread)
+            djobs, done = {}, 0
+            try:
+                while done < n_dyn and not errors:
+                    while True:
+                        try:
+                            i, comp = prepared_dyn.get_nowait()
+                        except queue.Empty:
+                            break
+                        rho0, obs = self._dynamics_inputs(comp)
+                        jid = deng.submit_dynamics(comp, rho0, (0.0, self.t_end), saveat=self.saveat,
+                                                   algorithm=self.dynamics_algorithm, observables=obs)
+                        djobs[jid] = (i, comp)
+                    c = deng.counts()
+                    if c["active"] + c["queued"] > 0:
+                        deng.run(max_ticks=1)
+                    else:
+                        time.sleep(2e-4)
+                    while (jid := deng.next_finished()) is not None:
+                        i, comp = djobs.pop(jid)
+                        res = deng.fetch_dynamics(jid, check=False)
+                        t = now()
+                        r = ServedResult(i, "dynamics", comp.basis.N, reqs[i].arrival, t, t - reqs[i].arrival,
+                                         bool(res.info.get("converged", 1)), state=res.final_state,
+                                         values=res.values, times=res.t, info=res.info)
+                        self._snapshot(r, reqs[i])
+                        with lock:
+                            results[i] = r
+                        done += 1
+            except Exception as e:
+                errors.append(e)
+
+        dyn_thread = threading.Thread(target=dyn_loop, daemon=True) if n_dyn else None
+        if dyn_thread is not None:
+            dyn_thread.start()
+        done_steady = 0
         try:
-            while done_steady < n_steady or done_dyn < n_dyn:
+            while done_steady < n_steady:
                 if errors:
                     raise errors[0]
                 # admit every prepared request that has arrived (between ticks)
@@ -218,24 +253,10 @@ class Server:
                         break
                     jid = eng.submit(comp, self.steady_algorithm, rtol=self.rtol)
                     jobs[jid] = (i, comp)
-                while deng is not None:
-                    try:
-                        i, comp = prepared_dyn.get_nowait()
-                    except queue.Empty:
-                        break
-                    rho0, obs = self._dynamics_inputs(comp)
-                    jid = deng.submit_dynamics(comp, rho0, (0.0, self.t_end), saveat=self.saveat,
-                                               algorithm=self.dynamics_algorithm, observables=obs)
-                    djobs[jid] = (i, comp)
-                busy = False
-                for e in (eng, deng):
-                    if e is None:
-                        continue
-                    c = e.counts()
-                    if c["active"] + c["queued"] > 0:
-                        e.run(max_ticks=1)
-                        busy = True
-                if not busy:
+                c = eng.counts()
+                if c["active"] + c["queued"] > 0:
+                    eng.run(max_ticks=1)
+                else:
                     time.sleep(2e-4)
                 while (jid := eng.next_finished()) is not None:
                     i, comp = jobs.pop(jid)
@@ -245,15 +266,8 @@ class Server:
                                               bool(r.info["converged"]), state=r.state, info=r.info)
                     self._snapshot(results[i], reqs[i])
                     done_steady += 1
-                while deng is not None and (jid := deng.next_finished()) is not None:
-                    i, comp = djobs.pop(jid)
-                    res = deng.fetch_dynamics(jid, check=False)
-                    t = now()
-                    results[i] = ServedResult(i, "dynamics", comp.basis.N, reqs[i].arrival, t, t - reqs[i].arrival,
-                                              bool(res.info.get("converged", 1)), state=res.final_state,
-                                              values=res.values, times=res.t, info=res.info)
-                    self._snapshot(results[i], reqs[i])
-                    done_dyn += 1
+            if dyn_thread is not None:
+                dyn_thread.join()
             dispatcher.join()
             pool.shutdown(wait=True)
             for _ in workers:
