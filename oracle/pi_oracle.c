@@ -4,9 +4,10 @@
  *
  * TEST INFRASTRUCTURE ONLY.  The product library never links this file; only
  * tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it, and
- * only as the checker.  Parity of each function is pinned by the tests named
- * in the oracle's README section of DESIGN.md (brute-force Hilbert-space
- * Lindbladians, closed forms and invariants).
+ * only as the checker.  Parity of each function is pinned by
+ * tests/test_oracle_pins.py (brute-force Hilbert-space Lindbladians, closed
+ * forms, dense norms and inverses, invariants) and the fixtures under
+ * tests/golden/.
  *
  * Restated reference functions (paths relative to the reference root):
  *   fused apply            src/liouvillian.jl:3070-3105, 3018-3028, 2461-2561
