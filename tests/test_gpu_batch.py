@@ -320,3 +320,62 @@ def test_batched_dynamics_rejects_bad_requests():
         assert st == N.PIMI_EINVAL
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("N_", [1, 2, 3])
+def test_batched_dynamics_small_spaces_and_special_times(N_):
+    """Dynamics jobs where the Krylov space is the whole space (exact
+    breakdowns: n = 4, 9, 16 below the Krylov dimension), a repeated save
+    time (a zero-length interval) and a zero initial state (the zero-state
+    projection), against the single-vector path."""
+    print(f"batched dynamics small spaces N={N_}", flush=True)
+    c = pimi.compile(_c2_model(N_))
+    alg = pimi.ExpvAlgorithm(krylovdim=30, rtol=1e-12)
+    obs = _dyn_observables(c)
+    times = [0.0, 0.5, 0.5, 1.25, 3.0]
+    zero = pimi.PIState(c.basis, np.zeros(len(c.basis), dtype=np.complex128))
+    eng = pimi.BatchEngine(precision=N.PIMI_C64, memory_bytes=1 << 26)
+    try:
+        j1 = eng.submit_dynamics(c, _excited(c), (0.0, 3.0), saveat=times, algorithm=alg, observables=obs)
+        j2 = eng.submit_dynamics(c, zero, (0.0, 3.0), saveat=times, algorithm=alg, observables=obs)
+        eng.run()
+        r1, r2 = eng.fetch_dynamics(j1), eng.fetch_dynamics(j2)
+    finally:
+        eng.close()
+    ref = pimi.solve_dynamics(c, _excited(c), (0.0, 3.0), saveat=times, algorithm=alg, observables=obs)
+    assert np.array_equal(r1.t, ref.t)
+    err = max(_relerr(r1.values[k], ref.values[k]) for k in ref.values)
+    assert err <= 1e-12, err
+    assert _relerr(r1.final_state.data, ref.final_state.data) <= 1e-12
+    assert np.array_equal(r1.values["n"][1], r1.values["n"][2])  # the repeated time sees the same state
+    assert all(np.all(v == 0) for v in r2.values.values()) and np.all(r2.final_state.data == 0)
+    assert r2.info["converged"]
+
+
+def test_batched_dynamics_max_steps_reports_failure():
+    """A dynamics job whose attempts run out (max_steps) ends as not converged
+    with the reference's message; the other jobs of the batch are unaffected."""
+    c = pimi.compile(_c2_model(10))
+    obs = _dyn_observables(c)
+    eng = pimi.BatchEngine(precision=N.PIMI_C64, memory_bytes=1 << 28)
+    try:
+        bad = eng.submit_dynamics(c, _excited(c), (0.0, 50.0), saveat=25.0, observables=obs,
+                                  algorithm=pimi.ExpvAlgorithm(krylovdim=4, rtol=1e-12, max_steps=3))
+        good = eng.submit_dynamics(c, _excited(c), (0.0, 1.0), saveat=0.5, observables=obs,
+                                   algorithm=pimi.ExpvAlgorithm(krylovdim=30, rtol=1e-10))
+        eng.run()
+        rb = eng.fetch_dynamics(bad, check=False)
+        assert not rb.info["converged"] and rb.info["iterations"] == 3
+        with pytest.raises(pimi.NotConverged):
+            eng2 = pimi.BatchEngine(precision=N.PIMI_C64, memory_bytes=1 << 28)
+            try:
+                j = eng2.submit_dynamics(c, _excited(c), (0.0, 50.0), saveat=25.0, observables=obs,
+                                         algorithm=pimi.ExpvAlgorithm(krylovdim=4, rtol=1e-12, max_steps=3))
+                eng2.run()
+                eng2.fetch_dynamics(j)
+            finally:
+                eng2.close()
+        rg = eng.fetch_dynamics(good)
+        assert rg.info["converged"]
+    finally:
+        eng.close()
