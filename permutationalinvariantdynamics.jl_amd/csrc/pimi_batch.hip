@@ -755,7 +755,8 @@ struct BJob {
     cx<double> *obs = nullptr;
     Factorization F;
     ExpvControl X;
-    bool have = false, near = false, enlarging = false;
+    DelayedArnoldi arn;
+    bool have = false;
     double err = 0;
     std::vector<cx<double>> coeffs;
     int64_t accepted = 0, rejected = 0, factorizations = 0, attempts = 0;
@@ -839,6 +840,24 @@ void dcgs_single_descs(void *host, int m, void *V, const void *w, void *partial,
         d[4 * j + 0] = BVec{0, V, w, partial, nullptr, n, nblk, nc, 0.0, 1};
         d[4 * j + 1] = BVec{0, partial, nullptr, mail, nullptr, n, nblk, nc, 0.0, 0};
         d[4 * j + 2] = BVec{0, mail, nullptr, h, slot, n, nblk, nc, 0.0, 0};
+        d[4 * j + 3] = BVec{0, V, h, const_cast<void *>(w), slot, n, nblk, nc, 0.0, 0};
+    }
+}
+
+static_assert(kArnMailHead == kMailHacc && kArnSlotRho == kSlotRho && kArnSlotE == kSlotE,
+              "the Arnoldi mail layout is the engine mailbox's");
+
+void dcgs_arnoldi_descs(void *host, int steps, void *V, const void *w, void *partial, cx<double> *mail,
+                        int64_t stride, void *h, int64_t n) {
+    BVec *d = static_cast<BVec *>(host);
+    const int nblk = blocks_for(n);
+    for (int j = 0; j < steps; ++j) {
+        const int nc = j + 1;
+        cx<double> *mb = mail + (size_t)j * (size_t)stride;
+        double *slot = reinterpret_cast<double *>(mb);
+        d[4 * j + 0] = BVec{0, V, w, partial, nullptr, n, nblk, nc, 0.0, 1};
+        d[4 * j + 1] = BVec{0, partial, nullptr, mb + kMailHacc, nullptr, n, nblk, nc, 0.0, 0};
+        d[4 * j + 2] = BVec{0, mb + kMailHacc, nullptr, h, slot, n, nblk, nc, 0.0, 0};
         d[4 * j + 3] = BVec{0, V, h, const_cast<void *>(w), slot, n, nblk, nc, 0.0, 0};
     }
 }
@@ -1424,7 +1443,7 @@ static void launch_tick(pimi_batch *E, BLane &L) {
         // step m); the enlargement after a near breakdown is one more step
         int most = 0;
         for (BJob *J : xarn) {
-            J->arn_steps = J->enlarging ? 1 : J->kdim + 1;
+            J->arn_steps = J->arn.enlarging ? 1 : J->kdim + 1;
             most = std::max(most, J->arn_steps);
         }
         for (int i = 0; i < most; ++i) {
@@ -1581,8 +1600,7 @@ static bool dyn_attempt(BJob *J) {
         } else {
             J->F.beta = J->xnorm;
             J->j = 0;
-            J->near = J->enlarging = false;
-            std::fill(J->Hraw.begin(), J->Hraw.end(), cd(0, 0));
+            J->arn.reset(J->kdim, J->n);
             J->phase = PX_ARNOLDI;
             return true;
         }
@@ -1609,78 +1627,28 @@ static bool dyn_next_interval(BJob *J, bool &done) {
     return dyn_attempt(J);
 }
 
-// An Arnoldi tick of a dynamics job's factorization: Hessenberg column j-1
-// final (as for the stationary jobs), breakdown tests as arnoldi() of the
-// single-vector path (exact: a zero subdiagonal or the whole space; near:
-// subdiagonal <= sqrt(eps) |A v_j|, with |A v_j| the norm of the final column),
-// and one more step to enlarge the factorization after a near breakdown.
+// One Arnoldi step of a dynamics job's factorization (its reductions at mb,
+// scalars at sc): the Hessenberg column it finalizes, and at the end of the
+// factorization the first projection.
 static void dyn_arnoldi(BJob *J, const cx<double> *mb, const double *sc, bool &done, bool &ok) {
-    const int m = J->kdim, jj = J->j, ld = m + 2;
-    auto Hw = [&](int i, int c) -> cd & { return J->Hraw[(size_t)i + (size_t)c * ld]; };
-    const cx<double> *sv = mb + kMailHacc, *tv = sv + (jj + 1);
-    auto S_ = [&](int i) { return cd(sv[i].x, sv[i].y); };
-    auto T_ = [&](int i) { return cd(tv[i].x, tv[i].y); };
-    const double rho = sc[kSlotRho];
-    const cd e(sc[kSlotE], sc[kSlotE + 1]);
-    bool complete = false, exact = false;
-    if (jj >= 1) {
-        for (int i = 0; i < jj; ++i) Hw(i, jj - 1) = J->cprov[(size_t)i] + S_(i);
-        Hw(jj, jj - 1) = rho;
-        if (J->enlarging) {
-            complete = true;
-        } else {
-            double img2 = 0;
-            for (int i = 0; i <= jj; ++i) img2 += std::norm(Hw(i, jj - 1));
-            const double breakfactor = std::sqrt(2.220446049250313e-16);
-            if (rho == 0 || (int64_t)jj == J->n) {
-                exact = complete = true;
-            } else {
-                J->near |= rho <= breakfactor * std::max(std::sqrt(img2), 2.2250738585072014e-308);
-                if (jj == m) {
-                    if (J->near) J->enlarging = true;
-                    else complete = true;
-                }
-            }
-        }
-    } else if (!(rho > 0) || !std::isfinite(rho)) {
+    const int jj = J->j;
+    const cx<double> *sv = mb + kMailHacc;
+    const int r = J->arn.step(jj, sv, sv + (jj + 1), sc[kSlotRho], cd(sc[kSlotE], sc[kSlotE + 1]));
+    if (r < 0) {
         J->error = "zero or non-finite start vector of a Krylov factorization";
         done = true;
         ok = false;
         return;
     }
-    if (!complete) {
-        for (int i = 0; i < jj; ++i) {
-            cd acc = T_(i);
-            for (int c = 0; c < jj; ++c) acc -= Hw(i, c) * S_(c);
-            J->cprov[(size_t)i] = acc / rho;
-        }
-        J->cprov[(size_t)jj] = e - (jj >= 1 ? S_(jj - 1) : cd(0, 0));
+    if (r == 0) {
         J->j = jj + 1;
         return;
     }
-    // the factorization, laid out as arnoldi() of the single-vector path
-    Factorization &F = J->F;
-    const bool enlarged = J->enlarging;
-    const int k = enlarged ? jj - 1 : jj, fl = m + 1;
-    F.S.assign((size_t)fl * fl, cd(0, 0));
-    for (int c = 0; c < k; ++c)
-        for (int i = 0; i < k; ++i) F.S[(size_t)i + (size_t)c * fl] = Hw(i, c);
-    const int q = k + (exact ? 0 : 1);
-    if (!exact) F.S[(size_t)k + (size_t)(k - 1) * fl] = Hw(k, k - 1);
-    F.used = k;
-    if (enlarged) {
-        F.enlarged = true;
-        F.used += 1;
-        for (int i = 0; i < q; ++i) F.S[(size_t)i + (size_t)(q - 1) * fl] += Hw(i, k);
-        F.next_remainder = Hw(k + 1, k).real();
-    }
-    F.k = k;
-    F.q = q;
-    F.coeff_count = enlarged ? q : k;
-    F.completed = exact || (enlarged && ((int64_t)q == J->n || F.next_remainder == 0));
-    J->applications += F.used;
-    J->err = expv_coefficients(F, fl, J->X.step(), J->coeffs);
-    for (auto &c : J->coeffs) c = cx<double>{F.beta * c.x, F.beta * c.y};
+    const double beta = J->F.beta;
+    J->arn.finish(jj, beta, J->F);
+    J->applications += J->F.used;
+    J->err = expv_coefficients(J->F, J->kdim + 1, J->X.step(), J->coeffs);
+    for (auto &c : J->coeffs) c = cx<double>{beta * c.x, beta * c.y};
     J->phase = PX_PROJECT;
 }
 
@@ -2121,9 +2089,6 @@ pimi_status pimi_batch_submit_expv(pimi_batch *E, pimi_plan *plan, const double 
         J->obs_host.resize((size_t)J->n * (size_t)nobs);
         if (nobs > 0) std::memcpy(J->obs_host.data(), obs, sizeof(cx<double>) * J->obs_host.size());
         J->obs_out.assign((size_t)nts * (size_t)nobs, cx<double>{0, 0});
-        const int m = J->kdim;
-        J->Hraw.assign((size_t)(m + 2) * (m + 1), cd(0, 0));
-        J->cprov.assign((size_t)m + 3, cd(0, 0));
         J->t_submit = std::chrono::steady_clock::now();
         J->id = E->next_id++;
         *job_id = J->id;

@@ -127,6 +127,93 @@ struct Factorization {
 };
 
 
+// The Arnoldi Hessenberg of a factorization from delayed-reorthogonalization
+// steps (the batch engine's kb_dcgs_* kernels, also used by the
+// single-vector path): step j reports s = Q^H p_j and t = Q^H w (j + 1
+// entries each), rho_j and e_j; column j - 1 is final after step j.  The
+// breakdown tests follow krylov_expv!'s Arnoldi (src/krylov_extensions.jl:
+// 1346-1401): an exact breakdown at a zero subdiagonal or when the whole
+// space is spanned, a near breakdown when a subdiagonal is at most
+// sqrt(eps) |A v_j| (here the norm of the final column, which equals |A v_j|
+// for an orthonormal basis); after a near breakdown the factorization takes
+// one more step (the enlargement).
+struct DelayedArnoldi {
+    int m = 0, ld = 0;
+    int64_t n = 0;
+    std::vector<cd> Hraw, cprov;  // final columns ((m + 2) x (m + 1), stride m + 2); first-pass coefficients
+    bool near = false, enlarging = false, exact = false;
+    void reset(int m_, int64_t n_) {
+        m = m_;
+        n = n_;
+        ld = m + 2;
+        Hraw.assign((size_t)(m + 2) * (m + 1), cd(0, 0));
+        cprov.assign((size_t)m + 3, cd(0, 0));
+        near = enlarging = exact = false;
+    }
+    cd &H(int i, int c) { return Hraw[(size_t)i + (size_t)c * ld]; }
+    // step jj's reductions; 1: the factorization is complete, 0: continue
+    // with step jj + 1, -1: the start vector was zero or not finite
+    int step(int jj, const cx<double> *sv, const cx<double> *tv, double rho, cd e) {
+        auto S_ = [&](int i) { return cd(sv[i].x, sv[i].y); };
+        auto T_ = [&](int i) { return cd(tv[i].x, tv[i].y); };
+        bool complete = false;
+        if (jj >= 1) {
+            for (int i = 0; i < jj; ++i) H(i, jj - 1) = cprov[(size_t)i] + S_(i);
+            H(jj, jj - 1) = rho;
+            if (enlarging) {
+                complete = true;
+            } else {
+                double img2 = 0;
+                for (int i = 0; i <= jj; ++i) img2 += std::norm(H(i, jj - 1));
+                const double breakfactor = std::sqrt(2.220446049250313e-16);
+                if (rho == 0 || (int64_t)jj == n) {
+                    exact = complete = true;
+                } else {
+                    near |= rho <= breakfactor * std::max(std::sqrt(img2), 2.2250738585072014e-308);
+                    if (jj == m) {
+                        if (near) enlarging = true;
+                        else complete = true;
+                    }
+                }
+            }
+        } else if (!(rho > 0) || !std::isfinite(rho)) {
+            return -1;
+        }
+        if (complete) return 1;
+        for (int i = 0; i < jj; ++i) {
+            cd acc = T_(i);
+            for (int c = 0; c < jj; ++c) acc -= H(i, c) * S_(c);
+            cprov[(size_t)i] = acc / rho;
+        }
+        cprov[(size_t)jj] = e - (jj >= 1 ? S_(jj - 1) : cd(0, 0));
+        return 0;
+    }
+    // the factorization after the step jj that completed it, laid out as
+    // the single-vector path's (S of stride m + 1)
+    void finish(int jj, double beta, Factorization &F) {
+        const bool enlarged = enlarging;
+        const int k = enlarged ? jj - 1 : jj, fl = m + 1;
+        F = Factorization();
+        F.beta = beta;
+        F.S.assign((size_t)fl * fl, cd(0, 0));
+        for (int c = 0; c < k; ++c)
+            for (int i = 0; i < k; ++i) F.S[(size_t)i + (size_t)c * fl] = H(i, c);
+        const int q = k + (exact ? 0 : 1);
+        if (!exact) F.S[(size_t)k + (size_t)(k - 1) * fl] = H(k, k - 1);
+        F.used = k;
+        if (enlarged) {
+            F.enlarged = true;
+            F.used += 1;
+            for (int i = 0; i < q; ++i) F.S[(size_t)i + (size_t)(q - 1) * fl] += H(i, k);
+            F.next_remainder = H(k + 1, k).real();
+        }
+        F.k = k;
+        F.q = q;
+        F.coeff_count = enlarged ? q : k;
+        F.completed = exact || (enlarged && ((int64_t)q == n || F.next_remainder == 0));
+    }
+};
+
 // The projection of one attempt with step h (signed): coeffs[j] = beta
 // exp(h S)[j, 0] for j < coeff_count, and the local error estimate; `ld` is
 // the column stride of F.S (m + 1).

@@ -586,6 +586,17 @@ size_t dcgs_single_desc_bytes(int m);
 void dcgs_single_descs(void *host, int m, void *V, const void *w, void *partial, void *mail, void *h, double *slot,
                        int64_t n);
 void dcgs_single_step(hipStream_t st, const void *desc_dev, int j, int64_t n);
+// The same steps for a Krylov exponential's factorization, every step
+// reporting in a region of its own so that all of them run before one read
+// back: region j (stride complex doubles from mail) holds the step's scalars
+// as doubles (rho at kArnSlotRho, e at kArnSlotE, kArnSlotE + 1) in its first
+// kArnMailHead complex entries, then [Q p]^H p (j + 1 entries) and [Q p]^H w
+// (j + 1 entries).  Descriptors (dcgs_single_desc_bytes(steps) bytes) are
+// stepped with dcgs_single_step; partial holds 2 steps nblk, h 2 steps
+// complex doubles.
+constexpr int kArnMailHead = 9, kArnSlotRho = 10, kArnSlotE = 12;
+void dcgs_arnoldi_descs(void *host, int steps, void *V, const void *w, void *partial, cx<double> *mail,
+                        int64_t stride, void *h, int64_t n);
 
 // Set up the banded programs of stencil S (of a plan on device `device`)
 // when wanted: per-sector program ids and the programs' source; kernels are

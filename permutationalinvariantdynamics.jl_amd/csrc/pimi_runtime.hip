@@ -423,8 +423,14 @@ struct pimi_workspace {
     pimi::DevBuf<pimi::cx<double>> V, w, r, x, b, trial, cur, tmp, obs;
     pimi::DevBuf<pimi::cx<float>> V32, w32, x32, y32;
     pimi::DevBuf<pimi::cx<double>> partial, hvec, hacc, coef;
-    pimi::DevBuf<pimi::cx<double>> hcols;  // expv Arnoldi: Hessenberg columns of one factorization, (m+1) x m
-    pimi::DevBuf<double> hnorms;           // expv Arnoldi: per step |A v_j| and the remainder norm
+    // Krylov exponential Arnoldi (delayed reorthogonalization): reduction
+    // partials, per-step mail regions, update coefficients, per-step
+    // descriptors and the pinned read-back of the regions
+    pimi::DevBuf<pimi::cx<double>> ex_partial, ex_mail, ex_h;
+    pimi::DevBuf<char> ex_desc;
+    pimi::cx<double> *ex_host = nullptr;
+    int64_t ex_stride = 0;
+    bool ex_ready = false;
     pimi::DevBuf<double> npartial, nscal, d_vval;
     pimi::DevBuf<pimi::cx<double>> zbuf, pt1, pt2;  // preconditioner work vectors
     // flexible GMRES: outer search directions Z and the inner solver's vectors
@@ -698,7 +704,7 @@ static void probe_vector(uint64_t &st, std::vector<cx<double>> &x) {
 static void ensure_workspace(pimi_workspace *W) {
     int64_t n = W->plan->n;
     int m = W->m;
-    W->V.alloc((size_t)n * (m + 1));
+    W->V.alloc((size_t)n * (m + 3));  // the Krylov exponential's Arnoldi takes up to m + 2 steps
     W->w.alloc(n);
     W->r.alloc(n);
     W->x.alloc(n);
@@ -1647,78 +1653,79 @@ static void steady_solve(pimi_workspace *W, const pimi_gmres_opts *o, const doub
 // ---------------------------------------------------------------------------
 // adaptive Krylov exponential (src/krylov_extensions.jl:1346-1568)
 // ---------------------------------------------------------------------------
+static void ensure_expv_arnoldi(pimi_workspace *W) {
+    if (W->ex_ready) return;
+    const int m = W->m, steps = m + 2;
+    const int64_t n = W->plan->n;
+    W->ex_stride = kArnMailHead + 2 * ((int64_t)steps + 1);
+    W->ex_partial.alloc((size_t)2 * steps * dcgs_single_nblk(n));
+    W->ex_mail.alloc((size_t)(W->ex_stride * steps));
+    W->ex_h.alloc((size_t)2 * steps);
+    std::vector<char> host(dcgs_single_desc_bytes(steps));
+    dcgs_arnoldi_descs(host.data(), steps, W->V.p, W->w.p, W->ex_partial.p, W->ex_mail.p, W->ex_stride, W->ex_h.p,
+                       n);
+    W->ex_desc.alloc(host.size());
+    h2d(W, W->ex_desc.p, host.data(), host.size());
+    HIPCHK(hipHostMalloc((void **)&W->ex_host, sizeof(cx<double>) * (size_t)(W->ex_stride * steps),
+                         hipHostMallocDefault));
+    W->ex_ready = true;
+}
+
+// The Krylov factorization of krylov_expv!'s Arnoldi (src/krylov_extensions.jl:
+// 1346-1401) from W->cur: beta = |cur|, V[:, 0] = cur / beta, then steps
+// 0..m of the delayed-reorthogonalization Arnoldi (two passes over the basis
+// per step; V[:, j] holds q_j once step j has run) on the device with no
+// host round trip, each step's reductions in a mail region of its own; the
+// host reads them once and finds the breakdowns (DelayedArnoldi).  Steps
+// after an exact breakdown divide by rho = 0 as multiplication by zero and
+// write columns that are never read.  After a near breakdown one more step
+// (the enlargement) runs.
 static void arnoldi(pimi_workspace *W, Factorization &F) {
     const int64_t n = W->plan->n;
     const int m = W->m;
     cx<double> *V = W->V.p;
     F = Factorization();
-    F.S.assign((size_t)(m + 1) * (m + 1), cd(0, 0));
     dev_norm<double>(W, W->cur.p, n, W->nscal.p);
-    F.beta = read_scalar(W, W->nscal.p);
-    if (F.beta == 0) { F.zero_state = true; F.completed = true; return; }
+    const double beta = read_scalar(W, W->nscal.p);
+    if (beta == 0) {
+        F.S.assign((size_t)(m + 1) * (m + 1), cd(0, 0));
+        F.zero_state = true;
+        F.completed = true;
+        return;
+    }
+    ensure_expv_arnoldi(W);
     hipLaunchKernelGGL(k_scale_div<double>, dim3(grid_for(n)), dim3(kRedBlock), 0, W->stream, V, W->cur.p, n,
                        W->nscal.p);
-    std::vector<cd> H((size_t)(m + 1) * m, cd(0, 0));
-    std::vector<cx<double>> hcol((size_t)m + 2);
-    int k = 0;
-    bool exact_breakdown = false, near_breakdown = false;
-    const double breakfactor = std::sqrt(2.220446049250313e-16);
-    // All m steps run on the device without a host round trip: column j of
-    // the Hessenberg matrix and the step's norms land in device buffers, and
-    // the host reads them once and finds the first (exact or dimension)
-    // breakdown afterwards.  Steps after a breakdown divide by a zero norm
-    // and compute values that are never read (columns beyond k are unused).
-    const int steps = (int)std::min<int64_t>(m, n);
-    if (W->hcols.count < (size_t)(m + 1) * m) W->hcols.alloc((size_t)(m + 1) * m);
-    if (W->hnorms.count < (size_t)2 * m) W->hnorms.alloc((size_t)2 * m);
-    HIPCHK(hipMemsetAsync(W->hcols.p, 0, sizeof(cx<double>) * (size_t)(m + 1) * m, W->stream));
-    for (int j = 0; j < steps; ++j) {
-        launch_apply<double>(W, V + (int64_t)j * n, W->w.p, 1.0);
-        dev_norm<double>(W, W->w.p, n, W->hnorms.p + 2 * j);
-        dev_cgs2<double>(W, V, n, j + 1, W->w.p, n, W->hcols.p + (size_t)j * (m + 1));
-        dev_scale_normalize<double>(W, V + (int64_t)(j + 1) * n, W->w.p, n, W->hnorms.p + 2 * j + 1);
-    }
-    std::vector<cx<double>> hall((size_t)(m + 1) * steps);
-    std::vector<double> nall((size_t)2 * steps);
-    HIPCHK(hipMemcpyAsync(hall.data(), W->hcols.p, sizeof(cx<double>) * hall.size(), hipMemcpyDeviceToHost,
-                          W->stream));
-    HIPCHK(hipMemcpyAsync(nall.data(), W->hnorms.p, sizeof(double) * nall.size(), hipMemcpyDeviceToHost,
-                          W->stream));
-    sync(W);
-    for (int j = 0; j < steps; ++j) {
-        const double image_norm = nall[(size_t)2 * j], rem = nall[(size_t)2 * j + 1];
-        for (int i = 0; i <= j; ++i) {
-            const cx<double> h = hall[(size_t)i + (size_t)j * (m + 1)];
-            H[(size_t)i + (size_t)j * (m + 1)] = cd(h.x, h.y);
+    const int64_t stride = W->ex_stride;
+    auto run = [&](int j0, int j1) {  // steps j0 .. j1 - 1, then their regions to the host
+        for (int j = j0; j < j1; ++j) {
+            launch_apply<double>(W, V + (int64_t)j * n, W->w.p, 1.0);
+            dcgs_single_step(W->stream, W->ex_desc.p, j, n);
         }
-        H[(size_t)(j + 1) + (size_t)j * (m + 1)] = rem;
-        k = j + 1;
-        if (rem == 0 || (int64_t)(j + 1) == n) { exact_breakdown = true; break; }
-        near_breakdown |= rem <= breakfactor * std::max(image_norm, 2.2250738585072014e-308);
-    }
-    int q = k + (exact_breakdown ? 0 : 1);
-    for (int j = 0; j < k; ++j)
-        for (int i = 0; i < k; ++i) F.S[(size_t)i + (size_t)j * (m + 1)] = H[(size_t)i + (size_t)j * (m + 1)];
-    if (!exact_breakdown) F.S[(size_t)k + (size_t)(k - 1) * (m + 1)] = H[(size_t)k + (size_t)(k - 1) * (m + 1)];
-    F.used = k;
-    if (!exact_breakdown && near_breakdown) {
-        F.enlarged = true;
-        F.used += 1;
-        launch_apply<double>(W, V + (int64_t)(q - 1) * n, W->w.p, 1.0);
-        HIPCHK(hipMemsetAsync(W->hacc.p, 0, sizeof(cx<double>) * q, W->stream));
-        dev_cgs2<double>(W, V, n, q, W->w.p, n, W->hacc.p);
-        hipLaunchKernelGGL(k_reduce_norm, dim3(1), dim3(kRedBlock), 0, W->stream, W->npartial.p, kNormBlocks,
-                           W->nscal.p + 3);
-        HIPCHK(hipMemcpyAsync(hcol.data(), W->hacc.p, sizeof(cx<double>) * q, hipMemcpyDeviceToHost, W->stream));
-        HIPCHK(hipMemcpyAsync(W->hpin, W->nscal.p + 3, sizeof(double), hipMemcpyDeviceToHost, W->stream));
+        HIPCHK(hipMemcpyAsync(W->ex_host + (size_t)(j0 * stride), W->ex_mail.p + (size_t)(j0 * stride),
+                              sizeof(cx<double>) * (size_t)((j1 - j0) * stride), hipMemcpyDeviceToHost,
+                              W->stream));
         sync(W);
-        for (int i = 0; i < q; ++i) F.S[(size_t)i + (size_t)(q - 1) * (m + 1)] += cd(hcol[i].x, hcol[i].y);
-        F.next_remainder = W->hpin[0];
+    };
+    DelayedArnoldi A;
+    A.reset(m, n);
+    auto feed = [&](int j) {
+        const cx<double> *mb = W->ex_host + (size_t)(j * stride);
+        const double *sc = reinterpret_cast<const double *>(mb);
+        const cx<double> *sv = mb + kArnMailHead;
+        return A.step(j, sv, sv + (j + 1), sc[kArnSlotRho], cd(sc[kArnSlotE], sc[kArnSlotE + 1]));
+    };
+    run(0, m + 1);
+    int j = 0, r = 0;
+    for (; j <= m; ++j)
+        if ((r = feed(j)) != 0) break;
+    if (r == 0) {  // a near breakdown at step m: the enlargement
+        j = m + 1;
+        run(j, j + 1);
+        r = feed(j);
     }
-    F.k = k;
-    F.q = q;
-    F.coeff_count = F.enlarged ? q : k;
-    F.completed = exact_breakdown || (F.enlarged && ((int64_t)q == n || F.next_remainder == 0));
+    if (r < 0) throw Error(PIMI_EINVAL, "the Krylov exponential's state is not finite");
+    A.finish(j, beta, F);
 }
 
 // trial = beta V exp(h S) e1 ; returns error estimate
@@ -2419,7 +2426,7 @@ pimi_status pimi_workspace_create(pimi_plan *P, int32_t precision, int32_t krylo
         size_t free_b = 0, total_b = 0;
         HIPCHK(hipSetDevice(P->ctx->device));
         HIPCHK(hipMemGetInfo(&free_b, &total_b));
-        double need = (double)P->n * 16.0 * (std::min<int64_t>(krylovdim, P->n) + 12) *
+        double need = (double)P->n * 16.0 * (std::min<int64_t>(krylovdim, P->n) + 14) *
                       (precision == PIMI_C32 ? 1.5 : 1.0);
         if (need > 0.9 * (double)free_b) throw Error(PIMI_ENOMEM, "Krylov workspace exceeds free device memory");
         HIPCHK(hipSetDevice(P->ctx->device));
@@ -2451,6 +2458,7 @@ void pimi_workspace_destroy(pimi_workspace *W) {
     if (W->hpin) (void)hipHostFree(W->hpin);
     if (W->hlook) (void)hipHostFree(W->hlook);
     if (W->dc_look) (void)hipHostFree(W->dc_look);
+    if (W->ex_host) (void)hipHostFree(W->ex_host);
     for (hipEvent_t e : W->look_ev)
         if (e) (void)hipEventDestroy(e);
     if (W->ev0) (void)hipEventDestroy(W->ev0);
