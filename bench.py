@@ -260,8 +260,11 @@ def batched(pimi, args):
     width64 = eng.autotune(comps)
     ms_spmv64, bytes_spmv64 = eng.apply_bench(comps, reps=20)
     eng.close()
-    eng = pimi.BatchEngine(precision=pimi.PIMI_C32, timing=True)
+    # per-operation device time: a timed engine (eager launches with events)
+    # over the same jobs, after the measured run
+    eng = pimi.BatchEngine(precision=pimi.PIMI_C32)
     eng.autotune(comps)
+    size_widths = {int(c.basis.N): w for c, w in zip(comps, eng.widths)}
     alg = pimi.BatchAlgorithm(krylovdim=300, inner_rtol=1e-4)
     t0 = time.perf_counter()
     ids = [eng.submit(c, alg, rtol=RTOL) for c in comps]
@@ -271,6 +274,14 @@ def batched(pimi, args):
         while (j := eng.next_finished()) is not None:
             out[j] = eng.fetch(j)
     wall = time.perf_counter() - t0
+    graph_ticks, graph_inst, graph_upd = eng.graph_counts()
+    eng.close()
+    eng = pimi.BatchEngine(precision=pimi.PIMI_C32, timing=True)
+    eng.autotune(comps)
+    tids = [eng.submit(c, alg, rtol=RTOL) for c in comps]
+    eng.run()
+    for j in tids:
+        eng.fetch(j)
     st = eng.stats()
     eng.close()
     res = [out[j] for j in ids]
@@ -297,6 +308,8 @@ def batched(pimi, args):
         "plain_fp64_single_solves_per_s": plain_rate,
         "plain_sample": f"first {BATCH_PLAIN_SAMPLE} of the same models, one at a time",
         "speedup_vs_plain_fp64": rate / plain_rate,
+        "tile_width_per_N": size_widths,
+        "graph_ticks": {"ticks": graph_ticks, "instantiated": graph_inst, "updated_in_place": graph_upd},
         "ops": ops,
     }, {"achieved": bytes_spmv / (ms_spmv * 1e-3) / 1e9, "avg_ms": ms_spmv, "algorithmic_bytes": bytes_spmv,
         "models": len(comps), "tile_width": width,

@@ -442,6 +442,11 @@ pimi_status pimi_batch_run(pimi_batch *batch, int32_t max_ticks, int32_t stop_wh
 /* counts[6]: queued, active, finished (not yet fetched), ticks so far,
  * reserved bytes in use, reservation size */
 pimi_status pimi_batch_counts(pimi_batch *batch, int64_t *counts);
+/* counts[3]: ticks launched as HIP graphs, graphs instantiated, graphs
+ * updated in place (a tick of a shape seen before).  The engine captures
+ * each tick's launches as a HIP graph unless PIMI_NO_GRAPH=1 or
+ * PIMI_BATCH_TIMING.  No counterpart in the reference. */
+pimi_status pimi_batch_graph_counts(pimi_batch *batch, int64_t *counts);
 /* Accumulated statistics per operation class c (PIMI_OP_*): out[c] = device
  * milliseconds (only with PIMI_BATCH_TIMING), out[8+c] = algorithmic bytes
  * (each array counted once per launch), out[16+c] = flops; 24 entries.
@@ -485,13 +490,18 @@ pimi_status pimi_batch_apply(pimi_batch *batch, pimi_plan *const *plans, int32_t
  * = sum over plans of 2 n s + stencil tables.  Measurement only. */
 pimi_status pimi_batch_apply_bench(pimi_batch *batch, pimi_plan *const *plans, int32_t nplans, int32_t reps,
                                    double *ms_avg, int64_t *algorithmic_bytes);
-/* Autotuner of the engine's banded-apply tile width: times one launch over a
- * synthetic vector of every plan (as pimi_batch_apply_bench) at each width
- * (4, 8, 16 columns), `reps` times each, and keeps the fastest for the
- * engine's later launches; *width_out = the chosen width.  No counterpart in
- * the reference. */
+/* Autotuner of the engine's banded-apply tile width, per plan size: times one
+ * launch over a synthetic vector of every plan (as pimi_batch_apply_bench) at
+ * each width (4, 8, 16 columns), `reps` times each, and keeps the fastest for
+ * plans of sizes it has not tuned (*width_out = that width); then the same
+ * over the plans of each distinct size (PI dimension and sector count, i.e.
+ * each N of one local dimension) on their own, and keeps that size's
+ * fastest for the engine's later launches, which group the jobs of one tick
+ * by width (one launch per width).  widths_out (nplans entries, or null) =
+ * the width now used for each plan (0: not banded).  Results do not depend
+ * on the widths.  No counterpart in the reference. */
 pimi_status pimi_batch_autotune_apply(pimi_batch *batch, pimi_plan *const *plans, int32_t nplans, int32_t reps,
-                                      int32_t *width_out);
+                                      int32_t *width_out, int32_t *widths_out);
 
 #ifdef __cplusplus
 }

@@ -813,8 +813,20 @@ struct BLane {
     std::vector<size_t> moff;       // mailbox offset of each active job
     std::vector<int> op_class;      // per launch (timing)
     double bytes[8] = {}, flops[8] = {};
+    // captured ticks: one executable graph per tick shape (the launch count
+    // and each launch's operation class); a tick of a known shape updates
+    // that graph's launch parameters in place (hipGraphExecUpdate) instead of
+    // instantiating a new one
+    std::map<std::string, hipGraphExec_t> graphs;
+    int64_t graph_ticks = 0, graph_instantiations = 0, graph_updates = 0;
+    void drop_graphs() {
+        for (auto &kv : graphs)
+            if (kv.second) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
     ~BLane() {
         if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
         for (auto ev : events) (void)hipEventDestroy(ev);
         if (hstage) (void)hipHostFree(hstage);
         if (hmail) (void)hipHostFree(hmail);
@@ -892,8 +904,16 @@ struct pimi_batch {
     std::atomic<int> busy{0};
     int64_t ticks = 0;
     int dia_w = 1;  // banded-apply tile width index (kDiaWidths) of the engine's launches
+    // the autotuner's width per plan size (PI dimension, sector count); plans
+    // of other sizes take dia_w
+    std::map<std::pair<int64_t, int>, int> dia_w_size;
+    int width_for(const pimi_plan *P) const {
+        const auto it = dia_w_size.find({P->n, P->ns});
+        return it == dia_w_size.end() ? dia_w : it->second;
+    }
     bool trace = false;
     bool timing = false;  // PIMI_BATCH_TIMING: HIP events between launches, per operation class
+    bool use_graphs = true;  // ticks captured as HIP graphs (PIMI_NO_GRAPH=1 or timing: eager launches)
     double stat_ms[8] = {}, stat_bytes[8] = {}, stat_flops[8] = {};
     size_t nactive() const {
         size_t k = 0;
@@ -976,15 +996,18 @@ struct Tick {
                bool unit) {
         if (js.empty()) return;
         std::vector<BApply> v;
-        std::vector<BDia> vd;
-        int64_t tot = 0, totd = 0;
-        const int w = E->dia_w;
+        // banded jobs grouped by their plan size's tile width: one launch per width
+        std::vector<BDia> vd[kNumDiaWidths];
+        std::vector<const pimi_plan *> vp[kNumDiaWidths];
+        int64_t tot = 0, totd[kNumDiaWidths] = {};
         for (BJob *J : js) {
             pimi_plan *P = J->plan;
             if (P->main.dia_ok) {
-                vd.push_back(dia_desc<R>(totd, P, in(J), out(J), !unit && J->weighted, unit ? 1.0 : 1.0 / J->scale,
-                                         w));
-                totd += P->ndtiles[w];
+                const int w = E->width_for(P);
+                vd[w].push_back(dia_desc<R>(totd[w], P, in(J), out(J), !unit && J->weighted,
+                                            unit ? 1.0 : 1.0 / J->scale, w));
+                vp[w].push_back(P);
+                totd[w] += P->ndtiles[w];
                 bytes[cls] += 2.0 * J->n * sizeof(cx<R>) + (sizeof(R) == 8 ? J->table_bytes64 : J->table_bytes32);
                 continue;
             }
@@ -997,13 +1020,14 @@ struct Tick {
             bytes[cls] += 2.0 * J->n * sizeof(cx<R>) + (sizeof(R) == 8 ? J->table_bytes64 : J->table_bytes32);
         }
         hipStream_t st = stream;
-        if (!vd.empty()) {
-            const size_t offd = put(vd);
-            const int njd = (int)vd.size();
-            std::vector<const pimi_plan *> ps;
-            for (BJob *J : js) ps.push_back(J->plan);
-            const DevStencil *M = common_programs(ps.begin(), ps.end());
-            push([=](const char *d) { launch_kb_dia<R>(st, (const BDia *)(d + offd), njd, totd, w, M); });
+        for (int w = 0; w < kNumDiaWidths; ++w) {
+            if (vd[w].empty()) continue;
+            const size_t offd = put(vd[w]);
+            const int njd = (int)vd[w].size();
+            const int64_t td = totd[w];
+            const DevStencil *M = common_programs(vp[w].begin(), vp[w].end());
+            if (M) (void)dia_program(*M, 1, sizeof(R) == 8 ? 0 : 1, w, st);  // compiled now, not under capture
+            push([=](const char *d) { launch_kb_dia<R>(st, (const BDia *)(d + offd), njd, td, w, M); });
         }
         if (v.empty()) return;
         const size_t off = put(v);
@@ -1302,6 +1326,57 @@ static void finish(pimi_batch *E, BJob *J, bool ok) {
 }
 
 // Stage and launch lane L's tick (no wait); complete_tick reads it back.
+// The tick's launches as one HIP graph on the lane's stream.  The launches
+// are captured (kernels only: the stage upload before and the mailbox and
+// result copies after stay ordinary stream operations); the lane keeps one
+// executable graph per tick shape and updates its parameters in place when a
+// tick of that shape recurs -- the Arnoldi ticks of a stable job set -- and
+// instantiates anew when the update is refused (a different kernel at some
+// position).  Every kernel a tick uses is resolved before the capture (the
+// compiled banded programs are compiled outside it).
+static void launch_graph(BLane &L, Tick &T, const char *dbase) {
+    std::string key = std::to_string(T.ops.size()) + ":";
+    for (int c : T.op_class) key += (char)('a' + c);
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        for (auto &op : T.ops) op(dbase);
+    } catch (...) {
+        (void)hipStreamEndCapture(L.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(L.stream, &g));
+    if (L.graphs.size() >= 64 && !L.graphs.count(key)) L.drop_graphs();
+    hipGraphExec_t &ex = L.graphs[key];
+    bool updated = false;
+    if (ex) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult r = hipGraphExecUpdateError;
+        updated = hipGraphExecUpdate(ex, g, &err_node, &r) == hipSuccess && r == hipGraphExecUpdateSuccess;
+        if (!updated) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(ex);
+            ex = nullptr;
+        }
+    }
+    if (!updated) {
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            ex = nullptr;
+            L.graphs.erase(key);
+            (void)hipGraphDestroy(g);
+            HIPCHK(e);
+        }
+        ++L.graph_instantiations;
+    } else {
+        ++L.graph_updates;
+    }
+    (void)hipGraphDestroy(g);
+    HIPCHK(hipGraphLaunch(ex, L.stream));
+    ++L.graph_ticks;
+}
+
 template <typename R>
 static void launch_tick(pimi_batch *E, BLane &L) {
     admit(E, L);
@@ -1551,10 +1626,14 @@ static void launch_tick(pimi_batch *E, BLane &L) {
     }
     std::memcpy(L.hstage, T.buf.data(), T.buf.size());
     HIPCHK(hipMemcpyAsync(L.dstage.p, L.hstage, T.buf.size(), hipMemcpyHostToDevice, L.stream));
-    if (timing) HIPCHK(hipEventRecord(L.events[0], L.stream));
-    for (size_t k = 0; k < T.ops.size(); ++k) {
-        T.ops[k](dbase);
-        if (timing) HIPCHK(hipEventRecord(L.events[k + 1], L.stream));
+    if (E->use_graphs) {
+        launch_graph(L, T, dbase);
+    } else {
+        if (timing) HIPCHK(hipEventRecord(L.events[0], L.stream));
+        for (size_t k = 0; k < T.ops.size(); ++k) {
+            T.ops[k](dbase);
+            if (timing) HIPCHK(hipEventRecord(L.events[k + 1], L.stream));
+        }
     }
     HIPCHK(hipGetLastError());
     if (L.hmail_cap < mtot) {
@@ -1963,6 +2042,8 @@ pimi_status pimi_batch_create(pimi_context *ctx, const pimi_batch_opts *opts, pi
             for (int w = 0; w < kNumDiaWidths; ++w)
                 if (kDiaWidths[w] == std::atoi(tb)) E->dia_w = w;
         E->timing = (opts->flags & PIMI_BATCH_TIMING) != 0;
+        const char *ng = std::getenv("PIMI_NO_GRAPH");
+        E->use_graphs = !(ng && ng[0] == '1') && !E->timing;
         HIPCHK(hipStreamCreate(&E->stream));
         for (auto &L : E->lanes) HIPCHK(hipStreamCreate(&L.stream));
         E->arena.init(align_up(bytes, 256));
@@ -2138,6 +2219,18 @@ pimi_status pimi_batch_counts(pimi_batch *E, int64_t *counts) {
     });
 }
 
+pimi_status pimi_batch_graph_counts(pimi_batch *E, int64_t *counts) {
+    return guarded([&] {
+        if (!E || !counts) throw Error(PIMI_EINVAL, "null argument");
+        counts[0] = counts[1] = counts[2] = 0;
+        for (const auto &L : E->lanes) {
+            counts[0] += L.graph_ticks;
+            counts[1] += L.graph_instantiations;
+            counts[2] += L.graph_updates;
+        }
+    });
+}
+
 pimi_status pimi_batch_stats(pimi_batch *E, double *out, int32_t reset) {
     return guarded([&] {
         if (!E || !out) throw Error(PIMI_EINVAL, "null argument");
@@ -2228,7 +2321,7 @@ pimi_status pimi_batch_fetch(pimi_batch *E, int64_t job_id, double *rho_out, pim
 }
 
 pimi_status pimi_batch_autotune_apply(pimi_batch *E, pimi_plan *const *plans, int32_t nplans, int32_t reps,
-                                      int32_t *width_out) {
+                                      int32_t *width_out, int32_t *widths_out) {
     return guarded([&] {
         if (!E || !plans || nplans < 1 || reps < 1 || !width_out) throw Error(PIMI_EINVAL, "invalid argument");
         BusyGuard guard(E->busy);
@@ -2246,47 +2339,70 @@ pimi_status pimi_batch_autotune_apply(pimi_batch *E, pimi_plan *const *plans, in
             HIPCHK(hipMemsetAsync(xs[banded.size()].p, 0x3f, es * (size_t)P->n, E->stream));
             banded.push_back(P);
         }
-        *width_out = kDiaWidths[E->dia_w];
-        if (banded.empty()) return;  // nothing banded: the width is not used
+        auto report = [&] {
+            *width_out = kDiaWidths[E->dia_w];
+            if (widths_out)
+                for (int32_t k = 0; k < nplans; ++k)
+                    widths_out[k] = plans[k]->main.dia_ok ? kDiaWidths[E->width_for(plans[k])] : 0;
+        };
+        if (banded.empty()) {  // nothing banded: the width is not used
+            report();
+            return;
+        }
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        int best = E->dia_w;
-        float best_ms = INFINITY;
         DevBuf<BDia> dvd;
         dvd.alloc(banded.size());
-        for (int w = 0; w < kNumDiaWidths; ++w) {
-            std::vector<BDia> vd;
-            int64_t totd = 0;
-            for (size_t k = 0; k < banded.size(); ++k) {
-                vd.push_back(E->prec == PIMI_C32 ? dia_desc<float>(totd, banded[k], xs[k].p, ys[k].p, false, 1.0, w)
-                                                 : dia_desc<double>(totd, banded[k], xs[k].p, ys[k].p, false, 1.0, w));
-                totd += banded[k]->ndtiles[w];
+        // the fastest width for one launch over the plans idx (ties: the first)
+        auto fastest = [&](const std::vector<size_t> &idx, int dflt) {
+            int best = dflt;
+            float best_ms = INFINITY;
+            for (int w = 0; w < kNumDiaWidths; ++w) {
+                std::vector<BDia> vd;
+                std::vector<const pimi_plan *> ps;
+                int64_t totd = 0;
+                for (size_t k : idx) {
+                    vd.push_back(E->prec == PIMI_C32
+                                     ? dia_desc<float>(totd, banded[k], xs[k].p, ys[k].p, false, 1.0, w)
+                                     : dia_desc<double>(totd, banded[k], xs[k].p, ys[k].p, false, 1.0, w));
+                    ps.push_back(banded[k]);
+                    totd += banded[k]->ndtiles[w];
+                }
+                HIPCHK(hipMemcpyAsync(dvd.p, vd.data(), sizeof(BDia) * vd.size(), hipMemcpyHostToDevice,
+                                      E->stream));
+                const int nvd = (int)vd.size();
+                const DevStencil *M = common_programs(ps.begin(), ps.end());
+                auto launch = [&] {
+                    if (E->prec == PIMI_C32) launch_kb_dia<float>(E->stream, dvd.p, nvd, totd, w, M);
+                    else launch_kb_dia<double>(E->stream, dvd.p, nvd, totd, w, M);
+                };
+                launch();  // warm
+                HIPCHK(hipEventRecord(e0, E->stream));
+                for (int32_t r = 0; r < reps; ++r) launch();
+                HIPCHK(hipEventRecord(e1, E->stream));
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = w;
+                }
             }
-            HIPCHK(hipMemcpyAsync(dvd.p, vd.data(), sizeof(BDia) * vd.size(), hipMemcpyHostToDevice, E->stream));
-            const int nvd = (int)vd.size();
-            const DevStencil *M = common_programs(banded.begin(), banded.end());
-            auto launch = [&] {
-                if (E->prec == PIMI_C32) launch_kb_dia<float>(E->stream, dvd.p, nvd, totd, w, M);
-                else launch_kb_dia<double>(E->stream, dvd.p, nvd, totd, w, M);
-            };
-            launch();  // warm
-            HIPCHK(hipEventRecord(e0, E->stream));
-            for (int32_t r = 0; r < reps; ++r) launch();
-            HIPCHK(hipEventRecord(e1, E->stream));
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = w;
-            }
-        }
+            return best;
+        };
+        // one launch over every plan: the width of sizes not tuned here
+        std::vector<size_t> all(banded.size());
+        for (size_t k = 0; k < all.size(); ++k) all[k] = k;
+        E->dia_w = fastest(all, E->dia_w);
+        // then each plan size on its own
+        std::map<std::pair<int64_t, int>, std::vector<size_t>> sizes;
+        for (size_t k = 0; k < banded.size(); ++k) sizes[{banded[k]->n, banded[k]->ns}].push_back(k);
+        for (const auto &kv : sizes) E->dia_w_size[kv.first] = fastest(kv.second, E->dia_w);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        E->dia_w = best;
-        *width_out = kDiaWidths[best];
+        report();
     });
 }
 

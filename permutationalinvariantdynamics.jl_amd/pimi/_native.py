@@ -28,7 +28,7 @@ EXPORTS = [
     "pimi_workspace_last_apply_ms", "pimi_apply_bench", "pimi_copy_bench",
     "pimi_sector_preconditioner_factors", "pimi_precond_apply",
     "pimi_batch_create", "pimi_batch_destroy", "pimi_batch_job_bytes", "pimi_batch_submit", "pimi_batch_run",
-    "pimi_batch_counts", "pimi_batch_next_finished", "pimi_batch_fetch", "pimi_batch_submit_expv",
+    "pimi_batch_counts", "pimi_batch_graph_counts", "pimi_batch_next_finished", "pimi_batch_fetch", "pimi_batch_submit_expv",
     "pimi_batch_fetch_expv", "pimi_batch_apply", "pimi_batch_apply_bench",
     "pimi_batch_stats", "pimi_plan_prepare", "pimi_precond_bench",
     "pimi_trajectory_channels", "pimi_conditional_action", "pimi_conditional_action_noalloc",

@@ -168,13 +168,25 @@ class BatchEngine:
             N.check(st)
         return res
 
+    def graph_counts(self):
+        """(ticks launched as HIP graphs, graphs instantiated, graphs updated
+        in place)."""
+        out = np.zeros(3, dtype=np.int64)
+        N.check(N.lib().pimi_batch_graph_counts(self._h, N.ptr(out)))
+        return tuple(int(v) for v in out)
+
     def autotune(self, compiled_list, reps=5):
-        """Pick the engine's banded-apply tile width by timing one launch over
-        a vector of every model at each width; returns the chosen width."""
+        """Pick the engine's banded-apply tile widths by timing launches at
+        each width: one over a vector of every model (the width of sizes not
+        tuned here, returned), then one per distinct model size (N), whose
+        fastest width the engine's later launches use for that size.  The
+        width of each model is left in `self.widths` (0: not banded)."""
         arr = (C.c_void_p * len(compiled_list))(*[c._plan.value for c in compiled_list])
         w = C.c_int32()
+        per = np.zeros(len(compiled_list), dtype=np.int32)
         N.check(N.lib().pimi_batch_autotune_apply(self._h, arr, C.c_int32(len(compiled_list)), C.c_int32(reps),
-                                                  C.byref(w)))
+                                                  C.byref(w), N.ptr(per)))
+        self.widths = [int(v) for v in per]
         return w.value
 
     def apply(self, compiled_list, xs):

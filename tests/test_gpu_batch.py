@@ -201,10 +201,61 @@ def test_batch_autotune_keeps_results():
     ref = pimi.batched_stationary_states(comps, alg, rtol=1e-9)
     eng = pimi.BatchEngine()
     assert eng.autotune(comps, reps=2) in (4, 8, 16)
+    assert len(eng.widths) == len(comps) and all(w in (4, 8, 16) for w in eng.widths)
     out = pimi.batched_stationary_states(comps, alg, rtol=1e-9, engine=eng)
     eng.close()
     for a, b in zip(ref, out):
         np.testing.assert_array_equal(a.state.data, b.state.data)
+
+
+def _engine_run(comps, dyn_comps, precision, graphs):
+    """Stationary jobs of `comps` and dynamics jobs of `dyn_comps` (FP64
+    engines only) in one engine, ticks captured as HIP graphs or launched
+    eagerly (PIMI_NO_GRAPH=1, read when the engine is created)."""
+    if graphs:
+        os.environ.pop("PIMI_NO_GRAPH", None)
+    else:
+        os.environ["PIMI_NO_GRAPH"] = "1"
+    try:
+        eng = pimi.BatchEngine(precision=precision, memory_bytes=1 << 30)
+    finally:
+        os.environ.pop("PIMI_NO_GRAPH", None)
+    try:
+        ids = [eng.submit(c, pimi.BatchAlgorithm(krylovdim=60, inner_rtol=1e-4), rtol=1e-9) for c in comps]
+        alg = pimi.ExpvAlgorithm(krylovdim=20, rtol=1e-10)
+        dids = [eng.submit_dynamics(c, _excited(c), (0.0, 2.0), saveat=0.5, algorithm=alg,
+                                    observables=_dyn_observables(c)) for c in dyn_comps]
+        eng.run()
+        out = [eng.fetch(j) for j in ids]
+        dout = [eng.fetch_dynamics(j) for j in dids]
+        return out, dout, eng.graph_counts()
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("precision", [N.PIMI_C64, N.PIMI_C32])
+def test_graph_ticks_equal_eager_ticks(precision):
+    """The engine captures each tick's launches as a HIP graph and, when a
+    tick of the same shape recurs (the Arnoldi ticks of a stable job set),
+    updates that graph's launch parameters in place.  Results are bitwise
+    those of the same ticks launched eagerly, for stationary jobs in both
+    precisions and dynamics jobs beside them in FP64, and most graph ticks
+    are in-place updates."""
+    print(f"graph vs eager ticks, precision {precision}", flush=True)
+    comps = [pimi.compile(_c4_model(n_, 0.7, 0.01)) for n_ in (16, 24, 32)]
+    dyn = [pimi.compile(_c2_model(n_)) for n_ in (8, 14)] if precision == N.PIMI_C64 else []
+    ref, dref, counts0 = _engine_run(comps, dyn, precision, graphs=False)
+    out, dout, counts = _engine_run(comps, dyn, precision, graphs=True)
+    assert counts0 == (0, 0, 0)
+    ticks, inst, upd = counts
+    assert ticks > 0 and inst + upd == ticks and upd >= ticks // 2, counts
+    for a, b in zip(ref, out):
+        assert a.info["converged"] and b.info["converged"]
+        np.testing.assert_array_equal(a.state.data, b.state.data)
+    for a, b in zip(dref, dout):
+        np.testing.assert_array_equal(a.final_state.data, b.final_state.data)
+        for k in a.values:
+            np.testing.assert_array_equal(a.values[k], b.values[k])
 
 
 def _excited(comp):
