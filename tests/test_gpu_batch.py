@@ -156,6 +156,44 @@ def test_batch_local_models_against_closed_form(precision):
         assert abs(val - ref) <= 1e-6 * ref, (p, val, ref)
 
 
+def test_batch_512_seeded_instances_one_engine():
+    """The batched form at its configured size: the 512 seeded driven-qubit
+    models of the batched workload (N in {20..120}, Omega ~ U[0,2], gamma_phi
+    log-uniform, PCG64 seed) in one FP32 engine with the tile width
+    autotuned per N and graph-captured ticks, each to relative residual
+    1e-8.  Every model converges within the bar, and <sum_i n_i> of every
+    model is checked against its product-state closed form per qubit (the
+    excited population): a relative residual of 1e-8 leaves weakly driven
+    models, whose <sum_i n_i> is tiny, with a large relative but a small
+    absolute error."""
+    from pimi.workloads import batched_instances
+    print("batch: 512 seeded instances in one engine", flush=True)
+    inst = batched_instances(512)
+    comps = [pimi.compile(i.model()) for i in inst]
+    eng = pimi.BatchEngine(precision=N.PIMI_C32)
+    try:
+        eng.autotune(comps, reps=2)
+        alg = pimi.BatchAlgorithm(krylovdim=300, inner_rtol=1e-4)
+        res = pimi.batched_stationary_states(comps, alg, rtol=1e-8, engine=eng)
+        ticks, inst_g, upd = eng.graph_counts()
+    finally:
+        eng.close()
+    assert ticks > 0 and upd > 0
+    worst, worst_rel, worst_res = 0.0, 0.0, 0.0
+    for i, comp, r in zip(inst, comps, res):
+        assert r.info["converged"]
+        worst_res = max(worst_res, r.info["relative_residual"])
+        val = pimi.expect(comp.collective_operator(SP2.jp @ SP2.jm), r.state).real
+        ref = _c4_closed_n(i.N, i.omega, i.gamma_phi)
+        worst = max(worst, abs(val - ref) / i.N)
+        worst_rel = max(worst_rel, abs(val - ref) / ref)
+    print(f"  largest relative error of <sum n_i>: {worst_rel:.2e}", flush=True)
+    parity_log.record("batch512_relative_residual", worst_res, 1e-8)
+    parity_log.record("batch512_excited_population_vs_closed_form", worst, 1e-4)
+    assert worst_res <= 1e-8
+    assert worst <= 1e-4
+
+
 def test_batch_not_converged_reports():
     comp = pimi.compile(_c2_model(20))
     eng = pimi.BatchEngine()
