@@ -29,9 +29,10 @@ Also reported:
                 oracle's CPU expv on a bounded sample of it;
   precision_form  spin-1 N=18: FP32 Arnoldi in FP64 refinement against the plain
                 FP64 device solve, observables against the FP64 reference values;
-  serving       the Poisson stream at 50, 200, 1000 and 2000 requests/s (24
-                requests each, 60 % stationary states, 40 % dynamics to
-                t=10) through the serving loop;
+  serving       the Poisson stream at 10, 50, 200, 1000 and 2000 requests/s
+                (24 requests each, 60 % stationary states, 40 % dynamics to
+                t=10) through the serving loop, and its sustained rate over
+                96 requests offered at once;
   reference_algorithm_c1  the reference's own GMRES(30) and expv on C1 (PI dim
                 165), the oracle on the host beside the device;
   cpu_baseline  the oracle's OpenMP CPU restatement solving the headline in
@@ -453,8 +454,9 @@ def dynamics_n120(pimi, comp, cpu=True):
     return out
 
 
-SERVE_RATES = (50.0, 200.0, 1000.0, 2000.0)
+SERVE_RATES = (10.0, 50.0, 200.0, 1000.0, 2000.0)  # the configured sweep (50-2000/s) and one rate below capacity
 SERVE_COUNT = 24
+SERVE_CAPACITY_COUNT = 96  # requests offered at once: the sustained service rate
 
 
 def batched_dynamics(pimi, count=64, single_sample=8):
@@ -523,6 +525,19 @@ def serving(pimi, rates=SERVE_RATES, count=SERVE_COUNT):
             out["rates"][f"{rate:g}"] = s
             print(f"# serving {rate:g}/s: {s['all']['per_s']:.2f} req/s, p50 {s['all']['p50_ms']:.0f} ms, "
                   f"p99 {s['all']['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
+        # capacity: a longer stream offered faster than it can be served, so
+        # the served rate is the sustained one (its latencies are queueing)
+        reqs = served_requests(100000.0, SERVE_CAPACITY_COUNT, seed=11)
+        t0 = time.perf_counter()
+        res = srv.serve(reqs)
+        wall = time.perf_counter() - t0
+        s = summarize(res, wall)
+        out["capacity"] = {"requests": SERVE_CAPACITY_COUNT, "offered": "all at once (100000/s)",
+                           "served_per_s": s["all"]["per_s"], "steady_per_s": s["steady"]["per_s"],
+                           "dynamics_per_s": s["dynamics"]["per_s"], "wall_s": wall,
+                           "converged": s["all"]["converged"]}
+        print(f"# serving capacity: {s['all']['per_s']:.2f} req/s over {SERVE_CAPACITY_COUNT} requests",
+              file=sys.stderr, flush=True)
     finally:
         srv.close()
     return out
