@@ -498,7 +498,11 @@ __global__ __launch_bounds__(64) void kb_dcgs_coef(const BVec *__restrict__ S) {
     }
 }
 
-// V[:, j] <- q_j and V[:, j+1] <- u_{j+1} (a = V, b = coef, c = w, d = slots, ncol = j + 1)
+// V[:, j] <- q_j and V[:, j+1] <- u_{j+1} (a = V, b = coef, c = w, d = slots, ncol = j + 1);
+// kUpdCols basis columns' loads in flight per thread (measured on the 512-model
+// FP32 batch: 8 -> 16 columns took the update pass from 3.9 to 4.3 TB/s; 24 and
+// 32 were slower)
+constexpr int kUpdCols = 16;
 template <typename R>
 __global__ __launch_bounds__(kRedBlock) void kb_dcgs_update(const BVec *__restrict__ S, int ns, int64_t total) {
     __shared__ cx<double> ca[kColChunk], cb[kColChunk];
@@ -513,7 +517,7 @@ __global__ __launch_bounds__(kRedBlock) void kb_dcgs_update(const BVec *__restri
         const double inv = slot[kSlotInvRho], er = slot[kSlotE], ei = slot[kSlotE + 1];
         cx<R> *pcol = V + (int64_t)j * J.n, *ucol = V + (int64_t)(j + 1) * J.n;
         // one coordinate per thread (ceil(n / kRedBlock) workgroups per job);
-        // eight basis columns' loads in flight, accumulated in column order
+        // kUpdCols basis columns' loads in flight, accumulated in column order
         const int64_t k = lb * kRedBlock + threadIdx.x;
         double ar = 0, ai = 0, br = 0, bi = 0;
         auto acc = [&](const cx<R> &v, int c) {
@@ -533,12 +537,12 @@ __global__ __launch_bounds__(kRedBlock) void kb_dcgs_update(const BVec *__restri
             if (k < J.n) {
                 const cx<R> *vk = V + k + (int64_t)c0 * J.n;
                 int c = 0;
-                for (; c + 8 <= cn; c += 8) {
-                    cx<R> v[8];
+                for (; c + kUpdCols <= cn; c += kUpdCols) {
+                    cx<R> v[kUpdCols];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) v[u] = vk[(int64_t)(c + u) * J.n];
+                    for (int u = 0; u < kUpdCols; ++u) v[u] = vk[(int64_t)(c + u) * J.n];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) acc(v[u], c + u);
+                    for (int u = 0; u < kUpdCols; ++u) acc(v[u], c + u);
                 }
                 for (; c < cn; ++c) acc(vk[(int64_t)c * J.n], c);
             }
