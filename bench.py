@@ -29,7 +29,7 @@ Also reported:
                 oracle's CPU expv on a bounded sample of it;
   precision_form  spin-1 N=18: FP32 Arnoldi in FP64 refinement against the plain
                 FP64 device solve, observables against the FP64 reference values;
-  serving       the Poisson stream at 10, 50, 200, 1000 and 2000 requests/s
+  serving       the Poisson stream at 10, 50, 100, 200, 500, 1000 and 2000 requests/s
                 (24 requests each, 60 % stationary states, 40 % dynamics to
                 t=10) through the serving loop, and its sustained rate over
                 96 requests offered at once;
@@ -454,7 +454,7 @@ def dynamics_n120(pimi, comp, cpu=True):
     return out
 
 
-SERVE_RATES = (10.0, 50.0, 200.0, 1000.0, 2000.0)  # the configured sweep (50-2000/s) and one rate below capacity
+SERVE_RATES = (10.0, 50.0, 100.0, 200.0, 500.0, 1000.0, 2000.0)  # the configured sweep (50-2000/s) and one rate below capacity
 SERVE_COUNT = 24
 SERVE_CAPACITY_COUNT = 96  # requests offered at once: the sustained service rate
 

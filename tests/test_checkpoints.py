@@ -122,3 +122,41 @@ def test_special_values_roundtrip_byte_for_byte(tmp_path, dtype):
                                   im.view(np.uint64 if R == np.float64 else np.uint32))
     pimi.save_checkpoint(p2, got)
     assert open(p1, "rb").read() == open(p2, "rb").read()
+
+
+def _golden(name):
+    import os
+    return os.path.join(os.path.dirname(__file__), "golden", name)
+
+
+def test_golden_pidchkpt_fixtures_reload_and_pin():
+    """The golden states exchanged in the reference's portable layout
+    (tests/golden/make_pidchkpt_fixtures.py): both load with every field
+    validated, carry the C1 basis (N=8, d=2, all five sectors, n = 165) and
+    their time; the oracle, rerun on one thread, reproduces them to 1e-14;
+    and <sum_i n_i> of each matches the closed form of the local-only C1
+    model (product states: steady state and t = 4)."""
+    import json
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_pidchkpt_fixtures as G
+    from oracle import lowering as L
+    from oracle import oracle as O
+    steady = pimi.load_checkpoint(_golden("c1_steady_state.pid"))
+    dyn = pimi.load_checkpoint(_golden("c1_expv_t4.pid"))
+    for cp in (steady, dyn):
+        b = cp.state.basis
+        assert (b.N, b.d, len(b.sectors), cp.state.data.shape) == (8, 2, 5, (165,))
+        assert cp.state.data.dtype == np.complex128 and cp.schema_version == 1
+    assert steady.time is None and dyn.time == 4.0
+    x, _, rho_t, _ = G.c1_states()
+    assert np.abs(steady.state.data - x).max() <= 1e-14 * np.abs(x).max()
+    assert np.abs(dyn.state.data - rho_t).max() <= 1e-14 * np.abs(rho_t).max()
+    gold = json.load(open(_golden("closed_form.json")))["c1"]
+    _, geom = O.compile_plan(8, 2, L.driven_qubits_terms(**G.C1_PARAMS))
+    sp = L.spin_matrices(2)
+    n_op = L.collective_operator(geom, sp["jp"] @ sp["jm"])
+    assert abs(np.vdot(n_op, steady.state.data).real - gold["steady_n"]) <= 1e-10 * abs(gold["steady_n"])
+    k4 = int(np.argmin(np.abs(np.asarray(gold["t"]) - 4.0)))
+    assert abs(np.vdot(n_op, dyn.state.data).real - gold["n"][k4]) <= 1e-10 * abs(gold["n"][k4])

@@ -282,6 +282,30 @@ def test_expv_series_parity_c1():
     assert e_obs <= 1e-12 and e_state <= 1e-12
 
 
+def test_device_matches_pidchkpt_golden_states():
+    """The golden C1 states exchanged in the reference's PIDCHKPT v1 layout
+    (tests/golden/c1_steady_state.pid, c1_expv_t4.pid, written by the
+    oracle): the device stationary state (full GMRES, rtol 1e-14) and the
+    device state at t = 4 (expv, Krylov 30, saveat 0.1, rtol 1e-14) each
+    within 1e-12 relative of the file's."""
+    gold_dir = os.path.join(os.path.dirname(__file__), "golden")
+    steady = pimi.load_checkpoint(os.path.join(gold_dir, "c1_steady_state.pid"))
+    dyn = pimi.load_checkpoint(os.path.join(gold_dir, "c1_expv_t4.pid"))
+    comp = pimi.compile(_c1_model())
+    assert comp.basis.sectors == steady.state.basis.sectors
+    n = comp.info["n"]
+    res = pimi.stationary_state(comp, algorithm=pimi.GMRESAlgorithm(n, n), atol=0.0, rtol=1e-14, return_info=True)
+    e_steady = _relmax(res.state.data, steady.state.data)
+    rho0 = pimi.computational_product_state(comp.basis, 2)
+    out = pimi.solve_dynamics(comp, rho0, (0.0, dyn.time), saveat=0.1,
+                              algorithm=pimi.ExpvAlgorithm(krylovdim=30, atol=0.0, rtol=1e-14),
+                              observables={"n": SP2.jp @ SP2.jm})
+    e_dyn = _relmax(out.final_state.data, dyn.state.data)
+    parity_log.record("pidchkpt_golden_c1_steady", e_steady, 1e-12)
+    parity_log.record("pidchkpt_golden_c1_t4", e_dyn, 1e-12)
+    assert e_steady <= 1e-12 and e_dyn <= 1e-12
+
+
 def test_expv_series_parity_n14():
     """The N=14 qubit ensemble (C2 model, n = 680) from |e><e|^14 over
     t in [0, 2], saveat 0.25, Krylov 30, both sides to rtol 1e-14."""
