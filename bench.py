@@ -245,6 +245,29 @@ def reference_algorithm_c1(pimi):
     return out
 
 
+def closed_form_errors(pimi, pairs):
+    """Error of <sum_i n_i> against the product-state closed form (local-only
+    driven qubits: the steady state is sigma_ss^N, from the 4 x 4 single-qubit
+    Lindbladian of tests/golden/make_closed_form.py) over (state, Instance)
+    pairs: the largest error per qubit (excited population) and the largest
+    relative error."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_closed_form as CF
+    sp = pimi.spin_matrices(2)
+    ops, worst, worst_rel = {}, 0.0, 0.0
+    for state, inst in pairs:
+        b = state.basis
+        if b.N not in ops:  # sum_i n_i on the basis (a host-only plan of any qubit model of that N)
+            ops[b.N] = pimi.compile(inst.model(), host_only=True).collective_operator(sp.jp @ sp.jm)
+        val = pimi.expect(ops[b.N], state).real
+        ref = CF.steady(inst.N, 2 * inst.omega * CF.jx + 1.0 * CF.jz,
+                        [(inst.gamma, CF.jm), (4 * inst.gamma_phi, CF.jz)])[0]
+        worst = max(worst, abs(val - ref) / inst.N)
+        worst_rel = max(worst_rel, abs(val - ref) / abs(ref))
+    return {"max_excited_population_err": worst, "max_rel_err_sum_n": worst_rel,
+            "reference": "closed form (product steady state), tests/golden/make_closed_form.py"}
+
+
 def batched(pimi, args):
     """512 seeded models through the batch engine; plain single-solve path on a sample."""
     from pimi.workloads import batched_instances
@@ -309,6 +332,7 @@ def batched(pimi, args):
         "plain_fp64_single_solves_per_s": plain_rate,
         "plain_sample": f"first {BATCH_PLAIN_SAMPLE} of the same models, one at a time",
         "speedup_vs_plain_fp64": rate / plain_rate,
+        "error_vs_closed_form": closed_form_errors(pimi, [(r.state, i) for r, i in zip(res, inst)]),
         "tile_width_per_N": size_widths,
         "graph_ticks": {"ticks": graph_ticks, "instantiated": graph_inst, "updated_in_place": graph_upd},
         "ops": ops,
@@ -355,7 +379,8 @@ def batched_n120(pimi, count=64, plain_sample=4):
             "converged": int(sum(r.info["converged"] for r in res)),
             "plain_fp64_single_solves_per_s": plain_rate,
             "plain_sample": f"first {plain_sample} of the same models, one at a time",
-            "speedup_vs_plain_fp64": rate / plain_rate}
+            "speedup_vs_plain_fp64": rate / plain_rate,
+            "error_vs_closed_form": closed_form_errors(pimi, [(r.state, q) for r, q in zip(res, inst)])}
 
 
 def precision_form(pimi):
@@ -522,6 +547,8 @@ def serving(pimi, rates=SERVE_RATES, count=SERVE_COUNT):
             s = summarize(res, wall)
             s["wall_s"] = wall
             s["max_steady_relative_residual"] = max(r.info["relative_residual"] for r in res if r.kind == "steady")
+            s["steady_error_vs_closed_form"] = closed_form_errors(
+                pimi, [(r.state, reqs[r.index].instance) for r in res if r.kind == "steady"])
             out["rates"][f"{rate:g}"] = s
             print(f"# serving {rate:g}/s: {s['all']['per_s']:.2f} req/s, p50 {s['all']['p50_ms']:.0f} ms, "
                   f"p99 {s['all']['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
