@@ -323,7 +323,9 @@ pimi_status pimi_precond_apply(pimi_workspace *ws, const double *x, double *y, d
 /* Throughput probe of the sector preconditioner (four grouped MFMA GEMMs per
  * application, in the workspace precision) on a synthetic device vector,
  * `reps` times, event-timed on the workspace stream: average milliseconds
- * per application and its algorithmic flops (4 x 8 sum_nu m_nu^3).
+ * per application and its algorithmic flops: 4 x 8 sum_nu m_nu^3 for complex
+ * factors U, 4 x 4 sum_nu m_nu^3 when every U entry is real (a real
+ * Hamiltonian block; each product is then complex by real).
  * Measurement only. */
 pimi_status pimi_precond_bench(pimi_workspace *ws, int32_t reps, double *ms_avg, double *flops);
 

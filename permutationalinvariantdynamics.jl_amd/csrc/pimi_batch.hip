@@ -80,6 +80,7 @@ struct BGemm {
     const int32_t *dims;
     const GemmTile *tiles;
     double dscale;
+    int32_t real_a, real_b;  // A (or B) is an array of R: the real factor U
 };
 
 struct BVec {
@@ -257,8 +258,17 @@ __global__ __launch_bounds__(256) void kb_gemm(const BGemm *__restrict__ G, int 
     }
     const BGemm J = Gc[lo];
     const GemmTile t = cptr(J.tiles)[w - J.first];
-    gemm_tile_mfma_buf<R>((const cx<R> *)J.A, (const cx<R> *)J.B, (cx<R> *)J.C, cptr(J.dims)[t.sector],
-                          cptr(J.offs)[t.sector], t, transA, transB, (const cx<R> *)J.D, (R)J.dscale, lane);
+    const int64_t m = cptr(J.dims)[t.sector], base = cptr(J.offs)[t.sector];
+    // a job whose factor U is real passes it as a real operand (wave-uniform branch)
+    if (J.real_a)
+        gemm_tile_mfma_buf<R, R, cx<R>>((const R *)J.A, (const cx<R> *)J.B, (cx<R> *)J.C, m, base, t, transA, transB,
+                                        (const cx<R> *)J.D, (R)J.dscale, lane);
+    else if (J.real_b)
+        gemm_tile_mfma_buf<R, cx<R>, R>((const cx<R> *)J.A, (const R *)J.B, (cx<R> *)J.C, m, base, t, transA, transB,
+                                        (const cx<R> *)J.D, (R)J.dscale, lane);
+    else
+        gemm_tile_mfma_buf<R>((const cx<R> *)J.A, (const cx<R> *)J.B, (cx<R> *)J.C, m, base, t, transA, transB,
+                              (const cx<R> *)J.D, (R)J.dscale, lane);
 }
 
 // The BLAS-1 kernels below run over "virtual blocks": entry e of the
@@ -1077,14 +1087,18 @@ struct Tick {
             int64_t tot = 0;
             for (BJob *J : js) {
                 pimi_plan *P = J->plan;
-                const void *U = sizeof(R) == 8 ? (const void *)P->pU64.p : (const void *)P->pU32.p;
+                const bool ur = P->u_real;
+                const void *U = sizeof(R) == 8 ? (ur ? (const void *)P->pUr64.p : (const void *)P->pU64.p)
+                                               : (ur ? (const void *)P->pUr32.p : (const void *)P->pU32.p);
                 const void *Inv = sizeof(R) == 8 ? (const void *)P->pInv64.p : (const void *)P->pInv32.p;
-                BGemm e{tot, nullptr, nullptr, nullptr, nullptr, P->offs.p, P->dims.p, P->mtiles.p, 1.0};
+                BGemm e{tot, nullptr, nullptr, nullptr, nullptr, P->offs.p, P->dims.p, P->mtiles.p, 1.0, 0, 0};
                 switch (stage) {
-                    case 0: e.A = U; e.B = in(J); e.C = J->t1; break;                            // U^H in
-                    case 1: e.A = J->t1; e.B = U; e.C = J->t2; e.D = Inv; e.dscale = J->scale; break;  // (. U) .* inv
-                    case 2: e.A = U; e.B = J->t2; e.C = J->t1; break;                           // U .
-                    default: e.A = J->t1; e.B = U; e.C = out(J); break;                         // . U^H
+                    case 0: e.A = U; e.B = in(J); e.C = J->t1; e.real_a = ur; break;             // U^H in
+                    case 1:                                                                      // (. U) .* inv
+                        e.A = J->t1; e.B = U; e.C = J->t2; e.D = Inv; e.dscale = J->scale; e.real_b = ur;
+                        break;
+                    case 2: e.A = U; e.B = J->t2; e.C = J->t1; e.real_a = ur; break;            // U .
+                    default: e.A = J->t1; e.B = U; e.C = out(J); e.real_b = ur; break;          // . U^H
                 }
                 v.push_back(e);
                 tot += P->nmtiles;
@@ -2101,13 +2115,16 @@ pimi_status pimi_batch_submit(pimi_batch *E, pimi_plan *plan, const pimi_gmres_o
         {
             J->table_bytes64 = (double)plan->main.table_bytes(16);
             J->table_bytes32 = (double)plan->main.table_bytes(8);
-            double m3 = 0;
-            for (auto m : plan->host.dims) m3 += (double)m * m * m;
-            J->gemm_flops = 4 * 8.0 * m3;  // four complex m x m products per sector, 8 flops per complex fma
         }
         if (J->bytes > E->arena.size)
             throw Error(PIMI_ENOMEM, "the job's device state exceeds the engine's memory reservation");
         ensure_plan_preconditioner(plan);
+        {
+            double m3 = 0;
+            for (auto m : plan->host.dims) m3 += (double)m * m * m;
+            // four m x m products per sector: 8 flops per complex fma, 4 with a real factor U
+            J->gemm_flops = 4 * (plan->u_real ? 4.0 : 8.0) * m3;
+        }
         J->weighted = o->weighting == PIMI_WEIGHT_SECTOR;
         if (J->weighted) ensure_plan_weighting(plan);
         double tt = 0;

@@ -490,6 +490,11 @@ struct pimi_plan {
     bool prec_ready = false;
     pimi::DevBuf<pimi::cx<double>> pU64, pInv64;
     pimi::DevBuf<pimi::cx<float>> pU32, pInv32;
+    // every U entry real (a real Hamiltonian block): the GEMMs then take U as a
+    // real operand, two MFMAs per complex-by-real product instead of three
+    bool u_real = false;
+    pimi::DevBuf<double> pUr64;
+    pimi::DevBuf<float> pUr32;
     pimi::DevBuf<pimi::GemmTile> gtiles;
     int64_t ngtiles = 0;
     pimi::DevBuf<pimi::GemmTile> mtiles;  // 16x16 tiles for the matrix-core GEMM

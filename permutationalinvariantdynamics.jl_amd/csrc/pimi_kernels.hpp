@@ -375,12 +375,34 @@ __device__ __forceinline__ cx<double> buf_load(__amdgpu_buffer_rsrc_t r, uint32_
 __device__ __forceinline__ cx<float> buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, cx<float> *) {
     return __builtin_bit_cast(cx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
 }
+__device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double *) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float *) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t elem_rsrc(const T *p, int64_t entries) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)(entries * (int64_t)sizeof(T)), 0x00020000);
+}
+
+// A sector GEMM operand is complex (cx<R>) or real (R: the preconditioner's
+// eigenvector factor of a real Hamiltonian block).  A complex product takes
+// three real MFMAs (Gauss), a complex-by-real one two.
+template <typename T>
+struct is_cx_t : std::false_type {};
+template <typename R>
+struct is_cx_t<cx<R>> : std::true_type {};
+__device__ __forceinline__ double op_conj(double a) { return a; }
+__device__ __forceinline__ float op_conj(float a) { return a; }
+template <typename R>
+__device__ __forceinline__ cx<R> op_conj(cx<R> a) { return cx<R>{a.x, -a.y}; }
 
 // The same 16x16 tile with its operands through raw buffers over the sector
 // block: an out-of-range row or k reads zero by the hardware range check (no
 // per-lane branch, 32-bit offsets).
-template <typename R>
-__device__ __forceinline__ void gemm_tile_mfma_buf(const cx<R> *A, const cx<R> *B, cx<R> *__restrict__ Cm,
+template <typename R, typename TA = cx<R>, typename TB = cx<R>>
+__device__ __forceinline__ void gemm_tile_mfma_buf(const TA *A, const TB *B, cx<R> *__restrict__ Cm,
                                                    int64_t m, int64_t base, GemmTile t, int transA, int transB,
                                                    const cx<R> *__restrict__ D, R dscale, int lane) {
     typedef MfmaOps<R> Ops;
@@ -388,30 +410,40 @@ __device__ __forceinline__ void gemm_tile_mfma_buf(const cx<R> *A, const cx<R> *
     const int mi = (int)m;
     const int li = lane & 15, lk = lane >> 4;
     const int gi = row0 + li, gj = col0 + li;
-    constexpr uint32_t E = sizeof(cx<R>);
-    cx<R> *const tag = nullptr;
-    const __amdgpu_buffer_rsrc_t ra = block_rsrc<R>(A + base, m * m);
-    const __amdgpu_buffer_rsrc_t rb = block_rsrc<R>(B + base, m * m);
+    constexpr uint32_t EA = sizeof(TA), EB = sizeof(TB);
+    TA *const tagA = nullptr;
+    TB *const tagB = nullptr;
+    const __amdgpu_buffer_rsrc_t ra = elem_rsrc(A + base, m * m);
+    const __amdgpu_buffer_rsrc_t rb = elem_rsrc(B + base, m * m);
     typename Ops::acc_t cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
     constexpr int kUnroll = 4;
     for (int k0 = 0; k0 < mi; k0 += 4 * kUnroll) {
-        cx<R> a[kUnroll], b[kUnroll];
+        TA a[kUnroll];
+        TB b[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const int gk = k0 + 4 * u + lk;
-            const uint32_t ia = (uint32_t)(transA ? gk + gi * mi : gi + gk * mi) * E;
-            const uint32_t ib = (uint32_t)(transB ? gj + gk * mi : gk + gj * mi) * E;
-            a[u] = buf_load(ra, (gi < mi && gk < mi) ? ia : kBufOut, tag);
-            b[u] = buf_load(rb, (gk < mi && gj < mi) ? ib : kBufOut, tag);
-            if (transA) a[u].y = -a[u].y;
-            if (transB) b[u].y = -b[u].y;
+            const uint32_t ia = (uint32_t)(transA ? gk + gi * mi : gi + gk * mi) * EA;
+            const uint32_t ib = (uint32_t)(transB ? gj + gk * mi : gk + gj * mi) * EB;
+            a[u] = buf_load(ra, (gi < mi && gk < mi) ? ia : kBufOut, tagA);
+            b[u] = buf_load(rb, (gk < mi && gj < mi) ? ib : kBufOut, tagB);
+            if (transA) a[u] = op_conj(a[u]);
+            if (transB) b[u] = op_conj(b[u]);
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-            cr = Ops::mac(a[u].x, b[u].x, cr);
-            cr = Ops::mac(-a[u].y, b[u].y, cr);
-            ci = Ops::mac(a[u].x, b[u].y, ci);
-            ci = Ops::mac(a[u].y, b[u].x, ci);
+            if constexpr (is_cx_t<TA>::value && is_cx_t<TB>::value) {
+                cr = Ops::mac(a[u].x, b[u].x, cr);
+                cr = Ops::mac(-a[u].y, b[u].y, cr);
+                ci = Ops::mac(a[u].x, b[u].y, ci);
+                ci = Ops::mac(a[u].y, b[u].x, ci);
+            } else if constexpr (is_cx_t<TA>::value) {
+                cr = Ops::mac(a[u].x, b[u], cr);
+                ci = Ops::mac(a[u].y, b[u], ci);
+            } else {
+                cr = Ops::mac(a[u], b[u].x, cr);
+                ci = Ops::mac(a[u], b[u].y, ci);
+            }
         }
     }
 #pragma unroll
@@ -443,8 +475,8 @@ struct MTile {
 // hundred) whose GEMMs have too few tiles to fill the chip.  A wave loads the
 // operands of two k-steps before their MFMAs, and wave 0 loads its D entries
 // before the k-loop, so the loads overlap.
-template <typename R, bool transA, bool transB>
-__global__ __launch_bounds__(256) void k_sector_gemm_mfma(const cx<R> *__restrict__ A, const cx<R> *__restrict__ B,
+template <typename R, bool transA, bool transB, typename TA = cx<R>, typename TB = cx<R>>
+__global__ __launch_bounds__(256) void k_sector_gemm_mfma(const TA *__restrict__ A, const TB *__restrict__ B,
                                                           cx<R> *__restrict__ Cm, const MTile *__restrict__ tiles,
                                                           int64_t ntiles, const cx<R> *__restrict__ D, R dscale) {
     typedef MfmaOps<R> Ops;
@@ -460,10 +492,12 @@ __global__ __launch_bounds__(256) void k_sector_gemm_mfma(const cx<R> *__restric
     const int row0 = t.ti * kMfmaTile, col0 = t.tj * kMfmaTile;
     const int li = lane & 15, lk = lane >> 4;
     const int gi = row0 + li, gj = col0 + li;
-    constexpr uint32_t E = sizeof(cx<R>);
+    constexpr uint32_t E = sizeof(cx<R>), EA = sizeof(TA), EB = sizeof(TB);
     cx<R> *const tag = nullptr;
-    const __amdgpu_buffer_rsrc_t ra = block_rsrc<R>(A + base, (int64_t)m * m);
-    const __amdgpu_buffer_rsrc_t rb = block_rsrc<R>(B + base, (int64_t)m * m);
+    TA *const tagA = nullptr;
+    TB *const tagB = nullptr;
+    const __amdgpu_buffer_rsrc_t ra = elem_rsrc(A + base, (int64_t)m * m);
+    const __amdgpu_buffer_rsrc_t rb = elem_rsrc(B + base, (int64_t)m * m);
     cx<R> dd[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) dd[r] = cx<R>{0, 0};
@@ -478,28 +512,39 @@ __global__ __launch_bounds__(256) void k_sector_gemm_mfma(const cx<R> *__restric
     constexpr int kUnroll = 4, kStep = 4 * kUnroll;
     const int nsteps = (m + kStep - 1) / kStep;
     const int s0 = nsteps * w / 4, s1 = nsteps * (w + 1) / 4;
-    // Three real products per complex one (Gauss): t1 = Re a Re b, t2 = Im a Im b,
-    // t3 = (Re a + Im a)(Re b + Im b); Re = t1 - t2, Im = t3 - t1 - t2.
+    // Complex operands: three real products per complex one (Gauss): t1 = Re a Re b,
+    // t2 = Im a Im b, t3 = (Re a + Im a)(Re b + Im b); Re = t1 - t2, Im = t3 - t1 - t2.
+    // One real operand: t1 and t2 are the real and imaginary parts.
+    constexpr bool kGauss = is_cx_t<TA>::value && is_cx_t<TB>::value;
     typename Ops::acc_t t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
     // G k-steps (4 G MFMA k-slices): all operand loads first, then the MFMAs.
     auto group = [&](int k0, auto G) {
         constexpr int NU = decltype(G)::value * kUnroll;
-        cx<R> a[NU], b[NU];
+        TA a[NU];
+        TB b[NU];
 #pragma unroll
         for (int u = 0; u < NU; ++u) {
             const int gk = k0 + 4 * u + lk;
-            const uint32_t ia = (uint32_t)(transA ? gk + gi * m : gi + gk * m) * E;
-            const uint32_t ib = (uint32_t)(transB ? gj + gk * m : gk + gj * m) * E;
-            a[u] = buf_load(ra, (gi < m && gk < m) ? ia : kBufOut, tag);
-            b[u] = buf_load(rb, (gk < m && gj < m) ? ib : kBufOut, tag);
-            if (transA) a[u].y = -a[u].y;
-            if (transB) b[u].y = -b[u].y;
+            const uint32_t ia = (uint32_t)(transA ? gk + gi * m : gi + gk * m) * EA;
+            const uint32_t ib = (uint32_t)(transB ? gj + gk * m : gk + gj * m) * EB;
+            a[u] = buf_load(ra, (gi < m && gk < m) ? ia : kBufOut, tagA);
+            b[u] = buf_load(rb, (gk < m && gj < m) ? ib : kBufOut, tagB);
+            if (transA) a[u] = op_conj(a[u]);
+            if (transB) b[u] = op_conj(b[u]);
         }
 #pragma unroll
         for (int u = 0; u < NU; ++u) {
-            t1 = Ops::mac(a[u].x, b[u].x, t1);
-            t2 = Ops::mac(a[u].y, b[u].y, t2);
-            t3 = Ops::mac(a[u].x + a[u].y, b[u].x + b[u].y, t3);
+            if constexpr (kGauss) {
+                t1 = Ops::mac(a[u].x, b[u].x, t1);
+                t2 = Ops::mac(a[u].y, b[u].y, t2);
+                t3 = Ops::mac(a[u].x + a[u].y, b[u].x + b[u].y, t3);
+            } else if constexpr (is_cx_t<TA>::value) {
+                t1 = Ops::mac(a[u].x, b[u], t1);
+                t2 = Ops::mac(a[u].y, b[u], t2);
+            } else {
+                t1 = Ops::mac(a[u], b[u].x, t1);
+                t2 = Ops::mac(a[u], b[u].y, t2);
+            }
         }
     };
     int st = s0;
@@ -508,8 +553,13 @@ __global__ __launch_bounds__(256) void k_sector_gemm_mfma(const cx<R> *__restric
     typename Ops::acc_t cr, ci;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        cr[r] = t1[r] - t2[r];
-        ci[r] = t3[r] - t1[r] - t2[r];
+        if constexpr (kGauss) {
+            cr[r] = t1[r] - t2[r];
+            ci[r] = t3[r] - t1[r] - t2[r];
+        } else {
+            cr[r] = t1[r];
+            ci[r] = t2[r];
+        }
     }
     if (w > 0)
 #pragma unroll

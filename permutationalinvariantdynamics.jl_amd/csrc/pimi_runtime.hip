@@ -789,6 +789,17 @@ void ensure_plan_preconditioner(pimi_plan *P) {
         P->pInv64.upload(i64.data(), i64.size());
         P->pU32.upload(u32.data(), u32.size());
         P->pInv32.upload(i32.data(), i32.size());
+        P->u_real = std::all_of(U.begin(), U.end(), [](const cd &z) { return z.imag() == 0.0; });
+        if (P->u_real) {
+            std::vector<double> r64(U.size());
+            std::vector<float> r32(U.size());
+            for (size_t k = 0; k < U.size(); ++k) {
+                r64[k] = U[k].real();
+                r32[k] = (float)U[k].real();
+            }
+            P->pUr64.upload(r64.data(), r64.size());
+            P->pUr32.upload(r32.data(), r32.size());
+        }
         P->gtiles.upload(tiles.data(), tiles.size());
         P->ngtiles = (int64_t)tiles.size();
         std::vector<GemmTile> mt;
@@ -873,14 +884,25 @@ static void apply_precond_t(pimi_workspace *W, const cx<R> *r, cx<R> *out, doubl
     dim3 g((unsigned)P->nxtiles), b(256);  // one 16x16 tile per workgroup (split K), XCD-grouped order
     const cx<R> *nul = nullptr;
     const cx<R> *U, *Inv;
-    if constexpr (sizeof(R) == 8) { U = P->pU64.p; Inv = P->pInv64.p; }
-    else { U = P->pU32.p; Inv = P->pInv32.p; }
+    const R *Ur;
+    if constexpr (sizeof(R) == 8) { U = P->pU64.p; Inv = P->pInv64.p; Ur = P->pUr64.p; }
+    else { U = P->pU32.p; Inv = P->pInv32.p; Ur = P->pUr32.p; }
     const int64_t nt = P->nxtiles;
     const MTile *xt = P->xtiles.p;
-    hipLaunchKernelGGL((k_sector_gemm_mfma<R, true, false>), g, b, 0, W->stream, U, r, t1, xt, nt, nul, (R)1);
-    hipLaunchKernelGGL((k_sector_gemm_mfma<R, false, false>), g, b, 0, W->stream, t1, U, t2, xt, nt, Inv, (R)scale);
-    hipLaunchKernelGGL((k_sector_gemm_mfma<R, false, false>), g, b, 0, W->stream, U, t2, t1, xt, nt, nul, (R)1);
-    hipLaunchKernelGGL((k_sector_gemm_mfma<R, false, true>), g, b, 0, W->stream, t1, U, out, xt, nt, nul, (R)1);
+    if (P->u_real) {
+        typedef cx<R> C;
+        hipLaunchKernelGGL((k_sector_gemm_mfma<R, true, false, R, C>), g, b, 0, W->stream, Ur, r, t1, xt, nt, nul, (R)1);
+        hipLaunchKernelGGL((k_sector_gemm_mfma<R, false, false, C, R>), g, b, 0, W->stream, t1, Ur, t2, xt, nt, Inv,
+                           (R)scale);
+        hipLaunchKernelGGL((k_sector_gemm_mfma<R, false, false, R, C>), g, b, 0, W->stream, Ur, t2, t1, xt, nt, nul, (R)1);
+        hipLaunchKernelGGL((k_sector_gemm_mfma<R, false, true, C, R>), g, b, 0, W->stream, t1, Ur, out, xt, nt, nul, (R)1);
+    } else {
+        hipLaunchKernelGGL((k_sector_gemm_mfma<R, true, false>), g, b, 0, W->stream, U, r, t1, xt, nt, nul, (R)1);
+        hipLaunchKernelGGL((k_sector_gemm_mfma<R, false, false>), g, b, 0, W->stream, t1, U, t2, xt, nt, Inv,
+                           (R)scale);
+        hipLaunchKernelGGL((k_sector_gemm_mfma<R, false, false>), g, b, 0, W->stream, U, t2, t1, xt, nt, nul, (R)1);
+        hipLaunchKernelGGL((k_sector_gemm_mfma<R, false, true>), g, b, 0, W->stream, t1, U, out, xt, nt, nul, (R)1);
+    }
     HIPCHK(hipGetLastError());
 }
 
@@ -2684,7 +2706,7 @@ pimi_status pimi_precond_bench(pimi_workspace *W, int32_t reps, double *ms_avg, 
         *ms_avg = ms / reps;
         double m3 = 0;
         for (auto m : W->plan->host.dims) m3 += (double)m * m * m;
-        *flops = 4 * 8.0 * m3;
+        *flops = 4 * (W->plan->u_real ? 4.0 : 8.0) * m3;  // complex-by-real products: 4 flops per complex entry fma
     });
 }
 

@@ -68,6 +68,35 @@ def test_batch_matches_oracle(precision, weighting):
 
 
 @pytest.mark.parametrize("precision", [N.PIMI_C64, N.PIMI_C32])
+def test_batch_real_and_complex_factors_share_launches(precision):
+    """Jobs whose preconditioner factor U is real (H = jz + jx: the GEMMs
+    take U as a real operand) and complex (H = jz + jx + 0.4 jy) run in the
+    same engine launches; every job reaches the oracle's solution."""
+    print(f"batch real/complex factors precision={precision}", flush=True)
+    sp = L.spin_matrices(2)
+    cases = []
+    for hy in (0.0, 0.4):
+        for n_ in (13, 20):
+            cases.append((n_, hy))
+    comps = [pimi.compile(pimi.qubit_ensemble_model(n_, hamiltonian=SP2.jz + SP2.jx + hy * SP2.jy, emission=0.1,
+                                                    dephasing=0.2, pumping=0.02, collective_emission=0.1))
+             for n_, hy in cases]
+    alg = pimi.BatchAlgorithm(krylovdim=150, inner_rtol=1e-4, weighting="sector")
+    res = pimi.batched_stationary_states(comps, alg, rtol=1e-11, precision=precision)
+    for (n_, hy), r in zip(cases, res):
+        ref, _ = O.compile_plan(n_, 2, L.qubit_ensemble_terms(hamiltonian=sp["jz"] + sp["jx"] + hy * sp["jy"],
+                                                              emission=0.1, dephasing=0.2, pumping=0.02,
+                                                              collective_emission=0.1))
+        x_ref, info_ref = ref.steady_state(krylovdim=400, maxiter=400, atol=0.0, rtol=1e-12,
+                                           preconditioner="sector", criterion="backward", weighting="sector")
+        assert info_ref["converged"] and r.info["converged"]
+        assert r.info["relative_residual"] <= 1e-11
+        err = np.linalg.norm(r.state.data - x_ref) / np.linalg.norm(x_ref)
+        print(f"  N={n_} jy={hy} iterations {r.info['iterations']} err {err:.2e}", flush=True)
+        assert err <= 1e-8, (n_, hy, err)
+
+
+@pytest.mark.parametrize("precision", [N.PIMI_C64, N.PIMI_C32])
 def test_batch_small_spaces(precision):
     """PI dimensions 4, 10 and 20 with krylovdim 150: each job's basis holds
     min(krylovdim, n) columns, as the reference's KrylovWorkspace."""

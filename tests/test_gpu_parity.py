@@ -406,12 +406,17 @@ def test_preconditioned_n120_backward():
 
 
 @pytest.mark.parametrize("precision,tol", [(0, 1e-13), (1, 2e-6)])
-@pytest.mark.parametrize("N,d", [(40, 2), (120, 2), (6, 3)])
+@pytest.mark.parametrize("N,d", [(40, 2), (120, 2), (6, 3), (40, "y"), (80, "y")])
 def test_device_preconditioner_matches_factors(N, d, precision, tol):
-    """The matrix-core sector GEMMs reproduce U ((U^H R U) .* inv_den) U^H."""
+    """The matrix-core sector GEMMs reproduce U ((U^H R U) .* inv_den) U^H,
+    with U a real operand (real Hamiltonian blocks: H = jz + jx, qutrit jz +
+    jx) and with U complex (H = jz + jx + 0.4 jy, d = "y")."""
     import pimi._native as Nn
     if d == 2:
         model = _c2_model(N)
+    elif d == "y":
+        model = pimi.qubit_ensemble_model(N, hamiltonian=SP2.jz + SP2.jx + 0.4 * SP2.jy, emission=0.1,
+                                          dephasing=0.2, pumping=0.02, collective_emission=0.1)
     else:
         model, _ = _qutrit(N)
     comp = pimi.compile(model)
@@ -419,6 +424,7 @@ def test_device_preconditioner_matches_factors(N, d, precision, tol):
     U = np.zeros(n, dtype=np.complex128)
     inv = np.zeros(n, dtype=np.complex128)
     Nn.check(Nn.lib().pimi_sector_preconditioner_factors(comp._plan, Nn.ptr(U), Nn.ptr(inv)))
+    assert (np.abs(U.imag).max() > 0) == (d == "y")  # which GEMM form runs
     x = np.ascontiguousarray(_rand(n, 21))
     y = np.zeros_like(x)
     Nn.check(Nn.lib().pimi_precond_apply(comp.workspace(precision, 8), Nn.ptr(x), Nn.ptr(y), C.c_double(2.5)))
@@ -429,7 +435,9 @@ def test_device_preconditioner_matches_factors(N, d, precision, tol):
         Us = U[o0:o1].reshape(m, m, order="F")
         ref[o0:o1] = (2.5 * Us @ ((Us.conj().T @ x[o0:o1].reshape(m, m, order="F") @ Us)
                                   * inv[o0:o1].reshape(m, m, order="F")) @ Us.conj().T).reshape(-1, order="F")
-    assert _relmax(y, ref) <= tol
+    err = _relmax(y, ref)
+    print(f"  preconditioner N={N} d={d} precision={precision} rel err {err:.2e} / {tol:g}", flush=True)
+    assert err <= tol
 
 
 @pytest.mark.parametrize("precision", [0, 1])
