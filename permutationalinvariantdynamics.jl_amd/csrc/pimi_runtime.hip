@@ -451,7 +451,7 @@ struct pimi_workspace {
     int inner_graph_k = 0, inner_graph_prec = -1;
     double inner_graph_scale = 0.0;
     int inner_graph_width = -1;  // banded tile width index the graph was captured with
-    bool use_graphs = true;  // PIMI_NO_GRAPH=1 runs the same kernels eagerly
+    bool use_graphs = false;  // PIMI_INNER_GRAPH=1: the inner solve as a captured HIP graph
     double *hpin = nullptr;  // pinned host staging (small results)
     bool trace_progress = false;  // PIMI_TRACE=1: solver progress lines on stderr
     bool weighted = false;        // current stationary solve iterates on u = S x
@@ -1081,8 +1081,9 @@ static void ensure_inner(pimi_workspace *W, int kin) {
     const size_t es = W->prec == PIMI_C32 ? sizeof(cx<float>) : sizeof(cx<double>);
     if (W->inner_k < kin) {
         if (W->inner_graph) { HIPCHK(hipGraphExecDestroy(W->inner_graph)); W->inner_graph = nullptr; }
-        W->inner_mem.alloc(es * (size_t)n * (size_t)(kin + 1 + 4));
-        HIPCHK(hipMemsetAsync(W->inner_mem.p, 0, es * (size_t)n * (size_t)(kin + 1 + 4), W->stream));
+        // V (kin + 1 vectors), Z (kin), then four work vectors
+        W->inner_mem.alloc(es * (size_t)n * (size_t)(2 * kin + 1 + 4));
+        HIPCHK(hipMemsetAsync(W->inner_mem.p, 0, es * (size_t)n * (size_t)(2 * kin + 1 + 4), W->stream));
         W->Hin.alloc((size_t)(kin + 1) * kin);
         W->inner_norms.alloc((size_t)kin + 2);
         W->inner_io.alloc((size_t)2 * n);
@@ -1093,30 +1094,32 @@ static void ensure_inner(pimi_workspace *W, int kin) {
 }
 
 // The inner solve with no host interaction: norms, Hessenberg, least squares
-// and the combination all stay on the device, so the sequence can be
-// captured once as a HIP graph and replayed for every outer step.
-// Reads W->inner_io[0:n], writes W->inner_io[n:2n].
+// and the combination all stay on the device.  Right-preconditioned GMRES on
+// D/s: the directions z_i = M^-1 v_i are kept (Z), so the result M^-1 V y is
+// the combination Z y and needs no fifth preconditioner application.
+// Reads rin, writes zout (n entries each, FP64).
 template <typename R>
-static void inner_solve_dev(pimi_workspace *W, int kin, double scale) {
+static void inner_solve_dev(pimi_workspace *W, int kin, double scale, const cx<double> *rin, cx<double> *zout) {
     const int64_t n = W->plan->n;
-    const cx<double> *rin = W->inner_io.p;
-    cx<double> *zout = W->inner_io.p + n;
     cx<R> *base = reinterpret_cast<cx<R> *>(W->inner_mem.p);
-    cx<R> *Vi = base, *zt = base + (size_t)n * (kin + 1), *wt = zt + n, *t1 = wt + n, *t2 = t1 + n;
+    cx<R> *Vi = base, *Zi = base + (size_t)n * (kin + 1), *zt = Zi + (size_t)n * kin, *wt = zt + n, *t1 = wt + n,
+          *t2 = t1 + n;
     double *nrm = W->inner_norms.p;
     const dim3 gv(grid_for(n)), bv(kRedBlock);
+    // v_0 = r / ||r||: norm partials, then the scaling kernel reduces them itself
     if constexpr (sizeof(R) == 8) {
-        dev_norm<double>(W, rin, n, nrm + kin);
-        hipLaunchKernelGGL(k_scale_div<double>, gv, bv, 0, W->stream, Vi, rin, n, nrm + kin);
+        hipLaunchKernelGGL(k_norm_partial<double>, dim3(kNormBlocks), bv, 0, W->stream, rin, n, W->npartial.p);
+        dev_scale_normalize<double>(W, Vi, rin, n, nrm + kin);
     } else {
         hipLaunchKernelGGL((k_convert<R, double>), dim3(1024), dim3(256), 0, W->stream, Vi, rin, n);
-        dev_norm<R>(W, Vi, n, nrm + kin);
-        hipLaunchKernelGGL(k_scale_div<R>, gv, bv, 0, W->stream, Vi, Vi, n, nrm + kin);
+        hipLaunchKernelGGL(k_norm_partial<R>, dim3(kNormBlocks), bv, 0, W->stream, Vi, n, W->npartial.p);
+        dev_scale_normalize<R>(W, Vi, Vi, n, nrm + kin);
     }
     hipLaunchKernelGGL(k_zero_c64, dim3(1), dim3(256), 0, W->stream, W->Hin.p, (int64_t)(kin + 1) * kin);
     for (int i = 0; i < kin; ++i) {
-        apply_precond_t<R>(W, Vi + (size_t)i * n, zt, scale, t1, t2);
-        launch_apply<R>(W, zt, wt, 1.0 / scale, 1, true);
+        cx<R> *zi = Zi + (size_t)i * n;
+        apply_precond_t<R>(W, Vi + (size_t)i * n, zi, scale, t1, t2);
+        launch_apply<R>(W, zi, wt, 1.0 / scale, 1, true);
         // one Gram-Schmidt pass: the inner solve only preconditions the
         // flexible outer iteration, and at <= 32 vectors one classical pass
         // keeps the inner basis orthogonal to far below its tolerance
@@ -1124,22 +1127,25 @@ static void inner_solve_dev(pimi_workspace *W, int kin, double scale) {
         dev_scale_normalize<R>(W, Vi + (size_t)(i + 1) * n, wt, n, nrm + i);
     }
     hipLaunchKernelGGL(k_inner_lsq, dim3(1), dim3(64), 0, W->stream, W->Hin.p, nrm, kin, W->inner_y.p);
-    hipLaunchKernelGGL(k_gemv<R>, gv, bv, 0, W->stream, wt, Vi, n, kin, W->inner_y.p, n, 1.0, 1);
     if constexpr (sizeof(R) == 8) {
-        apply_precond_t<R>(W, wt, zout, scale, t1, t2);
+        hipLaunchKernelGGL(k_gemv<R>, gv, bv, 0, W->stream, zout, Zi, n, kin, W->inner_y.p, n, 1.0, 1);
     } else {
-        apply_precond_t<R>(W, wt, zt, scale, t1, t2);
+        hipLaunchKernelGGL(k_gemv<R>, gv, bv, 0, W->stream, zt, Zi, n, kin, W->inner_y.p, n, 1.0, 1);
         hipLaunchKernelGGL((k_convert<double, R>), dim3(1024), dim3(256), 0, W->stream, zout, zt, n);
     }
     HIPCHK(hipGetLastError());
 }
 
-static void run_inner(pimi_workspace *W, int kin, double scale) {
-    auto body = [&]() {
-        if (W->prec == PIMI_C32) inner_solve_dev<float>(W, kin, scale);
-        else inner_solve_dev<double>(W, kin, scale);
+// z = inner(r).  Eager launches (the default) read r and write z in place; the
+// captured-graph form (PIMI_INNER_GRAPH=1) runs on the fixed pair inner_io and
+// copies r in and z out around the replay.
+static void run_inner(pimi_workspace *W, int kin, double scale, const cx<double> *r, cx<double> *z) {
+    auto body = [&](const cx<double> *rin, cx<double> *zout) {
+        if (W->prec == PIMI_C32) inner_solve_dev<float>(W, kin, scale, rin, zout);
+        else inner_solve_dev<double>(W, kin, scale, rin, zout);
     };
-    if (!W->use_graphs) { body(); return; }
+    if (!W->use_graphs) { body(r, z); return; }
+    const int64_t n = W->plan->n;
     // the key covers everything the captured launches bake in: the inner
     // Krylov dimension, the precision, the scale and the banded apply's tile
     // width (which the autotuner may change after a capture)
@@ -1150,7 +1156,7 @@ static void run_inner(pimi_workspace *W, int kin, double scale) {
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(W->stream, hipStreamCaptureModeThreadLocal));
         try {
-            body();
+            body(W->inner_io.p, W->inner_io.p + n);
         } catch (...) {
             (void)hipStreamEndCapture(W->stream, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -1165,7 +1171,9 @@ static void run_inner(pimi_workspace *W, int kin, double scale) {
         W->inner_graph_scale = scale;
         W->inner_graph_width = width;
     }
+    HIPCHK(hipMemcpyAsync(W->inner_io.p, r, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
     HIPCHK(hipGraphLaunch(W->inner_graph, W->stream));
+    HIPCHK(hipMemcpyAsync(z, W->inner_io.p + n, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
 }
 
 static GmresResult fgmres_device_cgs2(pimi_workspace *W, double invscale, double atol, double rtol,
@@ -1176,9 +1184,7 @@ static GmresResult fgmres_device_cgs2(pimi_workspace *W, double invscale, double
     ensure_inner(W, kin);
     cx<double> *V = W->V.p, *Z = W->Zo.p;
     auto inner = [&](const cx<double> *r, cx<double> *z) {
-        HIPCHK(hipMemcpyAsync(W->inner_io.p, r, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
-        run_inner(W, kin, scale);
-        HIPCHK(hipMemcpyAsync(z, W->inner_io.p + n, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
+        run_inner(W, kin, scale, r, z);
     };
     std::vector<cd> H((size_t)(m + 1) * m), g(m + 1), sn(m), yv(m);
     std::vector<double> cs(m);
@@ -1329,9 +1335,7 @@ static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol
     cx<double> *V = W->V.p, *Z = W->Zo.p;
     const size_t se = dc_slot_entries(m);
     auto inner = [&](const cx<double> *r, cx<double> *z) {
-        HIPCHK(hipMemcpyAsync(W->inner_io.p, r, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
-        run_inner(W, kin, scale);
-        HIPCHK(hipMemcpyAsync(z, W->inner_io.p + n, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, W->stream));
+        run_inner(W, kin, scale, r, z);
     };
     std::vector<cd> H((size_t)(m + 1) * m), g(m + 1), sn(m);
     std::vector<double> cs(m), rho_col(m + 1);
@@ -2458,8 +2462,13 @@ pimi_status pimi_workspace_create(pimi_plan *P, int32_t precision, int32_t krylo
         W->m = (int)std::min<int64_t>(krylovdim, P->n);
         const char *tr = std::getenv("PIMI_TRACE");
         W->trace_progress = tr && tr[0] == '1';
+        // The inner solve launches eagerly by default: measured at N = 10..120,
+        // the captured graph was 1-4 % slower per solve than the same kernels
+        // launched one by one behind the outer loop's lookahead, which already
+        // keeps the queue full.  PIMI_INNER_GRAPH=1 replays a captured graph.
+        const char *ig = std::getenv("PIMI_INNER_GRAPH");
         const char *ng = std::getenv("PIMI_NO_GRAPH");
-        W->use_graphs = !(ng && ng[0] == '1');
+        W->use_graphs = ig && ig[0] == '1' && !(ng && ng[0] == '1');
         HIPCHK(hipStreamCreateWithFlags(&W->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&W->ev0));
         HIPCHK(hipEventCreate(&W->ev1));

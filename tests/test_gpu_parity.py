@@ -497,11 +497,12 @@ def test_inner_outer_small_spaces(N_, preconditioner):
 def test_inner_graph_replays_match_eager(monkeypatch):
     """The captured inner-solve graph, replayed for every outer step (with the
     host's own work and synchronisation between replays), gives bitwise the
-    result of launching the same kernels eagerly."""
+    result of launching the same kernels eagerly (the default; PIMI_INNER_GRAPH=1,
+    read when the workspace is created, selects the graph)."""
     alg = pimi.GMRESAlgorithm(100, 100, "sector_inner", inner_steps=6)
     out = []
-    for flag in ("0", "1"):
-        monkeypatch.setenv("PIMI_NO_GRAPH", flag)
+    for flag in ("1", "0"):
+        monkeypatch.setenv("PIMI_INNER_GRAPH", flag)
         comp = pimi.compile(_c2_model(30))
         res = [pimi.stationary_state(comp, algorithm=alg, atol=0.0, rtol=1e-12, return_info=True,
                                      criterion="backward", precision=1) for _ in range(2)]
