@@ -161,6 +161,51 @@ __global__ __launch_bounds__(kRedBlock) void k_gs_update_r(cx<R> *__restrict__ w
     }
 }
 
+// One classical Gram-Schmidt pass with the normalization folded in.  The dot
+// partials dpart[c * G + q] (k_dots_partial) cover columns c = 0..ncol, column
+// ncol being w itself, so they hold h = V[:, 0:ncol]^H w and ||w||^2.  Every
+// workgroup reduces them in one fixed order and forms beta = sqrt(||w||^2 -
+// |h|^2) (w - V h and V h are orthogonal parts of w); v = (w - V h) / beta is
+// written over w (0 when beta is 0: an exact breakdown).  Workgroup 0 adds h
+// into hacc and writes beta to *nrm.  write_v = 0 runs the reductions only
+// (the last step of an inner solve, whose vector is not used).
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void k_gs_update_scale_r(cx<R> *w, const cx<R> *__restrict__ V, int64_t ldv,
+                                                                 int ncol, const cx<double> *__restrict__ dpart, int G,
+                                                                 cx<double> *hacc, double *nrm, int64_t n,
+                                                                 int write_v) {
+    __shared__ cx<double> hs[kFusedDotCols + 1];
+    __shared__ double beta_s;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int c = wid; c <= ncol; c += kRedBlock / 64) {
+        const double re = wave_total(&dpart[(int64_t)c * G].x, G, 2);
+        const double im = wave_total(&dpart[(int64_t)c * G].y, G, 2);
+        if (lane == 0) hs[c] = cx<double>{re, im};
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double hh = 0;
+        for (int c = 0; c < ncol; ++c) hh += hs[c].x * hs[c].x + hs[c].y * hs[c].y;
+        const double b2 = hs[ncol].x - hh;
+        beta_s = b2 > 0 ? sqrt(b2) : 0.0;
+        if (blockIdx.x == 0) *nrm = beta_s;
+    }
+    if (blockIdx.x == 0 && hacc && (int)threadIdx.x < ncol)
+        hacc[threadIdx.x] = cx<double>{hacc[threadIdx.x].x + hs[threadIdx.x].x, hacc[threadIdx.x].y + hs[threadIdx.x].y};
+    __syncthreads();
+    if (!write_v) return;
+    const double beta = beta_s;
+    for (int64_t k = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kRedBlock) {
+        double re = w[k].x, im = w[k].y;
+        for (int c = 0; c < ncol; ++c) {
+            const cx<R> v = V[k + (int64_t)c * ldv];
+            re -= (double)v.x * hs[c].x - (double)v.y * hs[c].y;
+            im -= (double)v.x * hs[c].y + (double)v.y * hs[c].x;
+        }
+        w[k] = beta == 0.0 ? cx<R>{0, 0} : cx<R>{(R)(re / beta), (R)(im / beta)};
+    }
+}
+
 // dst = src / ||src|| with ||src||^2 the reduction of npart[0:G]; workgroup 0
 // writes ||src|| to *nrm.  A zero norm (exact breakdown) gives dst = 0.
 template <typename R>
@@ -1117,14 +1162,25 @@ static void inner_solve_dev(pimi_workspace *W, int kin, double scale, const cx<d
     }
     hipLaunchKernelGGL(k_zero_c64, dim3(1), dim3(256), 0, W->stream, W->Hin.p, (int64_t)(kin + 1) * kin);
     for (int i = 0; i < kin; ++i) {
-        cx<R> *zi = Zi + (size_t)i * n;
+        cx<R> *zi = Zi + (size_t)i * n, *wi = Vi + (size_t)(i + 1) * n;
+        cx<double> *hcol = W->Hin.p + (size_t)i * (kin + 1);
         apply_precond_t<R>(W, Vi + (size_t)i * n, zi, scale, t1, t2);
-        launch_apply<R>(W, zi, wt, 1.0 / scale, 1, true);
+        launch_apply<R>(W, zi, wi, 1.0 / scale, 1, true);
         // one Gram-Schmidt pass: the inner solve only preconditions the
         // flexible outer iteration, and at <= 32 vectors one classical pass
-        // keeps the inner basis orthogonal to far below its tolerance
-        dev_cgs2<R>(W, Vi, n, i + 1, wt, n, W->Hin.p + (size_t)i * (kin + 1), 1);
-        dev_scale_normalize<R>(W, Vi + (size_t)(i + 1) * n, wt, n, nrm + i);
+        // keeps the inner basis orthogonal to far below its tolerance.  The
+        // candidate w sits in the basis slot it will occupy, so one reduction
+        // pass gives V^H w and ||w||^2 and one kernel updates and normalizes.
+        if (i + 1 <= kFusedDotCols) {
+            hipLaunchKernelGGL(k_dots_partial<R>, dim3(kDotBlocks, i + 2), bv, 0, W->stream, Vi, n, wi, n,
+                               W->partial.p);
+            const bool last = i + 1 == kin;
+            hipLaunchKernelGGL(k_gs_update_scale_r<R>, dim3(last ? 1 : kNormBlocks), bv, 0, W->stream, wi, Vi, n,
+                               i + 1, W->partial.p, kDotBlocks, hcol, nrm + i, n, last ? 0 : 1);
+        } else {
+            dev_cgs2<R>(W, Vi, n, i + 1, wi, n, hcol, 1);
+            dev_scale_normalize<R>(W, wi, wi, n, nrm + i);
+        }
     }
     hipLaunchKernelGGL(k_inner_lsq, dim3(1), dim3(64), 0, W->stream, W->Hin.p, nrm, kin, W->inner_y.p);
     if constexpr (sizeof(R) == 8) {
