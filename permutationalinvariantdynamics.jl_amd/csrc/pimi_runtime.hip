@@ -166,13 +166,13 @@ __global__ __launch_bounds__(kRedBlock) void k_gs_update_r(cx<R> *__restrict__ w
 // ncol being w itself, so they hold h = V[:, 0:ncol]^H w and ||w||^2.  Every
 // workgroup reduces them in one fixed order and forms beta = sqrt(||w||^2 -
 // |h|^2) (w - V h and V h are orthogonal parts of w); v = (w - V h) / beta is
-// written over w (0 when beta is 0: an exact breakdown).  Workgroup 0 adds h
-// into hacc and writes beta to *nrm.  write_v = 0 runs the reductions only
+// written over w (0 when beta is 0: an exact breakdown).  Workgroup 0 writes h
+// to hcol and beta to *nrm.  write_v = 0 runs the reductions only
 // (the last step of an inner solve, whose vector is not used).
 template <typename R>
 __global__ __launch_bounds__(kRedBlock) void k_gs_update_scale_r(cx<R> *w, const cx<R> *__restrict__ V, int64_t ldv,
                                                                  int ncol, const cx<double> *__restrict__ dpart, int G,
-                                                                 cx<double> *hacc, double *nrm, int64_t n,
+                                                                 cx<double> *hcol, double *nrm, int64_t n,
                                                                  int write_v) {
     __shared__ cx<double> hs[kFusedDotCols + 1];
     __shared__ double beta_s;
@@ -190,8 +190,7 @@ __global__ __launch_bounds__(kRedBlock) void k_gs_update_scale_r(cx<R> *w, const
         beta_s = b2 > 0 ? sqrt(b2) : 0.0;
         if (blockIdx.x == 0) *nrm = beta_s;
     }
-    if (blockIdx.x == 0 && hacc && (int)threadIdx.x < ncol)
-        hacc[threadIdx.x] = cx<double>{hacc[threadIdx.x].x + hs[threadIdx.x].x, hacc[threadIdx.x].y + hs[threadIdx.x].y};
+    if (blockIdx.x == 0 && (int)threadIdx.x < ncol) hcol[threadIdx.x] = hs[threadIdx.x];
     __syncthreads();
     if (!write_v) return;
     const double beta = beta_s;
@@ -336,19 +335,49 @@ template <typename R>
 __global__ __launch_bounds__(1024) void k_trace_axpy(cx<R> *__restrict__ y, const cx<R> *__restrict__ z,
                                                      const int64_t *__restrict__ idx, const double *__restrict__ t,
                                                      const double *__restrict__ v, int64_t nt) {
+    // one workgroup, latency-bound: each thread issues kTraceUnroll index
+    // loads, then their gathers, before it waits (nt = sum of the sector
+    // sizes, 3721 at N=120: one round)
+    constexpr int kTraceUnroll = 4;
+    constexpr int64_t kSpan = 1024 * kTraceUnroll;
     __shared__ double sh[1024 / 64];
     double re = 0, im = 0;
-    for (int64_t k = threadIdx.x; k < nt; k += 1024) {
-        cx<R> zz = z[idx[k]];
-        re += t[k] * (double)zz.x;
-        im += t[k] * (double)zz.y;
+    for (int64_t k0 = 0; k0 < nt; k0 += kSpan) {
+        int64_t ix[kTraceUnroll];
+        double tv[kTraceUnroll];
+        cx<R> zz[kTraceUnroll];
+#pragma unroll
+        for (int u = 0; u < kTraceUnroll; ++u) {
+            const int64_t k = k0 + u * 1024 + threadIdx.x;
+            ix[u] = k < nt ? idx[k] : -1;
+            tv[u] = k < nt ? t[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kTraceUnroll; ++u) zz[u] = ix[u] >= 0 ? z[ix[u]] : cx<R>{0, 0};
+#pragma unroll
+        for (int u = 0; u < kTraceUnroll; ++u) {
+            re += tv[u] * (double)zz[u].x;
+            im += tv[u] * (double)zz[u].y;
+        }
     }
     re = block_sum(re, sh);
     __syncthreads();
     im = block_sum(im, sh);
-    for (int64_t k = threadIdx.x; k < nt; k += 1024) {
-        cx<R> yy = y[idx[k]];
-        y[idx[k]] = cx<R>{(R)(yy.x + re * v[k]), (R)(yy.y + im * v[k])};
+    for (int64_t k0 = 0; k0 < nt; k0 += kSpan) {
+        int64_t ix[kTraceUnroll];
+        double vv[kTraceUnroll];
+        cx<R> yy[kTraceUnroll];
+#pragma unroll
+        for (int u = 0; u < kTraceUnroll; ++u) {
+            const int64_t k = k0 + u * 1024 + threadIdx.x;
+            ix[u] = k < nt ? idx[k] : -1;
+            vv[u] = k < nt ? v[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kTraceUnroll; ++u) yy[u] = ix[u] >= 0 ? y[ix[u]] : cx<R>{0, 0};
+#pragma unroll
+        for (int u = 0; u < kTraceUnroll; ++u)
+            if (ix[u] >= 0) y[ix[u]] = cx<R>{(R)(yy[u].x + re * vv[u]), (R)(yy[u].y + im * vv[u])};
     }
 }
 
@@ -1160,7 +1189,10 @@ static void inner_solve_dev(pimi_workspace *W, int kin, double scale, const cx<d
         hipLaunchKernelGGL(k_norm_partial<R>, dim3(kNormBlocks), bv, 0, W->stream, Vi, n, W->npartial.p);
         dev_scale_normalize<R>(W, Vi, Vi, n, nrm + kin);
     }
-    hipLaunchKernelGGL(k_zero_c64, dim3(1), dim3(256), 0, W->stream, W->Hin.p, (int64_t)(kin + 1) * kin);
+    // the fused steps (i < kFusedDotCols) write their Hessenberg column's rows
+    // 0..i (all that the least squares reads); wider steps accumulate
+    if (kin > kFusedDotCols)
+        hipLaunchKernelGGL(k_zero_c64, dim3(1), dim3(256), 0, W->stream, W->Hin.p, (int64_t)(kin + 1) * kin);
     for (int i = 0; i < kin; ++i) {
         cx<R> *zi = Zi + (size_t)i * n, *wi = Vi + (size_t)(i + 1) * n;
         cx<double> *hcol = W->Hin.p + (size_t)i * (kin + 1);
