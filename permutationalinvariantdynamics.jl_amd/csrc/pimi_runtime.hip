@@ -461,6 +461,153 @@ __global__ void k_inner_lsq(const cx<double> *__restrict__ Hin, const double *__
     for (int i = 0; i < k; ++i) y[i] = i < kk ? ys[i] : cx<double>{0, 0};
 }
 
+// min |beta e1 - H y| for a (k+1) x k upper Hessenberg H (k <= K <= 8) by
+// complex Givens rotations, on one wave: lane c < k holds column c of H, lane
+// k the right-hand side, each in registers indexed by row (K is a compile-time
+// bound and the row loops are unrolled, so no index is dynamic).  Rotation j
+// is formed by lane j and broadcast; every column to its right and the
+// right-hand side apply it to their rows j, j+1 (the same arithmetic, in the
+// same order per column, as applying rotations 0..c-1 to column c when it is
+// reached, as k_inner_lsq does).  Back substitution: y_i from lane i, the sum
+// of R(i, l) y_l over l > i reduced across lanes 0..7.  Hc(i, j) gives H(i, j)
+// for i <= j, sub(j) the real subdiagonal H(j+1, j); a zero subdiagonal ends
+// the space early.  Every lane of the wave must call it; lanes < k write ys.
+template <int K, typename HF, typename SF>
+__device__ __forceinline__ void hess_lsq_wave(int k, double beta, HF Hc, SF sub, cx<double> *ys) {
+    static_assert(K <= 8, "the back substitution reduces across 8 lanes");
+    const int lane = threadIdx.x & 63;
+    double hr[K + 1], hi[K + 1];
+#pragma unroll
+    for (int r = 0; r <= K; ++r) { hr[r] = 0; hi[r] = 0; }
+    int kk = k;
+    for (int j = k - 1; j >= 0; --j)
+        if (sub(j) == 0.0) kk = j + 1;
+    if (lane < k) {
+#pragma unroll
+        for (int r = 0; r <= K; ++r) {
+            if (r <= lane) {
+                const cx<double> h = Hc(r, lane);
+                hr[r] = h.x;
+                hi[r] = h.y;
+            } else if (r == lane + 1) {
+                hr[r] = sub(lane);
+            }
+        }
+    } else if (lane == k) {
+        hr[0] = beta;
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if (j >= kk) break;
+        double c = 0, sr = 0, si = 0;
+        if (lane == j) {
+            const double ar = hr[j], ai = hi[j], sub_r = hr[j + 1], sub_i = 0.0;
+            const double absa = sqrt(ar * ar + ai * ai), absb = sqrt(sub_r * sub_r + sub_i * sub_i);
+            if (absb == 0) { c = 1; sr = 0; si = 0; }
+            else if (absa == 0) { c = 0; sr = 1; si = 0; }
+            else {
+                const double scale = absa + absb;
+                const double nrmab = scale * sqrt((absa / scale) * (absa / scale) + (absb / scale) * (absb / scale));
+                const double alr = ar / absa, ali = ai / absa;
+                c = absa / nrmab;
+                sr = (alr * sub_r + ali * sub_i) / nrmab;
+                si = (ali * sub_r - alr * sub_i) / nrmab;
+            }
+            hr[j] = c * ar + (sr * sub_r - si * sub_i);
+            hi[j] = c * ai + (sr * sub_i + si * sub_r);
+            hr[j + 1] = 0;
+            hi[j + 1] = 0;
+        }
+        c = __shfl(c, j);
+        sr = __shfl(sr, j);
+        si = __shfl(si, j);
+        if (lane > j && lane <= k) {
+            const double h1r = hr[j], h1i = hi[j], h2r = hr[j + 1], h2i = hi[j + 1];
+            hr[j] = c * h1r + (sr * h2r - si * h2i);
+            hi[j] = c * h1i + (sr * h2i + si * h2r);
+            hr[j + 1] = -(sr * h1r + si * h1i) + c * h2r;
+            hi[j + 1] = -(sr * h1i - si * h1r) + c * h2i;
+        }
+    }
+    double yr = 0, yi = 0;
+#pragma unroll
+    for (int i = K - 1; i >= 0; --i) {
+        if (i >= kk) continue;
+        double pr = 0, pim = 0;
+        if (lane > i && lane < kk) {
+            pr = hr[i] * yr - hi[i] * yi;
+            pim = hr[i] * yi + hi[i] * yr;
+        }
+#pragma unroll
+        for (int m = 1; m < 8; m <<= 1) {
+            pr += __shfl_xor(pr, m);
+            pim += __shfl_xor(pim, m);
+        }
+        const double gir = __shfl(hr[i], k), gii = __shfl(hi[i], k);
+        if (lane == i) {
+            const double accr = gir - pr, acci = gii - pim;
+            const double dr = hr[i], di = hi[i];
+            const double den = dr * dr + di * di;
+            if (den != 0) {
+                yr = (accr * dr + acci * di) / den;
+                yi = (acci * dr - accr * di) / den;
+            }
+        }
+    }
+    if (lane < k) ys[lane] = cx<double>{yr, yi};
+}
+
+// The end of an inner solve of k <= kLsqRegs steps in one kernel: every
+// workgroup reduces the last step's dot partials (columns 0..k, column k the
+// candidate w itself: the Hessenberg column and ||w||^2, as
+// k_gs_update_scale_r does), solves the least squares (wave 0, registers),
+// and writes its share of z = Z y in FP64.  Replaces the last step's
+// normalization, the least-squares launch and the combination launch.
+constexpr int kLsqRegs = 8;
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void k_inner_finish(const cx<double> *__restrict__ dpart, int G,
+                                                            const cx<double> *__restrict__ Hin,
+                                                            const double *__restrict__ nrm, int k,
+                                                            const cx<R> *__restrict__ Z, int64_t n,
+                                                            cx<double> *__restrict__ zout) {
+    __shared__ cx<double> hl[kLsqRegs + 1];
+    __shared__ cx<double> ys[kLsqRegs];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int c = wid; c <= k; c += kRedBlock / 64) {
+        const double re = wave_total(&dpart[(int64_t)c * G].x, G, 2);
+        const double im = wave_total(&dpart[(int64_t)c * G].y, G, 2);
+        if (lane == 0) hl[c] = cx<double>{re, im};
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {  // wave 0
+        double hh = 0;
+        for (int c = 0; c < k; ++c) hh += hl[c].x * hl[c].x + hl[c].y * hl[c].y;
+        const double b2 = hl[k].x - hh;
+        const double beta_last = b2 > 0 ? sqrt(b2) : 0.0;
+        hess_lsq_wave<kLsqRegs>(
+            k, nrm[k],
+            [Hin, k](int i, int j) { return j < k - 1 ? Hin[i + (int64_t)j * (k + 1)] : hl[i]; },
+            [nrm, k, beta_last](int j) { return j < k - 1 ? nrm[j] : beta_last; }, ys);
+    }
+    __syncthreads();
+    cx<double> y[kLsqRegs];
+#pragma unroll
+    for (int u = 0; u < kLsqRegs; ++u) y[u] = ys[u];
+    for (int64_t q = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; q < n; q += (int64_t)gridDim.x * kRedBlock) {
+        cx<R> v[kLsqRegs];
+#pragma unroll
+        for (int u = 0; u < kLsqRegs; ++u) v[u] = u < k ? Z[q + (int64_t)u * n] : cx<R>{0, 0};
+        double re = 0, im = 0;
+#pragma unroll
+        for (int u = 0; u < kLsqRegs; ++u) {
+            if (u >= k) break;
+            re += (double)v[u].x * y[u].x - (double)v[u].y * y[u].y;
+            im += (double)v[u].x * y[u].y + (double)v[u].y * y[u].x;
+        }
+        zout[q] = cx<double>{re, im};
+    }
+}
+
 // scatter a sparse vector into a zeroed dense one
 template <typename R>
 __global__ void k_scatter(cx<R> *__restrict__ y, const int64_t *__restrict__ idx, const double *__restrict__ v,
@@ -526,6 +673,9 @@ struct pimi_workspace {
     double inner_graph_scale = 0.0;
     int inner_graph_width = -1;  // banded tile width index the graph was captured with
     bool use_graphs = false;  // PIMI_INNER_GRAPH=1: the inner solve as a captured HIP graph
+    // the inner solve's last step, least squares and combination in one launch
+    // (PIMI_INNER_FINISH=0: three launches, k_inner_lsq on one lane)
+    bool inner_finish = true;
     double *hpin = nullptr;  // pinned host staging (small results)
     bool trace_progress = false;  // PIMI_TRACE=1: solver progress lines on stderr
     bool weighted = false;        // current stationary solve iterates on u = S x
@@ -1203,10 +1353,17 @@ static void inner_solve_dev(pimi_workspace *W, int kin, double scale, const cx<d
         // keeps the inner basis orthogonal to far below its tolerance.  The
         // candidate w sits in the basis slot it will occupy, so one reduction
         // pass gives V^H w and ||w||^2 and one kernel updates and normalizes.
+        const bool last = i + 1 == kin;
         if (i + 1 <= kFusedDotCols) {
             hipLaunchKernelGGL(k_dots_partial<R>, dim3(kDotBlocks, i + 2), bv, 0, W->stream, Vi, n, wi, n,
                                W->partial.p);
-            const bool last = i + 1 == kin;
+            if (last && kin <= kLsqRegs && W->inner_finish) {
+                // the last normalization, the least squares and z = Z y in one launch
+                hipLaunchKernelGGL(k_inner_finish<R>, gv, bv, 0, W->stream, W->partial.p, kDotBlocks, W->Hin.p, nrm,
+                                   kin, Zi, n, zout);
+                HIPCHK(hipGetLastError());
+                return;
+            }
             hipLaunchKernelGGL(k_gs_update_scale_r<R>, dim3(last ? 1 : kNormBlocks), bv, 0, W->stream, wi, Vi, n,
                                i + 1, W->partial.p, kDotBlocks, hcol, nrm + i, n, last ? 0 : 1);
         } else {
@@ -2557,6 +2714,8 @@ pimi_status pimi_workspace_create(pimi_plan *P, int32_t precision, int32_t krylo
         const char *ig = std::getenv("PIMI_INNER_GRAPH");
         const char *ng = std::getenv("PIMI_NO_GRAPH");
         W->use_graphs = ig && ig[0] == '1' && !(ng && ng[0] == '1');
+        const char *fi = std::getenv("PIMI_INNER_FINISH");
+        W->inner_finish = !(fi && fi[0] == '0');
         HIPCHK(hipStreamCreateWithFlags(&W->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&W->ev0));
         HIPCHK(hipEventCreate(&W->ev1));

@@ -512,6 +512,27 @@ def test_inner_graph_replays_match_eager(monkeypatch):
     assert np.array_equal(out[0].state.data, out[1].state.data)
 
 
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-8)])
+def test_inner_finish_kernel_matches_three_launches(monkeypatch, precision, tol):
+    """The fused end of the inner solve (last reduction, wave-parallel Givens
+    least squares in registers, z = Z y in one launch) gives the directions of
+    the three-launch form (normalization, one-lane least squares, combination):
+    the same outer iteration count and the same stationary state."""
+    alg = pimi.GMRESAlgorithm(200, 400, "sector_inner", inner_steps=6)
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("PIMI_INNER_FINISH", flag)
+        comp = pimi.compile(_c2_model(40))
+        out.append(pimi.stationary_state(comp, algorithm=alg, atol=0.0, rtol=1e-12, return_info=True,
+                                         criterion="weighted", weighting="sector", precision=precision))
+    err = _relmax(out[0].state.data, out[1].state.data)
+    print(f"  inner finish precision={precision}: iterations {out[0].info['iterations']} / "
+          f"{out[1].info['iterations']}, state rel diff {err:.2e} / {tol:g}", flush=True)
+    assert out[0].info["converged"] and out[1].info["converged"]
+    assert abs(out[0].info["iterations"] - out[1].info["iterations"]) <= (1 if precision == 0 else 2)
+    assert err <= tol
+
+
 def _gold():
     import json
     import os
