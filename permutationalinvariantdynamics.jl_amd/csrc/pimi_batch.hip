@@ -474,7 +474,8 @@ constexpr int kSlotRho = 10, kSlotInvRho = 11, kSlotE = 12;  // e: complex at 12
 
 // rho, 1/rho, e per job (one wave each) and the update coefficients
 // coef[0:j] = s, coef[j:2j] = t - e s; a = reductions (V^H p, V^H w),
-// c = coef, d = scalar slots, ncol = j + 1
+// c = coef, d = scalar slots, ncol = j + 1; b, when set, a host mirror of a
+// and (from entry u0) the scalars
 __global__ __launch_bounds__(64) void kb_dcgs_coef(const BVec *__restrict__ S) {
     const BVec J = S[blockIdx.x];
     const int j = J.ncol - 1;
@@ -505,6 +506,17 @@ __global__ __launch_bounds__(64) void kb_dcgs_coef(const BVec *__restrict__ S) {
         slot[kSlotInvRho] = inv;
         slot[kSlotE] = e.x;
         slot[kSlotE + 1] = e.y;
+    }
+    if (J.b) {  // one job's outer loop: the mail and scalars straight into pinned host memory
+        cx<double> *hb = (cx<double> *)const_cast<void *>(J.b);
+        for (int c = threadIdx.x; c < 2 * J.ncol; c += 64) hb[c] = sp[c];
+        if (threadIdx.x == 0) {
+            double *hs = (double *)(hb + J.u0);
+            hs[kSlotRho] = rho;
+            hs[kSlotInvRho] = inv;
+            hs[kSlotE] = e.x;
+            hs[kSlotE + 1] = e.y;
+        }
     }
 }
 
@@ -858,14 +870,15 @@ int dcgs_single_nblk(int64_t n) { return blocks_for(n); }
 size_t dcgs_single_desc_bytes(int m) { return sizeof(BVec) * 4 * (size_t)m; }
 
 void dcgs_single_descs(void *host, int m, void *V, const void *w, void *partial, void *mail, void *h, double *slot,
-                       int64_t n) {
+                       int64_t n, cx<double> *look, int64_t look_stride) {
     BVec *d = static_cast<BVec *>(host);
     const int nblk = blocks_for(n);
     for (int j = 0; j < m; ++j) {
         const int nc = j + 1;
         d[4 * j + 0] = BVec{0, V, w, partial, nullptr, n, nblk, nc, 0.0, 1};
         d[4 * j + 1] = BVec{0, partial, nullptr, mail, nullptr, n, nblk, nc, 0.0, 0};
-        d[4 * j + 2] = BVec{0, mail, nullptr, h, slot, n, nblk, nc, 0.0, 0};
+        d[4 * j + 2] = BVec{0, mail, look + (size_t)(j & 1) * (size_t)look_stride, h, slot, n, nblk, nc, 0.0,
+                            (uint64_t)(2 * ((int64_t)m + 1))};
         d[4 * j + 3] = BVec{0, V, h, const_cast<void *>(w), slot, n, nblk, nc, 0.0, 0};
     }
 }

@@ -585,11 +585,14 @@ void ensure_plan_weighting(pimi_plan *P);
 // pass writes V[:, j] = (p - Q s)/rho and V[:, j+1] = (w - e p - Q(t - e s))/rho.
 // Descriptors for j = 0 .. m-1 are built once on the host (dcgs_single_descs,
 // dcgs_single_desc_bytes) and uploaded; partial holds 2 (m + 1) nblk, mail
-// 2 (m + 1), h 2 m complex doubles, slot 16 doubles.
+// 2 (m + 1), h 2 m complex doubles, slot 16 doubles.  look (pinned host
+// memory, two regions of look_stride complex doubles) receives step j's mail
+// and scalars in region j mod 2, written by the step's coefficient kernel:
+// mail entries first, the scalars as doubles from entry 2 (m + 1).
 int dcgs_single_nblk(int64_t n);
 size_t dcgs_single_desc_bytes(int m);
 void dcgs_single_descs(void *host, int m, void *V, const void *w, void *partial, void *mail, void *h, double *slot,
-                       int64_t n);
+                       int64_t n, cx<double> *look, int64_t look_stride);
 void dcgs_single_step(hipStream_t st, const void *desc_dev, int j, int64_t n);
 // The same steps for a Krylov exponential's factorization, every step
 // reporting in a region of its own so that all of them run before one read

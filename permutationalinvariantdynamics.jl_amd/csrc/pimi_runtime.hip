@@ -1545,11 +1545,14 @@ static void ensure_dcgs(pimi_workspace *W) {
     W->dc_mail.alloc((size_t)2 * (m + 1));
     W->dc_h.alloc((size_t)2 * m + 2);
     W->dc_slot.alloc(16);
+    // fine-grained pinned memory: the coefficient kernel writes each step's
+    // mail and scalars here, readable once the step's event has completed
+    HIPCHK(hipHostMalloc((void **)&W->dc_look, 2 * dc_slot_entries(m) * sizeof(cx<double>), hipHostMallocCoherent));
     std::vector<char> host(dcgs_single_desc_bytes(m));
-    dcgs_single_descs(host.data(), m, W->V.p, W->w.p, W->dc_partial.p, W->dc_mail.p, W->dc_h.p, W->dc_slot.p, n);
+    dcgs_single_descs(host.data(), m, W->V.p, W->w.p, W->dc_partial.p, W->dc_mail.p, W->dc_h.p, W->dc_slot.p, n,
+                      W->dc_look, (int64_t)dc_slot_entries(m));
     W->dc_desc.alloc(host.size());
     h2d(W, W->dc_desc.p, host.data(), host.size());
-    HIPCHK(hipHostMalloc((void **)&W->dc_look, 2 * dc_slot_entries(m) * sizeof(cx<double>), hipHostMallocDefault));
     W->dc_ready = true;
 }
 
@@ -1648,12 +1651,9 @@ static GmresResult fgmres_device(pimi_workspace *W, double invscale, double atol
         auto step = [&](int j) {
             inner(V + (int64_t)j * n, Z + (int64_t)j * n);
             steady_apply(W, Z + (int64_t)j * n, W->w.p, invscale);
+            // the step's coefficient kernel writes its mail and scalars into
+            // dc_look region j mod 2 (pinned host memory)
             dcgs_single_step(W->stream, W->dc_desc.p, j, n);
-            cx<double> *slot = W->dc_look + (size_t)(j & 1) * se;
-            HIPCHK(hipMemcpyAsync(slot, W->dc_mail.p, sizeof(cx<double>) * 2 * (j + 1), hipMemcpyDeviceToHost,
-                                  W->stream));
-            HIPCHK(hipMemcpyAsync(slot + 2 * ((size_t)m + 1), W->dc_slot.p, 16 * sizeof(double),
-                                  hipMemcpyDeviceToHost, W->stream));
             HIPCHK(hipEventRecord(W->look_ev[j & 1], W->stream));
         };
         std::vector<cd> tprev, col;
