@@ -739,7 +739,11 @@ def main():
         "roofline": {"bound": "mfma", "achieved": tfs_gemm, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
                      "frac": tfs_gemm / FP64_MFMA_PEAK_TFS,
                      "traffic": measured_traffic("k_sector_gemm_mfma<double>"),
-                     "kernel": "k_sector_gemm_mfma<double>", "avg_ms": ms_gemm, "algorithmic_flops": flops_gemm},
+                     "kernel": "k_sector_gemm_mfma<double>", "avg_ms": ms_gemm, "algorithmic_flops": flops_gemm,
+                     "flop_count": ("4 sum m^3 per launch: this model's eigenvector factor U is real, so each "
+                                    "product is complex by real (8 sum m^3 for a complex U)"
+                                    if flops_gemm < 0.75 * 8 * sum(m ** 3 for m in comp.basis.dims)
+                                    else "8 sum m^3 per launch (complex factor U)")},
         "spmv": spmv,
         "batched": batch,
         "batched_n120": batch120,
