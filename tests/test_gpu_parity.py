@@ -494,6 +494,27 @@ def test_inner_outer_small_spaces(N_, preconditioner):
     assert err <= 1e-9, err
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("inner_steps", [1, 8, 9, 16, 20])
+def test_inner_outer_every_inner_form(inner_steps, precision):
+    """Every form of the inner solve's Gram-Schmidt and ending against the
+    oracle: fused reduction + update-normalize steps ending in one launch
+    (<= 8 steps), ending in three launches (9..16), and steps past 16 on the
+    two-pass form with accumulated Hessenberg columns (20)."""
+    comp = pimi.compile(_c2_model(20))
+    ref, _ = O.compile_plan(20, 2, _c2_terms())
+    alg = pimi.GMRESAlgorithm(200, 400, "sector_inner", inner_steps=inner_steps)
+    res = pimi.stationary_state(comp, algorithm=alg, atol=0.0, rtol=1e-12, return_info=True,
+                                criterion="backward", precision=precision)
+    x_ref, info_ref = ref.steady_state(krylovdim=400, maxiter=400, atol=0.0, rtol=1e-13, preconditioner="sector",
+                                       criterion="backward")
+    assert res.info["converged"] and info_ref["converged"]
+    err = np.linalg.norm(res.state.data - x_ref) / np.linalg.norm(x_ref)
+    print(f"  inner_steps={inner_steps} precision={precision}: outer {res.info['iterations']} err {err:.2e}",
+          flush=True)
+    assert err <= 1e-9, err
+
+
 def test_inner_graph_replays_match_eager(monkeypatch):
     """The captured inner-solve graph, replayed for every outer step (with the
     host's own work and synchronisation between replays), gives bitwise the
