@@ -474,14 +474,13 @@ constexpr int kSlotRho = 10, kSlotInvRho = 11, kSlotE = 12;  // e: complex at 12
 
 // rho, 1/rho, e per job (one wave each) and the update coefficients
 // coef[0:j] = s, coef[j:2j] = t - e s; a = reductions (V^H p, V^H w),
-// c = coef, d = scalar slots, ncol = j + 1; b, when set, a host mirror of a
-// and (from entry u0) the scalars
-__global__ __launch_bounds__(64) void kb_dcgs_coef(const BVec *__restrict__ S) {
-    const BVec J = S[blockIdx.x];
-    const int j = J.ncol - 1;
-    const cx<double> *sp = (const cx<double> *)J.a, *tw = sp + J.ncol;
+// c = coef, d = scalar slots, ncol = j + 1
+// rho, 1/rho and e of a delayed step from its reductions sp = V^H p, tw = V^H w
+// (ncol = j + 1 entries each); one wave, every lane gets them
+__device__ __forceinline__ void dcgs_scalars(const cx<double> *sp, const cx<double> *tw, int j, double &rho,
+                                             double &inv, cx<double> &e) {
     double ss = 0, str = 0, sti = 0;
-    for (int c = threadIdx.x; c < j; c += 64) {
+    for (int c = (int)(threadIdx.x & 63); c < j; c += 64) {
         const cx<double> a = sp[c], b = tw[c];
         ss += a.x * a.x + a.y * a.y;
         str += a.x * b.x + a.y * b.y;
@@ -491,9 +490,18 @@ __global__ __launch_bounds__(64) void kb_dcgs_coef(const BVec *__restrict__ S) {
     str = wave_sum(str);
     sti = wave_sum(sti);
     const double r2 = sp[j].x - ss;
-    const double rho = r2 > 0 ? sqrt(r2) : 0.0;
-    const double inv = rho > 0 ? 1.0 / rho : 0.0;
-    const cx<double> e = rho > 0 ? cx<double>{(tw[j].x - str) / r2, (tw[j].y - sti) / r2} : cx<double>{0, 0};
+    rho = r2 > 0 ? sqrt(r2) : 0.0;
+    inv = rho > 0 ? 1.0 / rho : 0.0;
+    e = rho > 0 ? cx<double>{(tw[j].x - str) / r2, (tw[j].y - sti) / r2} : cx<double>{0, 0};
+}
+
+__global__ __launch_bounds__(64) void kb_dcgs_coef(const BVec *__restrict__ S) {
+    const BVec J = S[blockIdx.x];
+    const int j = J.ncol - 1;
+    const cx<double> *sp = (const cx<double> *)J.a, *tw = sp + J.ncol;
+    double rho, inv;
+    cx<double> e;
+    dcgs_scalars(sp, tw, j, rho, inv, e);
     cx<double> *coef = (cx<double> *)J.c;
     for (int c = threadIdx.x; c < j; c += 64) {
         const cx<double> a = sp[c], b = tw[c];
@@ -506,17 +514,6 @@ __global__ __launch_bounds__(64) void kb_dcgs_coef(const BVec *__restrict__ S) {
         slot[kSlotInvRho] = inv;
         slot[kSlotE] = e.x;
         slot[kSlotE + 1] = e.y;
-    }
-    if (J.b) {  // one job's outer loop: the mail and scalars straight into pinned host memory
-        cx<double> *hb = (cx<double> *)const_cast<void *>(J.b);
-        for (int c = threadIdx.x; c < 2 * J.ncol; c += 64) hb[c] = sp[c];
-        if (threadIdx.x == 0) {
-            double *hs = (double *)(hb + J.u0);
-            hs[kSlotRho] = rho;
-            hs[kSlotInvRho] = inv;
-            hs[kSlotE] = e.x;
-            hs[kSlotE + 1] = e.y;
-        }
     }
 }
 
@@ -554,6 +551,97 @@ __global__ __launch_bounds__(kRedBlock) void kb_dcgs_update(const BVec *__restri
             for (int c = threadIdx.x; c < cn; c += kRedBlock) {
                 ca[c] = coef[c0 + c];
                 cb[c] = coef[j + c0 + c];
+            }
+            __syncthreads();
+            if (k < J.n) {
+                const cx<R> *vk = V + k + (int64_t)c0 * J.n;
+                int c = 0;
+                for (; c + kUpdCols <= cn; c += kUpdCols) {
+                    cx<R> v[kUpdCols];
+#pragma unroll
+                    for (int u = 0; u < kUpdCols; ++u) v[u] = vk[(int64_t)(c + u) * J.n];
+#pragma unroll
+                    for (int u = 0; u < kUpdCols; ++u) acc(v[u], c + u);
+                }
+                for (; c < cn; ++c) acc(vk[(int64_t)c * J.n], c);
+            }
+        }
+        if (k < J.n) {
+            const double px = pcol[k].x, py = pcol[k].y, wx = w[k].x, wy = w[k].y;
+            pcol[k] = cx<R>{(R)((px - ar) * inv), (R)((py - ai) * inv)};
+            ucol[k] = cx<R>{(R)((wx - (er * px - ei * py) - br) * inv),
+                            (R)((wy - (er * py + ei * px) - bi) * inv)};
+        }
+        __syncthreads();
+    }
+}
+
+// kb_dcgs_coef and kb_dcgs_update of one job in one launch.  Every
+// workgroup's first wave forms rho, 1/rho and e from the reductions (the
+// coefficient kernel's arithmetic), the update coefficients s and t - e s are
+// formed per column chunk in LDS, and workgroup 0 writes the scalar slots and,
+// when the coefficient descriptor's b is set, mirrors the reductions and the
+// scalars into that pinned host memory (scalars from entry u0).  S[0] is the
+// step's coefficient descriptor, S[1] its update descriptor.  Bitwise the
+// results of the two launches.
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void kb_dcgs_coef_update(const BVec *__restrict__ S, int64_t total) {
+    __shared__ cx<double> ca[kColChunk], cb[kColChunk];
+    __shared__ double scal[3];
+    const BVec C = S[0], J = S[1];
+    const int j = C.ncol - 1;
+    const cx<double> *sp = (const cx<double> *)C.a, *tw = sp + C.ncol;
+    if (threadIdx.x < 64) {
+        double rho, inv;
+        cx<double> e;
+        dcgs_scalars(sp, tw, j, rho, inv, e);
+        if (threadIdx.x == 0) {
+            scal[0] = inv;
+            scal[1] = e.x;
+            scal[2] = e.y;
+        }
+        if (blockIdx.x == 0) {
+            if (threadIdx.x == 0) {
+                double *slot = (double *)C.d;
+                slot[kSlotRho] = rho;
+                slot[kSlotInvRho] = inv;
+                slot[kSlotE] = e.x;
+                slot[kSlotE + 1] = e.y;
+            }
+            if (C.b) {
+                cx<double> *hb = (cx<double> *)const_cast<void *>(C.b);
+                for (int c = threadIdx.x; c < 2 * C.ncol; c += 64) hb[c] = sp[c];
+                if (threadIdx.x == 0) {
+                    double *hs = (double *)(hb + C.u0);
+                    hs[kSlotRho] = rho;
+                    hs[kSlotInvRho] = inv;
+                    hs[kSlotE] = e.x;
+                    hs[kSlotE + 1] = e.y;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const double inv = scal[0], er = scal[1], ei = scal[2];
+    cx<R> *V = (cx<R> *)J.a;
+    const cx<R> *w = (const cx<R> *)J.c;
+    cx<R> *pcol = V + (int64_t)j * J.n, *ucol = V + (int64_t)(j + 1) * J.n;
+    for (int64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const int64_t k = vb * kRedBlock + threadIdx.x;
+        double ar = 0, ai = 0, br = 0, bi = 0;
+        auto acc = [&](const cx<R> &v, int c) {
+            ar += (double)v.x * ca[c].x - (double)v.y * ca[c].y;
+            ai += (double)v.x * ca[c].y + (double)v.y * ca[c].x;
+            br += (double)v.x * cb[c].x - (double)v.y * cb[c].y;
+            bi += (double)v.x * cb[c].y + (double)v.y * cb[c].x;
+        };
+        for (int c0 = 0; c0 < j; c0 += kColChunk) {
+            const int cn = min(kColChunk, j - c0);
+            __syncthreads();
+            for (int c = threadIdx.x; c < cn; c += kRedBlock) {
+                const cx<double> a = sp[c0 + c], b = tw[c0 + c];
+                ca[c] = a;
+                cb[c] = cx<double>{b.x - (er * a.x - ei * a.y), b.y - (er * a.y + ei * a.x)};
             }
             __syncthreads();
             if (k < J.n) {
@@ -910,9 +998,8 @@ void dcgs_single_step(hipStream_t st, const void *desc_dev, int j, int64_t n) {
     const int64_t tdots = nblk * ((nc + kDotCols - 1) / kDotCols);  // one chunk per workgroup (u0 = 1)
     hipLaunchKernelGGL(kb_dots2_partial<double>, dim3(grid8(tdots)), dim3(kRedBlock), 0, st, d + 0, 1, tdots);
     hipLaunchKernelGGL(kb_reduce_cols2, dim3(grid8(nc)), dim3(64), 0, st, d + 1, 1, nc);
-    hipLaunchKernelGGL(kb_dcgs_coef, dim3(1), dim3(64), 0, st, d + 2);
     const int64_t tupd = (n + kRedBlock - 1) / kRedBlock;
-    hipLaunchKernelGGL(kb_dcgs_update<double>, dim3(grid8(tupd)), dim3(kRedBlock), 0, st, d + 3, 1, tupd);
+    hipLaunchKernelGGL(kb_dcgs_coef_update<double>, dim3(grid8(tupd)), dim3(kRedBlock), 0, st, d + 2, tupd);
 }
 
 }  // namespace pimi
