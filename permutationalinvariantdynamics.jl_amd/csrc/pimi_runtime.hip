@@ -1320,7 +1320,7 @@ static void ensure_inner(pimi_workspace *W, int kin) {
 // The inner solve with no host interaction: norms, Hessenberg, least squares
 // and the combination all stay on the device.  Right-preconditioned GMRES on
 // D/s: the directions z_i = M^-1 v_i are kept (Z), so the result M^-1 V y is
-// the combination Z y and needs no fifth preconditioner application.
+// the combination Z y and needs no preconditioner application of its own.
 // Reads rin, writes zout (n entries each, FP64).
 template <typename R>
 static void inner_solve_dev(pimi_workspace *W, int kin, double scale, const cx<double> *rin, cx<double> *zout) {
